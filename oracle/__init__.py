@@ -1,0 +1,204 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper over ``liboracle.so``, the CPU restatement of Chroma's even-odd Wilson-clover path
+(``oracle.cpp``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, as the checker or as the timed CPU baseline.
+
+Host layout (identical to the product C-ABI): full-lattice arrays, checkerboard-major, x-fastest
+within a checkerboard; fermions ``complex128[V, 4, 3]``; gauge ``complex128[4, V, 3, 3]``.
+
+Parity status of every oracle function (what pins it) is listed in ``oracle/README.md``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
+CG, BICGSTAB, RELIABLE_CG = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_op_create.restype = ctypes.c_void_p
+        L.orc_op_create.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_op_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_op_apply.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        L.orc_op_apply_f32.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_int]
+        L.orc_op_eo_piece.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.orc_dslash.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_dslash_f32.argtypes = [_ip, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
+        L.orc_mesfield.argtypes = [_ip, _dp, _dp]
+        L.orc_make_clover.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_clover_invert.argtypes = [_ip, _dp, ctypes.c_int, _dp, _lp]
+        L.orc_clover_apply.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_op_logdet_even.argtypes = [ctypes.c_void_p, _dp, _lp]
+        L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_double, _ip, _dp, _dp, ctypes.c_int, _ip]
+        L.orc_qprop.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_double, _ip, _dp]
+        L.orc_quarkprop4_point.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_double, _ip, _dp, _dp, _dp]
+        L.orc_time.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_site_coords.argtypes = [_ip, ctypes.c_long, _ip]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def _dims(L):
+    return (ctypes.c_int * 4)(*[int(v) for v in L])
+
+
+def _d(a):
+    assert a.flags.c_contiguous and a.dtype in (np.complex128, np.float64)
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    assert a.flags.c_contiguous and a.dtype in (np.complex64, np.float32)
+    return a.ctypes.data_as(_fp)
+
+
+def set_threads(n=0):
+    return lib().orc_set_threads(int(n))
+
+
+def fermion_zeros(L):
+    return np.zeros((int(np.prod(L)), 4, 3), np.complex128)
+
+
+def dslash(L, U, psi, isign, cb, out=None):
+    """Hopping term D(isign) psi on output checkerboard ``cb``; other sites of ``out`` untouched."""
+    if U.dtype == np.complex64:
+        chi = np.zeros_like(psi) if out is None else out
+        _check(lib().orc_dslash_f32(_dims(L), _f(U), _f(chi), _f(psi), int(isign), int(cb)))
+        return chi
+    chi = np.zeros_like(psi) if out is None else out
+    _check(lib().orc_dslash(_dims(L), _d(U), _d(chi), _d(psi), int(isign), int(cb)))
+    return chi
+
+
+def mesfield(L, U):
+    V = int(np.prod(L))
+    F = np.zeros((6, V, 3, 3), np.complex128)
+    _check(lib().orc_mesfield(_dims(L), _d(U), _d(F)))
+    return F
+
+
+def make_clover(L, U, csw_r, csw_t, mass):
+    """Packed clover term, float64[V, 72] laid out as {diag[2][6], offd[2][15][2]}."""
+    V = int(np.prod(L))
+    tri = np.zeros((V, 72), np.float64)
+    _check(lib().orc_make_clover(_dims(L), _d(U), float(csw_r), float(csw_t), float(mass), _d(tri)))
+    return tri
+
+
+def clover_invert(L, tri, cb):
+    """In-place LDL^dag inversion on checkerboard cb; returns (sum log|d|, #sites with negative pivots)."""
+    ld = ctypes.c_double()
+    nn = ctypes.c_long()
+    _check(lib().orc_clover_invert(_dims(L), _d(tri), int(cb), ctypes.byref(ld), ctypes.byref(nn)))
+    return ld.value, nn.value
+
+
+def clover_apply(L, tri, psi, cb, out=None):
+    chi = np.zeros_like(psi) if out is None else out
+    _check(lib().orc_clover_apply(_dims(L), _d(tri), _d(chi), _d(psi), int(cb)))
+    return chi
+
+
+def site_coords(L, idx):
+    x = (ctypes.c_int * 4)()
+    lib().orc_site_coords(_dims(L), int(idx), x)
+    return tuple(x)
+
+
+class Op:
+    """An oracle linear operator (kind = UNPREC_WILSON, EO_WILSON or EO_CLOVER)."""
+
+    def __init__(self, kind, L, U, mass, csw_r=0.0, csw_t=0.0):
+        self.L = tuple(int(v) for v in L)
+        self.V = int(np.prod(L))
+        self.kind = kind
+        U = np.ascontiguousarray(U, np.complex128)
+        self.h = lib().orc_op_create(int(kind), _dims(L), _d(U), float(mass), float(csw_r), float(csw_t))
+        if not self.h:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_op_destroy(self.h)
+            self.h = None
+
+    def __call__(self, psi, isign=+1, out=None):
+        if psi.dtype == np.complex64:
+            chi = np.zeros_like(psi) if out is None else out
+            _check(lib().orc_op_apply_f32(self.h, _f(chi), _f(psi), int(isign)))
+            return chi
+        chi = np.zeros_like(psi) if out is None else out
+        _check(lib().orc_op_apply(self.h, _d(chi), _d(psi), int(isign)))
+        return chi
+
+    def piece(self, which, psi, isign=+1):
+        names = {"evenEven": 0, "evenEvenInv": 1, "evenOdd": 2, "oddEven": 3, "oddOdd": 4, "unprec": 5}
+        chi = np.zeros_like(psi)
+        _check(lib().orc_op_eo_piece(self.h, names[which], _d(chi), _d(psi), int(isign)))
+        return chi
+
+    def logdet_even(self):
+        ld = ctypes.c_double()
+        nn = ctypes.c_long()
+        _check(lib().orc_op_logdet_even(self.h, ctypes.byref(ld), ctypes.byref(nn)))
+        return ld.value, nn.value
+
+    def solve(self, chi, psi=None, solver=CG, rsd=1e-8, maxit=1000, delta=0.1, trace_cap=100000):
+        psi = np.zeros_like(chi) if psi is None else psi.copy()
+        nc = ctypes.c_int()
+        rs = ctypes.c_double()
+        tr = np.zeros(trace_cap, np.float64)
+        tl = ctypes.c_int()
+        _check(lib().orc_solve(self.h, int(solver), _d(psi), _d(chi), float(rsd), int(maxit), float(delta),
+                               ctypes.byref(nc), ctypes.byref(rs), _d(tr), int(trace_cap), ctypes.byref(tl)))
+        return psi, nc.value, rs.value, tr[: min(tl.value, trace_cap)]
+
+    def qprop(self, chi, solver=CG, rsd=1e-8, maxit=1000, delta=0.1):
+        psi = np.zeros_like(chi)
+        nc = ctypes.c_int()
+        rs = ctypes.c_double()
+        _check(lib().orc_qprop(self.h, int(solver), _d(psi), _d(chi), float(rsd), int(maxit), float(delta),
+                               ctypes.byref(nc), ctypes.byref(rs)))
+        return psi, nc.value, rs.value
+
+    def quarkprop4_point(self, src_site=0, solver=CG, rsd=1e-8, maxit=1000, delta=0.1, want_prop=False):
+        ncs = (ctypes.c_int * 12)()
+        res = np.zeros(12, np.float64)
+        corr = np.zeros(self.L[3], np.float64)
+        prop = np.zeros((self.V, 12, 12), np.complex128) if want_prop else None
+        _check(lib().orc_quarkprop4_point(self.h, int(solver), int(src_site), float(rsd), int(maxit), float(delta),
+                                          ncs, _d(res), _d(corr), _d(prop) if want_prop else None))
+        return list(ncs), res, corr, prop
+
+    def time(self, what, n):
+        t = ctypes.c_double()
+        _check(lib().orc_time(self.h, int(what), int(n), ctypes.byref(t)))
+        return t.value

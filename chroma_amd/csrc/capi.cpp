@@ -1,0 +1,347 @@
+// extern "C" entry points declared in include/chroma_mi355x.h.  No exception crosses this boundary:
+// every call returns CMX_OK or a CMX_ERR_* code and records the message for cmx_last_error().
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "op.h"
+#include "setup.h"
+
+struct cmx_gauge_s {
+  std::shared_ptr<cmx::GaugeData> gd;
+};
+struct cmx_op_s {
+  std::unique_ptr<cmx::Op> op;
+};
+
+namespace {
+thread_local std::string g_err;
+bool g_init = false;
+int g_device = -1;
+
+template <typename Fn> int guarded(Fn&& fn) {
+  try {
+    fn();
+    return CMX_OK;
+  } catch (const cmx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CMX_ERR_ARG;
+  } catch (...) {
+    g_err = "unknown error";
+    return CMX_ERR_ARG;
+  }
+}
+
+void require_init() { CMX_REQUIRE(g_init, CMX_ERR_STATE, "library not initialised: call cmx_init(device) first"); }
+cmx::Op& O(cmx_op h) {
+  CMX_REQUIRE(h && h->op, CMX_ERR_ARG, "null operator handle");
+  return *h->op;
+}
+}  // namespace
+
+extern "C" {
+
+const char* cmx_last_error(void) { return g_err.c_str(); }
+int cmx_version(void) { return 1; }
+
+int cmx_init(int device) {
+  return guarded([&] {
+    int n = 0;
+    CMX_HIP(hipGetDeviceCount(&n));
+    CMX_REQUIRE(device >= 0 && device < n, CMX_ERR_ARG, "cmx_init: no such device");
+    CMX_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CMX_HIP(hipGetDeviceProperties(&prop, device));
+    CMX_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, CMX_ERR_UNSUPPORTED,
+                std::string("cmx_init: built for gfx950, device is ") + prop.gcnArchName);
+    g_device = device;
+    g_init = true;
+  });
+}
+
+int cmx_finalize(void) {
+  return guarded([&] {
+    if (g_init) CMX_HIP(hipDeviceSynchronize());
+    g_init = false;
+  });
+}
+
+int cmx_gauge_create(const int dims[4], const double* links, cmx_gauge* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(out, CMX_ERR_ARG, "gauge_create: null output");
+    auto h = std::make_unique<cmx_gauge_s>();
+    h->gd = cmx::make_gauge(dims, links);
+    *out = h.release();
+  });
+}
+
+int cmx_gauge_destroy(cmx_gauge g) {
+  return guarded([&] { delete g; });
+}
+
+int cmx_op_create(cmx_gauge g, const cmx_op_params* p, cmx_op* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(g && p && out, CMX_ERR_ARG, "op_create: null argument");
+    auto h = std::make_unique<cmx_op_s>();
+    h->op.reset(cmx::make_op(g->gd, *p));
+    *out = h.release();
+  });
+}
+
+int cmx_op_destroy(cmx_op op) {
+  return guarded([&] { delete op; });
+}
+
+long cmx_op_subset_doubles(cmx_op op) {
+  if (!op || !op->op) return -1;
+  return 24L * op->op->subset_sites();
+}
+
+int cmx_op_apply(cmx_op h, double* chi, const double* psi, int isign, int prec) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(chi && psi, CMX_ERR_ARG, "op_apply: null array");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "op_apply: isign must be +1 or -1");
+    CMX_REQUIRE(prec >= 0 && prec <= 2, CMX_ERR_ARG, "op_apply: bad precision");
+    const int n = op.subset_sites();
+    cmx::Vec in(prec, n), out(prec, n);
+    op.pack(psi, n, in);
+    op.apply(prec, out.view(), in.view(), isign, nullptr);
+    op.unpack(chi, n, out);
+  });
+}
+
+int cmx_dslash_apply(cmx_op h, double* chi, const double* psi, int isign, int cb, int prec) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(chi && psi, CMX_ERR_ARG, "dslash_apply: null array");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "dslash_apply: isign must be +1 or -1");
+    CMX_REQUIRE(cb == 0 || cb == 1, CMX_ERR_ARG, "dslash_apply: cb must be 0 or 1");
+    CMX_REQUIRE(prec >= 0 && prec <= 2, CMX_ERR_ARG, "dslash_apply: bad precision");
+    const int n = (int)op.Vh;
+    cmx::Vec in(prec, n), out(prec, n);
+    op.pack(psi, n, in);
+    op.dslash(prec, out.view(), in.view(), isign, cb);
+    op.unpack(chi, n, out);
+  });
+}
+
+int cmx_op_eo_piece(cmx_op h, int which, double* chi, const double* psi, int isign) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.eo(), CMX_ERR_ARG, "op_eo_piece: operator is not even-odd preconditioned");
+    CMX_REQUIRE(chi && psi, CMX_ERR_ARG, "op_eo_piece: null array");
+    CMX_REQUIRE(which >= 0 && which <= 5, CMX_ERR_ARG, "op_eo_piece: bad block");
+    const int Vh = (int)op.Vh;
+    cmx::Vec in(cmx::P64, 2 * Vh), out(cmx::P64, 2 * Vh);
+    op.pack(psi, 2 * Vh, in);
+    cmx::HopLaunch h0 = op.hop_base(cmx::P64);
+    h0.isign = isign;
+    auto hop = [&](int cb, int epi, const void* clov, double a, double b, bool with_x) {
+      cmx::HopLaunch hh = h0;
+      hh.cb = cb;
+      cmx::VRef vin = in.view((1 - cb) * Vh), vout = out.view(cb * Vh), vx = in.view(cb * Vh);
+      hh.in = vin.v, hh.in_nrm = nullptr, hh.in_stride = vin.stride;
+      hh.out = vout.v, hh.out_nrm = nullptr, hh.out_stride = vout.stride;
+      if (with_x) hh.x = vx.v, hh.x_nrm = nullptr, hh.x_stride = vx.stride;
+      hh.epi = epi;
+      hh.clov = clov;
+      hh.a = a;
+      hh.b = b;
+      cmx::launch_hop(hh, op.stream);
+    };
+    // hop with a zero input gives b*D*0 = 0, so the "local" blocks below are the EPI_XPAY / EPI_CLOV_XPAY
+    // epilogues with b = 0 applied to a zero neighbour field
+    cmx::Vec zero(cmx::P64, 2 * Vh);
+    op.blas(cmx::B_ZERO, {&zero});
+    auto local = [&](int cb, const void* clov, double a) {
+      cmx::HopLaunch hh = h0;
+      hh.cb = cb;
+      cmx::VRef vz = zero.view((1 - cb) * Vh), vout = out.view(cb * Vh), vx = in.view(cb * Vh);
+      hh.in = vz.v, hh.in_stride = vz.stride;
+      hh.out = vout.v, hh.out_stride = vout.stride;
+      hh.x = vx.v, hh.x_stride = vx.stride;
+      hh.epi = clov ? cmx::EPI_CLOV_XPAY : cmx::EPI_XPAY;
+      hh.clov = clov;
+      hh.a = a;
+      hh.b = 0.0;
+      cmx::launch_hop(hh, op.stream);
+    };
+    const bool cl = op.clover();
+    switch (which) {
+      case 0: local(0, cl ? op.clov64_even.p : nullptr, op.fact); break;                    // A_ee
+      case 1: local(0, cl ? op.clovinv_even[cmx::P64].p : nullptr, 1.0 / op.fact); break;  // A_ee^-1
+      case 2: hop(0, cmx::EPI_NONE, nullptr, 0, -0.5, false); break;                        // D_eo = -1/2 D
+      case 3: hop(1, cmx::EPI_NONE, nullptr, 0, -0.5, false); break;                        // D_oe
+      case 4: local(1, cl ? op.clov_odd[cmx::P64].p : nullptr, op.fact); break;             // A_oo
+      case 5:  // unprecLinOp (eoprec_linop.h:147-165): chi_cb = A_cb psi_cb - 1/2 D psi_{1-cb}
+        hop(0, cl ? cmx::EPI_CLOV_XPAY : cmx::EPI_XPAY, cl ? op.clov64_even.p : nullptr, op.fact, -0.5, true);
+        hop(1, cl ? cmx::EPI_CLOV_XPAY : cmx::EPI_XPAY, cl ? op.clov_odd[cmx::P64].p : nullptr, op.fact, -0.5, true);
+        break;
+    }
+    CMX_HIP(hipGetLastError());
+    // write back only the target checkerboard(s)
+    std::unique_ptr<double[]> full(new double[(size_t)48 * Vh]);
+    op.unpack(full.get(), 2 * Vh, out);
+    const int lo = (which <= 2) ? 0 : (which == 5 ? 0 : Vh);
+    const int hi = (which <= 2) ? Vh : 2 * Vh;
+    std::memcpy(chi + (size_t)24 * lo, full.get() + (size_t)24 * lo, sizeof(double) * 24 * (hi - lo));
+  });
+}
+
+int cmx_clover_download(cmx_op h, int which, double* tri) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_download: not a clover operator");
+    CMX_REQUIRE(tri && (which == 0 || which == 1), CMX_ERR_ARG, "clover_download: bad argument");
+    const int Vh = (int)op.Vh;
+    cmx::DevBuf d((size_t)72 * sizeof(double) * Vh);
+    auto grab = [&](const cmx::DevBuf& src, int cb) {
+      cmx::clover_unpack(Vh, src.as<double2>(), d.as<double>(), op.stream);
+      CMX_HIP(hipGetLastError());
+      CMX_HIP(hipMemcpyAsync(tri + (size_t)72 * cb * Vh, d.p, (size_t)72 * sizeof(double) * Vh, hipMemcpyDeviceToHost,
+                             op.stream));
+      op.sync();
+    };
+    if (which == 0) {
+      grab(op.clov64_even, 0);
+      grab(op.clov_odd[cmx::P64], 1);
+    } else {
+      grab(op.clovinv_even[cmx::P64], 0);
+    }
+  });
+}
+
+int cmx_clover_logdet(cmx_op h, double* logdet, long* n_neg_sites) {
+  return guarded([&] {
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_logdet: not a clover operator");
+    if (logdet) *logdet = op.logdet_even;
+    if (n_neg_sites) *n_neg_sites = op.nneg_even;
+  });
+}
+
+int cmx_clover_pack_quda(cmx_op h, int which, int cb, double* out) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_pack_quda: not a clover operator");
+    CMX_REQUIRE(out && (which == 0 || (which == 1 && cb == 0)) && (cb == 0 || cb == 1), CMX_ERR_ARG,
+                "clover_pack_quda: bad argument");
+    const int Vh = (int)op.Vh;
+    std::unique_ptr<double[]> tri(new double[(size_t)72 * 2 * Vh]);
+    cmx::DevBuf d((size_t)72 * sizeof(double) * Vh);
+    const cmx::DevBuf& src = which == 1 ? op.clovinv_even[cmx::P64] : (cb == 0 ? op.clov64_even : op.clov_odd[cmx::P64]);
+    cmx::clover_unpack(Vh, src.as<double2>(), d.as<double>(), op.stream);
+    CMX_HIP(hipMemcpyAsync(tri.get(), d.p, (size_t)72 * sizeof(double) * Vh, hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+    // QUDAPackedClovSite: diag1[6], offDiag1[15][2], diag2[6], offDiag2[15][2]; the lower triangle
+    // column by column (packForQUDA, clover_term_qdp_w.h:2185-2222)
+    for (int i = 0; i < Vh; ++i) {
+      const double* t = tri.get() + (size_t)72 * i;
+      double* q = out + (size_t)72 * i;
+      for (int b = 0; b < 2; ++b) {
+        double* qb = q + 36 * b;
+        for (int k = 0; k < 6; ++k) qb[k] = t[6 * b + k];
+        int target = 0;
+        for (int col = 0; col < 5; ++col)
+          for (int row = col + 1; row < 6; ++row, ++target) {
+            const int src_i = row * (row - 1) / 2 + col;
+            qb[6 + 2 * target] = t[12 + 30 * b + 2 * src_i];
+            qb[6 + 2 * target + 1] = t[12 + 30 * b + 2 * src_i + 1];
+          }
+      }
+    }
+  });
+}
+
+int cmx_solve(cmx_op h, const cmx_solver_params* p, double* psi, const double* chi, int nrhs, cmx_solver_result* res) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(p && psi && chi && res && nrhs >= 1, CMX_ERR_ARG, "solve: bad argument");
+    const long nd = 24L * op.subset_sites();
+    for (int k = 0; k < nrhs; ++k) cmx::solve(op, *p, psi + k * nd, chi + k * nd, res[k]);
+  });
+}
+
+int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(ms_per_call && n > 0 && prec >= 0 && prec <= 2, CMX_ERR_ARG, "op_time: bad argument");
+    CMX_REQUIRE(what >= 0 && what <= 3, CMX_ERR_ARG, "op_time: bad selector");
+    const int Vh = (int)op.Vh;
+    const int ns = op.subset_sites();
+    cmx::Vec a(prec, ns), b(prec, ns);
+    // deterministic non-zero input
+    {
+      std::unique_ptr<double[]> hbuf(new double[(size_t)24 * ns]);
+      for (long k = 0; k < 24L * ns; ++k) hbuf[k] = std::sin(0.37 * (double)k) * 0.5;
+      op.pack(hbuf.get(), ns, a);
+    }
+    op.ensure_prec(prec);
+    auto body = [&]() {
+      switch (what) {
+        case 0: op.dslash(prec, b.view(), a.view(), +1, 1); break;
+        case 1: op.apply(prec, b.view(), a.view(), +1, nullptr); break;
+        case 2: {
+          cmx::HopLaunch hh = op.hop_base(prec);
+          hh.isign = 1;
+          hh.cb = 1;
+          cmx::Vec& t = op.tmp(prec, 0);
+          cmx::VRef vi = t.view(), vo = b.view(), vx = a.view();
+          hh.in = vi.v, hh.in_nrm = vi.nrm, hh.in_stride = vi.stride;
+          hh.out = vo.v, hh.out_nrm = vo.nrm, hh.out_stride = vo.stride;
+          hh.x = vx.v, hh.x_nrm = vx.nrm, hh.x_stride = vx.stride;
+          hh.epi = op.clover() ? cmx::EPI_CLOV_XPAY : cmx::EPI_XPAY;
+          hh.clov = op.clover() ? op.clov_odd[prec].p : nullptr;
+          hh.a = op.fact;
+          hh.b = -0.25;
+          cmx::launch_hop(hh, op.stream);
+          break;
+        }
+        case 3: cmx::stream_copy((long)cmx::spinor_chunks(prec) * Vh, a.buf.p, b.buf.p, op.stream); break;
+      }
+    };
+    body();  // warm-up
+    op.sync();
+    hipEvent_t e0, e1;
+    CMX_HIP(hipEventCreate(&e0));
+    CMX_HIP(hipEventCreate(&e1));
+    CMX_HIP(hipEventRecord(e0, op.stream));
+    for (int k = 0; k < n; ++k) body();
+    CMX_HIP(hipEventRecord(e1, op.stream));
+    CMX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CMX_HIP(hipGetLastError());
+    *ms_per_call = ms / n;
+  });
+}
+
+int cmx_layout_roundtrip(cmx_op h, int prec, const double* in, double* out) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(in && out && prec >= 0 && prec <= 2, CMX_ERR_ARG, "layout_roundtrip: bad argument");
+    const int n = (int)op.Vh;
+    cmx::Vec v(prec, n);
+    op.pack(in, n, v);
+    op.unpack(out, n, v);
+  });
+}
+
+}  // extern "C"

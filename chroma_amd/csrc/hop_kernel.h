@@ -1,0 +1,314 @@
+// Wilson hopping term on one checkerboard, one output site per lane, with fused epilogues.
+//
+//   chi(x) = sum_mu U_mu(x) (1 - isign gamma_mu) psi(x+mu) + U_mu^dag(x-mu) (1 + isign gamma_mu) psi(x-mu)
+// (QDPWilsonDslashT::apply, lwldslash_w.h:252-295, DeGrand-Rossi gammas).
+//
+// Spin projection: every DR gamma_mu is off-diagonal in 2x2 spin blocks, gamma_mu = [[0, B],[B^dag, 0]]
+// with B unitary and entries in {0, +-1, +-i}.  Hence (1 + s gamma_mu) psi = (h, s B^dag h) with the
+// half spinor h = psi_up + s B psi_down.  Only h (2 spins x 3 colours) is multiplied by the link,
+// and the lower two spins are rebuilt from it:
+//   mu=0: B = [[0,i],[i,0]]      h0 = p0 + s i p3, h1 = p1 + s i p2;  lo2 = -s i h1, lo3 = -s i h0
+//   mu=1: B = [[0,-1],[1,0]]     h0 = p0 - s p3,   h1 = p1 + s p2;    lo2 =  s h1,   lo3 = -s h0
+//   mu=2: B = [[i,0],[0,-i]]     h0 = p0 + s i p2, h1 = p1 - s i p3;  lo2 = -s i h0, lo3 =  s i h1
+//   mu=3: B = [[1,0],[0,1]]      h0 = p0 + s p2,   h1 = p1 + s p3;    lo2 =  s h0,   lo3 =  s h1
+// Forward terms use s = -isign, backward terms s = +isign.
+#pragma once
+#include "cmx_internal.h"
+
+namespace cmx {
+
+// h[2 spin][3 colour][re,im] = 12 reals; psi index 2*(spin*3+c)+ri
+template <int MU, int S, typename R>
+__device__ __forceinline__ void project(const R (&p)[24], R (&h)[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R p0r = p[2 * c], p0i = p[2 * c + 1];
+    const R p1r = p[2 * (3 + c)], p1i = p[2 * (3 + c) + 1];
+    const R p2r = p[2 * (6 + c)], p2i = p[2 * (6 + c) + 1];
+    const R p3r = p[2 * (9 + c)], p3i = p[2 * (9 + c) + 1];
+    R h0r, h0i, h1r, h1i;
+    if constexpr (MU == 0) {  // i*(a+ib) = -b + ia
+      h0r = p0r - S * p3i; h0i = p0i + S * p3r;
+      h1r = p1r - S * p2i; h1i = p1i + S * p2r;
+    } else if constexpr (MU == 1) {
+      h0r = p0r - S * p3r; h0i = p0i - S * p3i;
+      h1r = p1r + S * p2r; h1i = p1i + S * p2i;
+    } else if constexpr (MU == 2) {
+      h0r = p0r - S * p2i; h0i = p0i + S * p2r;
+      h1r = p1r + S * p3i; h1i = p1i - S * p3r;
+    } else {
+      h0r = p0r + S * p2r; h0i = p0i + S * p2i;
+      h1r = p1r + S * p3r; h1i = p1i + S * p3i;
+    }
+    h[2 * c] = h0r; h[2 * c + 1] = h0i;
+    h[6 + 2 * c] = h1r; h[6 + 2 * c + 1] = h1i;
+  }
+}
+
+template <int MU, int S, typename R>
+__device__ __forceinline__ void accumulate(R (&acc)[24], const R (&h)[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R h0r = h[2 * c], h0i = h[2 * c + 1], h1r = h[6 + 2 * c], h1i = h[6 + 2 * c + 1];
+    acc[2 * c] += h0r; acc[2 * c + 1] += h0i;
+    acc[2 * (3 + c)] += h1r; acc[2 * (3 + c) + 1] += h1i;
+    R l2r, l2i, l3r, l3i;
+    if constexpr (MU == 0) {  // -s i h: -s*(-b + ia) = s b - s i a
+      l2r = S * h1i; l2i = -S * h1r;
+      l3r = S * h0i; l3i = -S * h0r;
+    } else if constexpr (MU == 1) {
+      l2r = S * h1r; l2i = S * h1i;
+      l3r = -S * h0r; l3i = -S * h0i;
+    } else if constexpr (MU == 2) {
+      l2r = S * h0i; l2i = -S * h0r;   // -s i h0
+      l3r = -S * h1i; l3i = S * h1r;   //  s i h1
+    } else {
+      l2r = S * h0r; l2i = S * h0i;
+      l3r = S * h1r; l3i = S * h1i;
+    }
+    acc[2 * (6 + c)] += l2r; acc[2 * (6 + c) + 1] += l2i;
+    acc[2 * (9 + c)] += l3r; acc[2 * (9 + c) + 1] += l3i;
+  }
+}
+
+// out = U h (DAG=false) or U^dag h (DAG=true), for both spins of the half spinor
+template <bool DAG, typename R>
+__device__ __forceinline__ void link_mul(const R (&u)[18], const R (&h)[12], R (&o)[12]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R re = 0, im = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const R hr = h[6 * s + 2 * j], hi = h[6 * s + 2 * j + 1];
+        if constexpr (!DAG) {
+          const R ur = u[2 * (3 * i + j)], ui = u[2 * (3 * i + j) + 1];
+          re += ur * hr - ui * hi;
+          im += ur * hi + ui * hr;
+        } else {
+          const R ur = u[2 * (3 * j + i)], ui = u[2 * (3 * j + i) + 1];
+          re += ur * hr + ui * hi;
+          im += ur * hi - ui * hr;
+        }
+      }
+      o[6 * s + 2 * i] = re;
+      o[6 * s + 2 * i + 1] = im;
+    }
+}
+
+template <int P, int RECON> struct HopTypes {
+  using R = typename SpinorIO<P>::R;
+  static constexpr int LCH = link_chunks(P, RECON);
+  static constexpr int LCB = link_chunk_bytes(P, RECON);
+};
+
+template <int P, int RECON, int MU, int ISIGN>
+__device__ __forceinline__ void hop_dir(const HopLaunch& h, typename SpinorIO<P>::R (&acc)[24], int i, const int (&x)[4],
+                                        int rowbase_unused) {
+  using R = typename SpinorIO<P>::R;
+  constexpr int LCH = HopTypes<P, RECON>::LCH, LCB = HopTypes<P, RECON>::LCB;
+  const int Vh = h.g.Vh, Lxh = h.g.Lxh;
+  const int L = h.g.L[MU];
+  // neighbour indices on the other checkerboard
+  int xf[4] = {x[0], x[1], x[2], x[3]}, xb[4] = {x[0], x[1], x[2], x[3]};
+  xf[MU] = (x[MU] + 1 == L) ? 0 : x[MU] + 1;
+  xb[MU] = (x[MU] == 0) ? L - 1 : x[MU] - 1;
+  const int jf = ((xf[3] * h.g.L[2] + xf[2]) * h.g.L[1] + xf[1]) * Lxh + (xf[0] >> 1);
+  const int jb = ((xb[3] * h.g.L[2] + xb[2]) * h.g.L[1] + xb[1]) * Lxh + (xb[0] >> 1);
+
+  const char* gout = reinterpret_cast<const char*>(h.gauge[h.cb]) + (size_t)MU * LCH * LCB * Vh;
+  const char* gin = reinterpret_cast<const char*>(h.gauge[1 - h.cb]) + (size_t)MU * LCH * LCB * Vh;
+
+  R p[24], hs[12], o[12], u[18];
+  // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
+  {
+    SpinorIO<P>::load(h.in, h.in_nrm, h.in_stride, jf, p);
+    const int last = (x[MU] == L - 1);
+    LinkIO<P, RECON>::load(gout, Vh, i, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
+    project<MU, -ISIGN>(p, hs);
+    link_mul<false>(u, hs, o);
+    accumulate<MU, -ISIGN>(acc, o);
+  }
+  // backward: U_mu(x-mu)^dag (1 + isign gamma_mu) psi(x-mu)
+  {
+    SpinorIO<P>::load(h.in, h.in_nrm, h.in_stride, jb, p);
+    const int last = (x[MU] == 0);
+    LinkIO<P, RECON>::load(gin, Vh, jb, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
+    project<MU, ISIGN>(p, hs);
+    link_mul<true>(u, hs, o);
+    accumulate<MU, ISIGN>(acc, o);
+  }
+}
+
+template <int P> struct CloverBlockIO;
+template <> struct CloverBlockIO<P64> {
+  __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, double (&c)[36]) {
+    const double2* q = reinterpret_cast<const double2*>(base) + (size_t)18 * b * Vh;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+      double2 v = q[k * Vh + i];
+      c[2 * k] = v.x;
+      c[2 * k + 1] = v.y;
+    }
+  }
+};
+template <> struct CloverBlockIO<P32> {
+  __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, float (&c)[36]) {
+    const float4* q = reinterpret_cast<const float4*>(base) + (size_t)9 * b * Vh;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      float4 v = q[k * Vh + i];
+      c[4 * k] = v.x;
+      c[4 * k + 1] = v.y;
+      c[4 * k + 2] = v.z;
+      c[4 * k + 3] = v.w;
+    }
+  }
+};
+template <> struct CloverBlockIO<P16> {
+  // 18 chunks of 4 halves, then one float norm per site
+  __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, float (&c)[36]) {
+    const uint2* q = reinterpret_cast<const uint2*>(base);
+    const float n = reinterpret_cast<const float*>(q + 18 * Vh)[i];
+    q += (size_t)9 * b * Vh;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      H4 v;
+      v.u = q[k * Vh + i];
+      float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
+      c[4 * k] = f0.x * n;
+      c[4 * k + 1] = f0.y * n;
+      c[4 * k + 2] = f1.x * n;
+      c[4 * k + 3] = f1.y * n;
+    }
+  }
+};
+
+// out(6 components of block b) = (L + D + L^dag) in(block b); c = {diag[6], offd[15] complex}
+template <typename R>
+__device__ __forceinline__ void clover_block_mv(const R (&c)[36], const R (&in)[24], R (&out)[24], int b) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    R re = c[i] * in[2 * (6 * b + i)];
+    R im = c[i] * in[2 * (6 * b + i) + 1];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      if (j == i) continue;
+      const R pr = in[2 * (6 * b + j)], pi = in[2 * (6 * b + j) + 1];
+      if (j < i) {
+        const int e = 6 + 2 * (i * (i - 1) / 2 + j);
+        re += c[e] * pr - c[e + 1] * pi;
+        im += c[e] * pi + c[e + 1] * pr;
+      } else {
+        const int e = 6 + 2 * (j * (j - 1) / 2 + i);
+        re += c[e] * pr + c[e + 1] * pi;
+        im += c[e] * pi - c[e + 1] * pr;
+      }
+    }
+    out[2 * (6 * b + i)] = re;
+    out[2 * (6 * b + i) + 1] = im;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void clover_apply_site(const void* clov, long Vh, long i, const typename SpinorIO<P>::R (&in)[24],
+                                                  typename SpinorIO<P>::R (&out)[24]) {
+  using R = typename SpinorIO<P>::R;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    R c[36];
+    CloverBlockIO<P>::load(clov, Vh, i, b, c);
+    clover_block_mv<R>(c, in, out, b);
+  }
+}
+
+template <int P, int RECON, int ISIGN, int EPI, bool NORM>
+__global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
+  using R = typename SpinorIO<P>::R;
+  if (h.skip && *h.skip) return;
+  if (h.run && !*h.run) return;
+  const int blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = blk * blockDim.x + threadIdx.x;
+  double nsq = 0.0;
+  if (i < h.g.Vh) {
+    int x[4];
+    {
+      int r = i;
+      const int xh = r % h.g.Lxh;
+      r /= h.g.Lxh;
+      x[1] = r % h.g.L[1];
+      r /= h.g.L[1];
+      x[2] = r % h.g.L[2];
+      x[3] = r / h.g.L[2];
+      x[0] = 2 * xh + ((h.cb + x[1] + x[2] + x[3]) & 1);
+    }
+    R acc[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc[k] = 0;
+    hop_dir<P, RECON, 0, ISIGN>(h, acc, i, x, 0);
+    hop_dir<P, RECON, 1, ISIGN>(h, acc, i, x, 0);
+    hop_dir<P, RECON, 2, ISIGN>(h, acc, i, x, 0);
+    hop_dir<P, RECON, 3, ISIGN>(h, acc, i, x, 0);
+
+    R out[24];
+    const R b = R(h.b);
+    if constexpr (EPI == EPI_NONE) {
+#pragma unroll
+      for (int k = 0; k < 24; ++k) out[k] = b * acc[k];
+    } else if constexpr (EPI == EPI_CLOV) {
+#pragma unroll
+      for (int k = 0; k < 24; ++k) acc[k] *= b;
+      clover_apply_site<P>(h.clov, h.g.Vh, i, acc, out);
+    } else if constexpr (EPI == EPI_XPAY) {
+      R xs[24];
+      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
+      const R a = R(h.a);
+#pragma unroll
+      for (int k = 0; k < 24; ++k) out[k] = a * xs[k] + b * acc[k];
+    } else {
+      R xs[24];
+      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
+      clover_apply_site<P>(h.clov, h.g.Vh, i, xs, out);
+#pragma unroll
+      for (int k = 0; k < 24; ++k) out[k] += b * acc[k];
+    }
+    SpinorIO<P>::store(h.out, h.out_nrm, h.out_stride, i, out);
+    if constexpr (NORM) {
+      // |out|^2 of what was stored (FP16: the decoded value is what the next kernel reads; the
+      // difference is below the storage rounding and the reduction accumulates in FP64)
+#pragma unroll
+      for (int k = 0; k < 24; ++k) nsq += (double)out[k] * (double)out[k];
+    }
+  }
+  if constexpr (NORM) {
+    double v[1] = {nsq};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) h.partials[blockIdx.x] = v[0];
+  }
+}
+
+template <int P, int RECON> void launch_hop_t(const HopLaunch& h, hipStream_t s) {
+  const int nb = hop_blocks(h.g);
+  const bool norm = h.partials != nullptr;
+#define CMX_HOP_CASE(IS, EP)                                                                              \
+  if (h.isign == IS && h.epi == EP) {                                                                     \
+    if (norm)                                                                                             \
+      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, true>), dim3(nb), dim3(256), 0, s, h);                 \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, false>), dim3(nb), dim3(256), 0, s, h);                \
+    return;                                                                                               \
+  }
+  CMX_HOP_CASE(1, EPI_NONE)
+  CMX_HOP_CASE(1, EPI_CLOV)
+  CMX_HOP_CASE(1, EPI_XPAY)
+  CMX_HOP_CASE(1, EPI_CLOV_XPAY)
+  CMX_HOP_CASE(-1, EPI_NONE)
+  CMX_HOP_CASE(-1, EPI_CLOV)
+  CMX_HOP_CASE(-1, EPI_XPAY)
+  CMX_HOP_CASE(-1, EPI_CLOV_XPAY)
+#undef CMX_HOP_CASE
+  throw Error(CMX_ERR_ARG, "launch_hop: bad isign/epilogue");
+}
+
+}  // namespace cmx

@@ -1,0 +1,125 @@
+// Device-resident operator state behind the cmx_gauge / cmx_op handles.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "blas.h"
+#include "cmx_internal.h"
+
+namespace cmx {
+
+// Owning device allocation
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p, bytes = o.bytes;
+      o.p = nullptr, o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    release();
+    if (n) CMX_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr, bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// A fermion vector over n sites (one checkerboard, or a full lattice stored even-then-odd)
+struct Vec {
+  DevBuf buf;
+  int prec = P64;
+  int n = 0;
+  Vec() = default;
+  Vec(int prec_, int n_) : prec(prec_), n(n_) { buf.alloc(spinor_bytes(prec_, n_)); }
+  float* nrm() const {
+    return prec == P16 ? reinterpret_cast<float*>(buf.as<char>() + (size_t)spinor_chunks(prec) * 16 * n) : nullptr;
+  }
+  // view of the sites [off, off + count) with this vector's stride
+  VRef view(int off = 0) const {
+    VRef r;
+    r.v = buf.as<char>() + (size_t)off * 16;
+    r.nrm = prec == P16 ? nrm() + off : nullptr;
+    r.stride = n;
+    return r;
+  }
+};
+
+struct GaugeData {
+  Geom g;
+  long V = 0, Vh = 0;
+  DevBuf g64[2];           // FP64, per parity [mu][9 x double2][Vh]
+  double ph[4][2][2] = {}; // det(U) per direction: [bulk | last slice][re, im]
+  double su3_dev = 0;      // max deviation from (phase x SU(3)) over all links
+};
+
+struct Op {
+  std::shared_ptr<GaugeData> gd;
+  cmx_op_params par{};
+  Geom g;
+  long V = 0, Vh = 0;
+  double fact = 0;  // Nd + Mass
+  int recon[3] = {18, 12, 12};
+  // per precision: links (per parity) and the clover blocks used by the EO operator
+  DevBuf gauge[3][2];
+  DevBuf clov_odd[3];     // A_oo
+  DevBuf clovinv_even[3]; // A_ee^-1
+  DevBuf clov64_even;     // A_ee (FP64, for evenEvenLinOp / unprecLinOp)
+  double logdet_even = 0;
+  long nneg_even = 0;
+  hipStream_t stream = nullptr;
+  DevBuf partials;
+  int npartials = 0;
+  DevBuf state_S, state_F;
+  double* hS = nullptr;  // pinned mirrors
+  int* hF = nullptr;
+  DevBuf staging;        // V*24 doubles
+  std::vector<std::unique_ptr<Vec>> tmp_even[3];  // scratch even-checkerboard vectors
+
+  ~Op();
+  bool eo() const { return par.kind != CMX_OP_UNPREC_WILSON; }
+  bool clover() const { return par.kind == CMX_OP_EO_CLOVER; }
+  int subset_sites() const { return eo() ? (int)Vh : (int)V; }
+  void ensure_prec(int prec);
+  Vec& tmp(int prec, int k);
+  HopLaunch hop_base(int prec) const;
+
+  // chi = M(isign) psi on the subset; partials (optional) receives |chi|^2 per work-group;
+  // returns the number of partials written.
+  int apply(int prec, VRef out, VRef in, int isign, double* partials, const int* skip = nullptr,
+            const int* run = nullptr);
+  // raw hopping term on output checkerboard cb (views of one checkerboard each), scaled by b
+  void dslash(int prec, VRef out, VRef in, int isign, int cb, double b = 1.0);
+  // stage a host array (n doubles) into the device staging buffer and back
+  void upload(const double* host, size_t n);
+  void download(double* host, size_t n);
+  // host fermion (n sites x 24 doubles) <-> device vector view
+  void pack(const double* host, int nsites, const Vec& v, int off = 0);
+  void unpack(double* host, int nsites, const Vec& v, int off = 0);
+  // BLAS helper
+  void blas(int op, std::initializer_list<const Vec*> vs, int ia = 0, int ib = 0, int skip = -1, int run = -1,
+            int n = -1, int off = 0);
+  void finalize(int nparts, int nred, int prog, int slot = -1);
+  void sync();
+};
+
+std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);
+Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
+
+// Solvers (solvers.cpp)
+void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, cmx_solver_result& res);
+
+}  // namespace cmx

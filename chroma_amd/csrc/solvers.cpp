@@ -1,0 +1,255 @@
+// Krylov solvers on the device.  Each iteration is a fixed sequence of kernel launches whose
+// coefficients live in the device scalar array; the host launches iterations in batches and polls
+// the convergence flag in between (see blas.h).  Iteration counts and stopping rules are those of
+// the reference solvers:
+//   CG_INVERTER        LinOpSysSolverCG (syssolver_linop_cg.h:57-96) + InvCG2_a (invcg2.cc:70-232)
+//   BICGSTAB_INVERTER  InvBiCGStab_a (invbicgstab.cc:10-202)
+//   RELIABLE CG        LinOpSysSolverReliableCGClover (syssolver_linop_rel_cg_clover.h:104-152) +
+//                      RelInvCG_a (reliable_cg.cc:10-188): FP32/FP16 inner CG, FP64 reliable updates
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "op.h"
+
+namespace cmx {
+
+namespace {
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t s_) : s(s_) {
+    CMX_HIP(hipEventCreate(&a));
+    CMX_HIP(hipEventCreate(&b));
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  void start() { CMX_HIP(hipEventRecord(a, s)); }
+  double stop() {
+    CMX_HIP(hipEventRecord(b, s));
+    CMX_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    CMX_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+  }
+};
+
+void reset_state(Op& op, double rsd_sq, int maxit, double delta) {
+  std::memset(op.hS, 0, sizeof(double) * S_NSLOTS);
+  std::memset(op.hF, 0, sizeof(int) * F_NFLAGS);
+  op.hS[S_RSDSQ] = rsd_sq;
+  op.hS[S_DELTA] = delta;
+  // BiCGStab: rho_0 = alpha = omega = 1 (invbicgstab.cc:64-69)
+  op.hS[S_RHOP] = 1.0;
+  op.hS[S_ALPHA] = 1.0;
+  op.hS[S_OMEGA] = 1.0;
+  op.hF[F_MAXIT] = maxit;
+  CMX_HIP(hipMemcpyAsync(op.state_S.p, op.hS, sizeof(double) * S_NSLOTS, hipMemcpyHostToDevice, op.stream));
+  CMX_HIP(hipMemcpyAsync(op.state_F.p, op.hF, sizeof(int) * F_NFLAGS, hipMemcpyHostToDevice, op.stream));
+  op.sync();
+}
+
+void read_state(Op& op) {
+  CMX_HIP(hipMemcpyAsync(op.hS, op.state_S.p, sizeof(double) * S_NSLOTS, hipMemcpyDeviceToHost, op.stream));
+  CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+  op.sync();
+}
+
+// |v|^2 on the host (one synchronisation)
+double norm2(Op& op, const Vec& v) {
+  op.blas(B_NORM2, {&v});
+  op.finalize(blas_blocks(v.n), 1, PG_STORE, S_TR);
+  CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, sizeof(double),
+                         hipMemcpyDeviceToHost, op.stream));
+  op.sync();
+  return op.hS[S_TR];
+}
+
+// Runs iterations in batches until the device reports convergence (or a stop)
+template <typename Iter> int run_batches(Op& op, const cmx_solver_params& p, Iter&& iter) {
+  int batch = p.check_every > 0 ? p.check_every : 4;
+  long launched = 0;
+  const int* dF = op.state_F.as<int>();
+  (void)dF;
+  while (true) {
+    for (int j = 0; j < batch; ++j) iter();
+    launched += batch;
+    CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+    if (op.hF[F_CONV]) break;
+    if (launched > (long)p.max_iter + 8) break;  // the device stops itself at MaxIter; safety net
+    if (p.check_every <= 0 && batch < 32) batch *= 2;
+  }
+  return op.hF[F_NCOUNT];
+}
+
+double flops_per_apply(const Op& op) {
+  const double nc_ns = 12.0;
+  switch (op.par.kind) {
+    case CMX_OP_EO_CLOVER: return (2 * 1320.0 + 2 * 552.0 + 4 * nc_ns) * op.Vh;  // eoprec_clover_linop_w.cc:299-306
+    case CMX_OP_EO_WILSON: return (2 * 1320.0 + 6 * nc_ns) * op.Vh;             // eoprec_wilson_linop_w.cc:247-251
+    default: return (1320.0 + 4 * nc_ns) * op.V;                                 // unprec_wilson_linop_w.cc:109-113
+  }
+}
+
+// ---- CG_INVERTER: CGNE on M^dag M in FP64 ---------------------------------------------------------
+void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  Vec chi_tmp(P64, n), r(P64, n), pv(P64, n), mp(P64, n), mmp(P64, n);
+  const int nb = blas_blocks(n);
+  // chi_tmp = M^dag chi  (syssolver_linop_cg.h:65-66)
+  op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+  const double chi_sq = norm2(op, chi_tmp);
+  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+  Timer tm(op.stream);
+  tm.start();
+  // r = chi_tmp - M^dag M psi ; cp = |r|^2 ; p = r   (invcg2.cc:109-131)
+  op.apply(P64, mp.view(), psi.view(), +1, nullptr);
+  op.apply(P64, mmp.view(), mp.view(), -1, nullptr);
+  op.blas(B_XMY_NORM, {&r, &chi_tmp, &mmp});
+  op.finalize(nb, 1, PG_CG_INIT);
+  op.blas(B_COPY, {&pv, &r});
+  int* F = op.state_F.as<int>();
+  auto iter = [&]() {
+    const int np = op.apply(P64, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);  // Mp, |Mp|^2
+    op.finalize(np, 1, PG_CG_D);                                                                 // a = c/d
+    op.apply(P64, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);                               // M^dag M p
+    op.blas(B_AXPY_NORM, {&r, &mmp}, S_NEGA, 0, F_DONE);                                         // r -= a M^dag M p
+    op.finalize(nb, 1, PG_CG_R);                                                                 // cp, b, test
+    op.blas(B_CG_UPDATE, {&psi, &pv, &r}, S_A, S_B, F_DONE);                                     // psi += a p; p = r + b p
+  };
+  res.n_count = run_batches(op, p, iter);
+  read_state(op);
+  res.solve_seconds = tm.stop();
+  const int err = op.hF[F_ERR];
+  res.gflops = (res.n_count * (2 * flops_per_apply(op) + 10 * 12.0 * n) + 2 * flops_per_apply(op)) /
+               std::max(res.solve_seconds, 1e-12) * 1e-9;
+  if (err == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "CG: breakdown (|M p| = 0 or not finite)");
+  if (err == CMX_ERR_NOCONV) res.n_count = p.max_iter;  // InvCG2 returns MaxCG with a warning
+}
+
+// ---- BICGSTAB_INVERTER ----------------------------------------------------------------------------
+void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  Vec r(P64, n), r0(P64, n), pv(P64, n), v(P64, n), t(P64, n);
+  const int nb = blas_blocks(n);
+  const double chi_sq = norm2(op, chi);
+  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+  Timer tm(op.stream);
+  tm.start();
+  int* F = op.state_F.as<int>();
+  // r = r0 = chi - A psi ; p = v = 0   (invbicgstab.cc:37-62)
+  op.apply(P64, r0.view(), psi.view(), +1, nullptr);
+  op.blas(B_XMY_NORM, {&r, &chi, &r0});
+  op.blas(B_COPY, {&r0, &r});
+  op.blas(B_ZERO, {&pv});
+  op.blas(B_ZERO, {&v});
+  auto iter = [&]() {
+    op.blas(B_CDOT, {&r0, &r}, 0, 0, F_DONE);  // rho = <r0|r>
+    op.finalize(nb, 2, PG_BICG_RHO);           // beta = (rho/rho_prev)(alpha/omega)
+    op.blas(B_BICG_P, {&pv, &r, &v}, S_OMEGA, S_BETAC, F_DONE);  // p = r + beta (p - omega v)
+    op.apply(P64, v.view(), pv.view(), +1, nullptr, F + F_DONE);  // v = A p
+    op.blas(B_CDOT, {&r0, &v}, 0, 0, F_DONE);
+    op.finalize(nb, 2, PG_BICG_ALPHA);                            // alpha = rho / <r0|v>
+    op.blas(B_CAXPY, {&r, &v}, S_NALPHA, 0, F_DONE);              // s = r - alpha v
+    op.apply(P64, t.view(), r.view(), +1, nullptr, F + F_DONE);   // t = A s
+    op.blas(B_TNORM_TDOT, {&t, &r}, 0, 0, F_DONE);
+    op.finalize(nb, 3, PG_BICG_OMEGA);                            // omega = <t|s>/|t|^2
+    op.blas(B_BICG_UPDATE, {&psi, &r, &pv, &t}, S_OMEGA, S_ALPHA, F_DONE);  // psi += omega s + alpha p; r = s - omega t
+    op.finalize(nb, 1, PG_BICG_R);
+  };
+  res.n_count = run_batches(op, p, iter);
+  read_state(op);
+  res.solve_seconds = tm.stop();
+  res.gflops = res.n_count * (2 * flops_per_apply(op) + 80 * 12.0 * n) / std::max(res.solve_seconds, 1e-12) * 1e-9;
+  if (op.hF[F_ERR] == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "BiCGStab breakdown");
+  if (op.hF[F_ERR] == CMX_ERR_NOCONV) res.n_count = p.max_iter;
+}
+
+// ---- reliable-update mixed-precision CG on M^dag M ----------------------------------------------------
+void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  const int ps = p.sloppy_prec;
+  CMX_REQUIRE(ps == P32 || ps == P16, CMX_ERR_ARG, "reliable CG: sloppy precision must be single or half");
+  const int nb = blas_blocks(n);
+  Vec chi_tmp(P64, n), b(P64, n), r_dble(P64, n), x_dble(P64, n), tmp1(P64, n), tmp2(P64, n);
+  Vec r(P32, n), x(P32, n), pv(ps, n), mp(ps, n), mmp(ps, n);
+  // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125)
+  op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+  const double chi_sq = norm2(op, chi_tmp);
+  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+  Timer tm(op.stream);
+  tm.start();
+  int* F = op.state_F.as<int>();
+  // b = chi - A^dag A psi ; r = b ; x = 0   (reliable_cg.cc:44-64)
+  op.apply(P64, tmp1.view(), psi.view(), +1, nullptr);
+  op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr);
+  op.blas(B_XMY_NORM_COPY, {&b, &chi_tmp, &tmp2, &r});
+  op.finalize(nb, 1, PG_RCG_INIT);
+  op.blas(B_ZERO, {&x});
+  op.blas(B_ZERO, {&pv});
+  auto iter = [&]() {
+    op.blas(B_XPAY, {&pv, &r}, S_BETA, 0, F_DONE);                                          // p = r + beta p
+    const int np = op.apply(ps, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);
+    op.finalize(np, 1, PG_RCG_D);                                                           // c = r_sq; a = c/d
+    op.apply(ps, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);
+    op.blas(B_AXPY2_NORM, {&x, &r, &pv, &mmp}, S_A, 0, F_DONE);                             // x += a p; r -= a mmp
+    op.finalize(nb, 1, PG_RCG_R);                                                           // updateX / updateR
+    op.blas(B_COPY, {&x_dble, &x}, 0, 0, F_DONE, F_UPDR);                                   // x_dble = x
+    op.apply(P64, tmp1.view(), x_dble.view(), +1, nullptr, F + F_DONE, F + F_UPDR);
+    op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr, F + F_DONE, F + F_UPDR);
+    op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp2, &r}, 0, 0, F_DONE, F_UPDR);               // r = b - A x
+    op.finalize(nb, 1, PG_RCG_UPD);                                                         // norms, convergence
+    op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble}, 0, 0, F_DONE, F_UPDX);                // psi += x; x = 0; b = r
+    op.blas(B_ADD, {&psi, &x}, 0, 0, F_DONE, F_CONV);                                       // converged: psi += x
+  };
+  res.n_count = run_batches(op, p, iter);
+  read_state(op);
+  res.solve_seconds = tm.stop();
+  res.n_reliable = op.hF[F_NUPD];
+  res.gflops = (res.n_count + 1) * (2 * flops_per_apply(op) + 16 * 12.0 * n) / std::max(res.solve_seconds, 1e-12) * 1e-9;
+  if (op.hF[F_ERR] == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "reliable CG: breakdown");
+  if (op.hF[F_ERR] == CMX_ERR_NOCONV)
+    throw Error(CMX_ERR_NOCONV, "reliable CG: no convergence in " + std::to_string(p.max_iter) + " iterations");
+}
+
+}  // namespace
+
+void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_h, cmx_solver_result& res) {
+  CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = op.subset_sites();
+  Vec psi(P64, n), chi(P64, n);
+  op.pack(chi_h, n, chi);
+  op.pack(psi_h, n, psi);
+  std::memset(&res, 0, sizeof(res));
+  switch (p.solver) {
+    case CMX_SOLVER_CG: solve_cg(op, p, psi, chi, res); break;
+    case CMX_SOLVER_BICGSTAB: solve_bicgstab(op, p, psi, chi, res); break;
+    case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, p, psi, chi, res); break;
+    default: throw Error(CMX_ERR_ARG, "solve: unknown solver");
+  }
+  // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87)
+  Vec t(P64, n), rr(P64, n);
+  op.apply(P64, t.view(), psi.view(), +1, nullptr);
+  op.blas(B_XMY_NORM, {&rr, &chi, &t});
+  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
+  op.blas(B_NORM2, {&chi});
+  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR + 1);
+  CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, op.stream));
+  op.sync();
+  res.resid = std::sqrt(op.hS[S_TR]);
+  res.rel_resid = op.hS[S_TR + 1] > 0 ? res.resid / std::sqrt(op.hS[S_TR + 1]) : res.resid;
+  op.unpack(psi_h, n, psi);
+  res.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (p.verbose)
+    std::printf("CMX_SOLVER: %d iterations. Rsd = %g Relative Rsd = %g  (%.3f s, %.1f GFLOP/s)\n", res.n_count,
+                res.resid, res.rel_resid, res.solve_seconds, res.gflops);
+}
+
+}  // namespace cmx

@@ -1,0 +1,261 @@
+"""ctypes binding of libchroma_mi355x.so (declared in include/chroma_mi355x.h).
+
+There is no fallback path: if the HIP library is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchroma_mi355x.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "chroma_mi355x.h")
+
+CMX_OK = 0
+CMX_ERR_ARG, CMX_ERR_HIP, CMX_ERR_STATE, CMX_ERR_NOCONV, CMX_ERR_BREAKDOWN, CMX_ERR_UNSUPPORTED = 1, 2, 3, 4, 5, 6
+DOUBLE, SINGLE, HALF = 0, 1, 2
+RECONS_18, RECONS_12 = 18, 12
+UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
+CG, BICGSTAB, RELIABLE_CG = 0, 1, 2
+PLUS, MINUS = 1, -1
+
+
+class CmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[cmx error {code}] {msg}")
+        self.code = code
+
+
+class OpParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("mass", ctypes.c_double), ("clov_coeff_r", ctypes.c_double),
+                ("clov_coeff_t", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("sloppy_recon", ctypes.c_int)]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("max_iter", ctypes.c_int), ("rsd_target", ctypes.c_double),
+                ("delta", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("verbose", ctypes.c_int)]
+
+
+class SolverResult(ctypes.Structure):
+    _fields_ = [("n_count", ctypes.c_int), ("resid", ctypes.c_double), ("rel_resid", ctypes.c_double),
+                ("n_reliable", ctypes.c_int), ("solve_seconds", ctypes.c_double),
+                ("total_seconds", ctypes.c_double), ("gflops", ctypes.c_double)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+_L = None
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not found: build it with `python -m chroma_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "cmx_init": (ctypes.c_int, [ctypes.c_int]),
+        "cmx_finalize": (ctypes.c_int, []),
+        "cmx_last_error": (ctypes.c_char_p, []),
+        "cmx_version": (ctypes.c_int, []),
+        "cmx_gauge_create": (ctypes.c_int, [_ip, _dp, ctypes.POINTER(_vp)]),
+        "cmx_gauge_destroy": (ctypes.c_int, [_vp]),
+        "cmx_op_create": (ctypes.c_int, [_vp, ctypes.POINTER(OpParams), ctypes.POINTER(_vp)]),
+        "cmx_op_destroy": (ctypes.c_int, [_vp]),
+        "cmx_op_subset_doubles": (ctypes.c_long, [_vp]),
+        "cmx_op_apply": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+        "cmx_dslash_apply": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "cmx_op_eo_piece": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+        "cmx_clover_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "cmx_clover_logdet": (ctypes.c_int, [_vp, _dp, _lp]),
+        "cmx_clover_pack_quda": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+        "cmx_solve": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
+                                     ctypes.POINTER(SolverResult)]),
+        "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+        "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+        "cmx_gauge_generate": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_double, _dp]),
+        "cmx_gauge_apply_bc": (ctypes.c_int, [_ip, _dp, _ip, _dp, _ip]),
+        "cmx_plaquette": (ctypes.c_double, [_ip, _dp]),
+        "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _L = L
+    return L
+
+
+def check(rc):
+    if rc != CMX_OK:
+        raise CmxError(rc, lib().cmx_last_error().decode())
+
+
+def _dims(L):
+    return (ctypes.c_int * 4)(*[int(v) for v in L])
+
+
+def _d(a):
+    assert a.flags.c_contiguous and a.dtype in (np.float64, np.complex128), (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(lib().cmx_init(int(device)))
+        _initialised = True
+
+
+# ---- host-side synthetic inputs -----------------------------------------------------------------
+def gauge_generate(L, kind=1, seed=1, eps=0.3, nstout=0, rho=0.1):
+    """kind 0 unit, 1 exp(i eps H), 2 Haar; then nstout stout steps.  complex128[4, V, 3, 3]."""
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3), np.complex128)
+    check(lib().cmx_gauge_generate(_dims(L), int(kind), int(seed), float(eps), int(nstout), float(rho), _d(U)))
+    return U
+
+
+def gauge_apply_bc(L, U, boundary=(1, 1, 1, 1), twist_by_pi=None, twist_dir=(0, 1, 2)):
+    U = np.ascontiguousarray(U, np.complex128)
+    b = (ctypes.c_int * 4)(*boundary)
+    if twist_by_pi is None:
+        check(lib().cmx_gauge_apply_bc(_dims(L), _d(U), b, None, None))
+    else:
+        tw = np.ascontiguousarray(twist_by_pi, np.float64)
+        td = (ctypes.c_int * 3)(*twist_dir)
+        check(lib().cmx_gauge_apply_bc(_dims(L), _d(U), b, _d(tw), td))
+    return U
+
+
+def plaquette(L, U):
+    return lib().cmx_plaquette(_dims(L), _d(np.ascontiguousarray(U, np.complex128)))
+
+
+def z2_source(shape, seed):
+    out = np.zeros(shape, np.complex128)
+    check(lib().cmx_fill_z2(out.size * 2, int(seed), _d(out)))
+    return out
+
+
+def kappa_to_mass(kappa):
+    """kappaToMass, lib/io/param_io.cc:12-15."""
+    return 1.0 / (2.0 * kappa) - 4.0
+
+
+# ---- device objects -------------------------------------------------------------------------------
+class Gauge:
+    def __init__(self, L, U):
+        init()
+        self.L = tuple(int(v) for v in L)
+        self._U = np.ascontiguousarray(U, np.complex128)
+        h = _vp()
+        check(lib().cmx_gauge_create(_dims(L), _d(self._U), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _L is not None:
+            _L.cmx_gauge_destroy(self.h)
+            self.h = None
+
+
+class Operator:
+    """A device fermion operator (LinearOperator<LatticeFermion>, lib/linearop.h:25-49)."""
+
+    def __init__(self, gauge, kind, mass, csw_r=0.0, csw_t=None, sloppy_prec=SINGLE, sloppy_recon=RECONS_12):
+        self.gauge = gauge
+        self.L = gauge.L
+        self.V = int(np.prod(self.L))
+        self.kind = kind
+        p = OpParams(int(kind), float(mass), float(csw_r), float(csw_r if csw_t is None else csw_t),
+                     int(sloppy_prec), int(sloppy_recon))
+        h = _vp()
+        check(lib().cmx_op_create(gauge.h, ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.nsub = lib().cmx_op_subset_doubles(h) // 24
+
+    def __del__(self):
+        if getattr(self, "h", None) and _L is not None:
+            _L.cmx_op_destroy(self.h)
+            self.h = None
+
+    def subset_zeros(self):
+        return np.zeros((self.nsub, 4, 3), np.complex128)
+
+    def __call__(self, psi, isign=PLUS, prec=DOUBLE):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        chi = np.zeros_like(psi)
+        check(lib().cmx_op_apply(self.h, _d(chi), _d(psi), int(isign), int(prec)))
+        return chi
+
+    def dslash(self, psi_cb, isign, cb, prec=DOUBLE):
+        psi_cb = np.ascontiguousarray(psi_cb, np.complex128)
+        chi = np.zeros_like(psi_cb)
+        check(lib().cmx_dslash_apply(self.h, _d(chi), _d(psi_cb), int(isign), int(cb), int(prec)))
+        return chi
+
+    def piece(self, which, psi_full, isign=PLUS):
+        names = {"evenEven": 0, "evenEvenInv": 1, "evenOdd": 2, "oddEven": 3, "oddOdd": 4, "unprec": 5}
+        psi_full = np.ascontiguousarray(psi_full, np.complex128)
+        chi = np.zeros_like(psi_full)
+        check(lib().cmx_op_eo_piece(self.h, names[which], _d(chi), _d(psi_full), int(isign)))
+        return chi
+
+    def clover(self, which=0):
+        tri = np.zeros((self.V, 72), np.float64)
+        check(lib().cmx_clover_download(self.h, int(which), _d(tri)))
+        return tri
+
+    def clover_logdet(self):
+        ld = ctypes.c_double()
+        nn = ctypes.c_long()
+        check(lib().cmx_clover_logdet(self.h, ctypes.byref(ld), ctypes.byref(nn)))
+        return ld.value, nn.value
+
+    def clover_pack_quda(self, which, cb):
+        out = np.zeros((self.V // 2, 72), np.float64)
+        check(lib().cmx_clover_pack_quda(self.h, int(which), int(cb), _d(out)))
+        return out
+
+    def layout_roundtrip(self, psi_cb, prec):
+        psi_cb = np.ascontiguousarray(psi_cb, np.complex128)
+        out = np.zeros_like(psi_cb)
+        check(lib().cmx_layout_roundtrip(self.h, int(prec), _d(psi_cb), _d(out)))
+        return out
+
+    def time(self, what, prec, n):
+        ms = ctypes.c_double()
+        check(lib().cmx_op_time(self.h, int(what), int(prec), int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
+              check_every=0, verbose=False):
+        """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays."""
+        chi = np.ascontiguousarray(chi, np.complex128)
+        single = chi.ndim == 3
+        chis = chi[None] if single else chi
+        psis = np.zeros_like(chis) if psi is None else np.ascontiguousarray(psi, np.complex128).reshape(chis.shape).copy()
+        nrhs = chis.shape[0]
+        p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
+                         int(bool(verbose)))
+        res = (SolverResult * nrhs)()
+        check(lib().cmx_solve(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))
+        out = [dict(n_count=r.n_count, resid=r.resid, rel_resid=r.rel_resid, n_reliable=r.n_reliable,
+                    solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops) for r in res]
+        return (psis[0], out[0]) if single else (psis, out)
+
+
+def declared_symbols():
+    """Function names declared in include/chroma_mi355x.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|long|double|const char\*)\s+(cmx_\w+)\s*\(", txt, re.M)))

@@ -1,0 +1,170 @@
+/*
+ * chroma_mi355x.h -- C-ABI of the MI355X-native Wilson-clover fermion-matrix library.
+ *
+ * This is the drop-in boundary that a Chroma-side LinOpSystemSolver (registered in the InvertParam
+ * factory the way QUDA_CLOVER_INVERTER is) binds to.  Plain C types only: int status codes, host
+ * pointers and sizes, opaque handles.  Nothing throws across this boundary; on failure a call
+ * returns a non-zero CMX_ERR_* code and cmx_last_error() describes it (the Chroma adapter maps a
+ * failure to QDP_abort(1) unless SilentFail is set, as syssolver_linop_clover_quda_w.h:638-644 does).
+ *
+ * Host layout of every host array crossing this boundary (bit-exact permutation of the device
+ * layout; see cmx_layout_roundtrip):
+ *   lattice dims L = {Lx, Ly, Lz, Lt}, all even;  V = Lx*Ly*Lz*Lt,  Vh = V/2
+ *   checkerboard cb(x) = (x+y+z+t) mod 2;  full site index = cb*Vh + (((t*Lz+z)*Ly+y)*Lx + x)/2
+ *   fermion : per site 24 doubles, [spin 0..3][colour 0..2][re,im]   (QDP++ Fermion elem(s).elem(c))
+ *   link    : per site 18 doubles, 3x3 complex row-major            (QDP++ ColorMatrix elem(i,j))
+ *   clover  : per site 72 doubles, {diag[2][6], offd[2][15][re,im]}  (PrimitiveClovTriang,
+ *             clover_term_qdp_w.h:19-24)
+ * An even-odd operator acts on the odd checkerboard: its fermion arguments are the Vh*24 doubles
+ * that start at full site index Vh, i.e. &chi.elem(rb[1].start()) as syssolver_linop_clover_quda_w.cc:76,85
+ * passes them.
+ */
+#ifndef CHROMA_MI355X_H
+#define CHROMA_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+#define CMX_OK 0
+#define CMX_ERR_ARG 1          /* invalid argument or inconsistent sizes                          */
+#define CMX_ERR_HIP 2          /* HIP runtime failure                                             */
+#define CMX_ERR_STATE 3        /* library not initialised / handle in wrong state                 */
+#define CMX_ERR_NOCONV 4       /* solver reached MaxIter (reliable_cg.cc:180-183 aborts here)     */
+#define CMX_ERR_BREAKDOWN 5    /* BiCGStab breakdown (invbicgstab.cc:80-83,109-112,133-136)      */
+#define CMX_ERR_UNSUPPORTED 6  /* option not supported by this build                              */
+
+/* ---- enums ----------------------------------------------------------------------------------- */
+#define CMX_DOUBLE 0  /* FP64 storage and arithmetic                                                */
+#define CMX_SINGLE 1  /* FP32 storage and arithmetic                                                */
+#define CMX_HALF 2    /* FP16 storage (per-site FP32 norm for fermions), FP32 arithmetic            */
+
+#define CMX_RECONS_18 18 /* 3x3 complex links                                                        */
+#define CMX_RECONS_12 12 /* two rows stored, third rebuilt in registers (QUDA_RECONSTRUCT_12 role)  */
+
+#define CMX_OP_UNPREC_WILSON 0 /* UnprecWilsonLinOp, unprec_wilson_linop_w.cc:58-77                 */
+#define CMX_OP_EO_WILSON 1     /* EvenOddPrecWilsonLinOp, eoprec_wilson_linop_w.cc:126-166           */
+#define CMX_OP_EO_CLOVER 2     /* EvenOddPrecCloverLinOp, eoprec_clover_linop_w.cc:142-187           */
+
+#define CMX_SOLVER_CG 0           /* CG_INVERTER: LinOpSysSolverCG + InvCG2 (syssolver_linop_cg.h:57-96) */
+#define CMX_SOLVER_BICGSTAB 1     /* BICGSTAB_INVERTER: InvBiCGStab (invbicgstab.cc:10-202)             */
+#define CMX_SOLVER_RELIABLE_CG 2  /* RELIABLE_CG_MP_CLOVER_INVERTER: RelInvCG (reliable_cg.cc:10-188)    */
+
+#define CMX_PLUS 1
+#define CMX_MINUS (-1)
+
+typedef struct cmx_gauge_s* cmx_gauge;
+typedef struct cmx_op_s* cmx_op;
+
+/* ---- lifecycle: initQuda / endQuda hooks (lib/init/chroma_init.cc:215-250, 308-312) ----------- */
+int cmx_init(int device);
+int cmx_finalize(void);
+const char* cmx_last_error(void);
+int cmx_version(void);
+
+/* ---- gauge field: replaces loadGaugeQuda (syssolver_linop_clover_quda_w.h:510-523) -------------
+ * links: 4*V*18 doubles, [mu][site][3][3][re,im], with the fermion boundary conditions already
+ * applied (state->getLinks(), simple_fermbc.h:87-104 / twisted_fermbc_w.h:96-119).  Copied to the
+ * device at creation; the host array may be freed afterwards. */
+int cmx_gauge_create(const int dims[4], const double* links, cmx_gauge* out);
+int cmx_gauge_destroy(cmx_gauge g);
+
+/* ---- operator: EvenOddPrecCloverLinOp::create (eoprec_clover_linop_w.cc:19-39) ------------------
+ * The clover term is built on the device from the links (mesField + makeClov, FP64) and its even
+ * block is LDL^dag-inverted in FP64 (choles(0)); loadCloverQuda's role (:536-552). */
+typedef struct {
+  int kind;            /* CMX_OP_*                                                                  */
+  double mass;         /* Mass = 1/(2 Kappa) - Nd  (param_io.cc:12-15)                             */
+  double clov_coeff_r; /* CloverFermActParams clovCoeffR (clover_fermact_params_w.cc:65-77)         */
+  double clov_coeff_t; /* CloverFermActParams clovCoeffT                                            */
+  int sloppy_prec;     /* CMX_SINGLE or CMX_HALF: the inner (sloppy) operator copy                  */
+  int sloppy_recon;    /* CMX_RECONS_12 or CMX_RECONS_18                                            */
+} cmx_op_params;
+
+int cmx_op_create(cmx_gauge g, const cmx_op_params* p, cmx_op* out);
+int cmx_op_destroy(cmx_op op);
+/* Number of doubles of one fermion on the operator's subset (Vh*24 for EO, V*24 unpreconditioned) */
+long cmx_op_subset_doubles(cmx_op op);
+
+/* chi = M(isign) psi on the operator's subset, computed on the device at precision prec
+ * (LinearOperator::operator(), lib/linearop.h:33).  Host FP64 arrays of cmx_op_subset_doubles(). */
+int cmx_op_apply(cmx_op op, double* chi, const double* psi, int isign, int prec);
+
+/* Raw hopping term chi_cb = D(isign) psi_{1-cb} (QDPWilsonDslashT::apply, lwldslash_w.h:252-295).
+ * chi: Vh*24 doubles of checkerboard cb, psi: Vh*24 doubles of checkerboard 1-cb. */
+int cmx_dslash_apply(cmx_op op, double* chi, const double* psi, int isign, int cb, int prec);
+
+/* Even-odd blocks on full-lattice host arrays (V*24 doubles), FP64
+ * (EvenOddPrecLinearOperator, lib/eoprec_linop.h:102-165).  which: 0 evenEven, 1 evenEvenInv,
+ * 2 evenOdd, 3 oddEven, 4 oddOdd, 5 unprecLinOp.  Only the target checkerboard of chi is written. */
+int cmx_op_eo_piece(cmx_op op, int which, double* chi, const double* psi, int isign);
+
+/* Clover term download (FP64): which 0 = A on all sites, 1 = A_ee^-1 on the even sites.
+ * tri: V*72 doubles in PrimitiveClovTriang layout; sites not covered are left untouched. */
+int cmx_clover_download(cmx_op op, int which, double* tri);
+/* Sum over the even sites of log|d_i| of the LDL^dag pivots (cholesDet(0), clover_term_qdp_w.h:580-606;
+ * logDetEvenEvenLinOp, eoprec_clover_linop_w.cc:310-312) and the count of sites with negative pivots. */
+int cmx_clover_logdet(cmx_op op, double* logdet, long* n_neg_sites);
+/* QUDAPackedClovSite wire format (clover_term_qdp_w.h:26-32, packForQUDA :2171-2228) of checkerboard cb:
+ * which 0 = A, 1 = A^-1 (cb 0 only).  out: Vh*72 doubles. */
+int cmx_clover_pack_quda(cmx_op op, int which, int cb, double* out);
+
+/* ---- solver: replaces invertQuda (syssolver_linop_clover_quda_w.cc:98) --------------------------
+ * Solves A psi = chi on the operator's subset (SystemSolver::operator(), lib/syssolver.h:46).
+ * psi is the initial guess on entry.  nrhs right-hand sides are consecutive subset vectors.
+ * resid is |chi - A psi| recomputed in FP64 on the device (syssolver_linop_cg.h:80-87). */
+typedef struct {
+  int solver;        /* CMX_SOLVER_*                                                               */
+  int max_iter;      /* MaxCG / MaxIter                                                            */
+  double rsd_target; /* RsdCG / RsdTarget (relative)                                               */
+  double delta;      /* reliable-update Delta (syssolver_rel_bicgstab_clover_params.cc:11-19)      */
+  int sloppy_prec;   /* inner precision for the reliable solvers: CMX_SINGLE or CMX_HALF           */
+  int check_every;   /* iterations between host convergence checks (0 = library default)          */
+  int verbose;
+} cmx_solver_params;
+
+typedef struct {
+  int n_count;           /* iterations (SystemSolverResults_t::n_count)                            */
+  double resid;          /* |chi - A psi|, FP64                                                    */
+  double rel_resid;      /* resid / |chi|                                                          */
+  int n_reliable;        /* reliable updates performed                                             */
+  double solve_seconds;  /* device time of the solve (HIP events), fields already resident         */
+  double total_seconds;  /* wall time including host<->device transfers                            */
+  double gflops;         /* Chroma flop convention (invcg2.cc:67, eoprec_clover_linop_w.cc:299-306) */
+} cmx_solver_result;
+
+int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
+              cmx_solver_result* res);
+
+/* ---- measurement -------------------------------------------------------------------------------
+ * Average device milliseconds per call over n back-to-back launches on the operator's stream,
+ * bracketed by HIP events.  what: 0 hopping term (cb 1), 1 EO operator kernel pair (PLUS),
+ * 2 fused EO kernel 2 only, 3 streaming copy of one odd fermion (bandwidth reference). */
+int cmx_op_time(cmx_op op, int what, int prec, int n, double* ms_per_call);
+
+/* ---- layout ------------------------------------------------------------------------------------
+ * Uploads one checkerboard fermion (Vh*24 doubles) into the device SoA layout at precision prec and
+ * downloads it back.  For CMX_DOUBLE the result is bit-identical to the input. */
+int cmx_layout_roundtrip(cmx_op op, int prec, const double* in, double* out);
+
+/* ---- synthetic inputs (host): counter-based RNG keyed by (seed, site, mu, component) -------------
+ * kind 0: unit links; 1: U = exp(i eps H), H traceless Hermitian Gaussian, then reunitarised;
+ * 2: Haar-random SU(3).  Then nstout isotropic stout steps with parameter rho
+ * (stout_utils.cc:105-180,936-995).  links: 4*V*18 doubles (host layout). */
+int cmx_gauge_generate(const int dims[4], int kind, unsigned long long seed, double eps, int nstout,
+                       double rho, double* links);
+/* Fermion boundary conditions applied in place: SimpleFermBC::modify (simple_fermbc.h:87-104) with
+ * boundary[mu] = +1/-1, then TwistedFermBC::modify (twisted_fermbc_w.h:96-119) when twist != NULL:
+ * links in direction twist_dir[i] times exp(i pi twist_by_pi[i] / L[dir]). */
+int cmx_gauge_apply_bc(const int dims[4], double* links, const int boundary[4], const double* twist_by_pi,
+                       const int* twist_dir);
+/* Average plaquette Re tr(U_p)/3 over all sites and planes (MesPlq normalisation). */
+double cmx_plaquette(const int dims[4], const double* links);
+/* Z2 noise source: each real and imaginary part +-1/sqrt(2) from (seed, site, component). */
+int cmx_fill_z2(long n_doubles, unsigned long long seed, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHROMA_MI355X_H */

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP library (through its C-ABI) against the oracle on the same seeded inputs.
+
+Tolerances (relative, Frobenius norm over the output): FP64 1e-12, FP32 1e-6, FP16 storage 2e-3.
+Layout conversion in FP64 is checked bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from chroma_amd import lib as cmx
+from tests.util import rel, random_fermion
+
+pytestmark = pytest.mark.gpu
+
+TOL = {cmx.DOUBLE: 1e-12, cmx.SINGLE: 1e-6, cmx.HALF: 2e-3}
+LATS = [(4, 4, 4, 4), (4, 4, 4, 8), (8, 8, 8, 16), (6, 4, 8, 10)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gauge_field(L, seed, bc=(1, 1, 1, -1), twist=None):
+    U = cmx.gauge_generate(L, kind=1, seed=seed, eps=0.3)
+    return cmx.gauge_apply_bc(L, U, boundary=bc, twist_by_pi=twist)
+
+
+@pytest.mark.parametrize("prec", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
+def test_layout_roundtrip(prec):
+    L = (4, 4, 4, 8)
+    say("layout roundtrip", L, prec)
+    U = gauge_field(L, 1)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1)
+    V = int(np.prod(L))
+    psi = random_fermion(V // 2, 3)
+    out = op.layout_roundtrip(psi, prec)
+    if prec == cmx.DOUBLE:
+        assert np.array_equal(out.view(np.uint64), psi.view(np.uint64))  # bit exact
+    else:
+        assert rel(out, psi) < {cmx.SINGLE: 1e-7, cmx.HALF: 1e-3}[prec]
+
+
+@pytest.mark.parametrize("L", LATS)
+@pytest.mark.parametrize("prec,recon", [(cmx.DOUBLE, 18), (cmx.SINGLE, 18), (cmx.SINGLE, 12), (cmx.HALF, 12)])
+def test_dslash(L, prec, recon):
+    say("dslash", L, prec, recon)
+    V = int(np.prod(L))
+    U = gauge_field(L, 11)
+    sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1, sloppy_prec=sp, sloppy_recon=recon)
+    psi = random_fermion(V, 5)
+    for cb in (0, 1):
+        for isign in (+1, -1):
+            ref = oracle.dslash(L, U, psi, isign, cb)
+            Vh = V // 2
+            src = psi[(1 - cb) * Vh:(2 - cb) * Vh]
+            got = op.dslash(src, isign, cb, prec)
+            e = rel(got, ref[cb * Vh:(cb + 1) * Vh])
+            assert e < TOL[prec], (cb, isign, e)
+
+
+@pytest.mark.parametrize("twist", [None, (0.2, 0.4, 0.0)])
+def test_dslash_boundary_phases_recon12(twist):
+    """12-real reconstruction with antiperiodic time and twisted phases (det(U) != 1 on those links)."""
+    L = (4, 4, 6, 8)
+    say("dslash bc", twist)
+    V = int(np.prod(L))
+    U = gauge_field(L, 21, bc=(1, 1, 1, -1), twist=twist)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1, sloppy_prec=cmx.SINGLE, sloppy_recon=12)
+    psi = random_fermion(V, 6)
+    ref = oracle.dslash(L, U, psi, +1, 1)
+    got = op.dslash(psi[: V // 2], +1, 1, cmx.SINGLE)
+    assert rel(got, ref[V // 2:]) < 1e-6
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (4, 4, 4, 8), (8, 8, 8, 16)])
+def test_clover_build_and_inverse(L):
+    say("clover", L)
+    V = int(np.prod(L))
+    U = gauge_field(L, 31)
+    mass, csw = cmx.kappa_to_mass(0.125), 1.0
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw)
+    ref = oracle.make_clover(L, U, csw, csw, mass)
+    got = op.clover(0)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+    inv_ref = ref.copy()
+    ld_ref, nn_ref = oracle.clover_invert(L, inv_ref, 0)
+    inv = op.clover(1)
+    Vh = V // 2
+    assert np.max(np.abs(inv[:Vh] - inv_ref[:Vh])) < 1e-12 * np.max(np.abs(inv_ref[:Vh]))
+    ld, nn = op.clover_logdet()
+    assert abs(ld - ld_ref) < 1e-10 * max(1.0, abs(ld_ref)) and nn == nn_ref
+
+
+@pytest.mark.parametrize("kind", [cmx.EO_WILSON, cmx.EO_CLOVER])
+@pytest.mark.parametrize("prec", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
+def test_eo_operator(kind, prec):
+    L = (8, 8, 8, 16)
+    say("eo op", kind, prec)
+    V = int(np.prod(L))
+    U = gauge_field(L, 41)
+    mass = cmx.kappa_to_mass(0.13)
+    okind = {cmx.EO_WILSON: oracle.EO_WILSON, cmx.EO_CLOVER: oracle.EO_CLOVER}[kind]
+    ref_op = oracle.Op(okind, L, U, mass, 1.0, 1.0)
+    sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
+    op = cmx.Operator(cmx.Gauge(L, U), kind, mass, 1.0, sloppy_prec=sp)
+    psi = random_fermion(V, 7)
+    for isign in (+1, -1):
+        ref = ref_op(psi, isign)[V // 2:]
+        got = op(psi[V // 2:], isign, prec)
+        assert rel(got, ref) < TOL[prec], (isign, rel(got, ref))
+
+
+@pytest.mark.parametrize("kind", [cmx.EO_WILSON, cmx.EO_CLOVER])
+def test_eo_pieces(kind):
+    L = (4, 4, 4, 8)
+    say("eo pieces", kind)
+    V = int(np.prod(L))
+    U = gauge_field(L, 51)
+    mass = cmx.kappa_to_mass(0.12)
+    okind = {cmx.EO_WILSON: oracle.EO_WILSON, cmx.EO_CLOVER: oracle.EO_CLOVER}[kind]
+    ref_op = oracle.Op(okind, L, U, mass, 1.2, 1.2)
+    op = cmx.Operator(cmx.Gauge(L, U), kind, mass, 1.2)
+    psi = random_fermion(V, 8)
+    Vh = V // 2
+    for name, sl in [("evenEven", slice(0, Vh)), ("evenEvenInv", slice(0, Vh)), ("evenOdd", slice(0, Vh)),
+                     ("oddEven", slice(Vh, V)), ("oddOdd", slice(Vh, V)), ("unprec", slice(0, V))]:
+        for isign in (+1, -1):
+            ref = ref_op.piece(name, psi, isign)[sl]
+            got = cmx_piece = op.piece(name, psi, isign)[sl]
+            assert rel(got, ref) < 1e-12, (name, isign, rel(cmx_piece, ref))
+
+
+def test_unprec_wilson_operator():
+    L = (8, 8, 8, 16)
+    say("unprec wilson")
+    V = int(np.prod(L))
+    U = gauge_field(L, 61)
+    mass = cmx.kappa_to_mass(0.115)
+    ref_op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.UNPREC_WILSON, mass)
+    psi = random_fermion(V, 9)
+    for isign in (+1, -1):
+        for prec, tol in [(cmx.DOUBLE, 1e-12), (cmx.SINGLE, 1e-6), (cmx.HALF, 2e-3)]:
+            assert rel(op(psi, isign, prec), ref_op(psi, isign)) < tol
+
+
+def test_cg_unprec_wilson_config1_small():
+    """configs[0] recipe at 4^3x8: unpreconditioned Wilson CGNE, FP64, point source."""
+    L = (4, 4, 4, 8)
+    say("cg unprec")
+    V = int(np.prod(L))
+    U = gauge_field(L, 1)
+    mass = cmx.kappa_to_mass(0.115)
+    ref_op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.UNPREC_WILSON, mass)
+    chi = np.zeros((V, 4, 3), np.complex128)
+    chi[0, 0, 0] = 1.0
+    psi_ref, n_ref, r_ref, _ = ref_op.solve(chi, solver=oracle.CG, rsd=1e-10, maxit=2000)
+    psi, res = op.solve(chi, solver=cmx.CG, rsd=1e-10, maxit=2000)
+    assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)
+    assert res["rel_resid"] < 1e-10 * 2
+    assert rel(psi, psi_ref) < 1e-8
+
+
+@pytest.mark.parametrize("kind", [cmx.EO_WILSON, cmx.EO_CLOVER])
+def test_cg_eo(kind):
+    L = (8, 8, 8, 16)
+    say("cg eo", kind)
+    V = int(np.prod(L))
+    U = gauge_field(L, 71)
+    mass = cmx.kappa_to_mass(0.125)
+    okind = {cmx.EO_WILSON: oracle.EO_WILSON, cmx.EO_CLOVER: oracle.EO_CLOVER}[kind]
+    ref_op = oracle.Op(okind, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), kind, mass, 1.0)
+    chi_full = random_fermion(V, 12)
+    chi_full[: V // 2] = 0
+    psi_ref, n_ref, r_ref, _ = ref_op.solve(chi_full, solver=oracle.CG, rsd=1e-10, maxit=5000)
+    psi, res = op.solve(chi_full[V // 2:], solver=cmx.CG, rsd=1e-10, maxit=5000)
+    assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)
+    assert rel(psi, psi_ref[V // 2:]) < 1e-8
+    assert res["rel_resid"] < 1e-9
+
+
+def test_bicgstab_eo_clover():
+    L = (8, 8, 8, 16)
+    say("bicgstab")
+    V = int(np.prod(L))
+    U = gauge_field(L, 81)
+    mass = cmx.kappa_to_mass(0.125)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0)
+    chi_full = random_fermion(V, 13)
+    chi_full[: V // 2] = 0
+    psi_ref, n_ref, r_ref, _ = ref_op.solve(chi_full, solver=oracle.BICGSTAB, rsd=1e-10, maxit=2000)
+    psi, res = op.solve(chi_full[V // 2:], solver=cmx.BICGSTAB, rsd=1e-10, maxit=2000)
+    assert abs(res["n_count"] - n_ref) <= 2, (res["n_count"], n_ref)
+    assert rel(psi, psi_ref[V // 2:]) < 1e-8
+
+
+@pytest.mark.parametrize("sloppy", [cmx.SINGLE, cmx.HALF])
+def test_reliable_cg_mixed(sloppy):
+    """Mixed precision with FP64 reliable updates reaches a 1e-10 FP64 true residual."""
+    L = (8, 8, 8, 16)
+    say("reliable cg", sloppy)
+    V = int(np.prod(L))
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=91, nstout=6, rho=0.1), (1, 1, 1, -1))
+    mass = cmx.kappa_to_mass(0.1350)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sloppy)
+    chi_full = cmx.z2_source((V, 4, 3), 2)
+    chi_full[: V // 2] = 0
+    psi_ref, n_ref, r_ref, _ = ref_op.solve(chi_full, solver=oracle.CG, rsd=1e-11, maxit=10000)
+    psi, res = op.solve(chi_full[V // 2:], solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=20000, delta=0.1,
+                        sloppy_prec=sloppy)
+    say("  n_count", res["n_count"], "oracle CG", n_ref, "rel resid", res["rel_resid"], "updates", res["n_reliable"])
+    assert res["rel_resid"] <= 1e-10 * 1.5
+    assert rel(psi, psi_ref[V // 2:]) < 1e-7
+
+
+def test_twisted_golden_through_gpu():
+    """prec_wilson-twisted (tests/chroma/hadron/propagator/prec_wilson-twisted.{ini,out}.xml) on the GPU."""
+    from chroma_amd import qprop as qp
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "prec_wilson_twisted.json")))
+    L = tuple(gold["nrow"])
+    say("twisted golden")
+    V = int(np.prod(L))
+    U = cmx.gauge_generate(L, kind=0)
+    U = cmx.gauge_apply_bc(L, U, (1, 1, 1, 1), twist_by_pi=gold["phases_by_pi"], twist_dir=gold["phases_dir"])
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, cmx.kappa_to_mass(gold["kappa"]))
+    qprop = qp.PrecFermActQprop(op, cmx.CG, rsd=gold["RsdCG"], maxit=gold["MaxCG"])
+    prop, results = qp.quark_prop4(qprop, V, 0)
+    corr = qp.prop_corr(L, prop)
+    assert np.max(np.abs(corr - np.array(gold["prop_corr"])) / np.array(gold["prop_corr"])) < gold["tolerance"]
+    assert all(abs(r["n_count"] - gold["n_count"]) <= 1 for r in results)

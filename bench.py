@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the even-odd Wilson-clover hot path on one MI355X; prints one JSON line.
+
+Workload (BASELINE configuration "even-odd preconditioned clover, csw=1.0, 32^3x64"): Haar-random
+SU(3) start + 6 isotropic stout steps (rho = 0.1), antiperiodic in time, kappa = 0.1350, Z2 source on
+the odd checkerboard, mixed-precision CG (FP16 storage / FP32 arithmetic inner, FP64 reliable updates,
+delta = 0.1) to an FP64 true residual of 1e-10.
+
+value: achieved bandwidth of the hopping term D_eo (one application = one step) at FP16 storage with
+12-real links, in algorithmic bytes (each input spinor and link read once, output written once):
+48*2 + 8*12*2 + 8 (per-site FP32 norms of input and output) = 296 bytes per output site.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hop_bytes(prec, recon):
+    s = {0: 8, 1: 4, 2: 2}[prec]
+    b = 48 * s + 8 * recon * s
+    if prec == 2:
+        b += 8  # FP32 norm of the input and the output spinor
+    return b
+
+
+def eo_clover_bytes(prec, recon):
+    """Fused EO clover operator per odd site: 120 s + 16 L s_u + 144 s_c (two kernels)."""
+    s = {0: 8, 1: 4, 2: 2}[prec]
+    b = 120 * s + 16 * recon * s + 144 * s
+    if prec == 2:
+        b += 5 * 4 + 2 * 4  # spinor norms (5 spinor streams) and clover norms (2 clover streams)
+    return b
+
+
+def cpu_baseline(L, U, seconds_target=12.0):
+    """Oracle FP64 hopping term on the host cores (a bounded sample of the same workload)."""
+    import numpy as np
+    import oracle
+    ncores = len(os.sched_getaffinity(0))
+    oracle.set_threads(ncores)
+    op = oracle.Op(oracle.EO_WILSON, L, U, 0.0)  # the hop does not depend on the mass
+    t1 = op.time(0, 1)  # one call to size the sample
+    n = max(1, int(seconds_target / max(t1, 1e-6)))
+    t = op.time(0, n)
+    Vh = int(np.prod(L)) // 2
+    gbs = hop_bytes(0, 18) * Vh / t / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
+            "sample": f"{n} FP64 hopping-term applications at {'x'.join(map(str, L))} (oracle, {t * 1e3:.1f} ms each)",
+            "gflops": round(1320.0 * Vh / t / 1e9, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lattice", type=str, default="32,32,32,64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from chroma_amd import lib as cmx
+
+    L = tuple(int(v) for v in a.lattice.split(","))
+    V = int(np.prod(L))
+    Vh = V // 2
+    kappa, csw = 0.1350, 1.0
+    t0 = time.time()
+    U = cmx.gauge_generate(L, kind=2, seed=1, nstout=6, rho=0.1)
+    plaq = cmx.plaquette(L, U)
+    U = cmx.gauge_apply_bc(L, U, (1, 1, 1, -1))
+    t_gen = time.time() - t0
+    print(f"# gauge {L}: plaquette {plaq:.6f} ({t_gen:.1f} s)", file=sys.stderr, flush=True)
+
+    gauge = cmx.Gauge(L, U)
+    op = cmx.Operator(gauge, cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), csw, sloppy_prec=cmx.HALF,
+                      sloppy_recon=cmx.RECONS_12)
+    # warm-up, then exactly `steps` timed applications (HIP events on the operator's stream)
+    op.time(0, cmx.HALF, max(1, a.warmup))
+    ms = op.time(0, cmx.HALF, a.steps)
+    b16 = hop_bytes(cmx.HALF, 12) * Vh
+    gbs = b16 / (ms * 1e-3) / 1e9
+    print(f"# hop FP16/12: {ms * 1e3:.1f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+
+    extra = {}
+    for name, what, prec, nbytes in [
+        ("hop_fp32_recon12", 0, cmx.SINGLE, hop_bytes(1, 12) * Vh),
+        ("hop_fp64_recon18", 0, cmx.DOUBLE, hop_bytes(0, 18) * Vh),
+        ("eo_clover_op_fp16", 1, cmx.HALF, eo_clover_bytes(2, 12) * Vh),
+        ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
+        ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
+    ]:
+        op.time(what, prec, 5)
+        t = op.time(what, prec, a.steps)
+        extra[name] = {"us": round(t * 1e3, 2), "GB/s": round(nbytes / (t * 1e-3) / 1e9, 1)}
+        print(f"# {name}: {t * 1e3:.1f} us  {extra[name]['GB/s']} GB/s", file=sys.stderr, flush=True)
+    extra["hop_fp16_recon12"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1)}
+    extra["hop_gflops_fp16"] = round(1320.0 * Vh / (ms * 1e-3) / 1e9, 1)
+
+    solves = {}
+    if not a.no_solve:
+        chi = cmx.z2_source((Vh, 4, 3), 2)
+        for name, sp in [("plain_mixed_cg_fp16", cmx.HALF), ("plain_mixed_cg_fp32", cmx.SINGLE)]:
+            psi, res = op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=sp,
+                                true_residual=True)
+            solves[name] = {"seconds": round(res["solve_seconds"], 4), "iterations": res["n_count"],
+                            "true_rel_resid": res["rel_resid"], "reliable_updates": res["n_reliable"],
+                            "restarts": res["n_restarts"], "gflops": round(res["gflops"], 1)}
+            print(f"# solve {name}: {res['n_count']} it, {res['solve_seconds']:.3f} s, rel resid "
+                  f"{res['rel_resid']:.2e}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(L, U)
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "Dslash GB/s (% HBM peak); solve time to resid 1e-10, plain vs learned",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16-storage/f32",
+        "data": "synthetic (Haar + 6 stout steps, seeded)",
+        "config": {"workload": "EO Wilson-clover hopping term D_eo, 32^3x64 (configs[1] lattice)",
+                   "lattice": list(L), "kappa": kappa, "csw": csw, "plaquette": round(plaq, 6),
+                   "storage": "FP16 spinors + per-site FP32 norm, 12-real FP16 links",
+                   "bytes_per_site": hop_bytes(cmx.HALF, 12), "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "solve_time_plain_s": solves.get("plain_mixed_cg_fp16", {}).get("seconds"),
+        "solve_time_learned_s": None,
+        "solves": solves,
+        "kernels": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

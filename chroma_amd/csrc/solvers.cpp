@@ -227,24 +227,39 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
   op.pack(chi_h, n, chi);
   op.pack(psi_h, n, psi);
   std::memset(&res, 0, sizeof(res));
-  switch (p.solver) {
-    case CMX_SOLVER_CG: solve_cg(op, p, psi, chi, res); break;
-    case CMX_SOLVER_BICGSTAB: solve_bicgstab(op, p, psi, chi, res); break;
-    case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, p, psi, chi, res); break;
-    default: throw Error(CMX_ERR_ARG, "solve: unknown solver");
-  }
-  // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87)
+  cmx_solver_params q = p;
   Vec t(P64, n), rr(P64, n);
-  op.apply(P64, t.view(), psi.view(), +1, nullptr);
-  op.blas(B_XMY_NORM, {&rr, &chi, &t});
-  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
-  op.blas(B_NORM2, {&chi});
-  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR + 1);
-  CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, 2 * sizeof(double),
-                         hipMemcpyDeviceToHost, op.stream));
-  op.sync();
-  res.resid = std::sqrt(op.hS[S_TR]);
-  res.rel_resid = op.hS[S_TR + 1] > 0 ? res.resid / std::sqrt(op.hS[S_TR + 1]) : res.resid;
+  for (int restart = 0;; ++restart) {
+    cmx_solver_result part;
+    std::memset(&part, 0, sizeof(part));
+    switch (q.solver) {
+      case CMX_SOLVER_CG: solve_cg(op, q, psi, chi, part); break;
+      case CMX_SOLVER_BICGSTAB: solve_bicgstab(op, q, psi, chi, part); break;
+      case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, q, psi, chi, part); break;
+      default: throw Error(CMX_ERR_ARG, "solve: unknown solver");
+    }
+    res.n_count += part.n_count;
+    res.n_reliable += part.n_reliable;
+    res.solve_seconds += part.solve_seconds;
+    res.gflops += part.gflops * part.solve_seconds;
+    // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87)
+    op.apply(P64, t.view(), psi.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {&rr, &chi, &t});
+    op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
+    op.blas(B_NORM2, {&chi});
+    op.finalize(blas_blocks(n), 1, PG_STORE, S_TR + 1);
+    CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, 2 * sizeof(double),
+                           hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+    res.resid = std::sqrt(op.hS[S_TR]);
+    res.rel_resid = op.hS[S_TR + 1] > 0 ? res.resid / std::sqrt(op.hS[S_TR + 1]) : res.resid;
+    if (!p.true_residual || res.rel_resid <= p.rsd_target || restart >= 8 || res.n_count >= p.max_iter) break;
+    // restart from psi with the inner tolerance scaled by the observed true/inner ratio
+    q.rsd_target = q.rsd_target * std::max(0.02, 0.5 * p.rsd_target / res.rel_resid);
+    q.max_iter = std::max(1, p.max_iter - res.n_count);
+    res.n_restarts++;
+  }
+  res.gflops = res.solve_seconds > 0 ? res.gflops / res.solve_seconds : 0.0;
   op.unpack(psi_h, n, psi);
   res.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (p.verbose)

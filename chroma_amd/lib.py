@@ -34,13 +34,13 @@ class OpParams(ctypes.Structure):
 class SolverParams(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("max_iter", ctypes.c_int), ("rsd_target", ctypes.c_double),
                 ("delta", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("verbose", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("true_residual", ctypes.c_int)]
 
 
 class SolverResult(ctypes.Structure):
     _fields_ = [("n_count", ctypes.c_int), ("resid", ctypes.c_double), ("rel_resid", ctypes.c_double),
                 ("n_reliable", ctypes.c_int), ("solve_seconds", ctypes.c_double),
-                ("total_seconds", ctypes.c_double), ("gflops", ctypes.c_double)]
+                ("total_seconds", ctypes.c_double), ("gflops", ctypes.c_double), ("n_restarts", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -238,7 +238,7 @@ class Operator:
         return ms.value
 
     def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
-              check_every=0, verbose=False):
+              check_every=0, verbose=False, true_residual=False):
         """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays."""
         chi = np.ascontiguousarray(chi, np.complex128)
         single = chi.ndim == 3
@@ -246,11 +246,12 @@ class Operator:
         psis = np.zeros_like(chis) if psi is None else np.ascontiguousarray(psi, np.complex128).reshape(chis.shape).copy()
         nrhs = chis.shape[0]
         p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
-                         int(bool(verbose)))
+                         int(bool(verbose)), int(bool(true_residual)))
         res = (SolverResult * nrhs)()
         check(lib().cmx_solve(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))
         out = [dict(n_count=r.n_count, resid=r.resid, rel_resid=r.rel_resid, n_reliable=r.n_reliable,
-                    solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops) for r in res]
+                    solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
+                    n_restarts=r.n_restarts) for r in res]
         return (psis[0], out[0]) if single else (psis, out)
 
 

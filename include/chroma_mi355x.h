@@ -122,16 +122,21 @@ typedef struct {
   int sloppy_prec;   /* inner precision for the reliable solvers: CMX_SINGLE or CMX_HALF           */
   int check_every;   /* iterations between host convergence checks (0 = library default)          */
   int verbose;
+  int true_residual; /* 0: stop as the reference solver does (CG family: on the normal-equation
+                        residual, invcg2.cc:107,194).  1: additionally restart from the current psi
+                        with a tightened inner tolerance until |chi - A psi| <= rsd_target |chi| in
+                        FP64 (the generalisation of CHROMA_DO_ONE_CG_RESTART, syssolver_linop_cg.h:69-76) */
 } cmx_solver_params;
 
 typedef struct {
-  int n_count;           /* iterations (SystemSolverResults_t::n_count)                            */
+  int n_count;           /* iterations (SystemSolverResults_t::n_count), summed over restarts      */
   double resid;          /* |chi - A psi|, FP64                                                    */
   double rel_resid;      /* resid / |chi|                                                          */
   int n_reliable;        /* reliable updates performed                                             */
   double solve_seconds;  /* device time of the solve (HIP events), fields already resident         */
   double total_seconds;  /* wall time including host<->device transfers                            */
   double gflops;         /* Chroma flop convention (invcg2.cc:67, eoprec_clover_linop_w.cc:299-306) */
+  int n_restarts;        /* true_residual restarts performed                                       */
 } cmx_solver_result;
 
 int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,

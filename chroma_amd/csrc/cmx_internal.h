@@ -318,8 +318,22 @@ struct HopLaunch {
   double* partials;  // |out|^2 per work-group (nullable)
   const int* skip;   // device flag: return immediately when *skip != 0 (nullable)
   const int* run;    // device flag: run only when *run != 0 (nullable)
+  // launch shape (chosen by the operator's tuner, see hop_config)
+  int block;         // threads per work-group
+  int tiled;         // 1: tiled work-group order below, 0: linear order with the XCD remap
+  int tile_w;        // work-groups per (z,t) plane = Lx/2*Ly / block
+  int tile_zb;       // z planes per tile
+  int tile_t8;       // t slices per XCD slab = Lt/8
+  int prefetch;      // 1: double-buffered raw loads across the 8 directions
 };
-int hop_blocks(const Geom& g);
+
+// Launch-shape choice for the hopping kernels of one lattice and precision
+struct HopConfig {
+  int block = 256, tiled = 0, zb = 1, prefetch = 0;
+};
+// Fills the shape fields of h from c, falling back to the linear order when the lattice does not
+// admit the tiling (returns the number of work-groups)
+int hop_apply_config(HopLaunch& h, const HopConfig& c);
 void launch_hop(const HopLaunch& h, hipStream_t s);
 void launch_hop_p64(const HopLaunch& h, hipStream_t s);
 void launch_hop_p32(const HopLaunch& h, hipStream_t s);

@@ -223,13 +223,197 @@ __device__ __forceinline__ void clover_apply_site(const void* clov, long Vh, lon
   }
 }
 
-template <int P, int RECON, int ISIGN, int EPI, bool NORM>
+// ------------------------------------------------------------------------------------------------
+// Raw (undecoded) loads, so that the next direction's memory requests can be issued before the
+// current direction is decoded and multiplied (double-buffered across the 8 hopping directions).
+template <int P> struct RawSpinor;
+template <> struct RawSpinor<P32> {
+  float4 c[6];
+  __device__ __forceinline__ void load(const void* base, const float*, int stride, int j) {
+    const float4* b = reinterpret_cast<const float4*>(base);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = b[(size_t)k * stride + j];
+  }
+  __device__ __forceinline__ void decode(float (&s)[24]) const {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      s[4 * k] = c[k].x;
+      s[4 * k + 1] = c[k].y;
+      s[4 * k + 2] = c[k].z;
+      s[4 * k + 3] = c[k].w;
+    }
+  }
+};
+template <> struct RawSpinor<P16> {
+  uint4 c[3];
+  float n;
+  __device__ __forceinline__ void load(const void* base, const float* nrm, int stride, int j) {
+    const uint4* b = reinterpret_cast<const uint4*>(base);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = b[(size_t)k * stride + j];
+    n = nrm[j];
+  }
+  __device__ __forceinline__ void decode(float (&s)[24]) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      H8 v;
+      v.u = c[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float2 f = __half22float2(v.h[q]);
+        s[8 * k + 2 * q] = f.x * n;
+        s[8 * k + 2 * q + 1] = f.y * n;
+      }
+    }
+  }
+};
+template <int P, int RECON> struct RawLink;
+template <> struct RawLink<P32, 12> {
+  float4 c[3];
+  __device__ __forceinline__ void load(const void* base, int Vh, int j) {
+    const float4* b = reinterpret_cast<const float4*>(base);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = b[(size_t)k * Vh + j];
+  }
+  __device__ __forceinline__ void decode(float (&u)[18], float dre, float dim) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u[4 * k] = c[k].x;
+      u[4 * k + 1] = c[k].y;
+      u[4 * k + 2] = c[k].z;
+      u[4 * k + 3] = c[k].w;
+    }
+    recon_row2<float>(u, dre, dim);
+  }
+};
+template <> struct RawLink<P32, 18> {
+  float2 c[9];
+  __device__ __forceinline__ void load(const void* base, int Vh, int j) {
+    const float2* b = reinterpret_cast<const float2*>(base);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) c[k] = b[(size_t)k * Vh + j];
+  }
+  __device__ __forceinline__ void decode(float (&u)[18], float, float) const {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      u[2 * k] = c[k].x;
+      u[2 * k + 1] = c[k].y;
+    }
+  }
+};
+template <> struct RawLink<P16, 12> {
+  uint2 c[3];
+  __device__ __forceinline__ void load(const void* base, int Vh, int j) {
+    const uint2* b = reinterpret_cast<const uint2*>(base);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = b[(size_t)k * Vh + j];
+  }
+  __device__ __forceinline__ void decode(float (&u)[18], float dre, float dim) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      H4 v;
+      v.u = c[k];
+      const float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
+      u[4 * k] = f0.x;
+      u[4 * k + 1] = f0.y;
+      u[4 * k + 2] = f1.x;
+      u[4 * k + 3] = f1.y;
+    }
+    recon_row2<float>(u, dre, dim);
+  }
+};
+
+// Per-site context shared by the 8 directions
+struct HopSite {
+  int i;          // output site (checkerboard index)
+  int nb[8];      // neighbour index for step 2*mu (forward) and 2*mu+1 (backward)
+  int last[8];    // the link of this step sits on the last slice of its direction
+};
+
+template <int P, int RECON> struct RawStep {
+  RawSpinor<P> sp;
+  RawLink<P, RECON> ln;
+};
+
+template <int P, int RECON, int STEP>
+__device__ __forceinline__ void raw_step_load(const HopLaunch& h, const HopSite& st, RawStep<P, RECON>& r) {
+  constexpr int MU = STEP / 2;
+  constexpr bool FWD = (STEP % 2) == 0;
+  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
+  const int j = st.nb[STEP];
+  r.sp.load(h.in, h.in_nrm, h.in_stride, j);
+  const char* gb = reinterpret_cast<const char*>(h.gauge[FWD ? h.cb : 1 - h.cb]) + (size_t)MU * LCH * LCB * h.g.Vh;
+  r.ln.load(gb, h.g.Vh, FWD ? st.i : j);
+}
+
+template <int P, int RECON, int STEP, int ISIGN>
+__device__ __forceinline__ void raw_step_apply(const HopLaunch& h, const HopSite& st, const RawStep<P, RECON>& r,
+                                               float (&acc)[24]) {
+  constexpr int MU = STEP / 2;
+  constexpr bool FWD = (STEP % 2) == 0;
+  float p[24], u[18], hs[12], o[12];
+  r.sp.decode(p);
+  const int last = st.last[STEP];
+  r.ln.decode(u, (float)h.ph[MU][last][0], (float)h.ph[MU][last][1]);
+  if constexpr (FWD) {  // U_mu(x) (1 - isign gamma_mu) psi(x+mu)
+    project<MU, -ISIGN>(p, hs);
+    link_mul<false>(u, hs, o);
+    accumulate<MU, -ISIGN>(acc, o);
+  } else {  // U_mu(x-mu)^dag (1 + isign gamma_mu) psi(x-mu)
+    project<MU, ISIGN>(p, hs);
+    link_mul<true>(u, hs, o);
+    accumulate<MU, ISIGN>(acc, o);
+  }
+}
+
+// Double-buffered pass over the 8 steps: the loads of step s+1 are in flight while step s computes
+template <int P, int RECON, int ISIGN>
+__device__ __forceinline__ void hop_prefetched(const HopLaunch& h, const HopSite& st, float (&acc)[24]) {
+  RawStep<P, RECON> a, b;
+  raw_step_load<P, RECON, 0>(h, st, a);
+  raw_step_load<P, RECON, 1>(h, st, b);
+  raw_step_apply<P, RECON, 0, ISIGN>(h, st, a, acc);
+  raw_step_load<P, RECON, 2>(h, st, a);
+  raw_step_apply<P, RECON, 1, ISIGN>(h, st, b, acc);
+  raw_step_load<P, RECON, 3>(h, st, b);
+  raw_step_apply<P, RECON, 2, ISIGN>(h, st, a, acc);
+  raw_step_load<P, RECON, 4>(h, st, a);
+  raw_step_apply<P, RECON, 3, ISIGN>(h, st, b, acc);
+  raw_step_load<P, RECON, 5>(h, st, b);
+  raw_step_apply<P, RECON, 4, ISIGN>(h, st, a, acc);
+  raw_step_load<P, RECON, 6>(h, st, a);
+  raw_step_apply<P, RECON, 5, ISIGN>(h, st, b, acc);
+  raw_step_load<P, RECON, 7>(h, st, b);
+  raw_step_apply<P, RECON, 6, ISIGN>(h, st, a, acc);
+  raw_step_apply<P, RECON, 7, ISIGN>(h, st, b, acc);
+}
+
+// Output site of this lane.  Tiled order: work-group b runs on XCD b mod 8; XCD k owns the t-slab
+// [k*Lt/8, (k+1)*Lt/8) and walks it as (xy tile fastest, z within a block of tile_zb planes, t,
+// z block), so the t +- 1 and z +- 1 neighbours of a site are read again within a few work-groups
+// on the same XCD and hit its L2.  Linear order: contiguous sites with the XCD remap of xcd_tile.
+__device__ __forceinline__ int hop_site_index(const HopLaunch& h) {
+  if (h.tiled) {
+    const int b = blockIdx.x, xcd = b & 7;
+    int q = b >> 3;
+    const int wp = q % h.tile_w;
+    q /= h.tile_w;
+    const int zi = q % h.tile_zb;
+    q /= h.tile_zb;
+    const int tl = q % h.tile_t8;
+    const int zblk = q / h.tile_t8;
+    const int z = zblk * h.tile_zb + zi, t = xcd * h.tile_t8 + tl;
+    return (t * h.g.L[2] + z) * (h.g.Lxh * h.g.L[1]) + wp * (int)blockDim.x + (int)threadIdx.x;
+  }
+  return xcd_tile(blockIdx.x, gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
+}
+
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
 __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
   using R = typename SpinorIO<P>::R;
   if (h.skip && *h.skip) return;
   if (h.run && !*h.run) return;
-  const int blk = xcd_tile(blockIdx.x, gridDim.x);
-  const int i = blk * blockDim.x + threadIdx.x;
+  const int i = hop_site_index(h);
   double nsq = 0.0;
   if (i < h.g.Vh) {
     int x[4];
@@ -246,10 +430,27 @@ __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
     R acc[24];
 #pragma unroll
     for (int k = 0; k < 24; ++k) acc[k] = 0;
-    hop_dir<P, RECON, 0, ISIGN>(h, acc, i, x, 0);
-    hop_dir<P, RECON, 1, ISIGN>(h, acc, i, x, 0);
-    hop_dir<P, RECON, 2, ISIGN>(h, acc, i, x, 0);
-    hop_dir<P, RECON, 3, ISIGN>(h, acc, i, x, 0);
+    if constexpr (PF) {
+      HopSite st;
+      st.i = i;
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) {
+        const int L = h.g.L[mu];
+        int xf[4] = {x[0], x[1], x[2], x[3]}, xb[4] = {x[0], x[1], x[2], x[3]};
+        xf[mu] = (x[mu] + 1 == L) ? 0 : x[mu] + 1;
+        xb[mu] = (x[mu] == 0) ? L - 1 : x[mu] - 1;
+        st.nb[2 * mu] = ((xf[3] * h.g.L[2] + xf[2]) * h.g.L[1] + xf[1]) * h.g.Lxh + (xf[0] >> 1);
+        st.nb[2 * mu + 1] = ((xb[3] * h.g.L[2] + xb[2]) * h.g.L[1] + xb[1]) * h.g.Lxh + (xb[0] >> 1);
+        st.last[2 * mu] = (x[mu] == L - 1);
+        st.last[2 * mu + 1] = (x[mu] == 0);
+      }
+      hop_prefetched<P, RECON, ISIGN>(h, st, acc);
+    } else {
+      hop_dir<P, RECON, 0, ISIGN>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 1, ISIGN>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 2, ISIGN>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 3, ISIGN>(h, acc, i, x, 0);
+    }
 
     R out[24];
     const R b = R(h.b);
@@ -275,8 +476,7 @@ __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
     }
     SpinorIO<P>::store(h.out, h.out_nrm, h.out_stride, i, out);
     if constexpr (NORM) {
-      // |out|^2 of what was stored (FP16: the decoded value is what the next kernel reads; the
-      // difference is below the storage rounding and the reduction accumulates in FP64)
+      // |out|^2 of the values computed (below the FP16 storage rounding; FP64 accumulation)
 #pragma unroll
       for (int k = 0; k < 24; ++k) nsq += (double)out[k] * (double)out[k];
     }
@@ -288,15 +488,15 @@ __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
   }
 }
 
-template <int P, int RECON> void launch_hop_t(const HopLaunch& h, hipStream_t s) {
-  const int nb = hop_blocks(h.g);
+template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipStream_t s) {
+  const int nb = (h.g.Vh + h.block - 1) / h.block;
   const bool norm = h.partials != nullptr;
 #define CMX_HOP_CASE(IS, EP)                                                                              \
   if (h.isign == IS && h.epi == EP) {                                                                     \
     if (norm)                                                                                             \
-      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, true>), dim3(nb), dim3(256), 0, s, h);                 \
+      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, true, PF>), dim3(nb), dim3(h.block), 0, s, h);         \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, false>), dim3(nb), dim3(256), 0, s, h);                \
+      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, false, PF>), dim3(nb), dim3(h.block), 0, s, h);        \
     return;                                                                                               \
   }
   CMX_HOP_CASE(1, EPI_NONE)

@@ -2,5 +2,7 @@
 #include "hop_kernel.h"
 
 namespace cmx {
-void launch_hop_p16(const HopLaunch& h, hipStream_t s) { launch_hop_t<P16, 12>(h, s); }
+void launch_hop_p16(const HopLaunch& h, hipStream_t s) {
+  if (h.prefetch) launch_hop_t<P16, 12, true>(h, s); else launch_hop_t<P16, 12, false>(h, s);
+}
 }  // namespace cmx

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "parallel.h"
 #include "setup.h"
@@ -19,7 +21,19 @@ void launch_hop(const HopLaunch& h, hipStream_t s) {
     default: throw Error(CMX_ERR_ARG, "hop: bad precision");
   }
 }
-int hop_blocks(const Geom& g) { return (g.Vh + 255) / 256; }
+int hop_apply_config(HopLaunch& h, const HopConfig& c) {
+  h.block = c.block;
+  h.prefetch = c.prefetch;
+  h.tiled = 0;
+  const int P = h.g.Lxh * h.g.L[1];
+  if (c.tiled && P % c.block == 0 && h.g.L[3] % 8 == 0 && c.zb > 0 && h.g.L[2] % c.zb == 0) {
+    h.tiled = 1;
+    h.tile_w = P / c.block;
+    h.tile_zb = c.zb;
+    h.tile_t8 = h.g.L[3] / 8;
+  }
+  return (h.g.Vh + h.block - 1) / h.block;
+}
 
 namespace {
 
@@ -133,7 +147,7 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
                 "op_create: links are not SU(3) up to a per-direction boundary phase (deviation " +
                     std::to_string(gd->su3_dev) + "); use CMX_SINGLE with CMX_RECONS_18");
   CMX_HIP(hipStreamCreate(&op->stream));
-  op->npartials = std::max(2 * hop_blocks(op->g), blas_blocks((int)op->V)) * 4;
+  op->npartials = std::max(2 * (int)((op->Vh + 63) / 64), blas_blocks((int)op->V)) * 4;
   op->partials.alloc(sizeof(double) * op->npartials);
   op->state_S.alloc(sizeof(double) * S_NSLOTS);
   op->state_F.alloc(sizeof(int) * F_NFLAGS);
@@ -170,6 +184,7 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
     op->nneg_even = (long)std::llround(op->hS[1]);
   }
   op->ensure_prec(p.sloppy_prec);
+  if (p.autotune_dslash) op->autotune();
   CMX_HIP(hipStreamSynchronize(op->stream));
   return op.release();
 }
@@ -206,6 +221,7 @@ HopLaunch Op::hop_base(int prec) const {
   h.recon = recon[prec];
   for (int p_ = 0; p_ < 2; ++p_) h.gauge[p_] = prec == P64 ? gd->g64[p_].p : gauge[prec][p_].p;
   std::memcpy(h.ph, gd->ph, sizeof(h.ph));
+  hop_apply_config(h, hcfg[prec]);
   return h;
 }
 
@@ -221,8 +237,8 @@ static void set_x(HopLaunch& h, VRef r) {
 
 int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* skip, const int* run) {
   ensure_prec(prec);
-  const int nb = hop_blocks(g);
   HopLaunch h = hop_base(prec);
+  const int nb = (h.g.Vh + h.block - 1) / h.block;
   h.isign = isign;
   h.skip = skip;
   h.run = run;
@@ -342,6 +358,86 @@ void Op::finalize(int nparts, int nred, int prog, int slot) {
 void Op::sync() {
   CMX_HIP(hipGetLastError());
   CMX_HIP(hipStreamSynchronize(stream));
+}
+
+namespace {
+struct TuneKey {
+  int L[4], kind, prec;
+  bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(TuneKey)) < 0; }
+};
+std::map<TuneKey, HopConfig>& tune_cache() {
+  static std::map<TuneKey, HopConfig> m;
+  return m;
+}
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+}  // namespace
+
+void Op::autotune() {
+  std::vector<int> precs = {P64, par.sloppy_prec};
+  if (par.sloppy_prec == P16) precs.push_back(P32);
+  const int ns = subset_sites();
+  Vec a(P64, ns), b(P64, ns);
+  for (int prec : precs) {
+    TuneKey key;
+    std::memset(&key, 0, sizeof(key));
+    for (int m = 0; m < 4; ++m) key.L[m] = g.L[m];
+    key.kind = par.kind;
+    key.prec = prec;
+    {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end()) {
+        hcfg[prec] = it->second;
+        continue;
+      }
+    }
+    ensure_prec(prec);
+    Vec in(prec, ns), out(prec, ns);
+    blas(B_ZERO, {&in});
+    std::vector<HopConfig> cands;
+    for (int block : {128, 256})
+      for (int pf : {0, 1}) {
+        if (pf && prec == P64) continue;
+        cands.push_back(HopConfig{block, 0, 1, pf});
+        for (int zb : {1, 2, 4, 8}) cands.push_back(HopConfig{block, 1, zb, pf});
+      }
+    hipEvent_t e0, e1;
+    CMX_HIP(hipEventCreate(&e0));
+    CMX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    HopConfig bestc = hcfg[prec];
+    for (const HopConfig& c : cands) {
+      HopLaunch probe = hop_base(prec);
+      HopConfig saved = hcfg[prec];
+      hcfg[prec] = c;
+      hop_apply_config(probe, c);
+      if (c.tiled && !probe.tiled) {  // the lattice does not admit this tiling
+        hcfg[prec] = saved;
+        continue;
+      }
+      apply(prec, out.view(), in.view(), +1, nullptr);
+      CMX_HIP(hipEventRecord(e0, stream));
+      for (int k = 0; k < 4; ++k) apply(prec, out.view(), in.view(), +1, nullptr);
+      CMX_HIP(hipEventRecord(e1, stream));
+      CMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bestc = c;
+      }
+      hcfg[prec] = saved;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hcfg[prec] = bestc;
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    tune_cache()[key] = bestc;
+  }
+  CMX_HIP(hipGetLastError());
 }
 
 }  // namespace cmx

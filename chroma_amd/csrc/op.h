@@ -73,6 +73,7 @@ struct Op {
   long V = 0, Vh = 0;
   double fact = 0;  // Nd + Mass
   int recon[3] = {18, 12, 12};
+  HopConfig hcfg[3];      // hopping-kernel launch shape per precision (tuned by autotune())
   // per precision: links (per parity) and the clover blocks used by the EO operator
   DevBuf gauge[3][2];
   DevBuf clov_odd[3];     // A_oo
@@ -114,6 +115,10 @@ struct Op {
             int n = -1, int off = 0);
   void finalize(int nparts, int nred, int prog, int slot = -1);
   void sync();
+  // Times the candidate launch shapes of the EO operator at each precision in use and keeps the
+  // fastest (QUDA's AutotuneDslash role).  Results are cached per (lattice, kind, precision).
+  void autotune();
+  int hop_nblocks(int prec) const { return (int)((Vh + hcfg[prec].block - 1) / hcfg[prec].block); }
 };
 
 std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);

@@ -192,22 +192,32 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, c
   op.finalize(nb, 1, PG_RCG_INIT);
   op.blas(B_ZERO, {&x});
   op.blas(B_ZERO, {&pv});
-  auto iter = [&]() {
+  // One host synchronisation per iteration: the flags of the updateR / updateX decision are read
+  // back, and the FP64 reliable-update kernels are launched only when the device asks for them
+  // (an early-exit launch of a full-lattice grid costs tens of microseconds, more than the read-back).
+  long launched = 0;
+  while (true) {
     op.blas(B_XPAY, {&pv, &r}, S_BETA, 0, F_DONE);                                          // p = r + beta p
     const int np = op.apply(ps, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);
     op.finalize(np, 1, PG_RCG_D);                                                           // c = r_sq; a = c/d
     op.apply(ps, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);
     op.blas(B_AXPY2_NORM, {&x, &r, &pv, &mmp}, S_A, 0, F_DONE);                             // x += a p; r -= a mmp
     op.finalize(nb, 1, PG_RCG_R);                                                           // updateX / updateR
-    op.blas(B_COPY, {&x_dble, &x}, 0, 0, F_DONE, F_UPDR);                                   // x_dble = x
-    op.apply(P64, tmp1.view(), x_dble.view(), +1, nullptr, F + F_DONE, F + F_UPDR);
-    op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr, F + F_DONE, F + F_UPDR);
-    op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp2, &r}, 0, 0, F_DONE, F_UPDR);               // r = b - A x
+    CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+    if (op.hF[F_DONE]) break;  // converged (or stopped) in the previous iteration
+    if (op.hF[F_UPDR]) {       // r = b - A^dag A x in FP64 (reliable_cg.cc:120-136)
+      op.blas(B_COPY, {&x_dble, &x});
+      op.apply(P64, tmp1.view(), x_dble.view(), +1, nullptr);
+      op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr);
+      op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp2, &r});
+    }
     op.finalize(nb, 1, PG_RCG_UPD);                                                         // norms, convergence
-    op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble}, 0, 0, F_DONE, F_UPDX);                // psi += x; x = 0; b = r
-    op.blas(B_ADD, {&psi, &x}, 0, 0, F_DONE, F_CONV);                                       // converged: psi += x
-  };
-  res.n_count = run_batches(op, p, iter);
+    if (op.hF[F_UPDX]) op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble});                   // psi += x; x = 0; b = r
+    if (++launched > (long)p.max_iter + 4) break;                                           // device stops at MaxIter
+  }
+  op.blas(B_ADD, {&psi, &x});                                                               // psi += x (reliable_cg.cc:158-163)
+  res.n_count = op.hF[F_NCOUNT];
   read_state(op);
   res.solve_seconds = tm.stop();
   res.n_reliable = op.hF[F_NUPD];
@@ -228,6 +238,8 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
   op.pack(psi_h, n, psi);
   std::memset(&res, 0, sizeof(res));
   cmx_solver_params q = p;
+  // true-residual mode aims the first pass a little below the target, so that a restart is rare
+  if (p.true_residual) q.rsd_target = 0.5 * p.rsd_target;
   Vec t(P64, n), rr(P64, n);
   for (int restart = 0;; ++restart) {
     cmx_solver_result part;

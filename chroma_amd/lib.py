@@ -28,7 +28,8 @@ class CmxError(RuntimeError):
 
 class OpParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("mass", ctypes.c_double), ("clov_coeff_r", ctypes.c_double),
-                ("clov_coeff_t", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("sloppy_recon", ctypes.c_int)]
+                ("clov_coeff_t", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("sloppy_recon", ctypes.c_int),
+                ("autotune_dslash", ctypes.c_int)]
 
 
 class SolverParams(ctypes.Structure):
@@ -78,6 +79,9 @@ def lib():
                                      ctypes.POINTER(SolverResult)]),
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+        "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "cmx_op_set_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "cmx_op_autotune": (ctypes.c_int, [_vp]),
         "cmx_gauge_generate": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int,
                                               ctypes.c_double, _dp]),
         "cmx_gauge_apply_bc": (ctypes.c_int, [_ip, _dp, _ip, _dp, _ip]),
@@ -171,13 +175,14 @@ class Gauge:
 class Operator:
     """A device fermion operator (LinearOperator<LatticeFermion>, lib/linearop.h:25-49)."""
 
-    def __init__(self, gauge, kind, mass, csw_r=0.0, csw_t=None, sloppy_prec=SINGLE, sloppy_recon=RECONS_12):
+    def __init__(self, gauge, kind, mass, csw_r=0.0, csw_t=None, sloppy_prec=SINGLE, sloppy_recon=RECONS_12,
+                 autotune=False):
         self.gauge = gauge
         self.L = gauge.L
         self.V = int(np.prod(self.L))
         self.kind = kind
         p = OpParams(int(kind), float(mass), float(csw_r), float(csw_r if csw_t is None else csw_t),
-                     int(sloppy_prec), int(sloppy_recon))
+                     int(sloppy_prec), int(sloppy_recon), int(bool(autotune)))
         h = _vp()
         check(lib().cmx_op_create(gauge.h, ctypes.byref(p), ctypes.byref(h)))
         self.h = h
@@ -231,6 +236,18 @@ class Operator:
         out = np.zeros_like(psi_cb)
         check(lib().cmx_layout_roundtrip(self.h, int(prec), _d(psi_cb), _d(out)))
         return out
+
+    def hop_config(self, prec):
+        cfg = (ctypes.c_int * 4)()
+        check(lib().cmx_op_get_hop_config(self.h, int(prec), cfg))
+        return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3])
+
+    def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0):
+        cfg = (ctypes.c_int * 4)(int(block), int(tiled), int(zb), int(prefetch))
+        check(lib().cmx_op_set_hop_config(self.h, int(prec), cfg))
+
+    def autotune(self):
+        check(lib().cmx_op_autotune(self.h))
 
     def time(self, what, prec, n):
         ms = ctypes.c_double()

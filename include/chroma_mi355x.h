@@ -80,6 +80,8 @@ typedef struct {
   double clov_coeff_t; /* CloverFermActParams clovCoeffT                                            */
   int sloppy_prec;     /* CMX_SINGLE or CMX_HALF: the inner (sloppy) operator copy                  */
   int sloppy_recon;    /* CMX_RECONS_12 or CMX_RECONS_18                                            */
+  int autotune_dslash; /* 1: time the candidate launch shapes at creation and keep the fastest
+                          (AutotuneDslash, syssolver_quda_clover_params.cc:93-99)                  */
 } cmx_op_params;
 
 int cmx_op_create(cmx_gauge g, const cmx_op_params* p, cmx_op* out);
@@ -141,6 +143,14 @@ typedef struct {
 
 int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
               cmx_solver_result* res);
+
+/* ---- launch shape of the hopping kernels ---------------------------------------------------------
+ * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
+ * double-buffered direction loads (0/1; FP32/FP16 only)}.  No counterpart in the reference beyond
+ * QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
+int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[4]);
+int cmx_op_set_hop_config(cmx_op op, int prec, const int cfg[4]);
+int cmx_op_autotune(cmx_op op);
 
 /* ---- measurement -------------------------------------------------------------------------------
  * Average device milliseconds per call over n back-to-back launches on the operator's stream,

@@ -244,3 +244,39 @@ def test_twisted_golden_through_gpu():
     corr = qp.prop_corr(L, prop)
     assert np.max(np.abs(corr - np.array(gold["prop_corr"])) / np.array(gold["prop_corr"])) < gold["tolerance"]
     assert all(abs(r["n_count"] - gold["n_count"]) <= 1 for r in results)
+
+
+@pytest.mark.parametrize("prec", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
+@pytest.mark.parametrize("cfg", [(64, 0, 1, 0), (64, 1, 1, 0), (64, 1, 2, 1), (128, 1, 4, 1), (128, 0, 1, 1),
+                                 (128, 1, 8, 0), (256, 1, 2, 1)])
+def test_hop_launch_shapes(prec, cfg):
+    """Every launch shape the tuner may pick (block size, tiled work-group order, prefetch) gives the
+    same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and 128."""
+    L = (16, 16, 8, 16)
+    say("launch shape", prec, cfg)
+    V = int(np.prod(L))
+    U = gauge_field(L, 101)
+    mass = cmx.kappa_to_mass(0.13)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sp)
+    op.set_hop_config(prec, *cfg)
+    psi = random_fermion(V, 17)
+    for isign in (+1, -1):
+        ref = ref_op(psi, isign)[V // 2:]
+        assert rel(op(psi[V // 2:], isign, prec), ref) < TOL[prec]
+    ref = oracle.dslash(L, U, psi, +1, 1)[V // 2:]
+    assert rel(op.dslash(psi[: V // 2], +1, 1, prec), ref) < TOL[prec]
+
+
+def test_autotune_keeps_results():
+    L = (16, 16, 8, 16)
+    say("autotune")
+    V = int(np.prod(L))
+    U = gauge_field(L, 102)
+    mass = cmx.kappa_to_mass(0.13)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF, autotune=True)
+    ref = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    psi = random_fermion(V, 18)
+    for prec in (cmx.DOUBLE, cmx.HALF):
+        assert rel(op(psi[V // 2:], +1, prec), ref(psi)[V // 2:]) < TOL[prec]

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the even-odd Wilson-clover hot path on one MI355X; prints one JSON line.
 
-Workload (BASELINE configuration "even-odd preconditioned clover, csw=1.0, 32^3x64"): Haar-random
+Workload (even-odd preconditioned clover, csw = 1.0, 32^3x64): Haar-random
 SU(3) start + 6 isotropic stout steps (rho = 0.1), antiperiodic in time, kappa = 0.1350, Z2 source on
 the odd checkerboard, mixed-precision CG (FP16 storage / FP32 arithmetic inner, FP64 reliable updates,
 delta = 0.1) to an FP64 true residual of 1e-10.
@@ -67,7 +67,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md but not built)", file=sys.stderr)
+              "(the multi-GPU domain decomposition is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -139,7 +139,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f16-storage/f32",
         "data": "synthetic (Haar + 6 stout steps, seeded)",
-        "config": {"workload": "EO Wilson-clover hopping term D_eo, 32^3x64 (configs[1] lattice)",
+        "config": {"workload": "EO Wilson-clover hopping term D_eo, 32^3x64, stout-smeared Haar field",
                    "lattice": list(L), "kappa": kappa, "csw": csw, "plaquette": round(plaq, 6),
                    "storage": "FP16 spinors + per-site FP32 norm, 12-real FP16 links",
                    "bytes_per_site": hop_bytes(cmx.HALF, 12), "parallelism": "1 GPU"},

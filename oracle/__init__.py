@@ -7,7 +7,8 @@ ctypes wrapper over ``liboracle.so``, the CPU restatement of Chroma's even-odd W
 Host layout (identical to the product C-ABI): full-lattice arrays, checkerboard-major, x-fastest
 within a checkerboard; fermions ``complex128[V, 4, 3]``; gauge ``complex128[4, V, 3, 3]``.
 
-Parity status of every oracle function (what pins it) is listed in ``oracle/README.md``.
+Every oracle function is pinned by the golden vectors, closed forms and dense constructions in
+``tests/test_oracle.py``.
 """
 import ctypes
 import os

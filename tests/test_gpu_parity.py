@@ -149,7 +149,7 @@ def test_unprec_wilson_operator():
 
 
 def test_cg_unprec_wilson_config1_small():
-    """configs[0] recipe at 4^3x8: unpreconditioned Wilson CGNE, FP64, point source."""
+    """Unpreconditioned Wilson CGNE at 4^3x8, kappa = 0.115, FP64, point source at the origin."""
     L = (4, 4, 4, 8)
     say("cg unprec")
     V = int(np.prod(L))

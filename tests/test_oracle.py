@@ -1,0 +1,364 @@
+"""The oracle against what pins it (CPU only):
+
+* the reference's RNG-free regression golden prec_wilson-twisted (prop_corr to the metric file's 1e-5,
+  n_count 17 per solve);
+* the free-field closed form of the Wilson operator (plane waves, periodic / antiperiodic / twisted);
+* an independent dense construction of the Wilson-clover matrix from the DR gamma matrices (numpy);
+* gamma5-hermiticity, gauge covariance, clover identities (unit gauge, A A^-1 = 1, log det);
+* Schur-complement / EO-reconstruction identities and a dense solve for the solvers.
+
+Planted-mistake checks at the end show that each pinning test fails for a plausible oracle error
+(introduced here, on the CPU, in a numpy copy of the relevant formula).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.util import numpy_gauge, random_fermion, rel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# DeGrand-Rossi gammas (docs/notes/gamma_conventions.tex:74-79), written independently of oracle.cpp
+s1 = np.array([[0, 1], [1, 0]], complex)
+s2 = np.array([[0, -1j], [1j, 0]], complex)
+s3 = np.array([[1, 0], [0, -1]], complex)
+Z2 = np.zeros((2, 2), complex)
+I2 = np.eye(2)
+GAMMA = [np.block([[Z2, 1j * s1], [-1j * s1, Z2]]), np.block([[Z2, -1j * s2], [1j * s2, Z2]]),
+         np.block([[Z2, 1j * s3], [-1j * s3, Z2]]), np.block([[Z2, I2], [I2, Z2]])]
+G5 = GAMMA[0] @ GAMMA[1] @ GAMMA[2] @ GAMMA[3]
+
+
+def coords(L):
+    V = int(np.prod(L))
+    return np.array([oracle.site_coords(L, s) for s in range(V)])
+
+
+def index_of(L, x):
+    cb = int(sum(x)) & 1
+    lex = ((x[3] * L[2] + x[2]) * L[1] + x[1]) * L[0] + x[0]
+    return cb * (int(np.prod(L)) // 2) + lex // 2
+
+
+def dense_dslash(L, U, isign):
+    """D(isign) as a dense (12V x 12V) matrix built from the formula of lwldslash_w.h:21-43."""
+    V = int(np.prod(L))
+    X = coords(L)
+    D = np.zeros((V, 4, 3, V, 4, 3), complex)
+    for s in range(V):
+        for mu in range(4):
+            xp = X[s].copy(); xp[mu] = (xp[mu] + 1) % L[mu]
+            xm = X[s].copy(); xm[mu] = (xm[mu] - 1) % L[mu]
+            sp, sm = index_of(L, xp), index_of(L, xm)
+            D[s, :, :, sp, :, :] += np.einsum("ab,ij->aibj", np.eye(4) - isign * GAMMA[mu], U[mu, s])
+            D[s, :, :, sm, :, :] += np.einsum("ab,ij->aibj", np.eye(4) + isign * GAMMA[mu], U[mu, sm].conj().T)
+    return D.reshape(12 * V, 12 * V)
+
+
+def field_strength_numpy(L, U):
+    """F_{mu nu} = (Q - Q^dag)/8 with Q the four clover leaves of mesfield.cc:15-21 (numpy)."""
+    V = int(np.prod(L))
+    X = coords(L)
+
+    def sh(s, mu, d):
+        x = X[s].copy(); x[mu] = (x[mu] + d) % L[mu]
+        return index_of(L, x)
+
+    F = {}
+    for mu in range(3):
+        for nu in range(mu + 1, 4):
+            f = np.zeros((V, 3, 3), complex)
+            for s in range(V):
+                u = lambda m, t: U[m, t]
+                d = lambda m, t: U[m, t].conj().T
+                Q = u(mu, s) @ u(nu, sh(s, mu, 1)) @ d(mu, sh(s, nu, 1)) @ d(nu, s)
+                xm = sh(s, mu, -1)
+                Q += u(nu, s) @ d(mu, sh(xm, nu, 1)) @ d(nu, xm) @ u(mu, xm)
+                Q += d(mu, xm) @ d(nu, sh(xm, nu, -1)) @ u(mu, sh(xm, nu, -1)) @ u(nu, sh(s, nu, -1))
+                xn = sh(s, nu, -1)
+                Q += d(nu, xn) @ u(mu, xn) @ u(nu, sh(xn, mu, 1)) @ d(mu, s)
+                f[s] = (Q - Q.conj().T) / 8
+            F[(mu, nu)] = f
+    return F
+
+
+def tri_to_dense(tri):
+    """PrimitiveClovTriang (72 reals) -> dense 12x12 per site (block b on components 6b..6b+5)."""
+    V = tri.shape[0]
+    A = np.zeros((V, 12, 12), complex)
+    for b in range(2):
+        diag = tri[:, 6 * b:6 * b + 6]
+        off = tri[:, 12 + 30 * b:12 + 30 * b + 30].reshape(V, 15, 2)
+        off = off[..., 0] + 1j * off[..., 1]
+        for i in range(6):
+            A[:, 6 * b + i, 6 * b + i] = diag[:, i]
+            for j in range(i):
+                e = i * (i - 1) // 2 + j
+                A[:, 6 * b + i, 6 * b + j] = off[:, e]
+                A[:, 6 * b + j, 6 * b + i] = off[:, e].conj()
+    return A
+
+
+# ---------------------------------------------------------------------------------------------------
+def test_gamma_algebra():
+    for m in range(4):
+        for n in range(4):
+            ac = GAMMA[m] @ GAMMA[n] + GAMMA[n] @ GAMMA[m]
+            assert np.allclose(ac, 2 * np.eye(4) * (m == n))
+    assert np.allclose(G5, np.diag([1, 1, -1, -1]))
+
+
+def test_twisted_golden():
+    """tests/chroma/hadron/propagator/prec_wilson-twisted: prop_corr (metric 1e-5) and n_count."""
+    g = json.load(open(os.path.join(HERE, "golden", "prec_wilson_twisted.json")))
+    L = tuple(g["nrow"])
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:] = np.eye(3)
+    for ph, d in zip(g["phases_by_pi"], g["phases_dir"]):
+        U[d] *= np.exp(1j * np.pi * ph / L[d])
+    op = oracle.Op(oracle.EO_WILSON, L, U, 1 / (2 * g["kappa"]) - 4)
+    ncs, res, corr, _ = op.quarkprop4_point(index_of(L, g["t_srce"]), oracle.CG, g["RsdCG"], g["MaxCG"])
+    gold = np.array(g["prop_corr"])
+    assert np.max(np.abs(corr - gold) / gold) < g["tolerance"]
+    assert all(abs(n - g["n_count"]) <= 1 for n in ncs)  # the golden is an FP32 build
+
+
+@pytest.mark.parametrize("bc", ["periodic", "antiperiodic", "twisted"])
+def test_free_field_closed_form(bc):
+    """U = 1 (with BCs): plane waves psi = e^{ip.x} u are eigenvectors and
+    D(isign) psi = sum_mu [2 cos q_mu - 2 i isign gamma_mu sin q_mu] psi, q = p + (twist phase).
+    Antiperiodic time (link -1 on the last slice, simple_fermbc.h:87-104) admits p_t = pi(2n+1)/L_t."""
+    L = (4, 4, 6, 8)
+    V = int(np.prod(L))
+    X = coords(L)
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:] = np.eye(3)
+    n = np.array([1, 2, 1, 3])
+    p = 2 * np.pi * n / np.array(L)
+    theta = np.zeros(4)
+    if bc == "antiperiodic":
+        U[3, X[:, 3] == L[3] - 1] *= -1
+        p[3] = np.pi * (2 * n[3] + 1) / L[3]
+    if bc == "twisted":
+        for d, ph in zip((0, 1, 2), (0.2, 0.4, 0.3)):
+            U[d] *= np.exp(1j * np.pi * ph / L[d])
+            theta[d] = np.pi * ph / L[d]
+    u = np.random.default_rng(1).standard_normal((4, 3)) + 0j
+    psi = np.exp(1j * X @ p)[:, None, None] * u[None]
+    q = p + theta
+    for isign in (+1, -1):
+        full = np.zeros_like(psi)
+        for cb in (0, 1):
+            oracle.dslash(L, U, psi, isign, cb, out=full)
+        Mspin = sum(2 * np.cos(q[m]) * np.eye(4) - 2j * isign * GAMMA[m] * np.sin(q[m]) for m in range(4))
+        expect = np.exp(1j * X @ p)[:, None, None] * np.einsum("ab,bc->ac", Mspin, u)[None]
+        assert rel(full, expect) < 1e-13
+
+
+@pytest.mark.parametrize("L", [(2, 2, 2, 2), (2, 2, 4, 2)])
+def test_dslash_dense_bruteforce(L):
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 3)
+    psi = random_fermion(V, 4)
+    for isign in (+1, -1):
+        D = dense_dslash(L, U, isign)
+        ref = (D @ psi.reshape(-1)).reshape(V, 4, 3)
+        got = np.zeros_like(psi)
+        for cb in (0, 1):
+            oracle.dslash(L, U, psi, isign, cb, out=got)
+        assert rel(got, ref) < 1e-13
+
+
+def test_gamma5_hermiticity_and_gauge_covariance():
+    L = (4, 4, 4, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 5)
+    chi, psi = random_fermion(V, 6), random_fermion(V, 7)
+    m = oracle.Op(oracle.UNPREC_WILSON, L, U, 0.1)
+    # <chi, M(+) psi> = <M(-) chi, psi>
+    assert abs(np.vdot(chi, m(psi, +1)) - np.vdot(m(chi, -1), psi)) < 1e-12 * V
+    # gamma5 M gamma5 = M^dag
+    g5 = np.diag(G5)
+    assert rel(g5[None, :, None] * m(g5[None, :, None] * psi, +1), m(psi, -1)) < 1e-13
+    # gauge covariance: U -> g(x) U g^dag(x+mu), psi -> g psi
+    from tests.util import random_su3_numpy
+    g = random_su3_numpy(V, 8)
+    X = coords(L)
+    Ug = np.empty_like(U)
+    for mu in range(4):
+        xp = X.copy(); xp[:, mu] = (xp[:, mu] + 1) % L[mu]
+        idx = np.array([index_of(L, x) for x in xp])
+        Ug[mu] = g @ U[mu] @ g[idx].conj().transpose(0, 2, 1)
+    mg = oracle.Op(oracle.UNPREC_WILSON, L, Ug, 0.1)
+    gpsi = np.einsum("vij,vsj->vsi", g, psi)
+    assert rel(mg(gpsi), np.einsum("vij,vsj->vsi", g, m(psi))) < 1e-13
+
+
+def test_mesfield_against_numpy_leaves():
+    L = (2, 4, 2, 4)
+    U = numpy_gauge(L, 9)
+    F = oracle.mesfield(L, U)
+    Fn = field_strength_numpy(L, U)
+    for k, (mu, nu) in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]):
+        assert np.max(np.abs(F[k] - Fn[(mu, nu)])) < 1e-14
+
+
+def test_clover_dense_construction():
+    """A = (Nd+m) 1 - sum_{mu<nu} c_{mu nu} gamma_mu gamma_nu (x) F_{mu nu} with c = csw/2: the packing of
+    makeClovSiteLoop (clover_term_qdp_w.h:398-522) against a dense construction from the gammas."""
+    L = (2, 4, 2, 4)
+    U = numpy_gauge(L, 10)
+    mass, cR, cT = 0.07, 1.3, 0.9
+    tri = oracle.make_clover(L, U, cR, cT, mass)
+    Fn = field_strength_numpy(L, U)
+    V = int(np.prod(L))
+    A = np.zeros((V, 12, 12), complex)
+    A[:] = (4 + mass) * np.eye(12)
+    for (mu, nu), f in Fn.items():
+        c = 0.5 * (cT if 3 in (mu, nu) else cR)
+        A -= c * np.einsum("ab,vij->vaibj", GAMMA[mu] @ GAMMA[nu], f).reshape(V, 12, 12)
+    assert np.max(np.abs(tri_to_dense(tri) - A)) < 1e-13
+
+
+def test_clover_unit_gauge_and_inverse():
+    L = (4, 4, 4, 4)
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:] = np.eye(3)
+    mass = 0.25
+    tri = oracle.make_clover(L, U, 1.0, 1.0, mass)
+    assert np.allclose(tri_to_dense(tri), (4 + mass) * np.eye(12))
+    ld, nn = oracle.clover_invert(L, tri, 0)
+    # log det closed form: 12 Vh log|Nd + m| (cf. symm_prec_tests.cc:633-653)
+    assert abs(ld - 12 * (V // 2) * np.log(4 + mass)) < 1e-10 and nn == 0
+    U = numpy_gauge(L, 11)
+    tri = oracle.make_clover(L, U, 1.5, 1.5, -0.2)
+    inv = tri.copy()
+    ld, nn = oracle.clover_invert(L, inv, 0)
+    A, Ai = tri_to_dense(tri)[: V // 2], tri_to_dense(inv)[: V // 2]
+    assert np.max(np.abs(A @ Ai - np.eye(12))) < 1e-12
+    sign, logabs = np.linalg.slogdet(A)
+    assert abs(ld - logabs.sum()) < 1e-9
+    # A is Hermitian
+    assert np.allclose(A, A.conj().transpose(0, 2, 1))
+
+
+def test_clover_apply_matches_dense():
+    L = (4, 4, 2, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 12)
+    tri = oracle.make_clover(L, U, 1.0, 1.0, 0.1)
+    psi = random_fermion(V, 13)
+    out = np.zeros_like(psi)
+    for cb in (0, 1):
+        oracle.clover_apply(L, tri, psi, cb, out=out)
+    ref = np.einsum("vab,vb->va", tri_to_dense(tri), psi.reshape(V, 12)).reshape(V, 4, 3)
+    assert rel(out, ref) < 1e-14
+
+
+@pytest.mark.parametrize("kind", [oracle.EO_WILSON, oracle.EO_CLOVER])
+def test_schur_and_reconstruction(kind):
+    """EO pieces and the Schur complement: solving the preconditioned system and reconstructing the
+    even part solves the unpreconditioned system (eoprec_fermact_qprop.cc:41-80, eoprec_linop.h:147-165)."""
+    L = (4, 4, 4, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 14)
+    op = oracle.Op(kind, L, U, 0.2, 1.0, 1.0)
+    chi = random_fermion(V, 15)
+    psi, nc, resid = op.qprop(chi, oracle.CG, 1e-12, 2000)
+    assert resid / np.linalg.norm(chi) < 1e-10
+    r = op.piece("unprec", psi) - chi
+    assert np.linalg.norm(r) / np.linalg.norm(chi) < 1e-10
+    # M_oo = A_oo - D_oe A_ee^-1 D_eo on the odd sites
+    t = random_fermion(V, 16)
+    t[: V // 2] = 0
+    m = op(t)[V // 2:]
+    s = op.piece("oddEven", op.piece("evenEvenInv", op.piece("evenOdd", t)))
+    assert rel(m, (op.piece("oddOdd", t) - s)[V // 2:]) < 1e-13
+
+
+def test_prec_vs_unprec_propagator():
+    """prop_corr of the EO-preconditioned and unpreconditioned Wilson solves agree (the invariant of
+    prec_wilson.out.xml:487 vs unprec_wilson.out.xml:487)."""
+    L = (4, 4, 4, 4)
+    U = numpy_gauge(L, 17)
+    m = 0.3
+    a = oracle.Op(oracle.EO_WILSON, L, U, m).quarkprop4_point(0, oracle.CG, 1e-11, 3000)[2]
+    b = oracle.Op(oracle.UNPREC_WILSON, L, U, m)
+    V = int(np.prod(L))
+    corr = np.zeros(L[3])
+    X = coords(L)
+    for col in range(3):
+        for sp in range(4):
+            chi = np.zeros((V, 4, 3), complex)
+            chi[0, sp, col] = 1
+            psi = b.solve(chi, solver=oracle.CG, rsd=1e-11, maxit=3000)[0]
+            np.add.at(corr, X[:, 3], np.sum(np.abs(psi.reshape(V, 12)) ** 2, axis=1))
+    assert np.max(np.abs(a - corr) / corr) < 1e-8
+
+
+def test_cg_against_dense_solve_and_trace():
+    L = (2, 2, 2, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 18)
+    mass = 0.4
+    op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    D = dense_dslash(L, U, +1)
+    M = (4 + mass) * np.eye(12 * V) - 0.5 * D
+    chi = random_fermion(V, 19)
+    psi, nc, resid, trace = op.solve(chi, solver=oracle.CG, rsd=1e-12, maxit=500)
+    exact = np.linalg.solve(M, chi.reshape(-1)).reshape(V, 4, 3)
+    assert rel(psi, exact) < 1e-9
+    # CG residual norms decrease to the target; exact arithmetic terminates in <= 12V steps
+    assert trace[-1] <= 1e-24 * np.vdot(chi, chi).real * 1e2
+    assert nc <= 12 * V
+
+
+def test_bicgstab_and_reliable_cg():
+    L = (4, 4, 4, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 20)
+    op = oracle.Op(oracle.EO_CLOVER, L, U, 0.05, 1.0, 1.0)
+    chi = random_fermion(V, 21)
+    chi[: V // 2] = 0
+    ref = op.solve(chi, solver=oracle.CG, rsd=1e-12, maxit=5000)[0]
+    for solver in (oracle.BICGSTAB, oracle.RELIABLE_CG):
+        psi, nc, resid, _ = op.solve(chi, solver=solver, rsd=1e-10, maxit=5000, delta=0.1)
+        assert nc > 0
+        assert rel(psi[V // 2:], ref[V // 2:]) < 1e-8
+
+
+# ---- planted mistakes: each pinning check above fails for a plausible error -----------------------
+def test_planted_mistakes_are_caught():
+    L = (2, 2, 2, 2)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 22)
+    psi = random_fermion(V, 23)
+    good = dense_dslash(L, U, +1) @ psi.reshape(-1)
+    # (a) sign of the projector flipped: (1 + gamma) instead of (1 - gamma) in the forward hop
+    bad = (dense_dslash(L, U, -1) @ psi.reshape(-1))
+    assert rel(bad, good) > 1e-2
+    # (b) link not daggered in the backward hop
+    Ub = U.copy()
+    X = coords(L)
+    D_bad = np.zeros((V, 4, 3, V, 4, 3), complex)
+    for s in range(V):
+        for mu in range(4):
+            xp = X[s].copy(); xp[mu] = (xp[mu] + 1) % L[mu]
+            xm = X[s].copy(); xm[mu] = (xm[mu] - 1) % L[mu]
+            D_bad[s, :, :, index_of(L, xp)] += np.einsum("ab,ij->aibj", np.eye(4) - GAMMA[mu], Ub[mu, s])
+            D_bad[s, :, :, index_of(L, xm)] += np.einsum("ab,ij->aibj", np.eye(4) + GAMMA[mu], Ub[mu, index_of(L, xm)])
+    assert rel(D_bad.reshape(12 * V, 12 * V) @ psi.reshape(-1), good) > 1e-2
+    # (c) clover coefficient without the factor 1/2, or with the wrong sign, breaks the dense check
+    tri = oracle.make_clover(L, U, 1.0, 1.0, 0.0)
+    Fn = field_strength_numpy(L, U)
+    for c_bad in (1.0, -0.5):
+        A = np.zeros((V, 12, 12), complex)
+        A[:] = 4 * np.eye(12)
+        for (mu, nu), f in Fn.items():
+            A -= c_bad * np.einsum("ab,vij->vaibj", GAMMA[mu] @ GAMMA[nu], f).reshape(V, 12, 12)
+        assert np.max(np.abs(tri_to_dense(tri) - A)) > 1e-3

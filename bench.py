@@ -85,8 +85,11 @@ def main():
     print(f"# gauge {L}: plaquette {plaq:.6f} ({t_gen:.1f} s)", file=sys.stderr, flush=True)
 
     gauge = cmx.Gauge(L, U)
+    t0 = time.time()
     op = cmx.Operator(gauge, cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), csw, sloppy_prec=cmx.HALF,
-                      sloppy_recon=cmx.RECONS_12)
+                      sloppy_recon=cmx.RECONS_12, autotune=True)
+    print(f"# operator built and launch shapes tuned ({time.time() - t0:.1f} s): "
+          f"{ {p: (op.hop_config(p, 1), op.hop_config(p, 2)) for p in (0, 1, 2)} }", file=sys.stderr, flush=True)
     # warm-up, then exactly `steps` timed applications (HIP events on the operator's stream)
     op.time(0, cmx.HALF, max(1, a.warmup))
     ms = op.time(0, cmx.HALF, a.steps)
@@ -150,6 +153,8 @@ def main():
         "solve_time_learned_s": None,
         "solves": solves,
         "kernels": extra,
+        "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}
+                          for name, p in (("fp64", 0), ("fp32", 1), ("fp16", 2))},
     }
     print(json.dumps(out))
 

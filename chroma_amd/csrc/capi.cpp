@@ -332,23 +332,27 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
   });
 }
 
-int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[4]) {
+int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[6]) {
   return guarded([&] {
     cmx::Op& op = O(h);
     CMX_REQUIRE(cfg && prec >= 0 && prec <= 2, CMX_ERR_ARG, "get_hop_config: bad argument");
-    const cmx::HopConfig& c = op.hcfg[prec];
-    cfg[0] = c.block, cfg[1] = c.tiled, cfg[2] = c.zb, cfg[3] = c.prefetch;
+    const cmx::HopConfig& c = op.hcfg[prec][cfg[5] == 2 ? 1 : 0];
+    cfg[0] = c.block, cfg[1] = c.tiled, cfg[2] = c.zb, cfg[3] = c.prefetch, cfg[4] = c.fast;
   });
 }
 
-int cmx_op_set_hop_config(cmx_op h, int prec, const int cfg[4]) {
+int cmx_op_set_hop_config(cmx_op h, int prec, const int cfg[6]) {
   return guarded([&] {
     cmx::Op& op = O(h);
     CMX_REQUIRE(cfg && prec >= 0 && prec <= 2, CMX_ERR_ARG, "set_hop_config: bad argument");
     CMX_REQUIRE(cfg[0] == 64 || cfg[0] == 128 || cfg[0] == 256, CMX_ERR_ARG, "set_hop_config: block must be 64, 128 or 256");
-    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1), CMX_ERR_ARG,
+    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1) && (cfg[4] == 0 || cfg[4] == 1),
+                CMX_ERR_ARG,
                 "set_hop_config: bad tiling / prefetch flags");
-    op.hcfg[prec] = cmx::HopConfig{cfg[0], cfg[1], cfg[2], prec == cmx::P64 ? 0 : cfg[3]};
+    CMX_REQUIRE(cfg[5] >= 0 && cfg[5] <= 2, CMX_ERR_ARG, "set_hop_config: target must be 0, 1 or 2");
+    const cmx::HopConfig c{cfg[0], cfg[1], cfg[2], prec == cmx::P64 ? 0 : cfg[3], cfg[4]};
+    if (cfg[5] != 2) op.hcfg[prec][0] = c;
+    if (cfg[5] != 1) op.hcfg[prec][1] = c;
   });
 }
 

@@ -325,15 +325,20 @@ struct HopLaunch {
   int tile_zb;       // z planes per tile
   int tile_t8;       // t slices per XCD slab = Lt/8
   int prefetch;      // 1: double-buffered raw loads across the 8 directions
+  int fast;          // 1: 2-D grid kernel of hop_fast.h (FP16/FP32, 12-real links, real phases)
+  float inv_lxh;     // 1 / (Lx/2), for the fast kernel's plane decomposition
+  float sgn[4][2];   // real det(U) phase per direction: [bulk | last slice]
 };
 
 // Launch-shape choice for the hopping kernels of one lattice and precision
 struct HopConfig {
-  int block = 256, tiled = 0, zb = 1, prefetch = 0;
+  int block = 256, tiled = 0, zb = 1, prefetch = 0, fast = 1;
 };
 // Fills the shape fields of h from c, falling back to the linear order when the lattice does not
 // admit the tiling (returns the number of work-groups)
 int hop_apply_config(HopLaunch& h, const HopConfig& c);
+// Number of work-groups (= reduction partials) of a hopping launch
+int hop_num_blocks(const HopLaunch& h);
 void launch_hop(const HopLaunch& h, hipStream_t s);
 void launch_hop_p64(const HopLaunch& h, hipStream_t s);
 void launch_hop_p32(const HopLaunch& h, hipStream_t s);

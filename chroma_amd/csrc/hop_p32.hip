@@ -1,8 +1,12 @@
 // Hopping-term kernels instantiated for FP32 storage (kernel template in hop_kernel.h).
-#include "hop_kernel.h"
+#include "hop_fast.h"
 
 namespace cmx {
 void launch_hop_p32(const HopLaunch& h, hipStream_t s) {
+  if (h.fast) {
+    launch_hop_fast_t<P32>(h, s);
+    return;
+  }
   if (h.recon == 18) {
     if (h.prefetch) launch_hop_t<P32, 18, true>(h, s); else launch_hop_t<P32, 18, false>(h, s);
   } else {

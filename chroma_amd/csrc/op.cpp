@@ -21,10 +21,33 @@ void launch_hop(const HopLaunch& h, hipStream_t s) {
     default: throw Error(CMX_ERR_ARG, "hop: bad precision");
   }
 }
+int hop_num_blocks(const HopLaunch& h) {
+  if (h.fast) return ((h.g.Lxh * h.g.L[1] + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3];
+  return (h.g.Vh + h.block - 1) / h.block;
+}
+
 int hop_apply_config(HopLaunch& h, const HopConfig& c) {
   h.block = c.block;
   h.prefetch = c.prefetch;
   h.tiled = 0;
+  // the fast kernel needs 12-real links (or FP32/12), real boundary phases and plane indices < 2^16
+  bool real_phases = true;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int k = 0; k < 2; ++k) {
+      if (std::fabs(h.ph[mu][k][1]) > 1e-12 || std::fabs(std::fabs(h.ph[mu][k][0]) - 1.0) > 1e-6) real_phases = false;
+      h.sgn[mu][k] = (float)(h.ph[mu][k][0] < 0 ? -1.0 : 1.0);
+    }
+  h.fast = c.fast && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
+           h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536;
+  h.inv_lxh = 1.0f / (float)h.g.Lxh;
+  if (h.fast) {  // the plane-grid kernel tiles whole planes: any block size, Lt % 8 and Lz % zb only
+    if (c.tiled && h.g.L[3] % 8 == 0 && c.zb > 0 && h.g.L[2] % c.zb == 0) {
+      h.tiled = 1;
+      h.tile_zb = c.zb;
+      h.tile_t8 = h.g.L[3] / 8;
+    }
+    return hop_num_blocks(h);
+  }
   const int P = h.g.Lxh * h.g.L[1];
   if (c.tiled && P % c.block == 0 && h.g.L[3] % 8 == 0 && c.zb > 0 && h.g.L[2] % c.zb == 0) {
     h.tiled = 1;
@@ -32,7 +55,7 @@ int hop_apply_config(HopLaunch& h, const HopConfig& c) {
     h.tile_zb = c.zb;
     h.tile_t8 = h.g.L[3] / 8;
   }
-  return (h.g.Vh + h.block - 1) / h.block;
+  return hop_num_blocks(h);
 }
 
 namespace {
@@ -213,7 +236,7 @@ Vec& Op::tmp(int prec, int k) {
   return *v[k];
 }
 
-HopLaunch Op::hop_base(int prec) const {
+HopLaunch Op::hop_base(int prec, int which) const {
   HopLaunch h;
   std::memset(&h, 0, sizeof(h));
   h.g = g;
@@ -221,7 +244,7 @@ HopLaunch Op::hop_base(int prec) const {
   h.recon = recon[prec];
   for (int p_ = 0; p_ < 2; ++p_) h.gauge[p_] = prec == P64 ? gd->g64[p_].p : gauge[prec][p_].p;
   std::memcpy(h.ph, gd->ph, sizeof(h.ph));
-  hop_apply_config(h, hcfg[prec]);
+  hop_apply_config(h, hcfg[prec][which]);
   return h;
 }
 
@@ -238,7 +261,7 @@ static void set_x(HopLaunch& h, VRef r) {
 int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* skip, const int* run) {
   ensure_prec(prec);
   HopLaunch h = hop_base(prec);
-  const int nb = (h.g.Vh + h.block - 1) / h.block;
+  const int nb = hop_num_blocks(h);
   h.isign = isign;
   h.skip = skip;
   h.run = run;
@@ -293,7 +316,7 @@ int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* 
 
 void Op::dslash(int prec, VRef out, VRef in, int isign, int cb, double b) {
   ensure_prec(prec);
-  HopLaunch h = hop_base(prec);
+  HopLaunch h = hop_base(prec, 1);
   h.isign = isign;
   h.cb = cb;
   set_in(h, in);
@@ -379,64 +402,71 @@ void Op::autotune() {
   std::vector<int> precs = {P64, par.sloppy_prec};
   if (par.sloppy_prec == P16) precs.push_back(P32);
   const int ns = subset_sites();
-  Vec a(P64, ns), b(P64, ns);
-  for (int prec : precs) {
-    TuneKey key;
-    std::memset(&key, 0, sizeof(key));
-    for (int m = 0; m < 4; ++m) key.L[m] = g.L[m];
-    key.kind = par.kind;
-    key.prec = prec;
-    {
+  hipEvent_t e0, e1;
+  CMX_HIP(hipEventCreate(&e0));
+  CMX_HIP(hipEventCreate(&e1));
+  for (int prec : precs)
+    for (int which = 0; which < 2; ++which) {
+      TuneKey key;
+      std::memset(&key, 0, sizeof(key));
+      for (int m = 0; m < 4; ++m) key.L[m] = g.L[m];
+      key.kind = par.kind;
+      key.prec = prec + 3 * which;
+      {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+          hcfg[prec][which] = it->second;
+          continue;
+        }
+      }
+      ensure_prec(prec);
+      Vec in(prec, ns), out(prec, ns);
+      blas(B_ZERO, {&in});
+      std::vector<HopConfig> cands;
+      for (int block : {64, 128, 256}) {
+        if (prec != P64) {  // plane-grid kernel, linear or XCD-slab order
+          cands.push_back(HopConfig{block, 0, 1, 0, 1});
+          for (int zb : {1, 4, 8}) cands.push_back(HopConfig{block, 1, zb, 0, 1});
+        }
+        for (int pf : {0, 1}) {
+          if (pf && prec == P64) continue;
+          cands.push_back(HopConfig{block, 0, 1, pf, 0});
+          for (int zb : {2, 8}) cands.push_back(HopConfig{block, 1, zb, pf, 0});
+        }
+      }
+      float best = 1e30f;
+      HopConfig bestc = hcfg[prec][which];
+      const HopConfig saved = hcfg[prec][which];
+      for (const HopConfig& c : cands) {
+        HopLaunch probe = hop_base(prec, which);
+        hop_apply_config(probe, c);
+        if ((c.tiled && !probe.tiled) || (c.fast && !probe.fast)) continue;  // not admitted on this lattice
+        hcfg[prec][which] = c;
+        auto run = [&]() {
+          if (which == 0)
+            apply(prec, out.view(), in.view(), +1, nullptr);
+          else
+            dslash(prec, out.view(), in.view(), +1, 1);
+        };
+        run();
+        CMX_HIP(hipEventRecord(e0, stream));
+        for (int k = 0; k < 4; ++k) run();
+        CMX_HIP(hipEventRecord(e1, stream));
+        CMX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bestc = c;
+        }
+      }
+      hcfg[prec][which] = best < 1e29f ? bestc : saved;
       std::lock_guard<std::mutex> lk(tune_mutex());
-      auto it = tune_cache().find(key);
-      if (it != tune_cache().end()) {
-        hcfg[prec] = it->second;
-        continue;
-      }
+      tune_cache()[key] = hcfg[prec][which];
     }
-    ensure_prec(prec);
-    Vec in(prec, ns), out(prec, ns);
-    blas(B_ZERO, {&in});
-    std::vector<HopConfig> cands;
-    for (int block : {128, 256})
-      for (int pf : {0, 1}) {
-        if (pf && prec == P64) continue;
-        cands.push_back(HopConfig{block, 0, 1, pf});
-        for (int zb : {1, 2, 4, 8}) cands.push_back(HopConfig{block, 1, zb, pf});
-      }
-    hipEvent_t e0, e1;
-    CMX_HIP(hipEventCreate(&e0));
-    CMX_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    HopConfig bestc = hcfg[prec];
-    for (const HopConfig& c : cands) {
-      HopLaunch probe = hop_base(prec);
-      HopConfig saved = hcfg[prec];
-      hcfg[prec] = c;
-      hop_apply_config(probe, c);
-      if (c.tiled && !probe.tiled) {  // the lattice does not admit this tiling
-        hcfg[prec] = saved;
-        continue;
-      }
-      apply(prec, out.view(), in.view(), +1, nullptr);
-      CMX_HIP(hipEventRecord(e0, stream));
-      for (int k = 0; k < 4; ++k) apply(prec, out.view(), in.view(), +1, nullptr);
-      CMX_HIP(hipEventRecord(e1, stream));
-      CMX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bestc = c;
-      }
-      hcfg[prec] = saved;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    hcfg[prec] = bestc;
-    std::lock_guard<std::mutex> lk(tune_mutex());
-    tune_cache()[key] = bestc;
-  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   CMX_HIP(hipGetLastError());
 }
 

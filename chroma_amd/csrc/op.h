@@ -73,7 +73,9 @@ struct Op {
   long V = 0, Vh = 0;
   double fact = 0;  // Nd + Mass
   int recon[3] = {18, 12, 12};
-  HopConfig hcfg[3];      // hopping-kernel launch shape per precision (tuned by autotune())
+  // hopping-kernel launch shape per precision: [0] the EO operator's fused kernels, [1] the plain
+  // hopping term (tuned separately by autotune(): their best shapes differ)
+  HopConfig hcfg[3][2];
   // per precision: links (per parity) and the clover blocks used by the EO operator
   DevBuf gauge[3][2];
   DevBuf clov_odd[3];     // A_oo
@@ -96,7 +98,7 @@ struct Op {
   int subset_sites() const { return eo() ? (int)Vh : (int)V; }
   void ensure_prec(int prec);
   Vec& tmp(int prec, int k);
-  HopLaunch hop_base(int prec) const;
+  HopLaunch hop_base(int prec, int which = 0) const;
 
   // chi = M(isign) psi on the subset; partials (optional) receives |chi|^2 per work-group;
   // returns the number of partials written.
@@ -118,7 +120,6 @@ struct Op {
   // Times the candidate launch shapes of the EO operator at each precision in use and keeps the
   // fastest (QUDA's AutotuneDslash role).  Results are cached per (lattice, kind, precision).
   void autotune();
-  int hop_nblocks(int prec) const { return (int)((Vh + hcfg[prec].block - 1) / hcfg[prec].block); }
 };
 
 std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);

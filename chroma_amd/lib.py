@@ -237,13 +237,14 @@ class Operator:
         check(lib().cmx_layout_roundtrip(self.h, int(prec), _d(psi_cb), _d(out)))
         return out
 
-    def hop_config(self, prec):
-        cfg = (ctypes.c_int * 4)()
+    def hop_config(self, prec, target=1):
+        """Launch shape of the EO operator's kernels (target 1) or of the plain hopping term (2)."""
+        cfg = (ctypes.c_int * 6)(0, 0, 0, 0, 0, int(target))
         check(lib().cmx_op_get_hop_config(self.h, int(prec), cfg))
-        return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3])
+        return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
 
-    def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0):
-        cfg = (ctypes.c_int * 4)(int(block), int(tiled), int(zb), int(prefetch))
+    def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0, fast=0, target=0):
+        cfg = (ctypes.c_int * 6)(int(block), int(tiled), int(zb), int(prefetch), int(fast), int(target))
         check(lib().cmx_op_set_hop_config(self.h, int(prec), cfg))
 
     def autotune(self):

@@ -146,10 +146,12 @@ int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* 
 
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
- * double-buffered direction loads (0/1; FP32/FP16 only)}.  No counterpart in the reference beyond
- * QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
-int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[4]);
-int cmx_op_set_hop_config(cmx_op op, int prec, const int cfg[4]);
+ * double-buffered direction loads (0/1), plane-grid kernel (0/1; FP16 and FP32 with 12-real links and
+ * real boundary phases), target (0 both, 1 the EO operator's fused kernels, 2 the plain hopping
+ * term; for get, 2 reads the plain-hop shape and 0/1 the operator's)}.  No counterpart in the
+ * reference beyond QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
+int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[6]);
+int cmx_op_set_hop_config(cmx_op op, int prec, const int cfg[6]);
 int cmx_op_autotune(cmx_op op);
 
 /* ---- measurement -------------------------------------------------------------------------------

@@ -247,11 +247,14 @@ def test_twisted_golden_through_gpu():
 
 
 @pytest.mark.parametrize("prec", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
-@pytest.mark.parametrize("cfg", [(64, 0, 1, 0), (64, 1, 1, 0), (64, 1, 2, 1), (128, 1, 4, 1), (128, 0, 1, 1),
-                                 (128, 1, 8, 0), (256, 1, 2, 1)])
+@pytest.mark.parametrize("cfg", [(64, 0, 1, 0, 0), (64, 1, 1, 0, 0), (64, 1, 2, 1, 0), (128, 1, 4, 1, 0),
+                                 (128, 0, 1, 1, 0), (128, 1, 8, 0, 0), (256, 1, 2, 1, 0), (64, 0, 1, 0, 1),
+                                 (128, 0, 1, 0, 1), (256, 0, 1, 0, 1), (64, 1, 4, 0, 1), (256, 1, 2, 0, 1),
+                                 (128, 1, 8, 0, 1)])
 def test_hop_launch_shapes(prec, cfg):
-    """Every launch shape the tuner may pick (block size, tiled work-group order, prefetch) gives the
-    same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and 128."""
+    """Every launch shape the tuner may pick (block size, tiled work-group order, prefetch, plane-grid
+    kernel) gives the same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and
+    128, and its antiperiodic links admit the plane-grid kernel."""
     L = (16, 16, 8, 16)
     say("launch shape", prec, cfg)
     V = int(np.prod(L))
@@ -261,6 +264,8 @@ def test_hop_launch_shapes(prec, cfg):
     sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sp)
     op.set_hop_config(prec, *cfg)
+    if cfg[4] and prec != cmx.DOUBLE:
+        assert op.hop_config(prec)["fast"] == 1
     psi = random_fermion(V, 17)
     for isign in (+1, -1):
         ref = ref_op(psi, isign)[V // 2:]

@@ -48,6 +48,11 @@ enum Prog {
   PG_BICG_ALPHA,     // ctmp = <r0,v>; alpha = rho/ctmp; rho_prev = rho
   PG_BICG_OMEGA,     // |t|^2, <t,r>: omega = <t,r>/|t|^2
   PG_BICG_R,         // |r|^2: k++, convergence
+  // reliable BiCGStab (reliable_bicgstab.cc:14-290)
+  PG_RBICG_INIT,     // after b = chi - A psi: r_sq, norms, rho = r_sq, rho_prev = alpha = omega = 1, beta = 0
+  PG_RBICG_ALPHA,    // F_DONE = F_CONV; ctmp = <r0,v>; alpha = rho/ctmp; rho_prev = rho
+  PG_RBICG_R,        // |r|^2, rho = <r0,r>: updateX / updateR; without an update: k++, convergence, beta
+  PG_RBICG_UPD,      // after the FP64 update of r: r_sq, norms; k++, convergence, beta
 };
 
 // A fermion view for BLAS kernels
@@ -87,6 +92,9 @@ enum BlasOp {
   B_CAXPY,          // v0 += S[ia] v1 (complex)
   B_TNORM_TDOT,     // red0 = |v0|^2, red1,2 = <v0, v1>
   B_BICG_UPDATE,    // v0 += S[ia] v1 + S[ib] v2 ; v1 -= S[ia] v3 ; red0 = |v1|^2  (psi, r, p, t)
+  B_CAXPY2,         // v0 += S[ia] v1 ; v2 += S[ib] v3 (complex)    (r -= alpha v ; x += alpha p)
+  B_RBICG_UPDATE,   // v0 += S[ia] v1 ; v1 -= S[ia] v2 ; red0 = |v1|^2, red1,2 = <v3, v1>
+                    //                                     (x += omega s ; r = s - omega t ; |r|^2, <r0|r>)
   B_NOPS
 };
 

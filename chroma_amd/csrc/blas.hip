@@ -171,6 +171,34 @@ CMX_F(FBicgUpdate, 1, 15, 3) {
   red[0] += nrm2(v1);
 }
 };
+CMX_F(FCaxpy2, 0, 15, 5) {
+  const R ar = (R)S[ia], ai = (R)S[ia + 1], br = (R)S[ib], bi = (R)S[ib + 1];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const R xr = v1[2 * k], xi = v1[2 * k + 1], yr = v3[2 * k], yi = v3[2 * k + 1];
+    v0[2 * k] += ar * xr - ai * xi;
+    v0[2 * k + 1] += ar * xi + ai * xr;
+    v2[2 * k] += br * yr - bi * yi;
+    v2[2 * k + 1] += br * yi + bi * yr;
+  }
+}
+};
+CMX_F(FRbicgUpdate, 3, 15, 3) {
+  const R wr = (R)S[ia], wi = (R)S[ia + 1];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const R sr = v1[2 * k], si = v1[2 * k + 1], tr = v2[2 * k], ti = v2[2 * k + 1];
+    v0[2 * k] += wr * sr - wi * si;
+    v0[2 * k + 1] += wr * si + wi * sr;
+    v1[2 * k] = sr - (wr * tr - wi * ti);
+    v1[2 * k + 1] = si - (wr * ti + wi * tr);
+    const double rr = v1[2 * k], ri = v1[2 * k + 1], zr = v3[2 * k], zi = v3[2 * k + 1];
+    red[0] += rr * rr + ri * ri;
+    red[1] += zr * rr + zi * ri;
+    red[2] += zr * ri - zi * rr;
+  }
+}
+};
 #undef CMX_F
 
 template <class F, int P0, int P1, int P2, int P3>
@@ -229,6 +257,8 @@ void dispatch(const BlasLaunch& b, const int p[4], hipStream_t s) {
     CMX_T(P64, P64, P64, P16)
     CMX_T(P32, P32, P16, P16)
     CMX_T(P16, P32, P32, P32)
+    CMX_T(P16, P16, P32, P16)
+    CMX_T(P32, P16, P16, P16)
     default:
       break;
   }
@@ -432,6 +462,92 @@ __global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb
       }
       break;
     }
+    case PG_RBICG_INIT: {  // reliable_bicgstab.cc:61-104
+      S[S_RSQ] = red[0];
+      S[S_RNORM] = sqrt(red[0]);
+      S[S_R0NORM] = S[S_MAXRX] = S[S_MAXRR] = S[S_RNORM];
+      S[S_RHO] = red[0];  // <r0|r> = |r|^2 at the start (:109-113)
+      S[S_RHO + 1] = 0.0;
+      S[S_RHOP] = S[S_ALPHA] = S[S_OMEGA] = 1.0;
+      S[S_RHOP + 1] = S[S_ALPHA + 1] = S[S_OMEGA + 1] = 0.0;
+      S[S_BETAC] = S[S_BETAC + 1] = 0.0;  // first p = r (p = v = 0 on entry)
+      F[F_K] = 0;
+      F[F_NUPD] = 0;
+      break;
+    }
+    case PG_RBICG_ALPHA: {  // reliable_bicgstab.cc:135-150
+      F[F_DONE] = F[F_CONV];
+      if (F[F_DONE]) break;
+      const double cr = red[0], ci = red[1];
+      if (cr == 0.0 && ci == 0.0) {
+        F[F_ERR] = CMX_ERR_BREAKDOWN;
+        F[F_CONV] = F[F_DONE] = 1;
+        F[F_NCOUNT] = F[F_K];
+        break;
+      }
+      const double d = cr * cr + ci * ci;
+      S[S_ALPHA] = (S[S_RHO] * cr + S[S_RHO + 1] * ci) / d;
+      S[S_ALPHA + 1] = (S[S_RHO + 1] * cr - S[S_RHO] * ci) / d;
+      S[S_NALPHA] = -S[S_ALPHA];
+      S[S_NALPHA + 1] = -S[S_ALPHA + 1];
+      S[S_RHOP] = S[S_RHO];
+      S[S_RHOP + 1] = S[S_RHO + 1];
+      break;
+    }
+    case PG_RBICG_R:      // reliable_bicgstab.cc:199-214
+    case PG_RBICG_UPD: {  // reliable_bicgstab.cc:216-271
+      if (F[F_DONE]) break;
+      if (prog == PG_RBICG_R) {
+        S[S_RSQ] = red[0];
+        S[S_RHO] = red[1];
+        S[S_RHO + 1] = red[2];
+        const double rn = sqrt(red[0]);
+        S[S_RNORM] = rn;
+        if (rn > S[S_MAXRX]) S[S_MAXRX] = rn;
+        if (rn > S[S_MAXRR]) S[S_MAXRR] = rn;
+        const int updX = (rn < S[S_DELTA] * S[S_R0NORM] && S[S_R0NORM] <= S[S_MAXRX]);
+        const int updR = (rn < S[S_DELTA] * S[S_MAXRR] && S[S_R0NORM] <= S[S_MAXRR]) || updX;
+        F[F_UPDX] = updX;
+        F[F_UPDR] = updR;
+        if (updR) {  // the host launches the FP64 update, then PG_RBICG_UPD finishes the iteration
+          F[F_NUPD]++;
+          break;
+        }
+      } else {  // r = b - A x in FP64 (rho keeps the value from the sloppy r, as the reference does)
+        S[S_RSQ] = red[0];
+        S[S_RNORM] = sqrt(red[0]);
+        S[S_MAXRR] = S[S_RNORM];
+        if (F[F_UPDX]) {
+          S[S_R0NORM] = S[S_RNORM];
+          S[S_MAXRX] = S[S_RNORM];
+        }
+      }
+      const int k = F[F_K];
+      F[F_K] = k + 1;
+      if (S[S_RSQ] < S[S_RSDSQ]) {
+        F[F_CONV] = 1;
+        F[F_NCOUNT] = k;
+        break;
+      }
+      if (k + 1 >= F[F_MAXIT] || !isfinite(S[S_RSQ])) {
+        F[F_CONV] = 1;
+        F[F_NCOUNT] = k + 1;
+        F[F_ERR] = CMX_ERR_NOCONV;
+        break;
+      }
+      // beta = (rho / rho_prev) (alpha / omega) for the next p (:115-117)
+      auto cdiv = [](double ar, double ai, double br, double bi, double& cr, double& ci) {
+        const double d = br * br + bi * bi;
+        cr = (ar * br + ai * bi) / d;
+        ci = (ai * br - ar * bi) / d;
+      };
+      double q1r, q1i, q2r, q2i;
+      cdiv(S[S_RHO], S[S_RHO + 1], S[S_RHOP], S[S_RHOP + 1], q1r, q1i);
+      cdiv(S[S_ALPHA], S[S_ALPHA + 1], S[S_OMEGA], S[S_OMEGA + 1], q2r, q2i);
+      S[S_BETAC] = q1r * q2r - q1i * q2i;
+      S[S_BETAC + 1] = q1r * q2i + q1i * q2r;
+      break;
+    }
     default:
       break;
   }
@@ -445,7 +561,7 @@ int blas_nred(int op) {
       return 1;
     case B_CDOT:
       return 2;
-    case B_TNORM_TDOT:
+    case B_TNORM_TDOT: case B_RBICG_UPDATE:
       return 3;
     default:
       return 0;
@@ -473,6 +589,8 @@ void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s) 
     case B_CAXPY: dispatch<FCaxpy>(b, prec, s); break;
     case B_TNORM_TDOT: dispatch<FTnormTdot>(b, prec, s); break;
     case B_BICG_UPDATE: dispatch<FBicgUpdate>(b, prec, s); break;
+    case B_CAXPY2: dispatch<FCaxpy2>(b, prec, s); break;
+    case B_RBICG_UPDATE: dispatch<FRbicgUpdate>(b, prec, s); break;
     default: throw Error(CMX_ERR_ARG, "blas: unknown op");
   }
 }

@@ -271,7 +271,18 @@ int cmx_solve(cmx_op h, const cmx_solver_params* p, double* psi, const double* c
     cmx::Op& op = O(h);
     CMX_REQUIRE(p && psi && chi && res && nrhs >= 1, CMX_ERR_ARG, "solve: bad argument");
     const long nd = 24L * op.subset_sites();
-    for (int k = 0; k < nrhs; ++k) cmx::solve(op, *p, psi + k * nd, chi + k * nd, res[k]);
+    for (int k = 0; k < nrhs; ++k) cmx::solve(op, *p, psi + k * nd, chi + k * nd, false, res[k]);
+  });
+}
+
+int cmx_solve_mdagm(cmx_op h, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
+                    cmx_solver_result* res) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(p && psi && chi && res && nrhs >= 1, CMX_ERR_ARG, "solve_mdagm: bad argument");
+    const long nd = 24L * op.subset_sites();
+    for (int k = 0; k < nrhs; ++k) cmx::solve(op, *p, psi + k * nd, chi + k * nd, true, res[k]);
   });
 }
 

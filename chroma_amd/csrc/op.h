@@ -126,6 +126,7 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);
 Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
 
 // Solvers (solvers.cpp)
-void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, cmx_solver_result& res);
+// mdagm: solve M^dag M psi = chi instead of M psi = chi
+void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
 
 }  // namespace cmx

@@ -6,6 +6,11 @@
 //   BICGSTAB_INVERTER  InvBiCGStab_a (invbicgstab.cc:10-202)
 //   RELIABLE CG        LinOpSysSolverReliableCGClover (syssolver_linop_rel_cg_clover.h:104-152) +
 //                      RelInvCG_a (reliable_cg.cc:10-188): FP32/FP16 inner CG, FP64 reliable updates
+//   RELIABLE BICGSTAB  LinOpSysSolverReliableBiCGStabClover (syssolver_linop_rel_bicgstab_clover.h:104-140)
+//                      + RelInvBiCGStab_a (reliable_bicgstab.cc:14-290)
+// The MdagM system solvers (psi = (M^dag M)^-1 chi) are the same iterations without the M^dag chi
+// source preparation (CG, reliable CG), or two BiCGStab solves, M^dag Y = chi then M psi = Y
+// (syssolver_mdagm_bicgstab.h:64-100, syssolver_mdagm_rel_bicgstab_clover.h:104-176).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -97,12 +102,16 @@ double flops_per_apply(const Op& op) {
 }
 
 // ---- CG_INVERTER: CGNE on M^dag M in FP64 ---------------------------------------------------------
-void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+// mdagm: chi is already the source of M^dag M psi = chi (MdagMSysSolverCG, syssolver_mdagm_cg.h:70)
+void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
   const int n = op.subset_sites();
   Vec chi_tmp(P64, n), r(P64, n), pv(P64, n), mp(P64, n), mmp(P64, n);
   const int nb = blas_blocks(n);
   // chi_tmp = M^dag chi  (syssolver_linop_cg.h:65-66)
-  op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+  if (mdagm)
+    op.blas(B_COPY, {&chi_tmp, &chi});
+  else
+    op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
   const double chi_sq = norm2(op, chi_tmp);
   reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
   Timer tm(op.stream);
@@ -133,7 +142,7 @@ void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_
 }
 
 // ---- BICGSTAB_INVERTER ----------------------------------------------------------------------------
-void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, int isign, cmx_solver_result& res) {
   const int n = op.subset_sites();
   Vec r(P64, n), r0(P64, n), pv(P64, n), v(P64, n), t(P64, n);
   const int nb = blas_blocks(n);
@@ -143,7 +152,7 @@ void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   tm.start();
   int* F = op.state_F.as<int>();
   // r = r0 = chi - A psi ; p = v = 0   (invbicgstab.cc:37-62)
-  op.apply(P64, r0.view(), psi.view(), +1, nullptr);
+  op.apply(P64, r0.view(), psi.view(), isign, nullptr);
   op.blas(B_XMY_NORM, {&r, &chi, &r0});
   op.blas(B_COPY, {&r0, &r});
   op.blas(B_ZERO, {&pv});
@@ -152,11 +161,11 @@ void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     op.blas(B_CDOT, {&r0, &r}, 0, 0, F_DONE);  // rho = <r0|r>
     op.finalize(nb, 2, PG_BICG_RHO);           // beta = (rho/rho_prev)(alpha/omega)
     op.blas(B_BICG_P, {&pv, &r, &v}, S_OMEGA, S_BETAC, F_DONE);  // p = r + beta (p - omega v)
-    op.apply(P64, v.view(), pv.view(), +1, nullptr, F + F_DONE);  // v = A p
+    op.apply(P64, v.view(), pv.view(), isign, nullptr, F + F_DONE);  // v = A p
     op.blas(B_CDOT, {&r0, &v}, 0, 0, F_DONE);
     op.finalize(nb, 2, PG_BICG_ALPHA);                            // alpha = rho / <r0|v>
     op.blas(B_CAXPY, {&r, &v}, S_NALPHA, 0, F_DONE);              // s = r - alpha v
-    op.apply(P64, t.view(), r.view(), +1, nullptr, F + F_DONE);   // t = A s
+    op.apply(P64, t.view(), r.view(), isign, nullptr, F + F_DONE);   // t = A s
     op.blas(B_TNORM_TDOT, {&t, &r}, 0, 0, F_DONE);
     op.finalize(nb, 3, PG_BICG_OMEGA);                            // omega = <t|s>/|t|^2
     op.blas(B_BICG_UPDATE, {&psi, &r, &pv, &t}, S_OMEGA, S_ALPHA, F_DONE);  // psi += omega s + alpha p; r = s - omega t
@@ -171,15 +180,19 @@ void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
 }
 
 // ---- reliable-update mixed-precision CG on M^dag M ----------------------------------------------------
-void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
   const int n = op.subset_sites();
   const int ps = p.sloppy_prec;
   CMX_REQUIRE(ps == P32 || ps == P16, CMX_ERR_ARG, "reliable CG: sloppy precision must be single or half");
   const int nb = blas_blocks(n);
   Vec chi_tmp(P64, n), b(P64, n), r_dble(P64, n), x_dble(P64, n), tmp1(P64, n), tmp2(P64, n);
   Vec r(P32, n), x(P32, n), pv(ps, n), mp(ps, n), mmp(ps, n);
-  // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125)
-  op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+  // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125); the MdagM solver takes chi as is
+  // (syssolver_mdagm_rel_cg_clover.h:126-133)
+  if (mdagm)
+    op.blas(B_COPY, {&chi_tmp, &chi});
+  else
+    op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
   const double chi_sq = norm2(op, chi_tmp);
   reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
   Timer tm(op.stream);
@@ -227,9 +240,111 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, c
     throw Error(CMX_ERR_NOCONV, "reliable CG: no convergence in " + std::to_string(p.max_iter) + " iterations");
 }
 
+
+// ---- reliable-update mixed-precision BiCGStab on M (isign) ------------------------------------------
+// Vectors r, r0, p, v, t are stored in the sloppy precision (the operator's input and output), the
+// accumulated solution x in FP32 (FP64 when the sloppy precision is FP64); b, x_dble, r_dble, tmp are
+// FP64.  The reference's five BLAS passes per iteration are fused into four-vector passes:
+// r -= alpha v with x += alpha p, and x += omega s with r = s - omega t, |r|^2 and <r0|r>.
+void solve_relbicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, int isign,
+                       cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  const int ps = p.sloppy_prec;
+  CMX_REQUIRE(ps == P64 || ps == P32 || ps == P16, CMX_ERR_ARG, "reliable BiCGStab: bad sloppy precision");
+  const int px = ps == P64 ? P64 : P32;
+  const int nb = blas_blocks(n);
+  Vec b(P64, n), tmp(P64, n), r_dble(P64, n), x_dble(P64, n);
+  Vec r(ps, n), r0(ps, n), pv(ps, n), v(ps, n), t(ps, n), x(px, n);
+  const double chi_sq = norm2(op, chi);  // rsd_sq = Rsd^2 |chi|^2 (:65)
+  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+  Timer tm(op.stream);
+  tm.start();
+  int* F = op.state_F.as<int>();
+  // b = chi - A psi ; r = b ; r0 = b ; x = p = v = 0   (:57-79)
+  op.apply(P64, tmp.view(), psi.view(), isign, nullptr);
+  op.blas(B_XMY_NORM_COPY, {&b, &chi, &tmp, &r});
+  op.finalize(nb, 1, PG_RBICG_INIT);
+  op.blas(B_COPY, {&r0, &r});
+  op.blas(B_ZERO, {&x});
+  op.blas(B_ZERO, {&pv});
+  op.blas(B_ZERO, {&v});
+  long launched = 0;
+  while (true) {
+    op.blas(B_BICG_P, {&pv, &r, &v}, S_OMEGA, S_BETAC);                        // p = r + beta (p - omega v)
+    op.apply(ps, v.view(), pv.view(), isign, nullptr);                         // v = A p
+    op.blas(B_CDOT, {&r0, &v});
+    op.finalize(nb, 2, PG_RBICG_ALPHA);                                        // alpha = rho / <r0|v>
+    op.blas(B_CAXPY2, {&r, &v, &x, &pv}, S_NALPHA, S_ALPHA, F_DONE);           // s = r - alpha v; x += alpha p
+    op.apply(ps, t.view(), r.view(), isign, nullptr, F + F_DONE);              // t = A s
+    op.blas(B_TNORM_TDOT, {&t, &r}, 0, 0, F_DONE);
+    op.finalize(nb, 3, PG_BICG_OMEGA);                                         // omega = <t|s> / |t|^2
+    op.blas(B_RBICG_UPDATE, {&x, &r, &t, &r0}, S_OMEGA, 0, F_DONE);            // x += omega s; r = s - omega t
+    op.finalize(nb, 3, PG_RBICG_R);                                            // updateX / updateR
+    CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+    if (op.hF[F_DONE] || op.hF[F_ERR]) break;  // breakdown inside this iteration
+    if (op.hF[F_UPDR]) {                       // r = b - A x in FP64 (:216-256)
+      op.blas(B_COPY, {&x_dble, &x});
+      op.apply(P64, tmp.view(), x_dble.view(), isign, nullptr);
+      op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp, &r});
+      op.finalize(nb, 1, PG_RBICG_UPD);
+      if (op.hF[F_UPDX]) op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble});  // psi += x; x = 0; b = r
+      CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+      op.sync();
+    }
+    if (op.hF[F_CONV]) break;
+    if (++launched > (long)p.max_iter + 4) break;  // the device stops at MaxIter
+  }
+  op.blas(B_ADD, {&psi, &x});  // psi += x (:259-266)
+  res.n_count = op.hF[F_NCOUNT];
+  read_state(op);
+  res.solve_seconds = tm.stop();
+  res.n_reliable = op.hF[F_NUPD];
+  res.gflops = (res.n_count + 1) * (2 * flops_per_apply(op) + 80 * 12.0 * n) / std::max(res.solve_seconds, 1e-12) * 1e-9;
+  if (op.hF[F_ERR] == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "reliable BiCGStab: breakdown");
+  if (op.hF[F_ERR] == CMX_ERR_NOCONV)
+    throw Error(CMX_ERR_NOCONV, "reliable BiCGStab: no convergence in " + std::to_string(p.max_iter) + " iterations");
+}
+
+// one pass of the selected solver on A psi = chi (isign = +1), or on M^dag M psi = chi (mdagm)
+void solve_pass(Op& op, const cmx_solver_params& q, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& part) {
+  switch (q.solver) {
+    case CMX_SOLVER_CG: solve_cg(op, q, psi, chi, mdagm, part); return;
+    case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, q, psi, chi, mdagm, part); return;
+    case CMX_SOLVER_BICGSTAB:
+    case CMX_SOLVER_RELIABLE_BICGSTAB: break;
+    default: throw Error(CMX_ERR_ARG, "solve: unknown solver");
+  }
+  auto pass = [&](Vec& x, const Vec& rhs, int isign, cmx_solver_result& r) {
+    if (q.solver == CMX_SOLVER_BICGSTAB)
+      solve_bicgstab(op, q, x, rhs, isign, r);
+    else
+      solve_relbicgstab(op, q, x, rhs, isign, r);
+  };
+  if (!mdagm) {
+    pass(psi, chi, +1, part);
+    return;
+  }
+  // M^dag Y = chi from the guess Y = M psi, then M psi = Y from the guess psi
+  const int n = op.subset_sites();
+  Vec Y(P64, n);
+  op.apply(P64, Y.view(), psi.view(), +1, nullptr);
+  cmx_solver_result r1, r2;
+  std::memset(&r1, 0, sizeof(r1));
+  std::memset(&r2, 0, sizeof(r2));
+  pass(Y, chi, -1, r1);
+  pass(psi, Y, +1, r2);
+  part.n_count = r1.n_count + r2.n_count;
+  part.n_reliable = r1.n_reliable + r2.n_reliable;
+  part.solve_seconds = r1.solve_seconds + r2.solve_seconds;
+  part.gflops = part.solve_seconds > 0
+                    ? (r1.gflops * r1.solve_seconds + r2.gflops * r2.solve_seconds) / part.solve_seconds
+                    : 0.0;
+}
+
 }  // namespace
 
-void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_h, cmx_solver_result& res) {
+void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_h, bool mdagm, cmx_solver_result& res) {
   CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
   const auto t0 = std::chrono::steady_clock::now();
   const int n = op.subset_sites();
@@ -244,18 +359,18 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
   for (int restart = 0;; ++restart) {
     cmx_solver_result part;
     std::memset(&part, 0, sizeof(part));
-    switch (q.solver) {
-      case CMX_SOLVER_CG: solve_cg(op, q, psi, chi, part); break;
-      case CMX_SOLVER_BICGSTAB: solve_bicgstab(op, q, psi, chi, part); break;
-      case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, q, psi, chi, part); break;
-      default: throw Error(CMX_ERR_ARG, "solve: unknown solver");
-    }
+    solve_pass(op, q, psi, chi, mdagm, part);
     res.n_count += part.n_count;
     res.n_reliable += part.n_reliable;
     res.solve_seconds += part.solve_seconds;
     res.gflops += part.gflops * part.solve_seconds;
-    // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87)
+    // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87), or |chi - M^dag M psi|
+    // (syssolver_mdagm_cg.h:77-85)
     op.apply(P64, t.view(), psi.view(), +1, nullptr);
+    if (mdagm) {
+      op.apply(P64, rr.view(), t.view(), -1, nullptr);
+      op.blas(B_COPY, {&t, &rr});
+    }
     op.blas(B_XMY_NORM, {&rr, &chi, &t});
     op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
     op.blas(B_NORM2, {&chi});

@@ -16,7 +16,7 @@ CMX_ERR_ARG, CMX_ERR_HIP, CMX_ERR_STATE, CMX_ERR_NOCONV, CMX_ERR_BREAKDOWN, CMX_
 DOUBLE, SINGLE, HALF = 0, 1, 2
 RECONS_18, RECONS_12 = 18, 12
 UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
-CG, BICGSTAB, RELIABLE_CG = 0, 1, 2
+CG, BICGSTAB, RELIABLE_CG, RELIABLE_BICGSTAB = 0, 1, 2, 3
 PLUS, MINUS = 1, -1
 
 
@@ -77,6 +77,8 @@ def lib():
         "cmx_clover_pack_quda": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_solve": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
                                      ctypes.POINTER(SolverResult)]),
+        "cmx_solve_mdagm": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
+                                           ctypes.POINTER(SolverResult)]),
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
@@ -256,8 +258,10 @@ class Operator:
         return ms.value
 
     def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
-              check_every=0, verbose=False, true_residual=False):
-        """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays."""
+              check_every=0, verbose=False, true_residual=False, mdagm=False):
+        """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays.
+
+        mdagm=True solves M^dag M psi = chi (MdagMSystemSolver, lib/syssolver.h:72-84)."""
         chi = np.ascontiguousarray(chi, np.complex128)
         single = chi.ndim == 3
         chis = chi[None] if single else chi
@@ -266,7 +270,8 @@ class Operator:
         p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
                          int(bool(verbose)), int(bool(true_residual)))
         res = (SolverResult * nrhs)()
-        check(lib().cmx_solve(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))
+        fn = lib().cmx_solve_mdagm if mdagm else lib().cmx_solve
+        check(fn(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))
         out = [dict(n_count=r.n_count, resid=r.resid, rel_resid=r.rel_resid, n_reliable=r.n_reliable,
                     solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
                     n_restarts=r.n_restarts) for r in res]

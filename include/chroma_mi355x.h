@@ -50,6 +50,8 @@ extern "C" {
 #define CMX_SOLVER_CG 0           /* CG_INVERTER: LinOpSysSolverCG + InvCG2 (syssolver_linop_cg.h:57-96) */
 #define CMX_SOLVER_BICGSTAB 1     /* BICGSTAB_INVERTER: InvBiCGStab (invbicgstab.cc:10-202)             */
 #define CMX_SOLVER_RELIABLE_CG 2  /* RELIABLE_CG_MP_CLOVER_INVERTER: RelInvCG (reliable_cg.cc:10-188)    */
+#define CMX_SOLVER_RELIABLE_BICGSTAB 3 /* RELIABLE_BICGSTAB_MP_CLOVER_INVERTER: RelInvBiCGStab
+                                          (reliable_bicgstab.cc:14-290)                            */
 
 #define CMX_PLUS 1
 #define CMX_MINUS (-1)
@@ -121,7 +123,8 @@ typedef struct {
   int max_iter;      /* MaxCG / MaxIter                                                            */
   double rsd_target; /* RsdCG / RsdTarget (relative)                                               */
   double delta;      /* reliable-update Delta (syssolver_rel_bicgstab_clover_params.cc:11-19)      */
-  int sloppy_prec;   /* inner precision for the reliable solvers: CMX_SINGLE or CMX_HALF           */
+  int sloppy_prec;   /* inner precision for the reliable solvers: CMX_SINGLE or CMX_HALF (reliable
+                        BiCGStab also CMX_DOUBLE, its pure-double instance reliable_bicgstab.cc:309) */
   int check_every;   /* iterations between host convergence checks (0 = library default)          */
   int verbose;
   int true_residual; /* 0: stop as the reference solver does (CG family: on the normal-equation
@@ -143,6 +146,15 @@ typedef struct {
 
 int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
               cmx_solver_result* res);
+
+/* MdagMSystemSolver::operator() (lib/syssolver.h:72-84): solves M^dag M psi = chi on the subset, as
+ * MdagMSysSolverCG (syssolver_mdagm_cg.h:56-90), MdagMSysSolverBiCGStab (syssolver_mdagm_bicgstab.h:
+ * 64-100, two solves: M^dag Y = chi from Y = M psi, then M psi = Y), MdagMSysSolverReliableCGClover
+ * (syssolver_mdagm_rel_cg_clover.h:104-150) and MdagMSysSolverReliableBiCGStabClover
+ * (syssolver_mdagm_rel_bicgstab_clover.h:104-176).  resid = |chi - M^dag M psi| in FP64; n_count sums
+ * both solves of the two-step forms. */
+int cmx_solve_mdagm(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
+                    cmx_solver_result* res);
 
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,

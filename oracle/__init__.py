@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
-CG, BICGSTAB, RELIABLE_CG = 0, 1, 2
+CG, BICGSTAB, RELIABLE_CG, RELIABLE_BICGSTAB = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -50,6 +50,8 @@ def lib():
         L.orc_op_logdet_even.argtypes = [ctypes.c_void_p, _dp, _lp]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_double, _ip, _dp, _dp, ctypes.c_int, _ip]
+        L.orc_solve_mdagm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double, _ip, _dp]
         L.orc_qprop.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_double, _ip, _dp]
         L.orc_quarkprop4_point.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double,
@@ -181,6 +183,15 @@ class Op:
         _check(lib().orc_solve(self.h, int(solver), _d(psi), _d(chi), float(rsd), int(maxit), float(delta),
                                ctypes.byref(nc), ctypes.byref(rs), _d(tr), int(trace_cap), ctypes.byref(tl)))
         return psi, nc.value, rs.value, tr[: min(tl.value, trace_cap)]
+
+    def solve_mdagm(self, chi, psi=None, solver=CG, rsd=1e-8, maxit=1000, delta=0.1):
+        """psi = (M^dag M)^-1 chi with the MdagM system solvers; returns (psi, n_count, |chi - M^dag M psi|)."""
+        psi = np.zeros_like(chi) if psi is None else psi.copy()
+        nc = ctypes.c_int()
+        rs = ctypes.c_double()
+        _check(lib().orc_solve_mdagm(self.h, int(solver), _d(psi), _d(chi), float(rsd), int(maxit), float(delta),
+                                     ctypes.byref(nc), ctypes.byref(rs)))
+        return psi, nc.value, rs.value
 
     def qprop(self, chi, solver=CG, rsd=1e-8, maxit=1000, delta=0.1):
         psi = np.zeros_like(chi)

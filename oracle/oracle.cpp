@@ -730,6 +730,100 @@ Result syssolver_relcg(const LinOp<double>& A, const LinOp<float>& AF, cd* psi, 
   return res;
 }
 
+// RelInvBiCGStab_a, reliable_bicgstab.cc:14-290 (the single/double instance, :323-335): BiCGStab
+// on A (isign) with FP32 (AF) vectors r, r0, x, p, v, t and FP64 reliable updates of r and grouped
+// x.  Non-convergence and breakdown return n_count = -1 (Chroma calls QDP_abort, :140-143, :275-278).
+Result relbicgstab(const LinOp<double>& A, const LinOp<float>& AF, const cd* chi, cd* psi, double Rsd,
+                   double Delta, int MaxIt, int isign, std::vector<double>* trace, int* n_updates) {
+  using cf = std::complex<float>;
+  const Range s = A.subset();
+  const long n = A.g->V * 12;
+  Result ret{0, 0};
+  std::vector<cd> b(n), tmp(n), r_dble(n), x_dble(n);
+  std::vector<cf> r(n), r0(n), x(n), p(n), v(n), t(n);
+  const double rsd_sq = Rsd * Rsd * norm2<double>(chi, s);
+  A.apply(tmp.data(), psi, isign);
+  for (long q = s.lo * 12; q < s.hi * 12; ++q) b[q] = chi[q] - tmp[q];  // xymz_normx (:61-74)
+  double r_sq = norm2<double>(b.data(), s);
+  for (long q = s.lo * 12; q < s.hi * 12; ++q) r[q] = r0[q] = cf(b[q]);
+  double rNorm = std::sqrt(r_sq), r0Norm = rNorm, maxrx = rNorm, maxrr = rNorm;
+  cd rho = 1.0, rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  int k, nupd = 0;
+  bool convP = false;
+  for (k = 0; k < MaxIt && !convP; k++) {
+    if (k == 0) {  // (:109-113) <r0|r> = |r|^2 at the start
+      rho = r_sq;
+      copy<float>(p.data(), r.data(), s);
+    } else {       // p = r + beta (p - omega v), beta = (rho/rho_prev)(alpha/omega)  (:115-132)
+      const cd beta = (rho / rho_prev) * (alpha / omega);
+      const cf beta_r(beta), omega_r(omega);
+      for (long q = s.lo * 12; q < s.hi * 12; ++q) p[q] = r[q] + beta_r * (p[q] - omega_r * v[q]);
+    }
+    AF.apply(v.data(), p.data(), isign);
+    const cd ctmp = inner<float>(r0.data(), v.data(), s);
+    if (ctmp == cd(0)) return Result{-1, 0};
+    alpha = rho / ctmp;
+    rho_prev = rho;
+    const cf alpha_r(alpha);
+    for (long q = s.lo * 12; q < s.hi * 12; ++q) r[q] -= alpha_r * v[q];  // s = r - alpha v
+    AF.apply(t.data(), r.data(), isign);
+    const double t_norm = norm2<float>(t.data(), s);
+    if (t_norm == 0) return Result{-1, 0};
+    omega = inner<float>(t.data(), r.data(), s) / t_norm;
+    const cf omega_r(omega);
+    for (long q = s.lo * 12; q < s.hi * 12; ++q) x[q] = (x[q] + omega_r * r[q]) + alpha_r * p[q];  // xpaypbz
+    for (long q = s.lo * 12; q < s.hi * 12; ++q) r[q] -= omega_r * t[q];  // xmay_normx_cdotzx (:199)
+    r_sq = norm2<float>(r.data(), s);
+    rho = inner<float>(r0.data(), r.data(), s);
+    if (trace) trace->push_back(r_sq);
+    rNorm = std::sqrt(r_sq);
+    if (rNorm > maxrx) maxrx = rNorm;
+    if (rNorm > maxrr) maxrr = rNorm;
+    const bool updateX = (rNorm < Delta * r0Norm && r0Norm <= maxrx);  // (:213-214)
+    const bool updateR = (rNorm < Delta * maxrr && r0Norm <= maxrr) || updateX;
+    if (updateR) {  // (:216-256); rho is not recomputed from the new r
+      nupd++;
+      for (long q = s.lo * 12; q < s.hi * 12; ++q) x_dble[q] = cd(x[q]);
+      A.apply(tmp.data(), x_dble.data(), isign);
+      for (long q = s.lo * 12; q < s.hi * 12; ++q) r_dble[q] = b[q] - tmp[q];
+      r_sq = norm2<double>(r_dble.data(), s);
+      for (long q = s.lo * 12; q < s.hi * 12; ++q) r[q] = cf(r_dble[q]);
+      rNorm = std::sqrt(r_sq);
+      maxrr = rNorm;
+      if (updateX) {
+        for (long q = s.lo * 12; q < s.hi * 12; ++q) psi[q] += x_dble[q];
+        zero<float>(x.data(), s);
+        copy<double>(b.data(), r_dble.data(), s);
+        r0Norm = rNorm;
+        maxrx = rNorm;
+      }
+    }
+    if (r_sq < rsd_sq) {  // (:259-271)
+      convP = true;
+      for (long q = s.lo * 12; q < s.hi * 12; ++q) psi[q] += cd(x[q]);
+      ret.resid = rNorm;
+      ret.n_count = k;
+    }
+  }
+  if (n_updates) *n_updates = nupd;
+  if (!convP) ret.n_count = -1;
+  return ret;
+}
+
+// |chi - A psi| (isign = +1) or |chi - A^dag A psi| (mdagm) over the subset, in FP64
+double true_resid(const LinOp<double>& A, const cd* psi, const cd* chi, bool mdagm) {
+  const Range s = A.subset();
+  const long n = A.g->V * 12;
+  std::vector<cd> t1(n), t2(n);
+  A.apply(t1.data(), psi, +1);
+  if (mdagm) {
+    A.apply(t2.data(), t1.data(), -1);
+    t1.swap(t2);
+  }
+  for (long q = s.lo * 12; q < s.hi * 12; ++q) t1[q] = chi[q] - t1[q];
+  return std::sqrt(norm2<double>(t1.data(), s));
+}
+
 }  // namespace orc
 
 // ==================================================================================================
@@ -889,7 +983,8 @@ int orc_op_logdet_even(void* h, double* out, long* nneg) {
   });
 }
 
-// solver: 0 CG_INVERTER (LinOpSysSolverCG), 1 BICGSTAB_INVERTER, 2 RELIABLE_CG (FP32 inner, FP64 updates)
+// solver: 0 CG_INVERTER (LinOpSysSolverCG), 1 BICGSTAB_INVERTER, 2 RELIABLE_CG (FP32 inner, FP64 updates),
+//         3 RELIABLE_BICGSTAB (FP32 inner, FP64 updates)
 // psi: in/out initial guess; trace (optional, may be NULL): |r|^2 per iteration, up to trace_cap entries
 int orc_solve(void* h, int solver, double* psi, const double* chi, double rsd, int maxit, double delta, int* n_count,
               double* resid, double* trace, int trace_cap, int* trace_len) {
@@ -913,6 +1008,10 @@ int orc_solve(void* h, int solver, double* psi, const double* chi, double rsd, i
     } else if (solver == 2) {
       int nupd = 0;
       r = syssolver_relcg(*o->op, *o->opf, p, c, rsd, delta, maxit, &tr, &nupd);
+    } else if (solver == 3) {
+      // LinOpSysSolverReliableBiCGStabClover (syssolver_linop_rel_bicgstab_clover.h:104-140)
+      r = relbicgstab(*o->op, *o->opf, c, p, rsd, delta, maxit, +1, &tr, nullptr);
+      if (r.n_count >= 0) r.resid = true_resid(*o->op, p, c, false);
     } else {
       throw std::runtime_error("unknown solver");
     }
@@ -924,8 +1023,42 @@ int orc_solve(void* h, int solver, double* psi, const double* chi, double rsd, i
   });
 }
 
+// MdagM system solvers: psi = (A^dag A)^-1 chi.  resid = |chi - A^dag A psi|.
+//   0 MdagMSysSolverCG (syssolver_mdagm_cg.h:56-90): InvCG2 on chi directly
+//   1 MdagMSysSolverBiCGStab (syssolver_mdagm_bicgstab.h:64-100): Y = A psi; BiCGStab(MINUS) on chi
+//     for Y; BiCGStab(PLUS) on Y for psi
+//   2 MdagMSysSolverReliableCGClover (syssolver_mdagm_rel_cg_clover.h:104-150): RelInvCG on chi
+//   3 MdagMSysSolverReliableBiCGStabClover (syssolver_mdagm_rel_bicgstab_clover.h:104-176): the
+//     two-step form of 1 with the reliable BiCGStab
+int orc_solve_mdagm(void* h, int solver, double* psi, const double* chi, double rsd, int maxit, double delta,
+                    int* n_count, double* resid) {
+  return guard([&] {
+    auto* o = static_cast<OpHandle*>(h);
+    auto* p = reinterpret_cast<cd*>(psi);
+    auto* c = reinterpret_cast<const cd*>(chi);
+    Result r{0, 0};
+    if (solver == 0) {
+      r = invcg2<double>(*o->op, c, p, rsd, maxit, nullptr);
+    } else if (solver == 2) {
+      r = relcg(*o->op, *o->opf, c, p, rsd, delta, maxit, nullptr, nullptr);
+    } else if (solver == 1 || solver == 3) {
+      std::vector<cd> Y(o->g->V * 12);
+      o->op->apply(Y.data(), p, +1);
+      Result r1 = solver == 1 ? invbicgstab<double>(*o->op, c, Y.data(), rsd, maxit, -1, nullptr)
+                              : relbicgstab(*o->op, *o->opf, c, Y.data(), rsd, delta, maxit, -1, nullptr, nullptr);
+      Result r2 = solver == 1 ? invbicgstab<double>(*o->op, Y.data(), p, rsd, maxit, +1, nullptr)
+                              : relbicgstab(*o->op, *o->opf, Y.data(), p, rsd, delta, maxit, +1, nullptr, nullptr);
+      r.n_count = (r1.n_count < 0 || r2.n_count < 0) ? -1 : r1.n_count + r2.n_count;
+    } else {
+      throw std::runtime_error("unknown solver");
+    }
+    *n_count = r.n_count;
+    *resid = true_resid(*o->op, p, c, true);
+  });
+}
+
 // PrecFermActQprop::operator(), eoprec_fermact_qprop.cc:41-80, for an even-odd operator with the CG
-// inverter (solver 0) or reliable CG (solver 2).  psi, chi are full-lattice.  resid = |M_unprec psi - chi|.
+// inverter (solver 0), reliable CG (solver 2) or reliable BiCGStab (solver 3).  psi, chi are full-lattice.  resid = |M_unprec psi - chi|.
 int orc_qprop(void* h, int solver, double* psi, const double* chi, double rsd, int maxit, double delta, int* n_count,
               double* resid) {
   return guard([&] {
@@ -945,6 +1078,8 @@ int orc_qprop(void* h, int solver, double* psi, const double* chi, double rsd, i
       r = syssolver_cg<double>(*eo, p, chi_tmp.data(), rsd, maxit, nullptr);
     else if (solver == 2)
       r = syssolver_relcg(*eo, *o->opf, p, chi_tmp.data(), rsd, delta, maxit, nullptr, nullptr);
+    else if (solver == 3)
+      r = relbicgstab(*eo, *o->opf, chi_tmp.data(), p, rsd, delta, maxit, +1, nullptr, nullptr);
     else
       throw std::runtime_error("qprop: unsupported solver");
     eo->evenOddLinOp(t1.data(), p, +1);

@@ -227,6 +227,62 @@ def test_reliable_cg_mixed(sloppy):
     assert rel(psi, psi_ref[V // 2:]) < 1e-8
 
 
+@pytest.mark.parametrize("sloppy", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
+def test_reliable_bicgstab_mixed(sloppy):
+    """Reliable BiCGStab (FP64 / FP32 / FP16-storage inner iteration, FP64 reliable updates) against the
+    oracle's single/double RelInvBiCGStab: iteration count, solution, and a 1e-10 FP64 true residual."""
+    L = (8, 8, 8, 16)
+    say("reliable bicgstab", sloppy)
+    V = int(np.prod(L))
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=93, nstout=6, rho=0.1), (1, 1, 1, -1))
+    mass = cmx.kappa_to_mass(0.1350)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF if sloppy == cmx.HALF else cmx.SINGLE)
+    chi_full = cmx.z2_source((V, 4, 3), 4)
+    chi_full[: V // 2] = 0
+    psi_cg, _, _, _ = ref_op.solve(chi_full, solver=oracle.CG, rsd=1e-12, maxit=10000)
+    psi_ref, n_ref, r_ref, _ = ref_op.solve(chi_full, solver=oracle.RELIABLE_BICGSTAB, rsd=1e-10, maxit=5000,
+                                            delta=0.1)
+    psi0, res0 = op.solve(chi_full[V // 2:], solver=cmx.RELIABLE_BICGSTAB, rsd=1e-10, maxit=5000, delta=0.1,
+                          sloppy_prec=sloppy)
+    say("  n_count", res0["n_count"], "oracle", n_ref, "rel resid", res0["rel_resid"], "updates", res0["n_reliable"])
+    assert res0["n_reliable"] > 0
+    if sloppy == cmx.SINGLE:  # the same arithmetic as the oracle up to the order of the sums
+        assert abs(res0["n_count"] - n_ref) <= max(3, n_ref // 10), (res0["n_count"], n_ref)
+    assert res0["rel_resid"] < 1e-9
+    psi, res = op.solve(chi_full[V // 2:], solver=cmx.RELIABLE_BICGSTAB, rsd=1e-10, maxit=5000, delta=0.1,
+                        sloppy_prec=sloppy, true_residual=True)
+    assert res["rel_resid"] <= 1e-10
+    assert rel(psi, psi_cg[V // 2:]) < 1e-8
+
+
+@pytest.mark.parametrize("solver", [cmx.CG, cmx.BICGSTAB, cmx.RELIABLE_CG, cmx.RELIABLE_BICGSTAB])
+def test_mdagm_solvers(solver):
+    """MdagM system solvers (M^dag M psi = chi) against the oracle's, from a non-zero initial guess."""
+    L = (8, 8, 8, 16)
+    say("mdagm", solver)
+    V = int(np.prod(L))
+    U = gauge_field(L, 95)
+    mass = cmx.kappa_to_mass(0.125)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0)
+    chi_full = random_fermion(V, 14)
+    chi_full[: V // 2] = 0
+    guess = 0.05 * random_fermion(V, 15)
+    guess[: V // 2] = 0
+    osolver = {cmx.CG: oracle.CG, cmx.BICGSTAB: oracle.BICGSTAB, cmx.RELIABLE_CG: oracle.RELIABLE_CG,
+               cmx.RELIABLE_BICGSTAB: oracle.RELIABLE_BICGSTAB}[solver]
+    psi_ref, n_ref, r_ref = ref_op.solve_mdagm(chi_full, psi=guess, solver=osolver, rsd=1e-10, maxit=5000)
+    psi, res = op.solve(chi_full[V // 2:], psi=guess[V // 2:], solver=solver, rsd=1e-10, maxit=5000,
+                        sloppy_prec=cmx.SINGLE, mdagm=True)
+    say("  n_count", res["n_count"], "oracle", n_ref, "resid", res["resid"], "oracle", r_ref)
+    if solver == cmx.CG:
+        assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)
+    assert rel(psi, psi_ref[V // 2:]) < 1e-7
+    # the two-step forms bound |chi - M^dag M psi| by about (1 + cond M) x RsdTarget
+    assert res["rel_resid"] < (1e-9 if solver in (cmx.CG, cmx.RELIABLE_CG) else 1e-8)
+
+
 def test_twisted_golden_through_gpu():
     """prec_wilson-twisted (tests/chroma/hadron/propagator/prec_wilson-twisted.{ini,out}.xml) on the GPU."""
     from chroma_amd import qprop as qp

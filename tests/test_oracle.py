@@ -326,10 +326,47 @@ def test_bicgstab_and_reliable_cg():
     chi = random_fermion(V, 21)
     chi[: V // 2] = 0
     ref = op.solve(chi, solver=oracle.CG, rsd=1e-12, maxit=5000)[0]
-    for solver in (oracle.BICGSTAB, oracle.RELIABLE_CG):
+    for solver in (oracle.BICGSTAB, oracle.RELIABLE_CG, oracle.RELIABLE_BICGSTAB):
         psi, nc, resid, _ = op.solve(chi, solver=solver, rsd=1e-10, maxit=5000, delta=0.1)
         assert nc > 0
         assert rel(psi[V // 2:], ref[V // 2:]) < 1e-8
+
+
+def test_reliable_bicgstab_against_dense_solve():
+    # FP32 inner iteration with FP64 reliable updates reaches the FP64 solution of M psi = chi
+    L = (2, 2, 2, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 30)
+    mass = 0.3
+    op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    M = (4 + mass) * np.eye(12 * V) - 0.5 * dense_dslash(L, U, +1)
+    chi = random_fermion(V, 31)
+    exact = np.linalg.solve(M, chi.reshape(-1)).reshape(V, 4, 3)
+    for delta in (0.1, 0.01):
+        psi, nc, resid, trace = op.solve(chi, solver=oracle.RELIABLE_BICGSTAB, rsd=1e-11, maxit=2000, delta=delta)
+        assert nc > 0
+        # an FP32 solution vector alone is good to ~1e-7; 1e-9 needs the FP64 grouped x updates
+        assert rel(psi, exact) < 1e-9
+        assert resid <= 1e-10 * np.linalg.norm(chi)
+        assert trace[-1] < 1e-22 * np.vdot(chi, chi).real * 1e2
+
+
+@pytest.mark.parametrize("solver", ["CG", "BICGSTAB", "RELIABLE_CG", "RELIABLE_BICGSTAB"])
+def test_mdagm_solvers_against_dense_normal_equations(solver):
+    L = (2, 2, 2, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 32)
+    mass = 0.3
+    op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    M = (4 + mass) * np.eye(12 * V) - 0.5 * dense_dslash(L, U, +1)
+    chi = random_fermion(V, 33)
+    exact = np.linalg.solve(M.conj().T @ M, chi.reshape(-1)).reshape(V, 4, 3)
+    guess = 0.01 * random_fermion(V, 34)
+    psi, nc, resid = op.solve_mdagm(chi, psi=guess, solver=getattr(oracle, solver), rsd=1e-11, maxit=3000)
+    assert nc > 0
+    assert rel(psi, exact) < 1e-8
+    r = chi.reshape(-1) - M.conj().T @ (M @ psi.reshape(-1))
+    assert abs(resid - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(chi)
 
 
 # ---- planted mistakes: each pinning check above fails for a plausible error -----------------------

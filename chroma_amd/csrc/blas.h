@@ -18,7 +18,11 @@ enum Slot {
   // BiCGStab (invbicgstab.cc), complex pairs
   S_RHO = 20, S_RHOP = 22, S_ALPHA = 24, S_OMEGA = 26, S_BETAC = 28, S_CTMP = 30, S_TNORM = 32,
   S_NALPHA = 34, S_NOMEGA = 36, S_TR = 38,
-  S_NSLOTS = 64
+  // Krylov-basis solvers (FGMRES-DR): coefficient columns, norms and inverse norms
+  S_KRY = 64,
+  // MR preconditioner (invmr.cc:120-160): c = <Mr,r> (2), d, a (2), -a (2), |v|^2, cp, omega, RsdMR
+  S_MR = 480,
+  S_NSLOTS = 512
 };
 // Flags (ints)
 enum Flag {
@@ -31,6 +35,7 @@ enum Flag {
   F_UPDX,       // reliable update of x (group-wise) this iteration
   F_NUPD,       // number of reliable updates
   F_ERR,        // 0 ok, CMX_ERR_NOCONV, CMX_ERR_BREAKDOWN
+  F_MRDONE,     // MR preconditioner converged: its remaining iterations return immediately
   F_NFLAGS = 16
 };
 
@@ -53,6 +58,10 @@ enum Prog {
   PG_RBICG_ALPHA,    // F_DONE = F_CONV; ctmp = <r0,v>; alpha = rho/ctmp; rho_prev = rho
   PG_RBICG_R,        // |r|^2, rho = <r0,r>: updateX / updateR; without an update: k++, convergence, beta
   PG_RBICG_UPD,      // after the FP64 update of r: r_sq, norms; k++, convergence, beta
+  // MR (invmr.cc:100-160), run with the F_MRDONE skip flag
+  PG_MR_INIT,        // cp = |r|^2; rsd_sq = RsdMR^2 |v|^2; F_MRDONE = cp <= rsd_sq
+  PG_MR_A,           // d = |Mr|^2, c = <Mr, r>: a = omega c / d
+  PG_MR_R,           // cp = |r|^2; F_MRDONE = cp <= rsd_sq
 };
 
 // A fermion view for BLAS kernels
@@ -95,11 +104,36 @@ enum BlasOp {
   B_CAXPY2,         // v0 += S[ia] v1 ; v2 += S[ib] v3 (complex)    (r -= alpha v ; x += alpha p)
   B_RBICG_UPDATE,   // v0 += S[ia] v1 ; v1 -= S[ia] v2 ; red0 = |v1|^2, red1,2 = <v3, v1>
                     //                                     (x += omega s ; r = s - omega t ; |r|^2, <r0|r>)
+  B_SCALE,          // v0 = S[ia] v1 (real)
+  B_AXPY_CNORM,     // v0 += S[ia] v1 (complex) ; red0 = |v0|^2
   B_NOPS
 };
 
 int blas_nred(int op);
 int blas_blocks(int n);
+
+// Multi-vector kernels of the Krylov-basis solvers (krylov.hip).  A basis is up to KRY_MAXV views of
+// one precision; coefficients are complex pairs in the scalar array S.
+constexpr int KRY_MAXV = 48;
+constexpr int KRY_BLOCKS = 1024;  // work-groups of the reductions (grid-stride), 256 threads each
+struct MultiLaunch {
+  int n;                 // sites
+  int m;                 // basis vectors in use
+  VRef w;                // the single vector
+  VRef v[KRY_MAXV];      // the basis
+  double* S;
+  int coef;              // S slot of the first complex coefficient
+  double sgn;            // coefficient multiplier (+1 / -1)
+  int overwrite;         // maxpy: w = sum (instead of w += sum)
+  double* partials;      // KRY_BLOCKS x nred
+};
+// partials[b][2i, 2i+1] = <v_i, w> over work-group b's sites
+void launch_mdot(const MultiLaunch& L, int pv, int pw, hipStream_t s);
+// w (+)= sgn * sum_i S[coef+2i] v_i ; optionally partials[b][0] = |w|^2
+void launch_maxpy(const MultiLaunch& L, int pv, int pw, bool norm, hipStream_t s);
+// out[r] = sum_b partials[b][r] for r < nred (one work-group per column, fixed order);
+// mode 1 additionally writes out[nred] = 1 / sqrt(out[0])
+void launch_colsum(const double* partials, int nb, int nred, double* out, int mode, hipStream_t s);
 void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s);
 void launch_finalize(const double* partials, int nblocks, int nred, double* S, int* F, int prog, int slot,
                      hipStream_t s);

@@ -199,6 +199,23 @@ CMX_F(FRbicgUpdate, 3, 15, 3) {
   }
 }
 };
+CMX_F(FScale, 0, 2, 1) {
+  const R a = (R)S[ia];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) v0[k] = a * v1[k];
+}
+};
+CMX_F(FAxpyCNorm, 1, 3, 1) {
+  const R ar = (R)S[ia], ai = (R)S[ia + 1];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const R xr = v1[2 * k], xi = v1[2 * k + 1];
+    v0[2 * k] += ar * xr - ai * xi;
+    v0[2 * k + 1] += ar * xi + ai * xr;
+  }
+  red[0] += nrm2(v0);
+}
+};
 #undef CMX_F
 
 template <class F, int P0, int P1, int P2, int P3>
@@ -548,6 +565,32 @@ __global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb
       S[S_BETAC + 1] = q1r * q2i + q1i * q2r;
       break;
     }
+    case PG_MR_INIT: {
+      S[S_MR + 8] = red[0];
+      const double rsd = S[S_MR + 10];
+      S[S_MR + 11] = rsd * rsd * S[S_MR + 7];
+      F[F_MRDONE] = red[0] <= S[S_MR + 11] ? 1 : 0;
+      break;
+    }
+    case PG_MR_A: {
+      if (F[F_MRDONE]) break;
+      const double d = red[0], cr = red[1], ci = red[2], w = S[S_MR + 9];
+      if (!(d > 0.0)) {  // M r = 0: nothing left to reduce
+        F[F_MRDONE] = 1;
+        break;
+      }
+      S[S_MR + 3] = w * cr / d;
+      S[S_MR + 4] = w * ci / d;
+      S[S_MR + 5] = -S[S_MR + 3];
+      S[S_MR + 6] = -S[S_MR + 4];
+      break;
+    }
+    case PG_MR_R: {
+      if (F[F_MRDONE]) break;
+      S[S_MR + 8] = red[0];
+      if (red[0] <= S[S_MR + 11]) F[F_MRDONE] = 1;
+      break;
+    }
     default:
       break;
   }
@@ -558,6 +601,7 @@ __global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb
 int blas_nred(int op) {
   switch (op) {
     case B_NORM2: case B_XMY_NORM: case B_XMY_NORM_COPY: case B_AXPY_NORM: case B_AXPY2_NORM: case B_BICG_UPDATE:
+    case B_AXPY_CNORM:
       return 1;
     case B_CDOT:
       return 2;
@@ -591,6 +635,8 @@ void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s) 
     case B_BICG_UPDATE: dispatch<FBicgUpdate>(b, prec, s); break;
     case B_CAXPY2: dispatch<FCaxpy2>(b, prec, s); break;
     case B_RBICG_UPDATE: dispatch<FRbicgUpdate>(b, prec, s); break;
+    case B_SCALE: dispatch<FScale>(b, prec, s); break;
+    case B_AXPY_CNORM: dispatch<FAxpyCNorm>(b, prec, s); break;
     default: throw Error(CMX_ERR_ARG, "blas: unknown op");
   }
 }

@@ -1,0 +1,401 @@
+// FGMRES-DR: flexible (right-preconditioned) GMRES with deflated restarts, after
+// LinOpSysSolverFGMRESDR (syssolver_linop_fgmres_dr.cc:78-774, Givens rotations in
+// syssolver_linop_fgmres_dr.h:32-140).
+//
+// Device side: the Krylov basis V and the preconditioned basis Z live in the sloppy precision; one
+// Arnoldi step is the preconditioner, one operator application, one pass of all inner products
+// <V_i, w>, one pass w -= sum_i h_i V_i with |w|^2 (classical Gram-Schmidt, optionally repeated),
+// and the normalisation.  Host side: the small Hessenberg / Givens / least-squares / deflation
+// algebra (one read-back of the new column per step).  After every cycle the residual is recomputed
+// in FP64 (as the reference does, :740-745), which makes the sloppy basis safe.
+//
+// Differences from the reference, each deliberate:
+//  * deflated cycles apply only the Givens rotations of the current cycle to a new column (the
+//    reference also applies the previous cycle's rotations of the deflated columns, :151-153, which
+//    does not match the Q_k^H transformation it has already applied to those rows);
+//  * each solve starts with Z = 0 (the reference keeps Z from the previous solve as the
+//    preconditioner's initial guesses, because it zeroes Z only at construction, :197-225);
+//  * the identity preconditioner is offered (the reference needs a factory solver).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "dense.h"
+#include "op.h"
+
+namespace cmx {
+
+int kry_blocks(int n);
+void learned_precond_apply(Op& op, int prec, const Vec& in, Vec& out);
+
+namespace {
+
+using dense::cd;
+using dense::Mat;
+
+// Givens rotation with a real, non-negative sine (syssolver_linop_fgmres_dr.h:32-140)
+struct Givens {
+  int col = 0;
+  cd c = 1, s = 0, r = 0;
+  Givens() = default;
+  Givens(int col_, cd f, cd g) : col(col_) {
+    if (f == cd(0)) {
+      c = 0;
+      s = 1;
+      r = g;
+    } else if (g == cd(0)) {
+      s = 0;
+      c = std::conj(f) / std::abs(f);
+      r = std::abs(f);
+    } else {
+      const double t = std::sqrt(std::norm(f) + std::norm(g));
+      r = t;
+      c = f / t;
+      s = g / t;
+    }
+  }
+  // rows (col, col+1) of a column vector
+  void apply(std::vector<cd>& v) const {
+    const cd a = v[col], b = v[col + 1];
+    v[col] = std::conj(c) * a + std::conj(s) * b;
+    v[col + 1] = -s * a + c * b;
+  }
+};
+
+// Device helpers over a basis of sloppy vectors
+struct KryDev {
+  Op& op;
+  int n;
+  DevBuf part;
+  explicit KryDev(Op& o, int n_) : op(o), n(n_) { part.alloc(sizeof(double) * KRY_BLOCKS * 2 * KRY_MAXV); }
+  MultiLaunch base(const Vec& w, const std::vector<Vec*>& vs, int m) const {
+    MultiLaunch L;
+    std::memset(&L, 0, sizeof(L));
+    L.n = n;
+    L.m = m;
+    L.w = w.view();
+    for (int i = 0; i < m; ++i) L.v[i] = vs[i]->view();
+    L.S = op.state_S.as<double>();
+    L.partials = part.as<double>();
+    return L;
+  }
+  // S[slot + 2i, 2i+1] = <vs[i], w>, i < m
+  void mdot(const std::vector<Vec*>& vs, int m, const Vec& w, int slot) {
+    MultiLaunch L = base(w, vs, m);
+    launch_mdot(L, vs[0]->prec, w.prec, op.stream);
+    launch_colsum(part.as<double>(), kry_blocks(n), 2 * m, op.state_S.as<double>() + slot, 0, op.stream);
+  }
+  // w (+)= sgn sum_i S[slot+2i] vs[i]; if nslot >= 0: S[nslot] = |w|^2, S[nslot+1] = 1/|w|
+  void maxpy(Vec& w, const std::vector<Vec*>& vs, int m, int slot, double sgn, bool overwrite, int nslot) {
+    MultiLaunch L = base(w, vs, m);
+    L.coef = slot;
+    L.sgn = sgn;
+    L.overwrite = overwrite ? 1 : 0;
+    launch_maxpy(L, vs[0]->prec, w.prec, nslot >= 0, op.stream);
+    if (nslot >= 0) launch_colsum(part.as<double>(), kry_blocks(n), 1, op.state_S.as<double>() + nslot, 1, op.stream);
+  }
+  void put(int slot, const std::vector<cd>& c) {
+    for (size_t i = 0; i < c.size(); ++i) {
+      op.hS[slot + 2 * i] = c[i].real();
+      op.hS[slot + 2 * i + 1] = c[i].imag();
+    }
+    CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + slot, op.hS + slot, sizeof(double) * 2 * c.size(),
+                           hipMemcpyHostToDevice, op.stream));
+  }
+  void put1(int slot, double v) {
+    op.hS[slot] = v;
+    CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + slot, op.hS + slot, sizeof(double),
+                           hipMemcpyHostToDevice, op.stream));
+  }
+  void get(int slot, int count) {
+    CMX_HIP(hipMemcpyAsync(op.hS + slot, static_cast<double*>(op.state_S.p) + slot, sizeof(double) * count,
+                           hipMemcpyDeviceToHost, op.stream));
+    op.sync();
+  }
+};
+
+// Krylov-area layout of the scalar array
+constexpr int K_H = S_KRY;                 // new Hessenberg column (2 * KRY_MAXV)
+constexpr int K_H2 = K_H + 2 * KRY_MAXV;   // second Gram-Schmidt pass
+constexpr int K_WN = K_H2 + 2 * KRY_MAXV;  // |w|^2, 1/|w|
+constexpr int K_C = K_WN + 2;              // host-supplied coefficients (2 * KRY_MAXV)
+static_assert(K_C + 2 * KRY_MAXV <= S_MR, "Krylov slots overlap the MR slots");
+
+// MR preconditioner (LinOpSysSolverMR -> InvMR, invmr.cc:55-160) in precision ps: z is the initial
+// guess on entry (the flexible solver passes Z_j) and the result on exit.
+struct MRPrecond {
+  Op& op;
+  int ps, n, maxit;
+  double rsd, omega;
+  Vec r, mr;
+  MRPrecond(Op& o, int ps_, int n_, int maxit_, double rsd_, double omega_)
+      : op(o), ps(ps_), n(n_), maxit(maxit_), rsd(rsd_), omega(omega_), r(ps_, n_), mr(ps_, n_) {}
+  void apply(Vec& z, const Vec& v) {
+    const int nb = blas_blocks(n);
+    int* F = op.state_F.as<int>();
+    op.hS[S_MR + 9] = omega;
+    op.hS[S_MR + 10] = rsd;
+    CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + S_MR + 9, op.hS + S_MR + 9, 2 * sizeof(double),
+                           hipMemcpyHostToDevice, op.stream));
+    op.blas(B_NORM2, {&v});
+    op.finalize(nb, 1, PG_STORE, S_MR + 7);  // |chi|^2
+    op.apply(ps, mr.view(), z.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {&r, &v, &mr});      // r = chi - M psi
+    op.finalize(nb, 1, PG_MR_INIT);
+    for (int k = 0; k < maxit; ++k) {
+      op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
+      op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
+      op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
+      op.blas(B_CAXPY, {&z, &r}, S_MR + 3, 0, F_MRDONE);              // psi += a r
+      op.blas(B_AXPY_CNORM, {&r, &mr}, S_MR + 5, 0, F_MRDONE);        // r -= a M r
+      op.finalize(nb, 1, PG_MR_R);
+    }
+  }
+};
+
+}  // namespace
+
+void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  const int ps = p.sloppy_prec;
+  CMX_REQUIRE(ps == P64 || ps == P32 || ps == P16, CMX_ERR_ARG, "FGMRES-DR: bad sloppy precision");
+  const int nk = p.n_krylov, nd_max = p.n_defl;
+  CMX_REQUIRE(nk >= 1 && nd_max >= 0 && nd_max < nk && nk + nd_max + 1 <= KRY_MAXV, CMX_ERR_ARG,
+              "FGMRES-DR: need 1 <= NKrylov, 0 <= NDefl < NKrylov, NKrylov + NDefl < 48");
+  CMX_REQUIRE(p.precond >= CMX_PRECOND_NONE && p.precond <= CMX_PRECOND_LEARNED, CMX_ERR_ARG,
+              "FGMRES-DR: unknown preconditioner");
+  const int total = nk + nd_max;
+  std::vector<std::unique_ptr<Vec>> V(total + 1), Z(total);
+  for (auto& v : V) v.reset(new Vec(ps, n));
+  for (auto& z : Z) z.reset(new Vec(ps, n));
+  for (auto& z : Z) op.blas(B_ZERO, {z.get()});
+  Vec r(P64, n), tmp(P64, n);
+  KryDev kd(op, n);
+  std::unique_ptr<MRPrecond> mr;
+  if (p.precond == CMX_PRECOND_MR)
+    mr.reset(new MRPrecond(op, ps, n, std::max(0, p.precond_iters), p.precond_rsd,
+                           p.precond_omega != 0 ? p.precond_omega : 1.0));
+  const int reorth = p.reorth >= 2 ? 2 : 1;
+  const int nb = blas_blocks(n);
+
+  auto t0 = std::chrono::steady_clock::now();
+  hipEvent_t ev0, ev1;
+  CMX_HIP(hipEventCreate(&ev0));
+  CMX_HIP(hipEventCreate(&ev1));
+  CMX_HIP(hipEventRecord(ev0, op.stream));
+  (void)t0;
+
+  // |chi|, target (:463-465); r = chi - A psi, |r| (:468-475)
+  op.blas(B_NORM2, {&chi});
+  op.finalize(nb, 1, PG_STORE, K_WN);
+  kd.get(K_WN, 1);
+  const double norm_rhs = std::sqrt(op.hS[K_WN]);
+  const double target = norm_rhs * p.rsd_target;
+  auto true_residual = [&]() {
+    op.apply(P64, tmp.view(), psi.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {&r, &chi, &tmp});
+    op.finalize(nb, 1, PG_STORE, K_WN);
+    kd.get(K_WN, 1);
+    return std::sqrt(op.hS[K_WN]);
+  };
+  double r_norm = true_residual();
+
+  Mat H(total + 1, total), R(total + 1, total);
+  std::vector<cd> g(total + 1), c(total + 1), eta(total);
+  std::vector<Givens> rot(total);
+  Mat Qk;  // Q of the deflated block (nd+1 x nd+1)
+  int iters = 0, cycles = 0, prev_dim = nk, prev_nd = 0;
+  bool finished = r_norm <= target;
+  std::vector<Vec*> vs(total + 1);
+
+  auto precondition = [&](Vec& z, const Vec& v) {
+    switch (p.precond) {
+      case CMX_PRECOND_NONE: op.blas(B_COPY, {&z, &v}); break;
+      case CMX_PRECOND_MR: mr->apply(z, v); break;
+      default: learned_precond_apply(op, ps, v, z); break;
+    }
+  };
+
+  while (!finished) {
+    ++cycles;
+    int nd = nd_max;
+    if (cycles == 1 || nd_max == 0) {
+      // c = g = [|r|, 0, ...]; V_0 = r / |r|  (:491-513)
+      std::fill(c.begin(), c.end(), cd(0));
+      std::fill(g.begin(), g.end(), cd(0));
+      c[0] = g[0] = r_norm;
+      H = Mat(total + 1, total);
+      R = Mat(total + 1, total);
+      kd.put1(K_WN + 1, 1.0 / r_norm);
+      op.blas(B_SCALE, {V[0].get(), &r}, K_WN + 1);
+      nd = 0;
+    } else {
+      // deflated restart (:514-688): harmonic Ritz vectors of the previous cycle
+      const int m = prev_dim;
+      std::vector<cd> hm(m), f(m);
+      Mat Hm(m, m);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) Hm(i, j) = H(i, j);
+      for (int j = 0; j < m; ++j) hm[j] = std::conj(H(m, j));
+      f = dense::lu_solve(dense::adjoint(Hm), hm);  // H_m^H f = h_m
+      Mat Ht(m, m);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) Ht(i, j) = H(i, j) + f[i] * std::conj(hm[j]);
+      std::vector<cd> evals;
+      Mat evecs;
+      dense::eig(Ht, evals, evecs);
+      std::vector<int> order(m);
+      for (int i = 0; i < m; ++i) order[i] = i;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return std::norm(evals[a]) < std::norm(evals[b]); });
+      // G_{k+1} = [ [G_k; 0] | c - H eta ]  ((m+1) x (nd+1)), then Q_{k+1} from its QR
+      Mat G(m + 1, nd + 1);
+      for (int col = 0; col < nd; ++col)
+        for (int row = 0; row < m; ++row) G(row, col) = evecs(row, order[col]);
+      for (int row = 0; row <= m; ++row) {
+        cd s = c[row];
+        for (int col = 0; col < m; ++col) s -= H(row, col) * eta[col];
+        G(row, nd) = s;
+      }
+      Mat Q, Rq;
+      dense::qr(G, Q, Rq);
+      // H_new = Q_{k+1}^H H Q_k   ((nd+1) x nd)
+      Mat Hfull(m + 1, m), Qk_(m, nd);
+      for (int i = 0; i <= m; ++i)
+        for (int j = 0; j < m; ++j) Hfull(i, j) = H(i, j);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < nd; ++j) Qk_(i, j) = Q(i, j);
+      Mat Hn = dense::matmul(dense::adjoint(Q), dense::matmul(Hfull, Qk_));
+      // new V_i = sum_j V_j Q(j, i) (i <= nd); new Z_i = sum_j Z_j Q(j, i) (i < nd)
+      std::vector<std::unique_ptr<Vec>> nV(nd + 1), nZ(nd);
+      for (int j = 0; j <= m; ++j) vs[j] = V[j].get();
+      for (int i = 0; i <= nd; ++i) {
+        nV[i].reset(new Vec(ps, n));
+        std::vector<cd> col(m + 1);
+        for (int j = 0; j <= m; ++j) col[j] = Q(j, i);
+        kd.put(K_C, col);
+        kd.maxpy(*nV[i], vs, m + 1, K_C, 1.0, true, -1);
+      }
+      for (int j = 0; j < m; ++j) vs[j] = Z[j].get();
+      for (int i = 0; i < nd; ++i) {
+        nZ[i].reset(new Vec(ps, n));
+        std::vector<cd> col(m);
+        for (int j = 0; j < m; ++j) col[j] = Q(j, i);
+        kd.put(K_C, col);
+        kd.maxpy(*nZ[i], vs, m, K_C, 1.0, true, -1);
+      }
+      for (int i = 0; i <= nd; ++i) V[i].swap(nV[i]);
+      for (int i = 0; i < nd; ++i) Z[i].swap(nZ[i]);
+      for (int i = nd; i < total; ++i) op.blas(B_ZERO, {Z[i].get()});
+      // reset H, R, c, g; c = V_{k+1}^H r (:640-650)
+      H = Mat(total + 1, total);
+      R = Mat(total + 1, total);
+      std::fill(c.begin(), c.end(), cd(0));
+      std::fill(g.begin(), g.end(), cd(0));
+      {
+        // <V_i, r> with r in FP64: convert r once to the basis precision
+        Vec rs(ps, n);
+        op.blas(B_COPY, {&rs, &r});
+        for (int i = 0; i <= nd; ++i) vs[i] = V[i].get();
+        kd.mdot(vs, nd + 1, rs, K_H);
+        kd.get(K_H, 2 * (nd + 1));
+        for (int i = 0; i <= nd; ++i) c[i] = g[i] = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
+      }
+      for (int i = 0; i <= nd; ++i)
+        for (int j = 0; j < nd; ++j) H(i, j) = Hn(i, j);
+      // QR of H_k ((nd+1) x nd): R block and g = Q_k^H c (:652-670)
+      Mat Hk(nd + 1, nd);
+      for (int i = 0; i <= nd; ++i)
+        for (int j = 0; j < nd; ++j) Hk(i, j) = Hn(i, j);
+      Mat Rk;
+      dense::qr(Hk, Qk, Rk, true);
+      for (int i = 0; i < nd; ++i)
+        for (int j = i; j < nd; ++j) R(i, j) = Rk(i, j);
+      std::vector<cd> gq(nd + 1, cd(0));
+      for (int i = 0; i <= nd; ++i)
+        for (int j = 0; j <= nd; ++j) gq[i] += std::conj(Qk(j, i)) * g[j];
+      for (int i = 0; i <= nd; ++i) g[i] = gq[i];
+    }
+
+    // flexible Arnoldi (:78-183)
+    int dim = nd;
+    for (int j = nd; j < nd + nk; ++j) {
+      precondition(*Z[j], *V[j]);                                     // z_j = M v_j
+      op.apply(ps, V[j + 1]->view(), Z[j]->view(), +1, nullptr);      // w = A z_j
+      for (int i = 0; i <= j; ++i) vs[i] = V[i].get();
+      kd.mdot(vs, j + 1, *V[j + 1], K_H);                             // h_i = <v_i, w>
+      kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
+      if (reorth == 2) {
+        kd.mdot(vs, j + 1, *V[j + 1], K_H2);
+        kd.maxpy(*V[j + 1], vs, j + 1, K_H2, -1.0, false, K_WN);
+      }
+      kd.get(K_H, K_WN + 2 - K_H);  // the new column (both passes), |w|^2 and 1/|w| in one read-back
+      for (int i = 0; i <= j; ++i) {
+        H(i, j) = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
+        if (reorth == 2) H(i, j) += cd(op.hS[K_H2 + 2 * i], op.hS[K_H2 + 2 * i + 1]);
+      }
+      const double wnorm = std::sqrt(op.hS[K_WN]);
+      H(j + 1, j) = wnorm;
+      if (std::fabs(wnorm) < 1e-14) {  // exact convergence (:117-127)
+        dim = j;
+        break;
+      }
+      op.blas(B_SCALE, {V[j + 1].get(), V[j + 1].get()}, K_WN + 1);  // v_{j+1} = w / |w|
+      std::vector<cd> col(j + 2);
+      for (int i = 0; i <= j + 1; ++i) col[i] = H(i, j);
+      if (nd > 0) {  // R_col <- Q_k^H R_col on the deflated rows (:133-152)
+        std::vector<cd> t(nd + 1, cd(0));
+        for (int i = 0; i <= nd; ++i)
+          for (int k = 0; k <= nd; ++k) t[i] += std::conj(Qk(k, i)) * col[k];
+        for (int i = 0; i <= nd; ++i) col[i] = t[i];
+      }
+      for (int i = nd; i < j; ++i) rot[i].apply(col);
+      rot[j] = Givens(j, col[j], col[j + 1]);
+      col[j] = rot[j].r;
+      col[j + 1] = 0;
+      rot[j].apply(g);
+      for (int i = 0; i <= j + 1; ++i) R(i, j) = col[i];
+      dim = j + 1;
+      const double accum = std::fabs(g[j + 1].real());  // (:172)
+      if (p.verbose) std::printf("FGMRESDR: iter %d |r| ~ %g target %g\n", iters + j - nd + 1, accum, target);
+      if (accum <= target) break;
+    }
+    // eta = R^-1 g by back substitution (:240-262); psi += sum eta_j Z_j (:716-724)
+    if (dim > 0) {
+      for (int row = dim - 1; row >= 0; --row) {
+        cd s = g[row];
+        for (int col = row + 1; col < dim; ++col) s -= R(row, col) * eta[col];
+        eta[row] = s / R(row, row);
+      }
+      for (int j = 0; j < dim; ++j) vs[j] = Z[j].get();
+      kd.put(K_C, std::vector<cd>(eta.begin(), eta.begin() + dim));
+      kd.maxpy(psi, vs, dim, K_C, 1.0, false, -1);
+    }
+    r_norm = true_residual();  // (:727-736)
+    iters += dim - nd;
+    finished = r_norm <= target || iters >= p.max_iter || dim == nd;
+    prev_dim = dim;
+    prev_nd = nd;
+    if (p.verbose)
+      std::printf("FGMRESDR: cycle %d: %d iterations, |r| = %g target %g\n", cycles, dim - nd, r_norm, target);
+  }
+  (void)prev_nd;
+  CMX_HIP(hipEventRecord(ev1, op.stream));
+  CMX_HIP(hipEventSynchronize(ev1));
+  float ms = 0;
+  CMX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  res.n_count = iters;
+  res.resid = r_norm;
+  res.rel_resid = norm_rhs > 0 ? r_norm / norm_rhs : r_norm;
+  res.n_reliable = cycles;
+  res.solve_seconds = ms * 1e-3;
+  res.gflops = 0;
+  if (r_norm > target)
+    throw Error(CMX_ERR_NOCONV, "FGMRES-DR: no convergence in " + std::to_string(p.max_iter) + " iterations");
+}
+
+}  // namespace cmx

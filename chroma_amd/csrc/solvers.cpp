@@ -20,6 +20,8 @@
 
 namespace cmx {
 
+void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res);
+
 namespace {
 
 struct Timer {
@@ -308,6 +310,11 @@ void solve_relbicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& 
 
 // one pass of the selected solver on A psi = chi (isign = +1), or on M^dag M psi = chi (mdagm)
 void solve_pass(Op& op, const cmx_solver_params& q, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& part) {
+  if (q.solver == CMX_SOLVER_FGMRESDR) {
+    CMX_REQUIRE(!mdagm, CMX_ERR_UNSUPPORTED, "FGMRES-DR is a LinOp system solver (no MdagM form)");
+    solve_fgmresdr(op, q, psi, chi, part);
+    return;
+  }
   switch (q.solver) {
     case CMX_SOLVER_CG: solve_cg(op, q, psi, chi, mdagm, part); return;
     case CMX_SOLVER_RELIABLE_CG: solve_relcg(op, q, psi, chi, mdagm, part); return;

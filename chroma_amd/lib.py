@@ -16,7 +16,8 @@ CMX_ERR_ARG, CMX_ERR_HIP, CMX_ERR_STATE, CMX_ERR_NOCONV, CMX_ERR_BREAKDOWN, CMX_
 DOUBLE, SINGLE, HALF = 0, 1, 2
 RECONS_18, RECONS_12 = 18, 12
 UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
-CG, BICGSTAB, RELIABLE_CG, RELIABLE_BICGSTAB = 0, 1, 2, 3
+CG, BICGSTAB, RELIABLE_CG, RELIABLE_BICGSTAB, FGMRESDR = 0, 1, 2, 3, 4
+PRECOND_NONE, PRECOND_MR, PRECOND_LEARNED = 0, 1, 2
 PLUS, MINUS = 1, -1
 
 
@@ -35,7 +36,9 @@ class OpParams(ctypes.Structure):
 class SolverParams(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int), ("max_iter", ctypes.c_int), ("rsd_target", ctypes.c_double),
                 ("delta", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("verbose", ctypes.c_int), ("true_residual", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("true_residual", ctypes.c_int), ("n_krylov", ctypes.c_int),
+                ("n_defl", ctypes.c_int), ("precond", ctypes.c_int), ("precond_iters", ctypes.c_int),
+                ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int)]
 
 
 class SolverResult(ctypes.Structure):
@@ -258,7 +261,8 @@ class Operator:
         return ms.value
 
     def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
-              check_every=0, verbose=False, true_residual=False, mdagm=False):
+              check_every=0, verbose=False, true_residual=False, mdagm=False, n_krylov=16, n_defl=0,
+              precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1):
         """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays.
 
         mdagm=True solves M^dag M psi = chi (MdagMSystemSolver, lib/syssolver.h:72-84)."""
@@ -268,7 +272,8 @@ class Operator:
         psis = np.zeros_like(chis) if psi is None else np.ascontiguousarray(psi, np.complex128).reshape(chis.shape).copy()
         nrhs = chis.shape[0]
         p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
-                         int(bool(verbose)), int(bool(true_residual)))
+                         int(bool(verbose)), int(bool(true_residual)), int(n_krylov), int(n_defl), int(precond),
+                         int(precond_iters), float(precond_rsd), float(precond_omega), int(reorth))
         res = (SolverResult * nrhs)()
         fn = lib().cmx_solve_mdagm if mdagm else lib().cmx_solve
         check(fn(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))

@@ -52,6 +52,13 @@ extern "C" {
 #define CMX_SOLVER_RELIABLE_CG 2  /* RELIABLE_CG_MP_CLOVER_INVERTER: RelInvCG (reliable_cg.cc:10-188)    */
 #define CMX_SOLVER_RELIABLE_BICGSTAB 3 /* RELIABLE_BICGSTAB_MP_CLOVER_INVERTER: RelInvBiCGStab
                                           (reliable_bicgstab.cc:14-290)                            */
+#define CMX_SOLVER_FGMRESDR 4     /* FGMRESDR_INVERTER: LinOpSysSolverFGMRESDR
+                                     (syssolver_linop_fgmres_dr.cc:459-774), right-preconditioned  */
+
+/* Preconditioners of the flexible solver (its PrecondParams, syssolver_fgmres_dr_params.cc:21) */
+#define CMX_PRECOND_NONE 0        /* identity                                                       */
+#define CMX_PRECOND_MR 1          /* MR_INVERTER: InvMR (invmr.cc:55-160) in the sloppy precision     */
+#define CMX_PRECOND_LEARNED 2     /* gauge-equivariant network loaded with cmx_op_set_learned_precond */
 
 #define CMX_PLUS 1
 #define CMX_MINUS (-1)
@@ -131,6 +138,16 @@ typedef struct {
                         residual, invcg2.cc:107,194).  1: additionally restart from the current psi
                         with a tightened inner tolerance until |chi - A psi| <= rsd_target |chi| in
                         FP64 (the generalisation of CHROMA_DO_ONE_CG_RESTART, syssolver_linop_cg.h:69-76) */
+  /* FGMRES-DR (SysSolverFGMRESDRParams, syssolver_fgmres_dr_params.cc:13-21); the Krylov basis is
+     kept in sloppy_prec, the residual is recomputed in FP64 after every cycle */
+  int n_krylov;      /* NKrylov >= 1                                                               */
+  int n_defl;        /* NDefl < NKrylov, deflated (harmonic Ritz) vectors kept across restarts;
+                        NKrylov + NDefl <= 47                                                      */
+  int precond;       /* CMX_PRECOND_*                                                              */
+  int precond_iters; /* MR: MaxMR                                                                  */
+  double precond_rsd;   /* MR: RsdMR                                                               */
+  double precond_omega; /* MR: MROver (over-relaxation)                                            */
+  int reorth;        /* Gram-Schmidt passes per Arnoldi step: 1 (classical) or 2                   */
 } cmx_solver_params;
 
 typedef struct {

@@ -283,6 +283,62 @@ def test_mdagm_solvers(solver):
     assert res["rel_resid"] < (1e-9 if solver in (cmx.CG, cmx.RELIABLE_CG) else 1e-8)
 
 
+def _odd_op(ref_op, V):
+    def A(x):
+        full = np.zeros((V, 4, 3), np.complex128)
+        full[V // 2:] = x
+        return ref_op(full)[V // 2:]
+    return A
+
+
+@pytest.mark.parametrize("n_defl,precond", [(0, 0), (3, 0), (0, 1), (3, 1)])
+def test_fgmres_dr_vs_oracle(n_defl, precond):
+    """FGMRES-DR with an FP64 basis against the oracle's restatement: cycle structure, iteration count
+    and solution (the library orthogonalises by classical Gram-Schmidt twice, the oracle by modified
+    Gram-Schmidt, equal to rounding)."""
+    from oracle import krylov
+    L = (8, 8, 8, 16)
+    say("fgmres-dr", n_defl, precond)
+    V = int(np.prod(L))
+    U = gauge_field(L, 97)
+    mass = cmx.kappa_to_mass(0.125)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0)
+    chi = random_fermion(V, 16)[V // 2:]
+    A = _odd_op(ref_op, V)
+    pre = None if precond == 0 else (lambda z0, v: krylov.mr(A, v, z0, 1e-12, 3)[0])
+    psi_ref, n_ref, r_ref, cyc_ref = krylov.fgmres_dr(A, chi, np.zeros_like(chi), 1e-10, n_krylov=12,
+                                                      n_defl=n_defl, max_iter=3000, precond=pre)
+    psi, res = op.solve(chi, solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=cmx.DOUBLE, n_krylov=12,
+                        n_defl=n_defl, precond=precond, precond_iters=3, precond_rsd=1e-12, reorth=2)
+    say("  n_count", res["n_count"], "oracle", n_ref, "cycles", res["n_reliable"], "oracle", cyc_ref,
+        "rel resid", res["rel_resid"])
+    assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)
+    assert res["rel_resid"] <= 1e-10
+    assert rel(psi, psi_ref) < 1e-8
+
+
+@pytest.mark.parametrize("sloppy", [cmx.SINGLE, cmx.HALF])
+@pytest.mark.parametrize("n_defl", [0, 4])
+def test_fgmres_dr_sloppy_basis(sloppy, n_defl):
+    """FP32 / FP16 Krylov basis with the FP64 residual recomputed every cycle reaches a 1e-10 FP64 true
+    residual; the solution agrees with the FP64 CG solution."""
+    L = (8, 8, 8, 16)
+    say("fgmres-dr sloppy", sloppy, n_defl)
+    V = int(np.prod(L))
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=99, nstout=6, rho=0.1), (1, 1, 1, -1))
+    mass = cmx.kappa_to_mass(0.1350)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF if sloppy == cmx.HALF else cmx.SINGLE)
+    chi = cmx.z2_source((V // 2, 4, 3), 6)
+    psi_cg, r_cg = op.solve(chi, solver=cmx.CG, rsd=1e-12, maxit=20000)
+    psi, res = op.solve(chi, solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=sloppy, n_krylov=16,
+                        n_defl=n_defl, precond=cmx.PRECOND_MR, precond_iters=4, precond_rsd=1e-12)
+    say("  n_count", res["n_count"], "cycles", res["n_reliable"], "rel resid", res["rel_resid"],
+        "seconds", res["solve_seconds"])
+    assert res["rel_resid"] <= 1e-10
+    assert rel(psi, psi_cg) < 1e-8
+
+
 def test_twisted_golden_through_gpu():
     """prec_wilson-twisted (tests/chroma/hadron/propagator/prec_wilson-twisted.{ini,out}.xml) on the GPU."""
     from chroma_amd import qprop as qp

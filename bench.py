@@ -39,6 +39,18 @@ def eo_clover_bytes(prec, recon):
     return b
 
 
+def measured_traffic(key, lattice):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/hop_<lattice>_traffic.md), or None when no summary exists for this lattice."""
+    path = os.path.join(ROOT, "profiles", f"hop_{'x'.join(map(str, lattice))}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {key}:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(L, U, seconds_target=12.0):
     """Oracle FP64 hopping term on the host cores (a bounded sample of the same workload)."""
     import numpy as np
@@ -129,6 +141,7 @@ def main():
         cpu = cpu_baseline(L, U)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
+    traffic, traffic_src = measured_traffic("hop_fp16", L)
     out = {
         "metric": "Dslash GB/s (% HBM peak); solve time to resid 1e-10, plain vs learned",
         "value": round(gbs, 2),
@@ -147,7 +160,8 @@ def main():
                    "storage": "FP16 spinors + per-site FP32 norm, 12-real FP16 links",
                    "bytes_per_site": hop_bytes(cmx.HALF, 12), "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": hop_bytes(cmx.HALF, 12) * Vh, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "solve_time_plain_s": solves.get("plain_mixed_cg_fp16", {}).get("seconds"),
         "solve_time_learned_s": None,

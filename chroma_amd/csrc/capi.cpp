@@ -14,6 +14,9 @@ struct cmx_gauge_s {
 struct cmx_op_s {
   std::unique_ptr<cmx::Op> op;
 };
+struct cmx_net_s {
+  std::shared_ptr<cmx::LearnedNet> net;
+};
 
 namespace {
 thread_local std::string g_err;
@@ -40,6 +43,10 @@ void require_init() { CMX_REQUIRE(g_init, CMX_ERR_STATE, "library not initialise
 cmx::Op& O(cmx_op h) {
   CMX_REQUIRE(h && h->op, CMX_ERR_ARG, "null operator handle");
   return *h->op;
+}
+cmx::LearnedNet& N(cmx_net h) {
+  CMX_REQUIRE(h && h->net, CMX_ERR_ARG, "null network handle");
+  return *h->net;
 }
 }  // namespace
 
@@ -383,6 +390,88 @@ int cmx_layout_roundtrip(cmx_op h, int prec, const double* in, double* out) {
     cmx::Vec v(prec, n);
     op.pack(in, n, v);
     op.unpack(out, n, v);
+  });
+}
+
+
+// ---- learned preconditioner ------------------------------------------------------------------------
+int cmx_net_create(cmx_net* out) {
+  return guarded([&] {
+    CMX_REQUIRE(out, CMX_ERR_ARG, "net_create: null output");
+    auto* h = new cmx_net_s();
+    h->net = std::make_shared<cmx::LearnedNet>();
+    *out = h;
+  });
+}
+
+int cmx_net_destroy(cmx_net net) {
+  return guarded([&] { delete net; });
+}
+
+int cmx_net_init_neumann(cmx_net net, double diag) {
+  return guarded([&] {
+    CMX_REQUIRE(diag != 0, CMX_ERR_ARG, "net_init_neumann: diag must be non-zero");
+    cmx::net_init_neumann(N(net), diag);
+  });
+}
+
+int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0) {
+  return guarded([&] {
+    cmx::LearnedNet& n = N(net);
+    auto put = [](const std::vector<cmx::LearnedNet::cf>& w, double* o) {
+      if (!o) return;
+      for (size_t k = 0; k < w.size(); ++k) o[2 * k] = w[k].real(), o[2 * k + 1] = w[k].imag();
+    };
+    put(n.W1, w1);
+    put(n.W2, w2);
+    put(n.W0, w0);
+  });
+}
+
+int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0) {
+  return guarded([&] {
+    cmx::LearnedNet& n = N(net);
+    auto get = [](std::vector<cmx::LearnedNet::cf>& w, const double* i) {
+      if (!i) return;
+      for (size_t k = 0; k < w.size(); ++k) w[k] = cmx::LearnedNet::cf((float)i[2 * k], (float)i[2 * k + 1]);
+    };
+    get(n.W1, w1);
+    get(n.W2, w2);
+    get(n.W0, w0);
+    n.version++;
+  });
+}
+
+int cmx_op_set_net(cmx_op op, cmx_net net) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& o = O(op);
+    o.net = net ? net->net : nullptr;
+    o.net_version = -1;
+  });
+}
+
+int cmx_net_apply(cmx_op op, int prec, double* out, const double* in) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(out && in && prec >= 0 && prec <= 2, CMX_ERR_ARG, "net_apply: bad argument");
+    cmx::net_apply_host(O(op), prec, out, in);
+  });
+}
+
+int cmx_net_grad(cmx_op op, int prec, const double* v, const double* e, double* loss, double* g_wc, double* g_w0) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(v && e && loss && g_wc && g_w0 && prec >= 0 && prec <= 2, CMX_ERR_ARG, "net_grad: bad argument");
+    cmx::net_grad_host(O(op), prec, v, e, loss, g_wc, g_w0);
+  });
+}
+
+int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(p && res, CMX_ERR_ARG, "net_train: bad argument");
+    cmx::net_train(O(op), *p, *res);
   });
 }
 

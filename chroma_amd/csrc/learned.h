@@ -1,0 +1,54 @@
+// Gauge-equivariant parallel-transport network (the learned preconditioner): weights, device
+// buffers and kernel launchers.  The network and its kernels are described in learned.hip.
+#pragma once
+#include <complex>
+#include <vector>
+
+#include "cmx_internal.h"
+
+namespace cmx {
+
+constexpr int PTC_CH = 16;    // hidden channels
+constexpr int PTC_PATHS = 8;  // one-hop paths +-mu
+
+struct PtcLaunch {
+  Geom g;
+  int prec, recon;
+  const void* gauge[2];  // links per parity in the operator's layout for `prec`
+  double ph[4][2][2];    // det(U) per direction (bulk / last slice) for 12-real links
+  const void* in;        // odd-checkerboard input (gather: v or delta; scatter: v; w0 grad: v)
+  const float* in_nrm;
+  int in_stride;
+  void* out;             // odd-checkerboard output (scatter) / delta (w0 grad)
+  float* out_nrm;
+  int out_stride;
+  unsigned short* T;     // transported inputs, BF16 MFMA tiles (Vh/32 x 3 x 4 KiB)
+  unsigned short* dT;    // transported loss gradient, same layout
+  unsigned short* G;     // mixed field g_q, BF16 (8 x 3 x Vh x 16 B)
+  const unsigned short* Afr;  // real embedding of Wc in MFMA A-operand order (2 x 8 x 64 x 4 BF16)
+  const float* W0;       // 4x4 complex, row-major (re, im)
+  float* gpart;          // gradient partials (waves x 4096 floats)
+  double* w0part;        // W0 gradient partials (blocks x 32 doubles)
+};
+
+int ptc_waves();
+void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s);
+void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
+void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s);
+void launch_ptc_grad(const PtcLaunch& L, double* out4096, hipStream_t s);
+int launch_w0_grad(const PtcLaunch& L, hipStream_t s);
+
+// Network weights (host, FP32 complex) with Adam state.  Component indices: path/channel-major,
+// spin-minor: W1[(c*4 + s) * 32 + (p*4 + t)], W2[(q*4 + s) * 64 + (c*4 + t)], W0[s*4 + t].
+struct LearnedNet {
+  using cf = std::complex<float>;
+  std::vector<cf> W1, W2, W0;
+  std::vector<float> m1, v1, m2, v2, m0, v0;  // Adam moments over (re, im)
+  long step = 0;
+  long version = 0;  // bumped on every weight change; operators re-upload when it differs
+  LearnedNet();
+  // Wc = W2 W1 (32 x 32 complex), row-major over (q*4+s, p*4+t)
+  std::vector<cf> contracted() const;
+};
+
+}  // namespace cmx

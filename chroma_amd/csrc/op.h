@@ -5,6 +5,7 @@
 
 #include "blas.h"
 #include "cmx_internal.h"
+#include "learned.h"
 
 namespace cmx {
 
@@ -91,6 +92,10 @@ struct Op {
   int* hF = nullptr;
   DevBuf staging;        // V*24 doubles
   std::vector<std::unique_ptr<Vec>> tmp_even[3];  // scratch even-checkerboard vectors
+  // learned preconditioner: the attached network, the weight version last uploaded, device buffers
+  std::shared_ptr<LearnedNet> net;
+  long net_version = -1;
+  std::shared_ptr<void> ptc;
 
   ~Op();
   bool eo() const { return par.kind != CMX_OP_UNPREC_WILSON; }
@@ -126,6 +131,12 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);
 Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
 
 // Solvers (solvers.cpp)
+// Learned preconditioner (learned.cpp)
+void net_init_neumann(LearnedNet& net, double diag);
+void net_apply_host(Op& op, int prec, double* out, const double* in);
+void net_grad_host(Op& op, int prec, const double* v, const double* e, double* loss, double* gwc, double* gw0);
+void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
+
 // mdagm: solve M^dag M psi = chi instead of M psi = chi
 void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
 

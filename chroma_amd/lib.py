@@ -41,6 +41,17 @@ class SolverParams(ctypes.Structure):
                 ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int)]
 
 
+class TrainParams(ctypes.Structure):
+    _fields_ = [("n_sources", ctypes.c_int), ("snapshot_every", ctypes.c_int), ("solve_rsd", ctypes.c_double),
+                ("epochs", ctypes.c_int), ("lr", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+                ("max_pairs", ctypes.c_int), ("prec", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class TrainResult(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_int), ("n_steps", ctypes.c_int), ("loss_first", ctypes.c_double),
+                ("loss_last", ctypes.c_double), ("gen_seconds", ctypes.c_double), ("train_seconds", ctypes.c_double)]
+
+
 class SolverResult(ctypes.Structure):
     _fields_ = [("n_count", ctypes.c_int), ("resid", ctypes.c_double), ("rel_resid", ctypes.c_double),
                 ("n_reliable", ctypes.c_int), ("solve_seconds", ctypes.c_double),
@@ -82,6 +93,15 @@ def lib():
                                      ctypes.POINTER(SolverResult)]),
         "cmx_solve_mdagm": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
                                            ctypes.POINTER(SolverResult)]),
+        "cmx_net_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+        "cmx_net_destroy": (ctypes.c_int, [_vp]),
+        "cmx_net_init_neumann": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "cmx_net_get_weights": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+        "cmx_net_set_weights": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+        "cmx_op_set_net": (ctypes.c_int, [_vp, _vp]),
+        "cmx_net_apply": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+        "cmx_net_grad": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
+        "cmx_net_train": (ctypes.c_int, [_vp, ctypes.POINTER(TrainParams), ctypes.POINTER(TrainResult)]),
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
@@ -281,6 +301,73 @@ class Operator:
                     solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
                     n_restarts=r.n_restarts) for r in res]
         return (psis[0], out[0]) if single else (psis, out)
+
+
+class Net:
+    """Learned preconditioner: a gauge-equivariant parallel-transport network (see include/chroma_mi355x.h).
+
+    Weights: W1 complex[64, 32] over ((channel, spin), (path, spin)), W2 complex[32, 64], W0 complex[4, 4];
+    path p = 2 mu + (0 forward, 1 backward)."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().cmx_net_create(ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _L is not None:
+            _L.cmx_net_destroy(self.h)
+            self.h = None
+
+    def init_neumann(self, diag):
+        check(lib().cmx_net_init_neumann(self.h, float(diag)))
+
+    def get_weights(self):
+        w1 = np.zeros((64, 32), np.complex128)
+        w2 = np.zeros((32, 64), np.complex128)
+        w0 = np.zeros((4, 4), np.complex128)
+        check(lib().cmx_net_get_weights(self.h, _d(w1), _d(w2), _d(w0)))
+        return w1, w2, w0
+
+    def set_weights(self, w1, w2, w0):
+        w1 = np.ascontiguousarray(w1, np.complex128)
+        w2 = np.ascontiguousarray(w2, np.complex128)
+        w0 = np.ascontiguousarray(w0, np.complex128)
+        assert w1.shape == (64, 32) and w2.shape == (32, 64) and w0.shape == (4, 4)
+        check(lib().cmx_net_set_weights(self.h, _d(w1), _d(w2), _d(w0)))
+
+
+def net_attach(op, net):
+    check(lib().cmx_op_set_net(op.h, net.h if net is not None else None))
+    op.net = net
+
+
+def net_apply(op, v, prec=SINGLE):
+    v = np.ascontiguousarray(v, np.complex128)
+    out = np.zeros_like(v)
+    check(lib().cmx_net_apply(op.h, int(prec), _d(out), _d(v)))
+    return out
+
+
+def net_grad(op, v, e, prec=SINGLE):
+    """(|P v - e|^2, d/d conj(W2 W1) complex[32, 32], d/d conj(W0) complex[4, 4])"""
+    v = np.ascontiguousarray(v, np.complex128)
+    e = np.ascontiguousarray(e, np.complex128)
+    loss = ctypes.c_double()
+    gwc = np.zeros((32, 32), np.complex128)
+    gw0 = np.zeros((4, 4), np.complex128)
+    check(lib().cmx_net_grad(op.h, int(prec), _d(v), _d(e), ctypes.byref(loss), _d(gwc), _d(gw0)))
+    return loss.value, gwc, gw0
+
+
+def net_train(op, n_sources=4, snapshot_every=4, solve_rsd=1e-9, epochs=4, lr=1e-3, seed=1, max_pairs=64,
+              prec=SINGLE, verbose=False):
+    p = TrainParams(int(n_sources), int(snapshot_every), float(solve_rsd), int(epochs), float(lr), int(seed),
+                    int(max_pairs), int(prec), int(bool(verbose)))
+    r = TrainResult()
+    check(lib().cmx_net_train(op.h, ctypes.byref(p), ctypes.byref(r)))
+    return dict(n_pairs=r.n_pairs, n_steps=r.n_steps, loss_first=r.loss_first, loss_last=r.loss_last,
+                gen_seconds=r.gen_seconds, train_seconds=r.train_seconds)
 
 
 def declared_symbols():

@@ -173,6 +173,46 @@ int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* 
 int cmx_solve_mdagm(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
                     cmx_solver_result* res);
 
+/* ---- learned preconditioner (CMX_PRECOND_LEARNED): no counterpart in the reference ------------------
+ * A gauge-equivariant parallel-transport network on the odd checkerboard,
+ *   P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),  g_q(y) = sum_c W2[q][c] sum_p W1[c][p] U_p(y) v(y+p),
+ * with one-hop paths p, q in {+-mu}, 16 hidden channels and 4x4 complex spin matrices W0, W1, W2 (colour
+ * identity).  It is the right preconditioner of CMX_SOLVER_FGMRESDR (the slot of PrecondParams,
+ * syssolver_linop_fgmres_dr.h:175-185).  A network is independent of the gauge field: train it on some
+ * operators, attach it to others. */
+typedef struct cmx_net_s* cmx_net;
+int cmx_net_create(cmx_net* out);
+int cmx_net_destroy(cmx_net net);
+/* W0 = 1/diag, W2 W1 = (1/(4 diag^3)) (1 -+ gamma_q)(1 -+ gamma_p): the two-term Neumann series of the
+ * even-odd operator with the clover term replaced by its diagonal diag = Nd + m */
+int cmx_net_init_neumann(cmx_net net, double diag);
+/* weights as interleaved (re, im) doubles: W1[(c*4+s)*32 + p*4+t] (64 x 32), W2[(q*4+s)*64 + c*4+t]
+ * (32 x 64), W0[s*4+t] (4 x 4); path index p = 2 mu + (0 forward, 1 backward) */
+int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0);
+int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0);
+int cmx_op_set_net(cmx_op op, cmx_net net);
+/* out = P(in) on the odd checkerboard at storage precision prec (host arrays, Vh*24 doubles) */
+int cmx_net_apply(cmx_op op, int prec, double* out, const double* in);
+/* loss = |P(v) - e|^2; g_wc = d loss / d conj(W2 W1) (32 x 32 complex), g_w0 = d loss / d conj(W0) */
+int cmx_net_grad(cmx_op op, int prec, const double* v, const double* e, double* loss, double* g_wc, double* g_w0);
+typedef struct {
+  int n_sources;      /* Gaussian sources solved by plain FP64 CGNE                                 */
+  int snapshot_every; /* CG iterations between (residual, error) pairs                              */
+  double solve_rsd;   /* CGNE target defining the "exact" solution x*                               */
+  int epochs;         /* passes over the pairs (one Adam step per pair)                             */
+  double lr;          /* Adam step size                                                             */
+  unsigned long long seed;
+  int max_pairs;
+  int prec;           /* storage precision of the pairs and the network input: CMX_SINGLE/CMX_HALF  */
+  int verbose;
+} cmx_train_params;
+typedef struct {
+  int n_pairs, n_steps;
+  double loss_first, loss_last; /* relative loss |P r - e|^2 / |e|^2: first step, mean of last epoch */
+  double gen_seconds, train_seconds;
+} cmx_train_result;
+int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
+
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
  * double-buffered direction loads (0/1), plane-grid kernel (0/1; FP16 and FP32 with 12-real links and

@@ -1,0 +1,109 @@
+"""ORACLE -- test infrastructure only.
+
+numpy restatement of the learned preconditioner (the library's gauge-equivariant parallel-transport
+network) and of its loss gradient, in FP64, on odd-checkerboard vectors:
+
+    P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),        x odd,  q in {+-mu}
+    g_q(y)  = sum_c W2[q][c] h_c(y),   h_c(y) = sum_p W1[c][p] U_p(y) v(y+p),     y even
+
+U_{+mu}(y) = U_mu(y), U_{-mu}(y) = U_mu(y-mu)^dag.  Weights: W1 complex[64, 32] indexed
+((c*4 + s), (p*4 + t)), W2 complex[32, 64] indexed ((q*4 + s), (c*4 + t)), W0 complex[4, 4]; path
+p = 2 mu + (0 forward, 1 backward).  Spin matrices act on spin, the identity on colour.
+
+Host layout as in oracle/__init__.py: full lattice checkerboard-major, x-fastest; gauge
+complex128[4, V, 3, 3] with the boundary conditions applied.
+"""
+import numpy as np
+
+
+def neighbour_tables(L):
+    """fwd[mu][s], bwd[mu][s]: full-lattice index of s +- mu."""
+    L = tuple(int(v) for v in L)
+    V = int(np.prod(L))
+    Vh = V // 2
+    s = np.arange(V)
+    cb = s // Vh
+    i = s % Vh
+    Lxh = L[0] // 2
+    xh = i % Lxh
+    r = i // Lxh
+    y = r % L[1]
+    r //= L[1]
+    z = r % L[2]
+    t = r // L[2]
+    x = 2 * xh + ((cb + y + z + t) & 1)
+    X = np.stack([x, y, z, t], axis=1)
+
+    def index(C):
+        c = (C.sum(axis=1)) & 1
+        return c * Vh + (((C[:, 3] * L[2] + C[:, 2]) * L[1] + C[:, 1]) * L[0] + C[:, 0]) // 2
+
+    fwd, bwd = [], []
+    for mu in range(4):
+        C = X.copy()
+        C[:, mu] = (C[:, mu] + 1) % L[mu]
+        fwd.append(index(C))
+        C = X.copy()
+        C[:, mu] = (C[:, mu] - 1) % L[mu]
+        bwd.append(index(C))
+    return np.array(fwd), np.array(bwd)
+
+
+def _transports(L, U, v_full, sites):
+    """t[p][k] = U_p(y) v(y+p) for y = sites[k]: array [8, len(sites), 4, 3]."""
+    fwd, bwd = neighbour_tables(L)
+    out = np.zeros((8, len(sites), 4, 3), complex)
+    for mu in range(4):
+        yf = fwd[mu][sites]
+        out[2 * mu] = np.einsum("kab,ksb->ksa", U[mu, sites], v_full[yf])
+        yb = bwd[mu][sites]
+        out[2 * mu + 1] = np.einsum("kba,ksb->ksa", U[mu, yb].conj(), v_full[yb])
+    return out
+
+
+def ptc_apply(L, U, W1, W2, W0, v_odd, return_parts=False):
+    """P(v) on the odd checkerboard (v_odd: [Vh, 4, 3])."""
+    V = int(np.prod(L))
+    Vh = V // 2
+    full = np.zeros((V, 4, 3), complex)
+    full[Vh:] = v_odd
+    even = np.arange(Vh)
+    t = _transports(L, U, full, even)                       # [8, Vh, 4, 3]
+    tk = t.transpose(1, 0, 2, 3).reshape(Vh, 32, 3)          # (p*4 + t) components
+    Wc = W2 @ W1
+    g = np.einsum("qp,kpc->kqc", Wc, tk).reshape(Vh, 8, 4, 3)  # g_q(y)
+    gfull = np.zeros((8, V, 4, 3), complex)
+    gfull[:, :Vh] = g.transpose(1, 0, 2, 3)
+    fwd, bwd = neighbour_tables(L)
+    odd = np.arange(Vh, V)
+    out = np.einsum("st,kta->ksa", W0, v_odd)
+    for mu in range(4):
+        yf = fwd[mu][odd]
+        out += np.einsum("kab,ksb->ksa", U[mu, odd], gfull[2 * mu, yf])
+        yb = bwd[mu][odd]
+        out += np.einsum("kba,ksb->ksa", U[mu, yb].conj(), gfull[2 * mu + 1, yb])
+    if return_parts:
+        return out, tk
+    return out
+
+
+def ptc_grad(L, U, W1, W2, W0, v_odd, e_odd):
+    """(|P v - e|^2, dL/d conj(Wc) [32, 32], dL/d conj(W0) [4, 4]) with Wc = W2 W1."""
+    V = int(np.prod(L))
+    Vh = V // 2
+    out, tk = ptc_apply(L, U, W1, W2, W0, v_odd, return_parts=True)
+    delta = out - e_odd
+    loss = float(np.vdot(delta, delta).real)
+    # adjoint of the scatter: dg_q(y) = U_q(x)^dag delta(x) for y = x + q
+    fwd, bwd = neighbour_tables(L)
+    odd = np.arange(Vh, V)
+    dg = np.zeros((8, V, 4, 3), complex)
+    for mu in range(4):
+        yf = fwd[mu][odd]
+        np.add.at(dg[2 * mu], yf, np.einsum("kba,ksb->ksa", U[mu, odd].conj(), delta))
+        yb = bwd[mu][odd]
+        np.add.at(dg[2 * mu + 1], yb, np.einsum("kab,ksb->ksa", U[mu, yb], delta))
+    dgk = dg[:, :Vh].transpose(1, 0, 2, 3).reshape(Vh, 32, 3)
+    S = np.einsum("kqc,kpc->qp", dgk, tk.conj())
+    S0 = np.einsum("ksc,ktc->st", delta, v_odd.conj())
+    return loss, S, S0

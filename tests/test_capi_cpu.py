@@ -88,3 +88,23 @@ def test_bench_refuses_multi_gpu():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+def test_learned_net_neumann_initialisation():
+    """The library's Neumann weights equal the ones built from the DR gammas in the oracle tests
+    (host-side call: no device needed)."""
+    from tests.test_oracle_learned import neumann
+    a = 4.0 + cmx.kappa_to_mass(0.135)
+    net = cmx.Net()
+    net.init_neumann(a)
+    w1, w2, w0 = net.get_weights()
+    e1, e2, e0 = neumann(a)
+    assert np.allclose(w1, e1, rtol=1e-6, atol=1e-7)
+    assert np.allclose(w2, e2, rtol=1e-6, atol=1e-9)
+    assert np.allclose(w0, e0, rtol=1e-6)
+    # set / get round trip
+    rng = np.random.default_rng(3)
+    r1 = rng.standard_normal((64, 32)) + 1j * rng.standard_normal((64, 32))
+    net.set_weights(r1, e2, e0)
+    g1, _, _ = net.get_weights()
+    assert np.allclose(g1, r1.astype(np.complex64), rtol=0, atol=0)

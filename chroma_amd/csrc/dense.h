@@ -9,6 +9,7 @@
 #include <complex>
 #include <numeric>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace cmx {
@@ -179,7 +180,17 @@ inline void eig(const Mat& A0, std::vector<cd>& vals, Mat& vecs) {
       iter = 0;
       continue;
     }
-    if (++iter > 200 * n) throw std::runtime_error("dense::eig: QR iteration did not converge");
+    if (++iter > 200 * n) {
+      int bad = 0;
+      double mx = 0;
+      for (const cd& z : A0.a) {
+        if (!std::isfinite(z.real()) || !std::isfinite(z.imag())) ++bad;
+        else mx = std::max(mx, std::abs(z));
+      }
+      throw std::runtime_error("dense::eig: QR iteration did not converge (order " + std::to_string(n) +
+                               ", active block " + std::to_string(lo) + ".." + std::to_string(hi) +
+                               ", non-finite entries " + std::to_string(bad) + ", max |a| " + std::to_string(mx) + ")");
+    }
     // Wilkinson shift from the trailing 2x2 block (exceptional shift every 11 iterations)
     const cd a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
     cd mu;

@@ -15,7 +15,9 @@
 //    does not match the Q_k^H transformation it has already applied to those rows);
 //  * each solve starts with Z = 0 (the reference keeps Z from the previous solve as the
 //    preconditioner's initial guesses, because it zeroes Z only at construction, :197-225);
-//  * the identity preconditioner is offered (the reference needs a factory solver).
+//  * the identity preconditioner is offered (the reference needs a factory solver);
+//  * a deflated cycle that fails to halve the true residual is followed by a plain restart (in FP64
+//    arithmetic this does not trigger: the oracle's cycles are reproduced exactly).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -96,6 +98,8 @@ struct KryDev {
     launch_maxpy(L, vs[0]->prec, w.prec, nslot >= 0, op.stream);
     if (nslot >= 0) launch_colsum(part.as<double>(), kry_blocks(n), 1, op.state_S.as<double>() + nslot, 1, op.stream);
   }
+  // The copy reads the pinned host mirror when the stream reaches it, so the mirror must not be
+  // rewritten before then: put() waits for the copy (callers issue several puts back to back).
   void put(int slot, const std::vector<cd>& c) {
     for (size_t i = 0; i < c.size(); ++i) {
       op.hS[slot + 2 * i] = c[i].real();
@@ -103,11 +107,13 @@ struct KryDev {
     }
     CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + slot, op.hS + slot, sizeof(double) * 2 * c.size(),
                            hipMemcpyHostToDevice, op.stream));
+    op.sync();
   }
   void put1(int slot, double v) {
     op.hS[slot] = v;
     CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + slot, op.hS + slot, sizeof(double),
                            hipMemcpyHostToDevice, op.stream));
+    op.sync();
   }
   void get(int slot, int count) {
     CMX_HIP(hipMemcpyAsync(op.hS + slot, static_cast<double*>(op.state_S.p) + slot, sizeof(double) * count,
@@ -166,12 +172,20 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
               "FGMRES-DR: need 1 <= NKrylov, 0 <= NDefl < NKrylov, NKrylov + NDefl < 48");
   CMX_REQUIRE(p.precond >= CMX_PRECOND_NONE && p.precond <= CMX_PRECOND_LEARNED, CMX_ERR_ARG,
               "FGMRES-DR: unknown preconditioner");
+  // basis precision pb (V, Z) may be wider than the precision ps of the operator and preconditioner
+  const int pb = p.basis_prec < 0 ? ps : p.basis_prec;
+  CMX_REQUIRE(pb == P64 || pb == P32 || pb == P16, CMX_ERR_ARG, "FGMRES-DR: bad basis precision");
   const int total = nk + nd_max;
   std::vector<std::unique_ptr<Vec>> V(total + 1), Z(total);
-  for (auto& v : V) v.reset(new Vec(ps, n));
-  for (auto& z : Z) z.reset(new Vec(ps, n));
+  for (auto& v : V) v.reset(new Vec(pb, n));
+  for (auto& z : Z) z.reset(new Vec(pb, n));
   for (auto& z : Z) op.blas(B_ZERO, {z.get()});
   Vec r(P64, n), tmp(P64, n);
+  std::unique_ptr<Vec> xs, ys;  // operator-precision staging when pb != ps
+  if (pb != ps) {
+    xs.reset(new Vec(ps, n));
+    ys.reset(new Vec(ps, n));
+  }
   KryDev kd(op, n);
   std::unique_ptr<MRPrecond> mr;
   if (p.precond == CMX_PRECOND_MR)
@@ -210,18 +224,40 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   bool finished = r_norm <= target;
   std::vector<Vec*> vs(total + 1);
 
-  auto precondition = [&](Vec& z, const Vec& v) {
+  auto precondition_s = [&](Vec& z, const Vec& v) {  // both in precision ps
     switch (p.precond) {
       case CMX_PRECOND_NONE: op.blas(B_COPY, {&z, &v}); break;
       case CMX_PRECOND_MR: mr->apply(z, v); break;
       default: learned_precond_apply(op, ps, v, z); break;
     }
   };
+  auto precondition = [&](Vec& z, const Vec& v) {  // z, v in the basis precision
+    if (pb == ps || p.precond == CMX_PRECOND_NONE) {
+      if (pb == ps) precondition_s(z, v);
+      else op.blas(B_COPY, {&z, &v});
+      return;
+    }
+    op.blas(B_COPY, {xs.get(), &v});
+    if (p.precond == CMX_PRECOND_MR) op.blas(B_COPY, {ys.get(), &z});  // MR starts from z
+    precondition_s(*ys, *xs);
+    op.blas(B_COPY, {&z, ys.get()});
+  };
+  auto apply_a = [&](Vec& w, const Vec& z) {  // w = A z at the operator precision
+    if (pb == ps) {
+      op.apply(ps, w.view(), z.view(), +1, nullptr);
+      return;
+    }
+    op.blas(B_COPY, {xs.get(), &z});
+    op.apply(ps, ys->view(), xs->view(), +1, nullptr);
+    op.blas(B_COPY, {&w, ys.get()});
+  };
 
+  bool plain_next = true;  // the first cycle, and any cycle after a deflated one that stalled
+  double r_prev = r_norm;
   while (!finished) {
     ++cycles;
     int nd = nd_max;
-    if (cycles == 1 || nd_max == 0) {
+    if (plain_next || nd_max == 0) {
       // c = g = [|r|, 0, ...]; V_0 = r / |r|  (:491-513)
       std::fill(c.begin(), c.end(), cd(0));
       std::fill(g.begin(), g.end(), cd(0));
@@ -296,7 +332,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       std::fill(g.begin(), g.end(), cd(0));
       {
         // <V_i, r> with r in FP64: convert r once to the basis precision
-        Vec rs(ps, n);
+        Vec rs(pb, n);
         op.blas(B_COPY, {&rs, &r});
         for (int i = 0; i <= nd; ++i) vs[i] = V[i].get();
         kd.mdot(vs, nd + 1, rs, K_H);
@@ -319,11 +355,13 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       for (int i = 0; i <= nd; ++i) g[i] = gq[i];
     }
 
-    // flexible Arnoldi (:78-183)
+    // flexible Arnoldi (:78-183); with cycle_reduction > 0 the cycle also ends once its estimate has
+    // fallen by that factor below the residual it started from
+    const double cycle_stop = std::max(target, p.cycle_reduction > 0 ? p.cycle_reduction * r_norm : 0.0);
     int dim = nd;
     for (int j = nd; j < nd + nk; ++j) {
       precondition(*Z[j], *V[j]);                                     // z_j = M v_j
-      op.apply(ps, V[j + 1]->view(), Z[j]->view(), +1, nullptr);      // w = A z_j
+      apply_a(*V[j + 1], *Z[j]);                                     // w = A z_j
       for (int i = 0; i <= j; ++i) vs[i] = V[i].get();
       kd.mdot(vs, j + 1, *V[j + 1], K_H);                             // h_i = <v_i, w>
       kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
@@ -360,7 +398,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       dim = j + 1;
       const double accum = std::fabs(g[j + 1].real());  // (:172)
       if (p.verbose) std::printf("FGMRESDR: iter %d |r| ~ %g target %g\n", iters + j - nd + 1, accum, target);
-      if (accum <= target) break;
+      if (accum <= cycle_stop) break;
     }
     // eta = R^-1 g by back substitution (:240-262); psi += sum eta_j Z_j (:716-724)
     if (dim > 0) {
@@ -375,7 +413,12 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     }
     r_norm = true_residual();  // (:727-736)
     iters += dim - nd;
-    finished = r_norm <= target || iters >= p.max_iter || dim == nd;
+    // With a basis narrower than FP64 the true residual is not exactly in the span of the deflated
+    // basis, and a deflated cycle can stall on the part outside it; the next cycle then restarts
+    // from the true residual (a plain FGMRES cycle).
+    plain_next = nd > 0 && r_norm > 0.5 * r_prev;
+    r_prev = r_norm;
+    finished = r_norm <= target || iters >= p.max_iter || (dim == nd && !plain_next);
     prev_dim = dim;
     prev_nd = nd;
     if (p.verbose)

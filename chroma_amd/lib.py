@@ -38,13 +38,15 @@ class SolverParams(ctypes.Structure):
                 ("delta", ctypes.c_double), ("sloppy_prec", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("verbose", ctypes.c_int), ("true_residual", ctypes.c_int), ("n_krylov", ctypes.c_int),
                 ("n_defl", ctypes.c_int), ("precond", ctypes.c_int), ("precond_iters", ctypes.c_int),
-                ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int)]
+                ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int),
+                ("basis_prec", ctypes.c_int), ("cycle_reduction", ctypes.c_double)]
 
 
 class TrainParams(ctypes.Structure):
     _fields_ = [("n_sources", ctypes.c_int), ("snapshot_every", ctypes.c_int), ("solve_rsd", ctypes.c_double),
                 ("epochs", ctypes.c_int), ("lr", ctypes.c_double), ("seed", ctypes.c_ulonglong),
-                ("max_pairs", ctypes.c_int), ("prec", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("max_pairs", ctypes.c_int), ("prec", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("pairs_per_source", ctypes.c_int), ("normalize", ctypes.c_int)]
 
 
 class TrainResult(ctypes.Structure):
@@ -282,7 +284,8 @@ class Operator:
 
     def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
               check_every=0, verbose=False, true_residual=False, mdagm=False, n_krylov=16, n_defl=0,
-              precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1):
+              precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1,
+              basis_prec=-1, cycle_reduction=0.0):
         """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays.
 
         mdagm=True solves M^dag M psi = chi (MdagMSystemSolver, lib/syssolver.h:72-84)."""
@@ -293,7 +296,8 @@ class Operator:
         nrhs = chis.shape[0]
         p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
                          int(bool(verbose)), int(bool(true_residual)), int(n_krylov), int(n_defl), int(precond),
-                         int(precond_iters), float(precond_rsd), float(precond_omega), int(reorth))
+                         int(precond_iters), float(precond_rsd), float(precond_omega), int(reorth), int(basis_prec),
+                         float(cycle_reduction))
         res = (SolverResult * nrhs)()
         fn = lib().cmx_solve_mdagm if mdagm else lib().cmx_solve
         check(fn(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))
@@ -361,9 +365,9 @@ def net_grad(op, v, e, prec=SINGLE):
 
 
 def net_train(op, n_sources=4, snapshot_every=4, solve_rsd=1e-9, epochs=4, lr=1e-3, seed=1, max_pairs=64,
-              prec=SINGLE, verbose=False):
+              prec=SINGLE, verbose=False, pairs_per_source=0, normalize=True):
     p = TrainParams(int(n_sources), int(snapshot_every), float(solve_rsd), int(epochs), float(lr), int(seed),
-                    int(max_pairs), int(prec), int(bool(verbose)))
+                    int(max_pairs), int(prec), int(bool(verbose)), int(pairs_per_source), int(bool(normalize)))
     r = TrainResult()
     check(lib().cmx_net_train(op.h, ctypes.byref(p), ctypes.byref(r)))
     return dict(n_pairs=r.n_pairs, n_steps=r.n_steps, loss_first=r.loss_first, loss_last=r.loss_last,

@@ -148,6 +148,11 @@ typedef struct {
   double precond_rsd;   /* MR: RsdMR                                                               */
   double precond_omega; /* MR: MROver (over-relaxation)                                            */
   int reorth;        /* Gram-Schmidt passes per Arnoldi step: 1 (classical) or 2                   */
+  int basis_prec;    /* FGMRES-DR storage of the Krylov bases V, Z (CMX_DOUBLE/SINGLE/HALF); -1: the
+                        sloppy precision.  The operator and preconditioner run at sloppy_prec.     */
+  double cycle_reduction; /* FGMRES-DR: end a cycle once its residual estimate has fallen by this
+                        factor (then the FP64 residual is recomputed and the next cycle starts from
+                        it, as a reliable update); 0: only at RsdTarget or NKrylov, as the reference */
 } cmx_solver_params;
 
 typedef struct {
@@ -199,12 +204,15 @@ typedef struct {
   int n_sources;      /* Gaussian sources solved by plain FP64 CGNE                                 */
   int snapshot_every; /* CG iterations between (residual, error) pairs                              */
   double solve_rsd;   /* CGNE target defining the "exact" solution x*                               */
-  int epochs;         /* passes over the pairs (one Adam step per pair)                             */
+  int epochs;         /* passes over the pairs (one Adam step per pair on |M (P r - e)|^2 / |r|^2)  */
   double lr;          /* Adam step size                                                             */
   unsigned long long seed;
   int max_pairs;
   int prec;           /* storage precision of the pairs and the network input: CMX_SINGLE/CMX_HALF  */
   int verbose;
+  int pairs_per_source; /* cap on the pairs taken from one source's iterates (0: no cap)          */
+  int normalize;        /* 1: each pair's loss divided by |r|^2; 0: raw |M (P r - e)|^2, so early,
+                           large residuals weigh more                                               */
 } cmx_train_params;
 typedef struct {
   int n_pairs, n_steps;

@@ -7,12 +7,13 @@ over any linear operator given as a Python callable on odd-checkerboard arrays.
 The small dense algebra (LU solve, QR, eigen-decomposition) is numpy.linalg; the reference calls
 LAPACK zgetrf/zgetrs/zgeev/zgeqrf/zunmqr/zungqr at the same points.
 
-Two choices differ from the reference on purpose (the library makes the same ones):
+Three choices differ from the reference on purpose (the library makes the same ones):
   * in a deflated cycle only the current cycle's Givens rotations are applied to a new column
     (the reference also applies the previous cycle's rotations of the deflated columns, :151-153);
   * every solve starts with Z = 0 (the reference keeps Z across solves, zeroing it only at
     construction, :197-225); within a solve, a plain restart passes the previous cycle's Z_j to the
-    preconditioner as its initial guess, as the reference does.
+    preconditioner as its initial guess, as the reference does;
+  * a deflated cycle that fails to halve the true residual is followed by a plain restart.
 """
 import numpy as np
 
@@ -90,10 +91,11 @@ def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None,
     Qk = None
     iters, cycles, prev_dim = 0, 0, nk
     finished = r_norm <= target
+    plain_next, r_prev = True, r_norm
     while not finished:
         cycles += 1
         nd = nd_max
-        if cycles == 1 or nd_max == 0:
+        if plain_next or nd_max == 0:
             c[:] = 0
             g[:] = 0
             c[0] = g[0] = r_norm
@@ -172,6 +174,8 @@ def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None,
         r = chi - A(psi)
         r_norm = _norm(r)
         iters += dim - nd
-        finished = r_norm <= target or iters >= max_iter or dim == nd
+        plain_next = nd > 0 and r_norm > 0.5 * r_prev
+        r_prev = r_norm
+        finished = r_norm <= target or iters >= max_iter or (dim == nd and not plain_next)
         prev_dim = dim
     return psi, iters, r_norm, cycles

@@ -99,8 +99,8 @@ def test_learned_net_neumann_initialisation():
     net.init_neumann(a)
     w1, w2, w0 = net.get_weights()
     e1, e2, e0 = neumann(a)
-    assert np.allclose(w1, e1, rtol=1e-6, atol=1e-7)
-    assert np.allclose(w2, e2, rtol=1e-6, atol=1e-9)
+    # the factorisation W2 W1 is not unique; the contracted weights are what the network applies
+    assert np.allclose(w2 @ w1, e2 @ e1, rtol=1e-5, atol=1e-8)
     assert np.allclose(w0, e0, rtol=1e-6)
     # set / get round trip
     rng = np.random.default_rng(3)

@@ -310,7 +310,7 @@ def test_fgmres_dr_vs_oracle(n_defl, precond):
     psi_ref, n_ref, r_ref, cyc_ref = krylov.fgmres_dr(A, chi, np.zeros_like(chi), 1e-10, n_krylov=12,
                                                       n_defl=n_defl, max_iter=3000, precond=pre)
     psi, res = op.solve(chi, solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=cmx.DOUBLE, n_krylov=12,
-                        n_defl=n_defl, precond=precond, precond_iters=3, precond_rsd=1e-12, reorth=2)
+                        n_defl=n_defl, precond=precond, precond_iters=3, precond_rsd=1e-12, reorth=2, verbose=True)
     say("  n_count", res["n_count"], "oracle", n_ref, "cycles", res["n_reliable"], "oracle", cyc_ref,
         "rel resid", res["rel_resid"])
     assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)

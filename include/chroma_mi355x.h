@@ -216,7 +216,8 @@ typedef struct {
 } cmx_train_params;
 typedef struct {
   int n_pairs, n_steps;
-  double loss_first, loss_last; /* relative loss |P r - e|^2 / |e|^2: first step, mean of last epoch */
+  double loss_first, loss_last; /* the objective as optimised (mean weighted loss over the pairs)
+                                   before and after training                                     */
   double gen_seconds, train_seconds;
 } cmx_train_result;
 int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);

@@ -451,6 +451,13 @@ int cmx_op_set_net(cmx_op op, cmx_net net) {
   });
 }
 
+int cmx_op_set_net_variant(cmx_op op, int prec, int variant) {
+  return guarded([&] {
+    require_init();
+    cmx::net_set_variant(O(op), prec, variant);
+  });
+}
+
 int cmx_net_apply(cmx_op op, int prec, double* out, const double* in) {
   return guarded([&] {
     require_init();

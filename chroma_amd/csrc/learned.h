@@ -34,7 +34,8 @@ struct PtcLaunch {
 int ptc_waves();
 void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
-void launch_ptc_gather_mix(const PtcLaunch& L, hipStream_t s);  // inference: T never leaves registers
+// inference: variant 0 operands in registers, 1 transports staged in LDS
+void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s);
 void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s);
 void launch_ptc_grad(const PtcLaunch& L, double* out4096, hipStream_t s);
 int launch_w0_grad(const PtcLaunch& L, hipStream_t s);

@@ -16,6 +16,7 @@
 //   k_ptc_mix      (MFMA)        g = Wc t per (site, colour) column
 //   k_ptc_gather_mix             both of the above in one kernel, transports packed straight into the
 //                                MFMA operands (inference path)
+//   k_ptc_gather_mix_lds         the same with the transports staged in LDS tiles
 //   k_ptc_scatter  (odd sites)   P(v) = W0 v + sum_q U_q g_q
 //   k_ptc_grad     (MFMA)        S = sum_{y,c} dg(y,c) t(y,c)^dag, the gradient of |P(v) - e|^2 w.r.t. conj(Wc)
 //   k_w0_grad                    sum_x delta(x) v(x)^dag, the gradient w.r.t. conj(W0)
@@ -275,6 +276,105 @@ __global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
   }
 }
 
+// Fused gather + mix for inference: a work-group of 128 threads transports its 128 even sites into an
+// LDS copy of their four T tiles (48 KiB), then its two waves run the MFMA mix of those 12 (block,
+// colour) tiles and write G.  T never goes to HBM.
+template <int P, int RECON>
+__global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
+  __shared__ bf4 lds[4 * 3 * 8 * 64];
+  const Geom& g = L.g;
+  using R = typename SpinorIO<P>::R;
+  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
+  const int yi = blockIdx.x * 128 + threadIdx.x;
+  const int lb = threadIdx.x >> 5, n = threadIdx.x & 31;  // local block, site within the block
+  auto lds_at = [&](int c, int ks, int h) -> bf4& { return lds[((lb * 3 + c) * 8 + ks) * 64 + h * 32 + n]; };
+  if (yi < g.Vh) {
+    int x[4];
+    ptc_coords(g, 0, yi, x);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const bool fwd = d == 0;
+        bool last;
+        const int yn = ptc_neighbour(g, x, mu, fwd, last);
+        R v[24], u[18], t[24];
+        SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
+        if (fwd) {
+          const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)mu * LCH * LCB * g.Vh;
+          LinkIO<P, RECON>::load(gb, g.Vh, yi, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
+          transport4<false>(u, v, t);
+        } else {
+          const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)mu * LCH * LCB * g.Vh;
+          LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
+          transport4<true>(u, v, t);
+        }
+        const int p = 2 * mu + d;
+        const int ks = p >> 1, h = p & 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bf4 re, im;
+          re[0] = (short)f2bf((float)t[2 * (0 + c)]);
+          re[1] = (short)f2bf((float)t[2 * (3 + c)]);
+          re[2] = (short)f2bf((float)t[2 * (6 + c)]);
+          re[3] = (short)f2bf((float)t[2 * (9 + c)]);
+          im[0] = (short)f2bf((float)t[2 * (0 + c) + 1]);
+          im[1] = (short)f2bf((float)t[2 * (3 + c) + 1]);
+          im[2] = (short)f2bf((float)t[2 * (6 + c) + 1]);
+          im[3] = (short)f2bf((float)t[2 * (9 + c) + 1]);
+          lds_at(c, ks, h) = re;
+          lds_at(c, 4 + ks, h) = im;
+        }
+      }
+    }
+  } else {  // sites past the end of the lattice contribute zero columns
+    const bf4 z = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) lds_at(c, ks, h) = z;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bf4 a[2][8];
+  const bf4* A = reinterpret_cast<const bf4*>(L.Afr);
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) a[mt][ks] = A[(mt * 8 + ks) * 64 + lane];
+  for (int tile = wave; tile < 12; tile += 2) {  // (local block, colour)
+    const int tb = tile / 3, c = tile - 3 * tb;
+    f16v acc0 = {0}, acc1 = {0};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const bf4 b = lds[((tb * 3 + c) * 8 + ks) * 64 + lane];
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[0][ks], b, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[1][ks], b, acc1, 0, 0, 0);
+    }
+    const int y = blockIdx.x * 128 + tb * 32 + (lane & 31);
+    if (y < g.Vh) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = 2 * j + (lane >> 5);
+        ushort4 lo, hi;
+        lo.x = f2bf(acc0[4 * j + 0]);
+        lo.y = f2bf(acc1[4 * j + 0]);
+        lo.z = f2bf(acc0[4 * j + 1]);
+        lo.w = f2bf(acc1[4 * j + 1]);
+        hi.x = f2bf(acc0[4 * j + 2]);
+        hi.y = f2bf(acc1[4 * j + 2]);
+        hi.z = f2bf(acc0[4 * j + 3]);
+        hi.w = f2bf(acc1[4 * j + 3]);
+        ushort4* dst = reinterpret_cast<ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + y) * 8);
+        dst[0] = lo;
+        dst[1] = hi;
+      }
+    }
+  }
+}
+
 template <int P, int RECON>
 __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
   const int xi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,8 +559,15 @@ void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s) {
   });
 }
 
-void launch_ptc_gather_mix(const PtcLaunch& L, hipStream_t s) {
-  const dim3 grid((L.g.Vh + 127) / 128), block(256);  // 4 waves x 32 sites
+void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s) {
+  if (variant == 1) {  // transports staged in LDS
+    const dim3 grid((L.g.Vh + 127) / 128), block(128);
+    dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
+      hipLaunchKernelGGL((k_ptc_gather_mix_lds<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
+    });
+    return;
+  }
+  const dim3 grid((L.g.Vh + 127) / 128), block(256);  // 4 waves x 32 sites, operands in registers
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
     hipLaunchKernelGGL((k_ptc_gather_mix<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
   });

@@ -134,6 +134,7 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
 // Learned preconditioner (learned.cpp)
 void net_init_neumann(LearnedNet& net, double diag);
 void net_apply_host(Op& op, int prec, double* out, const double* in);
+void net_set_variant(Op& op, int prec, int variant);
 void net_grad_host(Op& op, int prec, const double* v, const double* e, double* loss, double* gwc, double* gw0);
 void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
 

@@ -102,6 +102,7 @@ def lib():
         "cmx_net_set_weights": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
         "cmx_op_set_net": (ctypes.c_int, [_vp, _vp]),
         "cmx_net_apply": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+        "cmx_op_set_net_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "cmx_net_grad": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
         "cmx_net_train": (ctypes.c_int, [_vp, ctypes.POINTER(TrainParams), ctypes.POINTER(TrainResult)]),
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
@@ -344,6 +345,10 @@ class Net:
 def net_attach(op, net):
     check(lib().cmx_op_set_net(op.h, net.h if net is not None else None))
     op.net = net
+
+
+def net_set_variant(op, prec, variant):
+    check(lib().cmx_op_set_net_variant(op.h, int(prec), int(variant)))
 
 
 def net_apply(op, v, prec=SINGLE):

@@ -196,6 +196,10 @@ int cmx_net_init_neumann(cmx_net net, double diag);
 int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0);
 int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0);
 int cmx_op_set_net(cmx_op op, cmx_net net);
+/* kernel path of the network's first layer at precision prec: 0 fused (MFMA operands in registers),
+ * 1 fused (transports staged in LDS), 2 separate gather and mix kernels, -1 time all three at the next
+ * application and keep the fastest (the default) */
+int cmx_op_set_net_variant(cmx_op op, int prec, int variant);
 /* out = P(in) on the odd checkerboard at storage precision prec (host arrays, Vh*24 doubles) */
 int cmx_net_apply(cmx_op op, int prec, double* out, const double* in);
 /* loss = |P(v) - e|^2; g_wc = d loss / d conj(W2 W1) (32 x 32 complex), g_w0 = d loss / d conj(W0) */

@@ -54,10 +54,12 @@ def test_net_forward(L, prec):
     net.set_weights(w1, w2, w0)
     cmx.net_attach(op, net)
     v = random_fermion(Vh, 72)
-    got = cmx.net_apply(op, v, prec)
     want = learned.ptc_apply(L, U, w1, w2, w0, v)
-    say("  rel", rel(got, want))
-    assert rel(got, want) < 2e-2
+    for variant in (0, 1, 2):  # fused (registers), fused (LDS), separate kernels
+        cmx.net_set_variant(op, prec, variant)
+        got = cmx.net_apply(op, v, prec)
+        say("  variant", variant, "rel", rel(got, want))
+        assert rel(got, want) < 2e-2
 
 
 def test_net_forward_neumann():

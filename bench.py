@@ -161,12 +161,17 @@ def main():
         ("eo_clover_op_fp16", 1, cmx.HALF, eo_clover_bytes(2, 12) * Vh),
         ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
         ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
+        # 12 sources: each spinor read and written once per source, each link once
+        ("hop_fp16_recon12_12src", 4, cmx.HALF, (12 * (48 + 4 + 48 + 4) + 8 * 12 * 2) * Vh),
     ]:
         op.time(what, prec, 5)
         t = op.time(what, prec, a.steps)
         extra[name] = {"us": round(t * 1e3, 2), "GB/s": round(nbytes / (t * 1e-3) / 1e9, 1)}
         print(f"# {name}: {t * 1e3:.1f} us  {extra[name]['GB/s']} GB/s", file=sys.stderr, flush=True)
     extra["hop_fp16_recon12"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1)}
+    m12 = extra["hop_fp16_recon12_12src"]
+    m12["us_per_source"] = round(m12["us"] / 12, 2)
+    m12["speedup_vs_12_single"] = round(12 * ms * 1e3 / m12["us"], 3)
     extra["hop_gflops_fp16"] = round(1320.0 * Vh / (ms * 1e-3) / 1e9, 1)
 
     solves = {}

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "op.h"
 #include "setup.h"
@@ -138,6 +139,31 @@ int cmx_dslash_apply(cmx_op h, double* chi, const double* psi, int isign, int cb
     op.pack(psi, n, in);
     op.dslash(prec, out.view(), in.view(), isign, cb);
     op.unpack(chi, n, out);
+  });
+}
+
+int cmx_dslash_multi(cmx_op h, int nrhs, double* chi, const double* psi, int isign, int cb, int prec) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(chi && psi && nrhs >= 1, CMX_ERR_ARG, "dslash_multi: bad argument");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "dslash_multi: isign must be +1 or -1");
+    CMX_REQUIRE(cb == 0 || cb == 1, CMX_ERR_ARG, "dslash_multi: cb must be 0 or 1");
+    CMX_REQUIRE(prec >= 0 && prec <= 2, CMX_ERR_ARG, "dslash_multi: bad precision");
+    const int n = (int)op.Vh;
+    const size_t nd = (size_t)24 * n;
+    std::vector<std::unique_ptr<cmx::Vec>> in, out;
+    std::vector<cmx::VRef> vi, vo;
+    for (int k = 0; k < nrhs; ++k) {
+      in.emplace_back(new cmx::Vec(prec, n));
+      out.emplace_back(new cmx::Vec(prec, n));
+      op.pack(psi + k * nd, n, *in.back());
+      vi.push_back(in.back()->view());
+      vo.push_back(out.back()->view());
+    }
+    if (!op.dslash_multi(prec, vo.data(), vi.data(), nrhs, isign, cb))
+      for (int k = 0; k < nrhs; ++k) op.dslash(prec, vo[k], vi[k], isign, cb);
+    for (int k = 0; k < nrhs; ++k) op.unpack(chi + k * nd, n, *out[k]);
   });
 }
 
@@ -298,15 +324,27 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
     require_init();
     cmx::Op& op = O(h);
     CMX_REQUIRE(ms_per_call && n > 0 && prec >= 0 && prec <= 2, CMX_ERR_ARG, "op_time: bad argument");
-    CMX_REQUIRE(what >= 0 && what <= 3, CMX_ERR_ARG, "op_time: bad selector");
+    CMX_REQUIRE(what >= 0 && what <= 4, CMX_ERR_ARG, "op_time: bad selector");
     const int Vh = (int)op.Vh;
     const int ns = op.subset_sites();
     cmx::Vec a(prec, ns), b(prec, ns);
+    // what = 4: the hopping term on 12 sources (distinct inputs and outputs)
+    std::vector<std::unique_ptr<cmx::Vec>> ma, mb;
+    std::vector<cmx::VRef> va, vb;
+    if (what == 4) {
+      for (int k = 0; k < 12; ++k) {
+        ma.emplace_back(new cmx::Vec(prec, Vh));
+        mb.emplace_back(new cmx::Vec(prec, Vh));
+        va.push_back(ma.back()->view());
+        vb.push_back(mb.back()->view());
+      }
+    }
     // deterministic non-zero input
     {
       std::unique_ptr<double[]> hbuf(new double[(size_t)24 * ns]);
       for (long k = 0; k < 24L * ns; ++k) hbuf[k] = std::sin(0.37 * (double)k) * 0.5;
       op.pack(hbuf.get(), ns, a);
+      for (auto& v : ma) op.pack(hbuf.get(), Vh, *v);
     }
     op.ensure_prec(prec);
     auto body = [&]() {
@@ -330,6 +368,10 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
           break;
         }
         case 3: cmx::stream_copy((long)cmx::spinor_chunks(prec) * Vh, a.buf.p, b.buf.p, op.stream); break;
+        case 4:
+          if (!op.dslash_multi(prec, vb.data(), va.data(), 12, +1, 1))
+            for (int k = 0; k < 12; ++k) op.dslash(prec, vb[k], va[k], +1, 1);
+          break;
       }
     };
     body();  // warm-up

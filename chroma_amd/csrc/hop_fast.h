@@ -124,9 +124,46 @@ template <int SS> __device__ __forceinline__ hf2 proj_addi(hf2 a, hf2 b) {  // a
   return SS > 0 ? a + ib : a - ib;
 }
 
+// third link row c = conj(u0 x u1) in FP32 from the FP16 rows (the boundary sign is applied by the caller)
+__device__ __forceinline__ void fast_row2_h(const FastLink<P16>& lk, float (&cr)[3], float (&ci)[3]) {
+  hf2 u[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) u[k] = as_hf2(lk.c[k]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    // chains start from fma(., ., +0): with signed zeros honoured this is not folded into a multiply,
+    // so FP16 operands stay in mixed-precision FMAs instead of being converted first
+    float xr = __fmaf_rn((float)u[a].x, (float)u[3 + b].x, 0.0f);
+    xr = __fmaf_rn(-(float)u[a].y, (float)u[3 + b].y, xr);
+    xr = __fmaf_rn(-(float)u[b].x, (float)u[3 + a].x, xr);
+    xr = __fmaf_rn((float)u[b].y, (float)u[3 + a].y, xr);
+    float xi = __fmaf_rn((float)u[a].x, (float)u[3 + b].y, 0.0f);
+    xi = __fmaf_rn((float)u[a].y, (float)u[3 + b].x, xi);
+    xi = __fmaf_rn(-(float)u[b].x, (float)u[3 + a].y, xi);
+    xi = __fmaf_rn(-(float)u[b].y, (float)u[3 + a].x, xi);
+    cr[k] = xr;
+    ci[k] = -xi;
+  }
+}
+
+template <int MU, bool FWD, int ISIGN>
+__device__ __forceinline__ void fast_step_h_body(const FastSpinor<P16>& ps, const FastLink<P16>& lk,
+                                                 const float (&cr)[3], const float (&ci)[3], float sg,
+                                                 float (&acc)[24]);
+
 template <int MU, bool FWD, int ISIGN>
 __device__ __forceinline__ void fast_step_h(const FastSpinor<P16>& ps, const FastLink<P16>& lk, float sg,
                                             float (&acc)[24]) {
+  float cr[3], ci[3];
+  fast_row2_h(lk, cr, ci);
+  fast_step_h_body<MU, FWD, ISIGN>(ps, lk, cr, ci, sg, acc);
+}
+
+template <int MU, bool FWD, int ISIGN>
+__device__ __forceinline__ void fast_step_h_body(const FastSpinor<P16>& ps, const FastLink<P16>& lk,
+                                                 const float (&cr)[3], const float (&ci)[3], float sg,
+                                                 float (&acc)[24]) {
   constexpr int S = FWD ? -ISIGN : ISIGN;  // (1 + S gamma_mu)
   hf2 h[2][3];
 #pragma unroll
@@ -149,24 +186,6 @@ __device__ __forceinline__ void fast_step_h(const FastSpinor<P16>& ps, const Fas
   hf2 u[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) u[k] = as_hf2(lk.c[k]);
-  // third row c = conj(u0 x u1) in FP32 from FP16 entries
-  float cr[3], ci[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int a = (k + 1) % 3, b = (k + 2) % 3;
-    // chains start from fma(., ., +0): with signed zeros honoured this is not folded into a multiply,
-    // so FP16 operands stay in mixed-precision FMAs instead of being converted first
-    float xr = __fmaf_rn((float)u[a].x, (float)u[3 + b].x, 0.0f);
-    xr = __fmaf_rn(-(float)u[a].y, (float)u[3 + b].y, xr);
-    xr = __fmaf_rn(-(float)u[b].x, (float)u[3 + a].x, xr);
-    xr = __fmaf_rn((float)u[b].y, (float)u[3 + a].y, xr);
-    float xi = __fmaf_rn((float)u[a].x, (float)u[3 + b].y, 0.0f);
-    xi = __fmaf_rn((float)u[a].y, (float)u[3 + b].x, xi);
-    xi = __fmaf_rn(-(float)u[b].x, (float)u[3 + a].y, xi);
-    xi = __fmaf_rn(-(float)u[b].y, (float)u[3 + a].x, xi);
-    cr[k] = xr;
-    ci[k] = -xi;
-  }
   const float n = ps.n;
   float orr[2][3], oi[2][3];
 #pragma unroll

@@ -1,5 +1,6 @@
 // Operator and gauge handles: device storage, precision copies and the fused even-odd operator.
 #include "op.h"
+#include "hop_multi.h"
 
 #include <algorithm>
 #include <cmath>
@@ -324,6 +325,55 @@ void Op::dslash(int prec, VRef out, VRef in, int isign, int cb, double b) {
   h.epi = EPI_NONE;
   h.b = b;
   launch_hop(h, stream);
+}
+
+bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isign, int cb, double b) {
+  if (prec != P16 || recon[P16] != 12) return false;
+  ensure_prec(prec);
+  MultiHopLaunch M;
+  std::memset(&M, 0, sizeof(M));
+  M.h = hop_base(prec, 1);
+  HopConfig c = hcfg[prec][1];
+  c.fast = 1;  // the multi-source kernel is a plane-grid kernel whatever the single-source choice
+  hop_apply_config(M.h, c);
+  if (!M.h.fast) return false;
+  M.h.isign = isign;
+  M.h.cb = cb;
+  M.h.epi = EPI_NONE;
+  M.h.b = b;
+  auto run = [&]() {
+    for (int r0 = 0; r0 < n; r0 += MULTI_MAXR) {
+      M.nrhs = std::min(MULTI_MAXR, n - r0);
+      for (int k = 0; k < M.nrhs; ++k) {
+        M.in[k] = in[r0 + k];
+        M.out[k] = out[r0 + k];
+      }
+      launch_hop_multi(M, stream);
+    }
+  };
+  if (multi_group == 0) {  // first use: time groups of 2, 3 and 4 sources (registers vs link reuse)
+    hipEvent_t e0, e1;
+    CMX_HIP(hipEventCreate(&e0));
+    CMX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int gsz = 2; gsz <= 4; ++gsz) {
+      M.group = gsz;
+      run();
+      CMX_HIP(hipEventRecord(e0, stream));
+      run();
+      run();
+      CMX_HIP(hipEventRecord(e1, stream));
+      CMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) best = ms, multi_group = gsz;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  M.group = multi_group;
+  run();
+  return true;
 }
 
 void Op::upload(const double* host, size_t n) {

@@ -77,6 +77,7 @@ struct Op {
   // hopping-kernel launch shape per precision: [0] the EO operator's fused kernels, [1] the plain
   // hopping term (tuned separately by autotune(): their best shapes differ)
   HopConfig hcfg[3][2];
+  int multi_group = 0;  // sources per multi-source hop kernel; 0: timed at first use
   // per precision: links (per parity) and the clover blocks used by the EO operator
   DevBuf gauge[3][2];
   DevBuf clov_odd[3];     // A_oo
@@ -111,6 +112,9 @@ struct Op {
             const int* run = nullptr);
   // raw hopping term on output checkerboard cb (views of one checkerboard each), scaled by b
   void dslash(int prec, VRef out, VRef in, int isign, int cb, double b = 1.0);
+  // the same on n sources at once (FP16 storage: links loaded once per group of sources); false when
+  // the multi-source kernel does not apply (then the caller uses dslash per source)
+  bool dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isign, int cb, double b = 1.0);
   // stage a host array (n doubles) into the device staging buffer and back
   void upload(const double* host, size_t n);
   void download(double* host, size_t n);

@@ -88,6 +88,7 @@ def lib():
         "cmx_op_apply": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
         "cmx_dslash_apply": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "cmx_op_eo_piece": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+        "cmx_dslash_multi": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "cmx_clover_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "cmx_clover_logdet": (ctypes.c_int, [_vp, _dp, _lp]),
         "cmx_clover_pack_quda": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
@@ -234,6 +235,14 @@ class Operator:
         psi_cb = np.ascontiguousarray(psi_cb, np.complex128)
         chi = np.zeros_like(psi_cb)
         check(lib().cmx_dslash_apply(self.h, _d(chi), _d(psi_cb), int(isign), int(cb), int(prec)))
+        return chi
+
+    def dslash_multi(self, psis_cb, isign, cb, prec=HALF):
+        """D on n sources at once: psis_cb complex[n, Vh, 4, 3]."""
+        psis_cb = np.ascontiguousarray(psis_cb, np.complex128)
+        chi = np.zeros_like(psis_cb)
+        check(lib().cmx_dslash_multi(self.h, int(psis_cb.shape[0]), _d(chi), _d(psis_cb), int(isign), int(cb),
+                                     int(prec)))
         return chi
 
     def piece(self, which, psi_full, isign=PLUS):

@@ -105,6 +105,11 @@ int cmx_op_apply(cmx_op op, double* chi, const double* psi, int isign, int prec)
 /* Raw hopping term chi_cb = D(isign) psi_{1-cb} (QDPWilsonDslashT::apply, lwldslash_w.h:252-295).
  * chi: Vh*24 doubles of checkerboard cb, psi: Vh*24 doubles of checkerboard 1-cb. */
 int cmx_dslash_apply(cmx_op op, double* chi, const double* psi, int isign, int cb, int prec);
+/* The hopping term on nrhs sources at once (the 12 spin-colour sources of quarkProp4_a,
+ * quarkprop4_w.cc:70-117): chi[k] = D psi[k], consecutive Vh*24-double arrays.  At FP16 storage with
+ * 12-real links and real boundary phases each link is loaded once per group of 4 sources; otherwise the
+ * single-source kernel runs per source.  Results equal nrhs calls of cmx_dslash_apply. */
+int cmx_dslash_multi(cmx_op op, int nrhs, double* chi, const double* psi, int isign, int cb, int prec);
 
 /* Even-odd blocks on full-lattice host arrays (V*24 doubles), FP64
  * (EvenOddPrecLinearOperator, lib/eoprec_linop.h:102-165).  which: 0 evenEven, 1 evenEvenInv,
@@ -239,7 +244,8 @@ int cmx_op_autotune(cmx_op op);
 /* ---- measurement -------------------------------------------------------------------------------
  * Average device milliseconds per call over n back-to-back launches on the operator's stream,
  * bracketed by HIP events.  what: 0 hopping term (cb 1), 1 EO operator kernel pair (PLUS),
- * 2 fused EO kernel 2 only, 3 streaming copy of one odd fermion (bandwidth reference). */
+ * 2 fused EO kernel 2 only, 3 streaming copy of one odd fermion (bandwidth reference), 4 the hopping
+ * term on 12 sources (cmx_dslash_multi; one call covers all 12). */
 int cmx_op_time(cmx_op op, int what, int prec, int n, double* ms_per_call);
 
 /* ---- layout ------------------------------------------------------------------------------------

@@ -339,6 +339,25 @@ def test_fgmres_dr_sloppy_basis(sloppy, n_defl):
     assert rel(psi, psi_cg) < 1e-8
 
 
+@pytest.mark.parametrize("L", [(8, 8, 8, 16), (16, 16, 8, 16), (6, 4, 8, 10)])
+@pytest.mark.parametrize("nrhs", [1, 5, 12])
+def test_dslash_multi_source(L, nrhs):
+    """The multi-source hop (links loaded once per group of sources) against the oracle, per source."""
+    say("dslash multi", L, nrhs)
+    V = int(np.prod(L))
+    Vh = V // 2
+    U = gauge_field(L, 21)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1, sloppy_prec=cmx.HALF, sloppy_recon=12)
+    psis = np.stack([random_fermion(V, 100 + k) for k in range(nrhs)])
+    for cb in (0, 1):
+        for isign in (+1, -1):
+            src = np.ascontiguousarray(psis[:, (1 - cb) * Vh:(2 - cb) * Vh])
+            got = op.dslash_multi(src, isign, cb, cmx.HALF)
+            for k in range(nrhs):
+                ref = oracle.dslash(L, U, psis[k], isign, cb)[cb * Vh:(cb + 1) * Vh]
+                assert rel(got[k], ref) < TOL[cmx.HALF], (cb, isign, k)
+
+
 def test_twisted_golden_through_gpu():
     """prec_wilson-twisted (tests/chroma/hadron/propagator/prec_wilson-twisted.{ini,out}.xml) on the GPU."""
     from chroma_amd import qprop as qp

@@ -68,6 +68,28 @@ def cpu_baseline(L, U, seconds_target=12.0):
             "gflops": round(1320.0 * Vh / t / 1e9, 3)}
 
 
+def small_volume_solves(cmx):
+    """The 8^3x16 unpreconditioned Wilson CGNE (kappa 0.115, FP64, point source at the origin, FP64 true
+    residual 1e-10): launch-bound, so the iterations are replayed from captured HIP graphs and timed
+    against eager launches."""
+    import numpy as np
+    L = (8, 8, 8, 16)
+    V = int(np.prod(L))
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=1, seed=1, eps=0.3), (1, 1, 1, -1))
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.UNPREC_WILSON, cmx.kappa_to_mass(0.115))
+    chi = np.zeros((V, 4, 3), np.complex128)
+    chi[0, 0, 0] = 1.0
+    out = {"lattice": list(L), "operator": "unpreconditioned Wilson, kappa 0.115, FP64 CGNE"}
+    for name, graph in [("eager", False), ("hip_graph", True)]:
+        for _ in range(2):
+            psi, res = op.solve(chi, solver=cmx.CG, rsd=1e-10, maxit=5000, true_residual=True, graph=graph)
+        out[name] = {"seconds": round(res["solve_seconds"], 5), "iterations": res["n_count"],
+                     "true_rel_resid": res["rel_resid"]}
+        print(f"# small volume CG {name}: {res['n_count']} it, {res['solve_seconds'] * 1e3:.2f} ms",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def learned_solve(cmx, op, L, kappa, csw, solves, a):
     """Learned preconditioner: trained on separate fields of the same recipe (Haar + 6 stout steps,
     seeds 10, 11, ...) on a smaller lattice, then used as the right preconditioner of FGMRES-DR on the
@@ -187,6 +209,10 @@ def main():
             print(f"# solve {name}: {res['n_count']} it, {res['solve_seconds']:.3f} s, rel resid "
                   f"{res['rel_resid']:.2e}", file=sys.stderr, flush=True)
 
+    small = None
+    if not a.no_solve:
+        small = small_volume_solves(cmx)
+
     learned = None
     if not a.no_solve and not a.no_learned:
         learned = learned_solve(cmx, op, L, kappa, csw, solves, a)
@@ -221,6 +247,7 @@ def main():
         "solve_time_plain_s": solves.get("plain_mixed_cg_fp16", {}).get("seconds"),
         "solve_time_learned_s": (learned or {}).get("seconds"),
         "learned": learned,
+        "small_volume": small,
         "solves": solves,
         "kernels": extra,
         "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}

@@ -69,25 +69,33 @@ __global__ void __launch_bounds__(256) k_hop_multi(MultiHopLaunch M, int r0) {
   for (int k = 0; k < NB; ++k)
 #pragma unroll
     for (int c = 0; c < 24; ++c) acc[k][c] = 0.f;
+  // raw loads of step STEP into buffer slot (STEP & 1); the next step's loads are issued before the
+  // current step is computed, so two steps of raw data are live at a time
+  FastLink<P16> lk[2];
+  FastSpinor<P16> sp[2][NB];
+  auto load_step = [&](int step, int slot) {
+    const int mu = step >> 1;
+    const bool fwd = (step & 1) == 0;
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(
+        static_cast<const char*>(h.gauge[fwd ? h.cb : 1 - h.cb]) + (size_t)mu * 24 * Vh, (unsigned)(24 * Vh));
+    fast_load_link<P16>(rl, Vh, fwd ? i : nb[step], lk[slot]);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) fast_load_spinor<P16>(rin[k], rnin[k], M.in[r0 + k].stride, nb[step], sp[slot][k]);
+  };
 #define CMX_MULTI_STEP(STEP)                                                                          \
   {                                                                                                   \
+    if ((STEP) < 7) load_step((STEP) + 1, ((STEP) + 1) & 1);                                          \
     constexpr int MU_ = (STEP) >> 1;                                                                  \
     constexpr bool FWD_ = ((STEP) & 1) == 0;                                                          \
-    const __amdgpu_buffer_rsrc_t rl = make_rsrc(                                                      \
-        static_cast<const char*>(h.gauge[FWD_ ? h.cb : 1 - h.cb]) + (size_t)MU_ * 24 * Vh, (unsigned)(24 * Vh)); \
-    FastLink<P16> lk;                                                                                 \
-    fast_load_link<P16>(rl, Vh, FWD_ ? i : nb[STEP], lk);                                             \
-    FastSpinor<P16> sp[NB];                                                                           \
-    _Pragma("unroll") for (int k = 0; k < NB; ++k)                                                    \
-        fast_load_spinor<P16>(rin[k], rnin[k], M.in[r0 + k].stride, nb[STEP], sp[k]);                 \
     float cr[3], ci[3];                                                                               \
-    fast_row2_h(lk, cr, ci);                                                                          \
+    fast_row2_h(lk[(STEP) & 1], cr, ci);                                                              \
     const float sg_ = (FWD_ ? lastf[MU_] : lastb[MU_]) ? h.sgn[MU_][1] : h.sgn[MU_][0];               \
     _Pragma("unroll") for (int k = 0; k < NB; ++k) {                                                  \
-      fast_step_h_body<MU_, FWD_, ISIGN>(sp[k], lk, cr, ci, sg_, acc[k]);                             \
+      fast_step_h_body<MU_, FWD_, ISIGN>(sp[(STEP) & 1][k], lk[(STEP) & 1], cr, ci, sg_, acc[k]);     \
       pin_acc(acc[k]);                                                                                \
     }                                                                                                 \
   }
+  load_step(0, 0);
   CMX_MULTI_STEP(0)
   CMX_MULTI_STEP(1)
   CMX_MULTI_STEP(2)

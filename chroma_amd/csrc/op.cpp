@@ -351,12 +351,12 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
       launch_hop_multi(M, stream);
     }
   };
-  if (multi_group == 0) {  // first use: time groups of 2, 3 and 4 sources (registers vs link reuse)
+  if (multi_group == 0) {  // first use: time groups of 1 to 4 sources (registers vs link reuse)
     hipEvent_t e0, e1;
     CMX_HIP(hipEventCreate(&e0));
     CMX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    for (int gsz = 2; gsz <= 4; ++gsz) {
+    for (int gsz = 1; gsz <= 4; ++gsz) {
       M.group = gsz;
       run();
       CMX_HIP(hipEventRecord(e0, stream));

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "op.h"
 
@@ -76,14 +78,62 @@ double norm2(Op& op, const Vec& v) {
   return op.hS[S_TR];
 }
 
-// Runs iterations in batches until the device reports convergence (or a stop)
+// Replays a fixed sequence of launches on the operator's stream, optionally from a captured HIP graph
+// (one graph per sequence length).  The sequence must not allocate, synchronise or copy to the host;
+// the solvers run it once eagerly before the first capture, so every lazy allocation has happened.
+struct GraphCache {
+  Op& op;
+  bool on;
+  std::vector<std::pair<int, hipGraphExec_t>> execs;
+  GraphCache(Op& o, bool use) : op(o), on(use) {}
+  ~GraphCache() {
+    for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+  }
+  template <typename Seq> void run(int key, Seq&& seq) {
+    if (!on) {
+      seq();
+      return;
+    }
+    for (auto& e : execs)
+      if (e.first == key) {
+        CMX_HIP(hipGraphLaunch(e.second, op.stream));
+        return;
+      }
+    hipGraph_t gr = nullptr;
+    CMX_HIP(hipStreamBeginCapture(op.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      seq();
+    } catch (...) {
+      (void)hipStreamEndCapture(op.stream, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    CMX_HIP(hipStreamEndCapture(op.stream, &gr));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    CMX_HIP(e);
+    execs.emplace_back(key, ex);
+    CMX_HIP(hipGraphLaunch(ex, op.stream));
+  }
+};
+
+// Runs iterations in batches until the device reports convergence (or a stop).  With p.graph the
+// batches after the first are replayed from captured graphs.
 template <typename Iter> int run_batches(Op& op, const cmx_solver_params& p, Iter&& iter) {
   int batch = p.check_every > 0 ? p.check_every : 4;
   long launched = 0;
-  const int* dF = op.state_F.as<int>();
-  (void)dF;
+  GraphCache gc(op, p.graph != 0);
+  bool first = true;
   while (true) {
-    for (int j = 0; j < batch; ++j) iter();
+    if (first) {
+      for (int j = 0; j < batch; ++j) iter();
+      first = false;
+    } else {
+      gc.run(batch, [&] {
+        for (int j = 0; j < batch; ++j) iter();
+      });
+    }
     launched += batch;
     CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
     op.sync();
@@ -211,13 +261,20 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, b
   // back, and the FP64 reliable-update kernels are launched only when the device asks for them
   // (an early-exit launch of a full-lattice grid costs tens of microseconds, more than the read-back).
   long launched = 0;
-  while (true) {
+  GraphCache gc(op, p.graph != 0);
+  auto inner = [&]() {
     op.blas(B_XPAY, {&pv, &r}, S_BETA, 0, F_DONE);                                          // p = r + beta p
     const int np = op.apply(ps, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);
     op.finalize(np, 1, PG_RCG_D);                                                           // c = r_sq; a = c/d
     op.apply(ps, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);
     op.blas(B_AXPY2_NORM, {&x, &r, &pv, &mmp}, S_A, 0, F_DONE);                             // x += a p; r -= a mmp
     op.finalize(nb, 1, PG_RCG_R);                                                           // updateX / updateR
+  };
+  while (true) {
+    if (launched == 0)
+      inner();  // eager: performs the lazy allocations of the operator's scratch vectors
+    else
+      gc.run(0, inner);
     CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
     op.sync();
     if (op.hF[F_DONE]) break;  // converged (or stopped) in the previous iteration

@@ -39,7 +39,7 @@ class SolverParams(ctypes.Structure):
                 ("verbose", ctypes.c_int), ("true_residual", ctypes.c_int), ("n_krylov", ctypes.c_int),
                 ("n_defl", ctypes.c_int), ("precond", ctypes.c_int), ("precond_iters", ctypes.c_int),
                 ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int),
-                ("basis_prec", ctypes.c_int), ("cycle_reduction", ctypes.c_double)]
+                ("basis_prec", ctypes.c_int), ("cycle_reduction", ctypes.c_double), ("graph", ctypes.c_int)]
 
 
 class TrainParams(ctypes.Structure):
@@ -295,7 +295,7 @@ class Operator:
     def solve(self, chi, psi=None, solver=CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=SINGLE,
               check_every=0, verbose=False, true_residual=False, mdagm=False, n_krylov=16, n_defl=0,
               precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1,
-              basis_prec=-1, cycle_reduction=0.0):
+              basis_prec=-1, cycle_reduction=0.0, graph=False):
         """SystemSolver<T>::operator()(psi, chi) (lib/syssolver.h:46); chi/psi: [nrhs,] subset arrays.
 
         mdagm=True solves M^dag M psi = chi (MdagMSystemSolver, lib/syssolver.h:72-84)."""
@@ -307,7 +307,7 @@ class Operator:
         p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
                          int(bool(verbose)), int(bool(true_residual)), int(n_krylov), int(n_defl), int(precond),
                          int(precond_iters), float(precond_rsd), float(precond_omega), int(reorth), int(basis_prec),
-                         float(cycle_reduction))
+                         float(cycle_reduction), int(bool(graph)))
         res = (SolverResult * nrhs)()
         fn = lib().cmx_solve_mdagm if mdagm else lib().cmx_solve
         check(fn(self.h, ctypes.byref(p), _d(psis), _d(chis), nrhs, res))

@@ -158,6 +158,8 @@ typedef struct {
   double cycle_reduction; /* FGMRES-DR: end a cycle once its residual estimate has fallen by this
                         factor (then the FP64 residual is recomputed and the next cycle starts from
                         it, as a reliable update); 0: only at RsdTarget or NKrylov, as the reference */
+  int graph;         /* 1: CG, BiCGStab and reliable CG replay their iterations from captured HIP graphs
+                        (fewer launch overheads on small volumes; results are identical)           */
 } cmx_solver_params;
 
 typedef struct {

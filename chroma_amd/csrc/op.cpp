@@ -144,6 +144,7 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
 }
 
 Op::~Op() {
+  for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
   if (stream) (void)hipStreamDestroy(stream);
   if (hS) (void)hipHostFree(hS);
   if (hF) (void)hipHostFree(hF);
@@ -229,6 +230,16 @@ void Op::ensure_prec(int prec) {
     clover_convert((int)Vh, prec, clovinv_even[P64].as<double2>(), clovinv_even[prec].p, stream);
   }
   CMX_HIP(hipGetLastError());
+}
+
+Vec& Op::work(int slot, int prec, int n) {
+  if ((int)ws.size() <= slot) ws.resize(slot + 1);
+  auto& v = ws[slot];
+  if (!v || v->prec != prec || v->n != n) {
+    v.reset(new Vec(prec, n));
+    ++ws_gen;
+  }
+  return *v;
 }
 
 Vec& Op::tmp(int prec, int k) {

@@ -93,6 +93,18 @@ struct Op {
   int* hF = nullptr;
   DevBuf staging;        // V*24 doubles
   std::vector<std::unique_ptr<Vec>> tmp_even[3];  // scratch even-checkerboard vectors
+  // solver workspace kept across solves (so that captured iteration graphs stay valid), and the
+  // captured graphs: key = (solver tag, batch length, solution and source buffers); an entry is
+  // re-captured when the workspace generation changed
+  std::vector<std::unique_ptr<Vec>> ws;
+  long ws_gen = 0;
+  struct GraphEntry {
+    long tag, batch, gen;
+    const void *psi, *chi;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  Vec& work(int slot, int prec, int n);
   // learned preconditioner: the attached network, the weight version last uploaded, device buffers
   std::shared_ptr<LearnedNet> net;
   long net_version = -1;

@@ -79,25 +79,32 @@ double norm2(Op& op, const Vec& v) {
 }
 
 // Replays a fixed sequence of launches on the operator's stream, optionally from a captured HIP graph
-// (one graph per sequence length).  The sequence must not allocate, synchronise or copy to the host;
-// the solvers run it once eagerly before the first capture, so every lazy allocation has happened.
+// kept in the operator (keyed by solver tag, sequence length and the solution / source buffers; the
+// solver's other vectors are the operator's persistent workspace).  The sequence must not allocate,
+// synchronise or copy to the host; the solvers run it once eagerly before the first capture, so every
+// lazy allocation has happened.
 struct GraphCache {
   Op& op;
   bool on;
-  std::vector<std::pair<int, hipGraphExec_t>> execs;
-  GraphCache(Op& o, bool use) : op(o), on(use) {}
-  ~GraphCache() {
-    for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
-  }
-  template <typename Seq> void run(int key, Seq&& seq) {
+  long tag;
+  const void *psi, *chi;
+  GraphCache(Op& o, bool use, long tag_, const Vec& psi_, const Vec& chi_)
+      : op(o), on(use), tag(tag_), psi(psi_.buf.p), chi(chi_.buf.p) {}
+  template <typename Seq> void run(long key, Seq&& seq) {
     if (!on) {
       seq();
       return;
     }
-    for (auto& e : execs)
-      if (e.first == key) {
-        CMX_HIP(hipGraphLaunch(e.second, op.stream));
-        return;
+    for (auto& e : op.graphs)
+      if (e.tag == tag && e.batch == key && e.psi == psi && e.chi == chi) {
+        if (e.gen == op.ws_gen) {
+          CMX_HIP(hipGraphLaunch(e.exec, op.stream));
+          return;
+        }
+        (void)hipGraphExecDestroy(e.exec);  // the workspace moved: capture again
+        e = op.graphs.back();
+        op.graphs.pop_back();
+        break;
       }
     hipGraph_t gr = nullptr;
     CMX_HIP(hipStreamBeginCapture(op.stream, hipStreamCaptureModeThreadLocal));
@@ -113,17 +120,18 @@ struct GraphCache {
     const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
     CMX_HIP(e);
-    execs.emplace_back(key, ex);
+    op.graphs.push_back(Op::GraphEntry{tag, key, op.ws_gen, psi, chi, ex});
     CMX_HIP(hipGraphLaunch(ex, op.stream));
   }
 };
 
 // Runs iterations in batches until the device reports convergence (or a stop).  With p.graph the
 // batches after the first are replayed from captured graphs.
-template <typename Iter> int run_batches(Op& op, const cmx_solver_params& p, Iter&& iter) {
+template <typename Iter>
+int run_batches(Op& op, const cmx_solver_params& p, long tag, const Vec& psi, const Vec& chi, Iter&& iter) {
   int batch = p.check_every > 0 ? p.check_every : 4;
   long launched = 0;
-  GraphCache gc(op, p.graph != 0);
+  GraphCache gc(op, p.graph != 0, tag, psi, chi);
   bool first = true;
   while (true) {
     if (first) {
@@ -157,7 +165,8 @@ double flops_per_apply(const Op& op) {
 // mdagm: chi is already the source of M^dag M psi = chi (MdagMSysSolverCG, syssolver_mdagm_cg.h:70)
 void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
   const int n = op.subset_sites();
-  Vec chi_tmp(P64, n), r(P64, n), pv(P64, n), mp(P64, n), mmp(P64, n);
+  Vec &chi_tmp = op.work(10, P64, n), &r = op.work(11, P64, n), &pv = op.work(12, P64, n),
+      &mp = op.work(13, P64, n), &mmp = op.work(14, P64, n);
   const int nb = blas_blocks(n);
   // chi_tmp = M^dag chi  (syssolver_linop_cg.h:65-66)
   if (mdagm)
@@ -183,7 +192,7 @@ void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool
     op.finalize(nb, 1, PG_CG_R);                                                                 // cp, b, test
     op.blas(B_CG_UPDATE, {&psi, &pv, &r}, S_A, S_B, F_DONE);                                     // psi += a p; p = r + b p
   };
-  res.n_count = run_batches(op, p, iter);
+  res.n_count = run_batches(op, p, 1 + (mdagm ? 100 : 0), psi, chi, iter);
   read_state(op);
   res.solve_seconds = tm.stop();
   const int err = op.hF[F_ERR];
@@ -196,7 +205,8 @@ void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool
 // ---- BICGSTAB_INVERTER ----------------------------------------------------------------------------
 void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, int isign, cmx_solver_result& res) {
   const int n = op.subset_sites();
-  Vec r(P64, n), r0(P64, n), pv(P64, n), v(P64, n), t(P64, n);
+  Vec &r = op.work(20, P64, n), &r0 = op.work(21, P64, n), &pv = op.work(22, P64, n), &v = op.work(23, P64, n),
+      &t = op.work(24, P64, n);
   const int nb = blas_blocks(n);
   const double chi_sq = norm2(op, chi);
   reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
@@ -223,7 +233,7 @@ void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     op.blas(B_BICG_UPDATE, {&psi, &r, &pv, &t}, S_OMEGA, S_ALPHA, F_DONE);  // psi += omega s + alpha p; r = s - omega t
     op.finalize(nb, 1, PG_BICG_R);
   };
-  res.n_count = run_batches(op, p, iter);
+  res.n_count = run_batches(op, p, 2 + (isign < 0 ? 100 : 0), psi, chi, iter);
   read_state(op);
   res.solve_seconds = tm.stop();
   res.gflops = res.n_count * (2 * flops_per_apply(op) + 80 * 12.0 * n) / std::max(res.solve_seconds, 1e-12) * 1e-9;
@@ -237,8 +247,10 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, b
   const int ps = p.sloppy_prec;
   CMX_REQUIRE(ps == P32 || ps == P16, CMX_ERR_ARG, "reliable CG: sloppy precision must be single or half");
   const int nb = blas_blocks(n);
-  Vec chi_tmp(P64, n), b(P64, n), r_dble(P64, n), x_dble(P64, n), tmp1(P64, n), tmp2(P64, n);
-  Vec r(P32, n), x(P32, n), pv(ps, n), mp(ps, n), mmp(ps, n);
+  Vec &chi_tmp = op.work(30, P64, n), &b = op.work(31, P64, n), &r_dble = op.work(32, P64, n),
+      &x_dble = op.work(33, P64, n), &tmp1 = op.work(34, P64, n), &tmp2 = op.work(35, P64, n);
+  Vec &r = op.work(36, P32, n), &x = op.work(37, P32, n), &pv = op.work(38, ps, n), &mp = op.work(39, ps, n),
+      &mmp = op.work(40, ps, n);
   // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125); the MdagM solver takes chi as is
   // (syssolver_mdagm_rel_cg_clover.h:126-133)
   if (mdagm)
@@ -261,7 +273,7 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, b
   // back, and the FP64 reliable-update kernels are launched only when the device asks for them
   // (an early-exit launch of a full-lattice grid costs tens of microseconds, more than the read-back).
   long launched = 0;
-  GraphCache gc(op, p.graph != 0);
+  GraphCache gc(op, p.graph != 0, 3 + 10 * ps + (mdagm ? 100 : 0), psi, chi);
   auto inner = [&]() {
     op.blas(B_XPAY, {&pv, &r}, S_BETA, 0, F_DONE);                                          // p = r + beta p
     const int np = op.apply(ps, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);
@@ -391,7 +403,7 @@ void solve_pass(Op& op, const cmx_solver_params& q, Vec& psi, const Vec& chi, bo
   }
   // M^dag Y = chi from the guess Y = M psi, then M psi = Y from the guess psi
   const int n = op.subset_sites();
-  Vec Y(P64, n);
+  Vec& Y = op.work(4, P64, n);
   op.apply(P64, Y.view(), psi.view(), +1, nullptr);
   cmx_solver_result r1, r2;
   std::memset(&r1, 0, sizeof(r1));
@@ -412,14 +424,14 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
   CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
   const auto t0 = std::chrono::steady_clock::now();
   const int n = op.subset_sites();
-  Vec psi(P64, n), chi(P64, n);
+  Vec &psi = op.work(0, P64, n), &chi = op.work(1, P64, n);
   op.pack(chi_h, n, chi);
   op.pack(psi_h, n, psi);
   std::memset(&res, 0, sizeof(res));
   cmx_solver_params q = p;
   // true-residual mode aims the first pass a little below the target, so that a restart is rare
   if (p.true_residual) q.rsd_target = 0.5 * p.rsd_target;
-  Vec t(P64, n), rr(P64, n);
+  Vec &t = op.work(2, P64, n), &rr = op.work(3, P64, n);
   for (int restart = 0;; ++restart) {
     cmx_solver_result part;
     std::memset(&part, 0, sizeof(part));

@@ -1,0 +1,113 @@
+// Launch recording and batched replay (see batch.h).
+#include "batch.h"
+
+#include <cstring>
+#include <string>
+
+namespace cmx {
+
+Recorder*& active_recorder() {
+  thread_local Recorder* r = nullptr;
+  return r;
+}
+
+namespace {
+template <typename T> LaunchRec make_rec(int kind, const T& v, int nblocks) {
+  LaunchRec r;
+  r.kind = kind;
+  r.nblocks = nblocks;
+  r.bytes.resize(sizeof(T));
+  std::memcpy(r.bytes.data(), &v, sizeof(T));
+  return r;
+}
+}  // namespace
+
+void record_hop(Recorder& r, const HopLaunch& h) {
+  LaunchRec x = make_rec(REC_HOP, h, hop_num_blocks(h));
+  // everything that selects the kernel template and the work-group size
+  const int k[8] = {h.prec, h.recon, h.fast, h.prefetch, h.isign, h.epi, h.partials != nullptr, h.block};
+  std::memcpy(x.key, k, sizeof(k));
+  r.recs.push_back(std::move(x));
+}
+
+void record_blas(Recorder& r, int op, const BlasLaunch& b, const int prec[4]) {
+  LaunchRec x = make_rec(REC_BLAS, b, blas_blocks(b.n));
+  const int k[8] = {op, prec[0], prec[1], prec[2], prec[3], 0, 0, 0};
+  std::memcpy(x.key, k, sizeof(k));
+  r.recs.push_back(std::move(x));
+}
+
+void record_fin(Recorder& r, const FinLaunch& f) { r.recs.push_back(make_rec(REC_FIN, f, 1)); }
+
+void BatchPlan::build(const std::vector<Recorder>& recs) {
+  st_.clear();
+  CMX_REQUIRE(!recs.empty(), CMX_ERR_ARG, "batch: no problems");
+  const size_t ns = recs[0].recs.size();
+  const int np = (int)recs.size();
+  for (const Recorder& r : recs)
+    CMX_REQUIRE(r.recs.size() == ns, CMX_ERR_ARG, "batch: the problems' kernel sequences differ in length");
+  for (size_t j = 0; j < ns; ++j) {
+    const LaunchRec& r0 = recs[0].recs[j];
+    Stage s;
+    s.kind = r0.kind;
+    s.nprob = np;
+    s.rec0 = r0.bytes;
+    if (s.kind == REC_BLAS) {
+      s.opc = r0.key[0];
+      for (int k = 0; k < 4; ++k) s.prec[k] = r0.key[1 + k];
+    }
+    std::vector<unsigned char> tab;
+    tab.reserve(r0.bytes.size() * np);
+    std::vector<int4> map;
+    for (int p = 0; p < np; ++p) {
+      const LaunchRec& r = recs[p].recs[j];
+      CMX_REQUIRE(r.kind == r0.kind && std::memcmp(r.key, r0.key, sizeof(r.key)) == 0 && r.bytes.size() == r0.bytes.size(),
+                  CMX_ERR_ARG,
+                  "batch: stage " + std::to_string(j) + " selects a different kernel for problem " + std::to_string(p) +
+                      " (launch shape, precision or epilogue)");
+      tab.insert(tab.end(), r.bytes.begin(), r.bytes.end());
+      if (s.kind != REC_FIN)
+        for (int b = 0; b < r.nblocks; ++b) map.push_back(make_int4(p, b, r.nblocks, 0));
+    }
+    s.nblocks = s.kind == REC_FIN ? np : (int)map.size();
+    s.tab.alloc(tab.size());
+    CMX_HIP(hipMemcpy(s.tab.p, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    if (!map.empty()) {
+      s.map.alloc(sizeof(int4) * map.size());
+      CMX_HIP(hipMemcpy(s.map.p, map.data(), sizeof(int4) * map.size(), hipMemcpyHostToDevice));
+    }
+    st_.push_back(std::move(s));
+  }
+}
+
+long BatchPlan::blocks() const {
+  long n = 0;
+  for (const Stage& s : st_) n += s.nblocks;
+  return n;
+}
+
+void BatchPlan::run(hipStream_t strm) const {
+  CMX_REQUIRE(active_recorder() == nullptr, CMX_ERR_ARG, "batch: replay while recording");
+  for (const Stage& s : st_) {
+    const BatchGrid bg{s.tab.p, s.map.as<int4>(), s.nblocks};
+    switch (s.kind) {
+      case REC_HOP: {
+        HopLaunch h;
+        std::memcpy(&h, s.rec0.data(), sizeof(h));
+        launch_hop(h, strm, &bg);
+        break;
+      }
+      case REC_BLAS: {
+        BlasLaunch b;
+        std::memcpy(&b, s.rec0.data(), sizeof(b));
+        launch_blas(s.opc, b, s.prec, strm, &bg);
+        break;
+      }
+      default:
+        launch_finalize_batch(s.tab.as<FinLaunch>(), s.nprob, strm);
+        break;
+    }
+  }
+}
+
+}  // namespace cmx

@@ -134,8 +134,20 @@ void launch_maxpy(const MultiLaunch& L, int pv, int pw, bool norm, hipStream_t s
 // out[r] = sum_b partials[b][r] for r < nred (one work-group per column, fixed order);
 // mode 1 additionally writes out[nred] = 1 / sqrt(out[0])
 void launch_colsum(const double* partials, int nb, int nred, double* out, int mode, hipStream_t s);
-void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s);
-void launch_finalize(const double* partials, int nblocks, int nred, double* S, int* F, int prog, int slot,
-                     hipStream_t s);
+// bg != nullptr: the batched grid of bg (records of the same op and precisions)
+void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s, const BatchGrid* bg = nullptr);
+
+// Finalize: one work-group sums nb x nred partials in a fixed order into red[], stores them at
+// S[slot..] (slot >= 0) and runs the scalar program prog
+struct FinLaunch {
+  const double* partials;
+  int nb, nred;
+  double* S;
+  int* F;
+  int prog, slot;
+};
+void launch_finalize(const FinLaunch& L, hipStream_t s);
+// one finalize per problem: tab is a device array of nproblems records
+void launch_finalize_batch(const FinLaunch* tab, int nproblems, hipStream_t s);
 
 }  // namespace cmx

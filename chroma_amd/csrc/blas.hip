@@ -218,8 +218,9 @@ CMX_F(FAxpyCNorm, 1, 3, 1) {
 };
 #undef CMX_F
 
+// work-group blk of the launch b
 template <class F, int P0, int P1, int P2, int P3>
-__global__ void __launch_bounds__(256) k_blas(BlasLaunch b) {
+__device__ __forceinline__ void blas_body(const BlasLaunch& b, int blk) {
   if (b.skip_flag >= 0 && b.F[b.skip_flag]) return;
   if (b.run_flag >= 0 && !b.F[b.run_flag]) return;
   constexpr int U = F::LOAD | F::STORE;
@@ -227,7 +228,7 @@ __global__ void __launch_bounds__(256) k_blas(BlasLaunch b) {
                      ((U & 8) && P3 == P64);
   using R = typename std::conditional<D, double, float>::type;
   double red[4] = {0, 0, 0, 0};
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blk * blockDim.x + threadIdx.x;
   if (i < b.n) {
     R v0[24], v1[24], v2[24], v3[24];
     if constexpr (F::LOAD & 1) vload<P0>(b.v[0], i, v0);
@@ -247,17 +248,31 @@ __global__ void __launch_bounds__(256) k_blas(BlasLaunch b) {
     block_sum<F::NRED>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-      for (int r = 0; r < F::NRED; ++r) b.partials[(size_t)blockIdx.x * F::NRED + r] = v[r];
+      for (int r = 0; r < F::NRED; ++r) b.partials[(size_t)blk * F::NRED + r] = v[r];
   }
 }
 
+template <class F, int P0, int P1, int P2, int P3>
+__global__ void __launch_bounds__(256) k_blas(BlasLaunch b) {
+  blas_body<F, P0, P1, P2, P3>(b, blockIdx.x);
+}
+template <class F, int P0, int P1, int P2, int P3>
+__global__ void __launch_bounds__(256) k_blas_b(const BlasLaunch* __restrict__ tab, const int4* __restrict__ map) {
+  const int4 m = map[blockIdx.x];
+  blas_body<F, P0, P1, P2, P3>(tab[m.x], m.y);
+}
+
 template <class F>
-void dispatch(const BlasLaunch& b, const int p[4], hipStream_t s) {
+void dispatch(const BlasLaunch& b, const int p[4], hipStream_t s, const BatchGrid* bg) {
   const dim3 grid(blas_blocks(b.n)), block(256);
   const int key = p[0] + 3 * p[1] + 9 * p[2] + 27 * p[3];
 #define CMX_T(A, B, C, D)                                                                  \
   case A + 3 * B + 9 * C + 27 * D:                                                         \
-    hipLaunchKernelGGL((k_blas<F, A, B, C, D>), grid, block, 0, s, b);                    \
+    if (bg)                                                                                \
+      hipLaunchKernelGGL((k_blas_b<F, A, B, C, D>), dim3(bg->nblocks), block, 0, s,       \
+                         static_cast<const BlasLaunch*>(bg->tab), bg->map);               \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_blas<F, A, B, C, D>), grid, block, 0, s, b);                  \
     return;
   switch (key) {
     CMX_T(P64, P64, P64, P64)
@@ -284,8 +299,11 @@ void dispatch(const BlasLaunch& b, const int p[4], hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb, int nred, double* S, int* F, int prog,
-                                                  int slot) {
+__device__ __forceinline__ void finalize_body(const FinLaunch& L) {
+  const double* partials = L.partials;
+  const int nb = L.nb, nred = L.nred, prog = L.prog, slot = L.slot;
+  double* S = L.S;
+  int* F = L.F;
   double red[3] = {0, 0, 0};
   for (int r = 0; r < nred; ++r) {
     double acc = 0;
@@ -596,6 +614,10 @@ __global__ void __launch_bounds__(256) k_finalize(const double* partials, int nb
   }
 }
 
+__global__ void __launch_bounds__(256) k_finalize(FinLaunch L) { finalize_body(L); }
+// one work-group per problem
+__global__ void __launch_bounds__(256) k_finalize_b(const FinLaunch* __restrict__ tab) { finalize_body(tab[blockIdx.x]); }
+
 }  // namespace
 
 int blas_nred(int op) {
@@ -614,36 +636,36 @@ int blas_nred(int op) {
 
 int blas_blocks(int n) { return (n + 255) / 256; }
 
-void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s) {
+void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s, const BatchGrid* bg) {
   switch (op) {
-    case B_COPY: dispatch<FCopy>(b, prec, s); break;
-    case B_ZERO: dispatch<FZero>(b, prec, s); break;
-    case B_NORM2: dispatch<FNorm2>(b, prec, s); break;
-    case B_XMY_NORM: dispatch<FXmyNorm>(b, prec, s); break;
-    case B_XMY_NORM_COPY: dispatch<FXmyNormCopy>(b, prec, s); break;
-    case B_AXPY_NORM: dispatch<FAxpyNorm>(b, prec, s); break;
-    case B_AXPY: dispatch<FAxpy>(b, prec, s); break;
-    case B_XPAY: dispatch<FXpay>(b, prec, s); break;
-    case B_CG_UPDATE: dispatch<FCgUpdate>(b, prec, s); break;
-    case B_AXPY2_NORM: dispatch<FAxpy2Norm>(b, prec, s); break;
-    case B_ADD_ZERO_COPY: dispatch<FAddZeroCopy>(b, prec, s); break;
-    case B_ADD: dispatch<FAdd>(b, prec, s); break;
-    case B_CDOT: dispatch<FCdot>(b, prec, s); break;
-    case B_BICG_P: dispatch<FBicgP>(b, prec, s); break;
-    case B_CAXPY: dispatch<FCaxpy>(b, prec, s); break;
-    case B_TNORM_TDOT: dispatch<FTnormTdot>(b, prec, s); break;
-    case B_BICG_UPDATE: dispatch<FBicgUpdate>(b, prec, s); break;
-    case B_CAXPY2: dispatch<FCaxpy2>(b, prec, s); break;
-    case B_RBICG_UPDATE: dispatch<FRbicgUpdate>(b, prec, s); break;
-    case B_SCALE: dispatch<FScale>(b, prec, s); break;
-    case B_AXPY_CNORM: dispatch<FAxpyCNorm>(b, prec, s); break;
+    case B_COPY: dispatch<FCopy>(b, prec, s, bg); break;
+    case B_ZERO: dispatch<FZero>(b, prec, s, bg); break;
+    case B_NORM2: dispatch<FNorm2>(b, prec, s, bg); break;
+    case B_XMY_NORM: dispatch<FXmyNorm>(b, prec, s, bg); break;
+    case B_XMY_NORM_COPY: dispatch<FXmyNormCopy>(b, prec, s, bg); break;
+    case B_AXPY_NORM: dispatch<FAxpyNorm>(b, prec, s, bg); break;
+    case B_AXPY: dispatch<FAxpy>(b, prec, s, bg); break;
+    case B_XPAY: dispatch<FXpay>(b, prec, s, bg); break;
+    case B_CG_UPDATE: dispatch<FCgUpdate>(b, prec, s, bg); break;
+    case B_AXPY2_NORM: dispatch<FAxpy2Norm>(b, prec, s, bg); break;
+    case B_ADD_ZERO_COPY: dispatch<FAddZeroCopy>(b, prec, s, bg); break;
+    case B_ADD: dispatch<FAdd>(b, prec, s, bg); break;
+    case B_CDOT: dispatch<FCdot>(b, prec, s, bg); break;
+    case B_BICG_P: dispatch<FBicgP>(b, prec, s, bg); break;
+    case B_CAXPY: dispatch<FCaxpy>(b, prec, s, bg); break;
+    case B_TNORM_TDOT: dispatch<FTnormTdot>(b, prec, s, bg); break;
+    case B_BICG_UPDATE: dispatch<FBicgUpdate>(b, prec, s, bg); break;
+    case B_CAXPY2: dispatch<FCaxpy2>(b, prec, s, bg); break;
+    case B_RBICG_UPDATE: dispatch<FRbicgUpdate>(b, prec, s, bg); break;
+    case B_SCALE: dispatch<FScale>(b, prec, s, bg); break;
+    case B_AXPY_CNORM: dispatch<FAxpyCNorm>(b, prec, s, bg); break;
     default: throw Error(CMX_ERR_ARG, "blas: unknown op");
   }
 }
 
-void launch_finalize(const double* partials, int nblocks, int nred, double* S, int* F, int prog, int slot,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nblocks, nred, S, F, prog, slot);
+void launch_finalize(const FinLaunch& L, hipStream_t s) { hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, L); }
+void launch_finalize_batch(const FinLaunch* tab, int nproblems, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_b, dim3(nproblems), dim3(256), 0, s, tab);
 }
 
 }  // namespace cmx

@@ -319,6 +319,32 @@ int cmx_solve_mdagm(cmx_op h, const cmx_solver_params* p, double* psi, const dou
   });
 }
 
+int cmx_solve_batch(const cmx_op* hs, int n, const cmx_solver_params* p, const cmx_batch_params* bp,
+                    double* const* psi, const double* const* chi, int mdagm, cmx_solver_result* res,
+                    cmx_batch_result* bres) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(hs && n >= 1 && p && bp && psi && chi && res && bres, CMX_ERR_ARG, "solve_batch: bad argument");
+    // a repeated operator gets a solver context of its own (sharing its fields and stream)
+    std::vector<std::unique_ptr<cmx::Op>> clones;
+    std::vector<cmx::Op*> ops(n);
+    const int ps = p->solver == CMX_SOLVER_RELIABLE_CG ? p->sloppy_prec : cmx::P64;
+    for (int i = 0; i < n; ++i) {
+      CMX_REQUIRE(hs[i] && psi[i] && chi[i], CMX_ERR_ARG, "solve_batch: null operator or field");
+      cmx::Op& o = O(hs[i]);
+      bool seen = false;
+      for (int j = 0; j < i; ++j) seen = seen || hs[j] == hs[i];
+      if (seen) {
+        clones.emplace_back(cmx::clone_op(o, ps));
+        ops[i] = clones.back().get();
+      } else {
+        ops[i] = &o;
+      }
+    }
+    cmx::solve_batch(ops, *p, *bp, psi, chi, mdagm != 0, res, *bres);
+  });
+}
+
 int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
   return guarded([&] {
     require_init();
@@ -406,7 +432,7 @@ int cmx_op_set_hop_config(cmx_op h, int prec, const int cfg[6]) {
     cmx::Op& op = O(h);
     CMX_REQUIRE(cfg && prec >= 0 && prec <= 2, CMX_ERR_ARG, "set_hop_config: bad argument");
     CMX_REQUIRE(cfg[0] == 64 || cfg[0] == 128 || cfg[0] == 256, CMX_ERR_ARG, "set_hop_config: block must be 64, 128 or 256");
-    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1) && (cfg[4] == 0 || cfg[4] == 1),
+    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1) && (cfg[4] == 0 || cfg[4] == 1 || cfg[4] == 3),
                 CMX_ERR_ARG,
                 "set_hop_config: bad tiling / prefetch flags");
     CMX_REQUIRE(cfg[5] >= 0 && cfg[5] <= 2, CMX_ERR_ARG, "set_hop_config: target must be 0, 1 or 2");

@@ -88,6 +88,7 @@ template <int P> struct SpinorIO;
 
 template <> struct SpinorIO<P64> {
   using R = double;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, const float*, long Vh, long i, double (&s)[24]) {
     const double2* b = reinterpret_cast<const double2*>(base);
 #pragma unroll
@@ -104,13 +105,28 @@ template <> struct SpinorIO<P64> {
   }
 };
 
+// Load with a cache policy: CP = 0 an ordinary load, otherwise non-temporal (data read once)
+template <int CP, typename T> __device__ __forceinline__ T ld_cp(const T* p) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "ld_cp: 4, 8 or 16 bytes");
+  if constexpr (CP == 0) {
+    return *p;
+  } else {
+    typedef unsigned int W __attribute__((ext_vector_type(sizeof(T) / 4)));
+    const W w = __builtin_nontemporal_load(reinterpret_cast<const W*>(p));
+    T r;
+    __builtin_memcpy(&r, &w, sizeof(T));
+    return r;
+  }
+}
+
 template <> struct SpinorIO<P32> {
   using R = float;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, const float*, long Vh, long i, float (&s)[24]) {
     const float4* b = reinterpret_cast<const float4*>(base);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      float4 v = b[k * Vh + i];
+      float4 v = ld_cp<CP>(&b[k * Vh + i]);
       s[4 * k] = v.x;
       s[4 * k + 1] = v.y;
       s[4 * k + 2] = v.z;
@@ -131,13 +147,14 @@ union H8 {
 
 template <> struct SpinorIO<P16> {
   using R = float;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, const float* nrm, long Vh, long i, float (&s)[24]) {
     const uint4* b = reinterpret_cast<const uint4*>(base);
-    const float n = nrm[i];
+    const float n = ld_cp<CP>(&nrm[i]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       H8 v;
-      v.u = b[k * Vh + i];
+      v.u = ld_cp<CP>(&b[k * Vh + i]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         float2 f = __half22float2(v.h[j]);
@@ -286,6 +303,16 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Batched launches: one grid covers the work-groups of several problems (solves on different
+// lattices, or several solves on one).  Work-group b runs work-group map[b].y (of map[b].z) of
+// problem map[b].x, whose launch record is tab[map[b].x].
+struct BatchGrid {
+  const void* tab;   // device array of launch records (HopLaunch, BlasLaunch or FinLaunch)
+  const int4* map;   // device array, one entry per work-group of the batched grid
+  int nblocks;       // work-groups of the batched grid
+};
+
+// ------------------------------------------------------------------------------------------------
 // Host-side launch interfaces (implemented in the .hip files)
 
 // Hop kernel epilogues
@@ -325,11 +352,14 @@ struct HopLaunch {
   int tile_zb;       // z planes per tile
   int tile_t8;       // t slices per XCD slab = Lt/8
   int prefetch;      // 1: double-buffered raw loads across the 8 directions
-  int fast;          // 1: 2-D grid kernel of hop_fast.h (FP16/FP32, 12-real links, real phases)
+  int fast;          // bit 0: 2-D grid kernel of hop_fast.h (FP16/FP32, 12-real links, real phases);
+                     // bit 1 (HOP_FAST_STREAM): its loads of data read once per launch (links,
+                     // clover, the xpay input) carry the non-temporal hint
   float inv_lxh;     // 1 / (Lx/2), for the fast kernel's plane decomposition
   float sgn[4][2];   // real det(U) phase per direction: [bulk | last slice]
 };
 
+constexpr int HOP_FAST_STREAM = 2;
 // Launch-shape choice for the hopping kernels of one lattice and precision
 struct HopConfig {
   int block = 256, tiled = 0, zb = 1, prefetch = 0, fast = 1;
@@ -339,10 +369,11 @@ struct HopConfig {
 int hop_apply_config(HopLaunch& h, const HopConfig& c);
 // Number of work-groups (= reduction partials) of a hopping launch
 int hop_num_blocks(const HopLaunch& h);
-void launch_hop(const HopLaunch& h, hipStream_t s);
-void launch_hop_p64(const HopLaunch& h, hipStream_t s);
-void launch_hop_p32(const HopLaunch& h, hipStream_t s);
-void launch_hop_p16(const HopLaunch& h, hipStream_t s);
+// bg != nullptr: the batched grid of bg, whose records all select the kernel that h selects
+void launch_hop(const HopLaunch& h, hipStream_t s, const BatchGrid* bg = nullptr);
+void launch_hop_p64(const HopLaunch& h, hipStream_t s, const BatchGrid* bg);
+void launch_hop_p32(const HopLaunch& h, hipStream_t s, const BatchGrid* bg);
+void launch_hop_p16(const HopLaunch& h, hipStream_t s, const BatchGrid* bg);
 
 // Device solver scalars live in one double array; flags in one int array
 struct DevState {

@@ -20,12 +20,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsig
   // raw buffer, stride 0; word 3 = 0x00020000 for the gfx9 family (loads past `bytes` return 0)
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
+// CP: cache-policy operand of the buffer loads; 0 = default, CMX_NT = non-temporal (streamed through
+// L2 with early eviction: for data read once, so that it displaces less of the reused data)
+#define CMX_NT 2
+template <int CP = 0>
 __device__ __forceinline__ uint4 bload128(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CP);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
+template <int CP = 0>
 __device__ __forceinline__ uint2 bload64(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CP);
   return make_uint2(v.x, v.y);
 }
 __device__ __forceinline__ float bload32f(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
@@ -89,20 +94,20 @@ __device__ __forceinline__ void fast_load_spinor(__amdgpu_buffer_rsrc_t rs, __am
   }
 }
 
-template <int P>
+template <int P, int CP = 0>
 __device__ __forceinline__ void fast_load_link(__amdgpu_buffer_rsrc_t rl, int Vh, int j, FastLink<P>& u) {
   if constexpr (P == P16) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       H4x v;
-      v.u = bload64(rl, j * 8, k * Vh * 8);
+      v.u = bload64<CP>(rl, j * 8, k * Vh * 8);
       u.c[2 * k] = v.h[0];
       u.c[2 * k + 1] = v.h[1];
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const uint4 v = bload128(rl, j * 16, k * Vh * 16);
+      const uint4 v = bload128<CP>(rl, j * 16, k * Vh * 16);
       u.c[2 * k] = make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
       u.c[2 * k + 1] = make_float2(__uint_as_float(v.z), __uint_as_float(v.w));
     }
@@ -420,8 +425,9 @@ __device__ __forceinline__ void pin_acc(float (&acc)[24]) {
   for (int k = 0; k < 24; ++k) asm volatile("" : "+v"(acc[k]));
 }
 
-template <int P, int ISIGN, int EPI, bool NORM>
-__global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
+// work-group blk of the launch h
+template <int P, int ISIGN, int EPI, bool NORM, int LCP>
+__device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
   if (h.skip && *h.skip) return;
   if (h.run && !*h.run) return;
   const int Lxh = h.g.Lxh, Ly = h.g.L[1], Lz = h.g.L[2], Lt = h.g.L[3];
@@ -432,7 +438,7 @@ __global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
   const int gx = (Pl + (int)blockDim.x - 1) / (int)blockDim.x;
   int bx, z, t;
   {
-    const int b = blockIdx.x;
+    const int b = blk;
     if (h.tiled) {
       const int xcd = b & 7;
       int q = b >> 3;
@@ -490,7 +496,7 @@ __global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
       const __amdgpu_buffer_rsrc_t rl = make_rsrc(
           static_cast<const char*>(h.gauge[fwd ? h.cb : 1 - h.cb]) + (size_t)mu * LB * Vh, (unsigned)(LB * Vh));
       fast_load_spinor<P>(rin, rnin, h.in_stride, nb[step], sp[slot]);
-      fast_load_link<P>(rl, Vh, fwd ? i : nb[step], lk[slot]);
+      fast_load_link<P, LCP>(rl, Vh, fwd ? i : nb[step], lk[slot]);
     };
 #define CMX_FAST_STEP(STEP)                                                                            \
     {                                                                                                  \
@@ -523,17 +529,17 @@ __global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
     } else if constexpr (EPI == EPI_CLOV) {
 #pragma unroll
       for (int k = 0; k < 24; ++k) acc[k] *= b;
-      clover_apply_site<P>(h.clov, Vh, i, acc, out);
+      clover_apply_site<P, LCP>(h.clov, Vh, i, acc, out);
     } else if constexpr (EPI == EPI_XPAY) {
       float xs[24];
-      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
+      SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
       const float a = (float)h.a;
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = __fmaf_rn(a, xs[k], b * acc[k]);
     } else {
       float xs[24];
-      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
-      clover_apply_site<P>(h.clov, Vh, i, xs, out);
+      SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
+      clover_apply_site<P, LCP>(h.clov, Vh, i, xs, out);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = __fmaf_rn(b, acc[k], out[k]);
     }
@@ -546,20 +552,44 @@ __global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
   if constexpr (NORM) {
     double v[1] = {nsq};
     block_sum<1>(v);
-    if (threadIdx.x == 0) h.partials[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) h.partials[blk] = v[0];
   }
 }
 
-template <int P> void launch_hop_fast_t(const HopLaunch& h, hipStream_t s) {
+template <int P, int ISIGN, int EPI, bool NORM, int LCP>
+__global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
+  hop_fast_body<P, ISIGN, EPI, NORM, LCP>(h, blockIdx.x);
+}
+template <int P, int ISIGN, int EPI, bool NORM, int LCP>
+__global__ void __launch_bounds__(256) k_hop_fast_b(const HopLaunch* __restrict__ tab, const int4* __restrict__ map) {
+  const int4 m = map[blockIdx.x];
+  hop_fast_body<P, ISIGN, EPI, NORM, LCP>(tab[m.x], m.y);
+}
+
+template <int P> void launch_hop_fast_t(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
   const int Pl = h.g.Lxh * h.g.L[1];
   const dim3 grid(((Pl + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3]);
   const bool norm = h.partials != nullptr;
+  const bool lnt = (h.fast & HOP_FAST_STREAM) != 0;
+#define CMX_HOPF_GO(IS, EP, NM, CP)                                                             \
+  if (bg)                                                                                       \
+    hipLaunchKernelGGL((k_hop_fast_b<P, IS, EP, NM, CP>), dim3(bg->nblocks), dim3(h.block), 0, \
+                       s, static_cast<const HopLaunch*>(bg->tab), bg->map);                     \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_hop_fast<P, IS, EP, NM, CP>), grid, dim3(h.block), 0, s, h);
+#define CMX_HOPF_NT(IS, EP, NM)                                                                 \
+  if (lnt) {                                                                                    \
+    CMX_HOPF_GO(IS, EP, NM, CMX_NT)                                                             \
+  } else {                                                                                      \
+    CMX_HOPF_GO(IS, EP, NM, 0)                                                                  \
+  }
 #define CMX_HOPF_CASE(IS, EP)                                                                   \
   if (h.isign == IS && h.epi == EP) {                                                           \
-    if (norm)                                                                                   \
-      hipLaunchKernelGGL((k_hop_fast<P, IS, EP, true>), grid, dim3(h.block), 0, s, h);         \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_hop_fast<P, IS, EP, false>), grid, dim3(h.block), 0, s, h);        \
+    if (norm) {                                                                                 \
+      CMX_HOPF_NT(IS, EP, true)                                                                 \
+    } else {                                                                                    \
+      CMX_HOPF_NT(IS, EP, false)                                                                \
+    }                                                                                           \
     return;                                                                                     \
   }
   CMX_HOPF_CASE(1, EPI_NONE)
@@ -571,6 +601,8 @@ template <int P> void launch_hop_fast_t(const HopLaunch& h, hipStream_t s) {
   CMX_HOPF_CASE(-1, EPI_XPAY)
   CMX_HOPF_CASE(-1, EPI_CLOV_XPAY)
 #undef CMX_HOPF_CASE
+#undef CMX_HOPF_NT
+#undef CMX_HOPF_GO
   throw Error(CMX_ERR_ARG, "launch_hop_fast: bad isign/epilogue");
 }
 

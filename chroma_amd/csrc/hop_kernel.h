@@ -143,6 +143,7 @@ __device__ __forceinline__ void hop_dir(const HopLaunch& h, typename SpinorIO<P>
 
 template <int P> struct CloverBlockIO;
 template <> struct CloverBlockIO<P64> {
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, double (&c)[36]) {
     const double2* q = reinterpret_cast<const double2*>(base) + (size_t)18 * b * Vh;
 #pragma unroll
@@ -154,11 +155,12 @@ template <> struct CloverBlockIO<P64> {
   }
 };
 template <> struct CloverBlockIO<P32> {
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, float (&c)[36]) {
     const float4* q = reinterpret_cast<const float4*>(base) + (size_t)9 * b * Vh;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      float4 v = q[k * Vh + i];
+      float4 v = ld_cp<CP>(&q[k * Vh + i]);
       c[4 * k] = v.x;
       c[4 * k + 1] = v.y;
       c[4 * k + 2] = v.z;
@@ -168,14 +170,15 @@ template <> struct CloverBlockIO<P32> {
 };
 template <> struct CloverBlockIO<P16> {
   // 18 chunks of 4 halves, then one float norm per site
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, float (&c)[36]) {
     const uint2* q = reinterpret_cast<const uint2*>(base);
-    const float n = reinterpret_cast<const float*>(q + 18 * Vh)[i];
+    const float n = ld_cp<CP>(&reinterpret_cast<const float*>(q + 18 * Vh)[i]);
     q += (size_t)9 * b * Vh;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       H4 v;
-      v.u = q[k * Vh + i];
+      v.u = ld_cp<CP>(&q[k * Vh + i]);
       float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
       c[4 * k] = f0.x * n;
       c[4 * k + 1] = f0.y * n;
@@ -211,14 +214,14 @@ __device__ __forceinline__ void clover_block_mv(const R (&c)[36], const R (&in)[
   }
 }
 
-template <int P>
+template <int P, int CP = 0>
 __device__ __forceinline__ void clover_apply_site(const void* clov, long Vh, long i, const typename SpinorIO<P>::R (&in)[24],
                                                   typename SpinorIO<P>::R (&out)[24]) {
   using R = typename SpinorIO<P>::R;
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     R c[36];
-    CloverBlockIO<P>::load(clov, Vh, i, b, c);
+    CloverBlockIO<P>::template load<CP>(clov, Vh, i, b, c);
     clover_block_mv<R>(c, in, out, b);
   }
 }
@@ -392,9 +395,9 @@ __device__ __forceinline__ void hop_prefetched(const HopLaunch& h, const HopSite
 // [k*Lt/8, (k+1)*Lt/8) and walks it as (xy tile fastest, z within a block of tile_zb planes, t,
 // z block), so the t +- 1 and z +- 1 neighbours of a site are read again within a few work-groups
 // on the same XCD and hit its L2.  Linear order: contiguous sites with the XCD remap of xcd_tile.
-__device__ __forceinline__ int hop_site_index(const HopLaunch& h) {
+__device__ __forceinline__ int hop_site_index(const HopLaunch& h, int blk, int nblk) {
   if (h.tiled) {
-    const int b = blockIdx.x, xcd = b & 7;
+    const int b = blk, xcd = b & 7;
     int q = b >> 3;
     const int wp = q % h.tile_w;
     q /= h.tile_w;
@@ -405,15 +408,16 @@ __device__ __forceinline__ int hop_site_index(const HopLaunch& h) {
     const int z = zblk * h.tile_zb + zi, t = xcd * h.tile_t8 + tl;
     return (t * h.g.L[2] + z) * (h.g.Lxh * h.g.L[1]) + wp * (int)blockDim.x + (int)threadIdx.x;
   }
-  return xcd_tile(blockIdx.x, gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
+  return xcd_tile(blk, nblk) * (int)blockDim.x + (int)threadIdx.x;
 }
 
+// work-group blk of nblk of the launch h
 template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
-__global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
+__device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) {
   using R = typename SpinorIO<P>::R;
   if (h.skip && *h.skip) return;
   if (h.run && !*h.run) return;
-  const int i = hop_site_index(h);
+  const int i = hop_site_index(h, blk, nblk);
   double nsq = 0.0;
   if (i < h.g.Vh) {
     int x[4];
@@ -484,19 +488,36 @@ __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
   if constexpr (NORM) {
     double v[1] = {nsq};
     block_sum<1>(v);
-    if (threadIdx.x == 0) h.partials[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) h.partials[blk] = v[0];
   }
 }
 
-template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipStream_t s) {
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
+__global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
+  hop_body<P, RECON, ISIGN, EPI, NORM, PF>(h, blockIdx.x, gridDim.x);
+}
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
+__global__ void __launch_bounds__(256) k_hop_b(const HopLaunch* __restrict__ tab, const int4* __restrict__ map) {
+  const int4 m = map[blockIdx.x];
+  hop_body<P, RECON, ISIGN, EPI, NORM, PF>(tab[m.x], m.y, m.z);
+}
+
+template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
   const int nb = (h.g.Vh + h.block - 1) / h.block;
   const bool norm = h.partials != nullptr;
+#define CMX_HOP_GO(IS, EP, NM)                                                                            \
+  if (bg)                                                                                                 \
+    hipLaunchKernelGGL((k_hop_b<P, RECON, IS, EP, NM, PF>), dim3(bg->nblocks), dim3(h.block), 0, s,      \
+                       static_cast<const HopLaunch*>(bg->tab), bg->map);                                  \
+  else                                                                                                    \
+    hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, NM, PF>), dim3(nb), dim3(h.block), 0, s, h);
 #define CMX_HOP_CASE(IS, EP)                                                                              \
   if (h.isign == IS && h.epi == EP) {                                                                     \
-    if (norm)                                                                                             \
-      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, true, PF>), dim3(nb), dim3(h.block), 0, s, h);         \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, false, PF>), dim3(nb), dim3(h.block), 0, s, h);        \
+    if (norm) {                                                                                           \
+      CMX_HOP_GO(IS, EP, true)                                                                            \
+    } else {                                                                                              \
+      CMX_HOP_GO(IS, EP, false)                                                                           \
+    }                                                                                                     \
     return;                                                                                               \
   }
   CMX_HOP_CASE(1, EPI_NONE)
@@ -508,6 +529,7 @@ template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipSt
   CMX_HOP_CASE(-1, EPI_XPAY)
   CMX_HOP_CASE(-1, EPI_CLOV_XPAY)
 #undef CMX_HOP_CASE
+#undef CMX_HOP_GO
   throw Error(CMX_ERR_ARG, "launch_hop: bad isign/epilogue");
 }
 

@@ -11,7 +11,7 @@ namespace cmx {
 
 namespace {
 
-template <int NB, int ISIGN>
+template <int NB, int ISIGN, int LCP>
 __global__ void __launch_bounds__(256) k_hop_multi(MultiHopLaunch M, int r0) {
   const HopLaunch& h = M.h;
   const int Lxh = h.g.Lxh, Ly = h.g.L[1], Lz = h.g.L[2], Lt = h.g.L[3];
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_hop_multi(MultiHopLaunch M, int r0) {
     const bool fwd = (step & 1) == 0;
     const __amdgpu_buffer_rsrc_t rl = make_rsrc(
         static_cast<const char*>(h.gauge[fwd ? h.cb : 1 - h.cb]) + (size_t)mu * 24 * Vh, (unsigned)(24 * Vh));
-    fast_load_link<P16>(rl, Vh, fwd ? i : nb[step], lk[slot]);
+    fast_load_link<P16, LCP>(rl, Vh, fwd ? i : nb[step], lk[slot]);
 #pragma unroll
     for (int k = 0; k < NB; ++k) fast_load_spinor<P16>(rin[k], rnin[k], M.in[r0 + k].stride, nb[step], sp[slot][k]);
   };
@@ -119,10 +119,18 @@ template <int NB> void launch_nb(const MultiHopLaunch& M, int r0, hipStream_t s)
   const HopLaunch& h = M.h;
   const int Pl = h.g.Lxh * h.g.L[1];
   const dim3 grid(((Pl + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3]);
-  if (h.isign == 1)
-    hipLaunchKernelGGL((k_hop_multi<NB, 1>), grid, dim3(h.block), 0, s, M, r0);
-  else
-    hipLaunchKernelGGL((k_hop_multi<NB, -1>), grid, dim3(h.block), 0, s, M, r0);
+  const bool lnt = (h.fast & HOP_FAST_STREAM) != 0;
+  if (h.isign == 1) {
+    if (lnt)
+      hipLaunchKernelGGL((k_hop_multi<NB, 1, CMX_NT>), grid, dim3(h.block), 0, s, M, r0);
+    else
+      hipLaunchKernelGGL((k_hop_multi<NB, 1, 0>), grid, dim3(h.block), 0, s, M, r0);
+  } else {
+    if (lnt)
+      hipLaunchKernelGGL((k_hop_multi<NB, -1, CMX_NT>), grid, dim3(h.block), 0, s, M, r0);
+    else
+      hipLaunchKernelGGL((k_hop_multi<NB, -1, 0>), grid, dim3(h.block), 0, s, M, r0);
+  }
 }
 
 }  // namespace

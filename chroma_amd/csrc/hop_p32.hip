@@ -2,15 +2,15 @@
 #include "hop_fast.h"
 
 namespace cmx {
-void launch_hop_p32(const HopLaunch& h, hipStream_t s) {
+void launch_hop_p32(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
   if (h.fast) {
-    launch_hop_fast_t<P32>(h, s);
+    launch_hop_fast_t<P32>(h, s, bg);
     return;
   }
   if (h.recon == 18) {
-    if (h.prefetch) launch_hop_t<P32, 18, true>(h, s); else launch_hop_t<P32, 18, false>(h, s);
+    if (h.prefetch) launch_hop_t<P32, 18, true>(h, s, bg); else launch_hop_t<P32, 18, false>(h, s, bg);
   } else {
-    if (h.prefetch) launch_hop_t<P32, 12, true>(h, s); else launch_hop_t<P32, 12, false>(h, s);
+    if (h.prefetch) launch_hop_t<P32, 12, true>(h, s, bg); else launch_hop_t<P32, 12, false>(h, s, bg);
   }
 }
 }  // namespace cmx

@@ -2,5 +2,5 @@
 #include "hop_kernel.h"
 
 namespace cmx {
-void launch_hop_p64(const HopLaunch& h, hipStream_t s) { launch_hop_t<P64, 18, false>(h, s); }
+void launch_hop_p64(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) { launch_hop_t<P64, 18, false>(h, s, bg); }
 }  // namespace cmx

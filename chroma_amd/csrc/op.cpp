@@ -9,16 +9,22 @@
 #include <map>
 #include <mutex>
 
+#include "batch.h"
 #include "parallel.h"
 #include "setup.h"
 
 namespace cmx {
 
-void launch_hop(const HopLaunch& h, hipStream_t s) {
+void launch_hop(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
+  if (!bg)
+    if (Recorder* r = active_recorder()) {
+      record_hop(*r, h);
+      return;
+    }
   switch (h.prec) {
-    case P64: launch_hop_p64(h, s); break;
-    case P32: launch_hop_p32(h, s); break;
-    case P16: launch_hop_p16(h, s); break;
+    case P64: launch_hop_p64(h, s, bg); break;
+    case P32: launch_hop_p32(h, s, bg); break;
+    case P16: launch_hop_p16(h, s, bg); break;
     default: throw Error(CMX_ERR_ARG, "hop: bad precision");
   }
 }
@@ -38,8 +44,10 @@ int hop_apply_config(HopLaunch& h, const HopConfig& c) {
       if (std::fabs(h.ph[mu][k][1]) > 1e-12 || std::fabs(std::fabs(h.ph[mu][k][0]) - 1.0) > 1e-6) real_phases = false;
       h.sgn[mu][k] = (float)(h.ph[mu][k][0] < 0 ? -1.0 : 1.0);
     }
-  h.fast = c.fast && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
-           h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536;
+  h.fast = ((c.fast & 1) && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
+            h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536)
+               ? c.fast
+               : 0;
   h.inv_lxh = 1.0f / (float)h.g.Lxh;
   if (h.fast) {  // the plane-grid kernel tiles whole planes: any block size, Lt % 8 and Lz % zb only
     if (c.tiled && h.g.L[3] % 8 == 0 && c.zb > 0 && h.g.L[2] % c.zb == 0) {
@@ -145,7 +153,7 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
 
 Op::~Op() {
   for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream && !shared_stream) (void)hipStreamDestroy(stream);
   if (hS) (void)hipHostFree(hS);
   if (hF) (void)hipHostFree(hF);
 }
@@ -197,10 +205,12 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
                   nn.as<double>(), op->stream);
     // deterministic sums of the per-site log|det| and negative-pivot indicators
     int nb = sum_partials((int)op->Vh, ld.as<double>(), op->partials.as<double>(), op->stream);
-    launch_finalize(op->partials.as<double>(), nb, 1, op->state_S.as<double>(), op->state_F.as<int>(), PG_STORE, 0,
+    launch_finalize(FinLaunch{op->partials.as<double>(), nb, 1, op->state_S.as<double>(), op->state_F.as<int>(),
+                              PG_STORE, 0},
                     op->stream);
     nb = sum_partials((int)op->Vh, nn.as<double>(), op->partials.as<double>(), op->stream);
-    launch_finalize(op->partials.as<double>(), nb, 1, op->state_S.as<double>(), op->state_F.as<int>(), PG_STORE, 1,
+    launch_finalize(FinLaunch{op->partials.as<double>(), nb, 1, op->state_S.as<double>(), op->state_F.as<int>(),
+                              PG_STORE, 1},
                     op->stream);
     CMX_HIP(hipGetLastError());
     CMX_HIP(hipMemcpyAsync(op->hS, op->state_S.p, 2 * sizeof(double), hipMemcpyDeviceToHost, op->stream));
@@ -212,6 +222,38 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
   if (p.autotune_dslash) op->autotune();
   CMX_HIP(hipStreamSynchronize(op->stream));
   return op.release();
+}
+
+Op* clone_op(Op& o, int prec) {
+  o.ensure_prec(prec);
+  std::unique_ptr<Op> c(new Op());
+  c->gd = o.gd;
+  c->par = o.par;
+  c->g = o.g;
+  c->V = o.V;
+  c->Vh = o.Vh;
+  c->fact = o.fact;
+  std::memcpy(c->recon, o.recon, sizeof(o.recon));
+  std::memcpy(c->hcfg, o.hcfg, sizeof(o.hcfg));
+  c->multi_group = o.multi_group;
+  for (int p_ = 0; p_ < 3; ++p_) {
+    for (int q = 0; q < 2; ++q) c->gauge[p_][q].alias(o.gauge[p_][q]);
+    c->clov_odd[p_].alias(o.clov_odd[p_]);
+    c->clovinv_even[p_].alias(o.clovinv_even[p_]);
+  }
+  c->clov64_even.alias(o.clov64_even);
+  c->logdet_even = o.logdet_even;
+  c->nneg_even = o.nneg_even;
+  c->stream = o.stream;
+  c->shared_stream = true;
+  c->npartials = o.npartials;
+  c->partials.alloc(sizeof(double) * c->npartials);
+  c->state_S.alloc(sizeof(double) * S_NSLOTS);
+  c->state_F.alloc(sizeof(int) * F_NFLAGS);
+  CMX_HIP(hipHostMalloc((void**)&c->hS, sizeof(double) * S_NSLOTS, 0));
+  CMX_HIP(hipHostMalloc((void**)&c->hF, sizeof(int) * F_NFLAGS, 0));
+  c->staging.alias(o.staging);
+  return c.release();
 }
 
 void Op::ensure_prec(int prec) {
@@ -345,7 +387,9 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
   std::memset(&M, 0, sizeof(M));
   M.h = hop_base(prec, 1);
   HopConfig c = hcfg[prec][1];
-  c.fast = 1;  // the multi-source kernel is a plane-grid kernel whatever the single-source choice
+  // the multi-source kernel is a plane-grid kernel whatever the single-source choice; it keeps the
+  // link cache policy of that choice
+  c.fast = 1 | (c.fast & HOP_FAST_STREAM);
   hop_apply_config(M.h, c);
   if (!M.h.fast) return false;
   M.h.isign = isign;
@@ -432,14 +476,24 @@ void Op::blas(int opc, std::initializer_list<const Vec*> vs, int ia, int ib, int
   b.partials = partials.as<double>();
   b.skip_flag = skip;
   b.run_flag = run;
+  if (Recorder* r = active_recorder()) {
+    record_blas(*r, opc, b, prec);
+    return;
+  }
   launch_blas(opc, b, prec, stream);
 }
 
 void Op::finalize(int nparts, int nred, int prog, int slot) {
-  launch_finalize(partials.as<double>(), nparts, nred, state_S.as<double>(), state_F.as<int>(), prog, slot, stream);
+  const FinLaunch f{partials.as<double>(), nparts, nred, state_S.as<double>(), state_F.as<int>(), prog, slot};
+  if (Recorder* r = active_recorder()) {
+    record_fin(*r, f);
+    return;
+  }
+  launch_finalize(f, stream);
 }
 
 void Op::sync() {
+  CMX_REQUIRE(active_recorder() == nullptr, CMX_ERR_ARG, "synchronisation inside a recorded sequence");
   CMX_HIP(hipGetLastError());
   CMX_HIP(hipStreamSynchronize(stream));
 }
@@ -486,9 +540,11 @@ void Op::autotune() {
       blas(B_ZERO, {&in});
       std::vector<HopConfig> cands;
       for (int block : {64, 128, 256}) {
-        if (prec != P64) {  // plane-grid kernel, linear or XCD-slab order
-          cands.push_back(HopConfig{block, 0, 1, 0, 1});
-          for (int zb : {1, 4, 8}) cands.push_back(HopConfig{block, 1, zb, 0, 1});
+        if (prec != P64) {  // plane-grid kernel, linear or XCD-slab order, links streamed or not
+          for (int fm : {1, 1 | HOP_FAST_STREAM}) {
+            cands.push_back(HopConfig{block, 0, 1, 0, fm});
+            for (int zb : {1, 4, 8}) cands.push_back(HopConfig{block, 1, zb, 0, fm});
+          }
         }
         for (int pf : {0, 1}) {
           if (pf && prec == P64) continue;

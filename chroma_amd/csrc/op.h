@@ -10,19 +10,21 @@
 namespace cmx {
 
 // Owning device allocation
+// Owning device allocation (or, after alias(), a non-owning view of another one)
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool own = true;
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), own(o.own) { o.p = nullptr, o.bytes = 0, o.own = true; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p = o.p, bytes = o.bytes;
-      o.p = nullptr, o.bytes = 0;
+      p = o.p, bytes = o.bytes, own = o.own;
+      o.p = nullptr, o.bytes = 0, o.own = true;
     }
     return *this;
   }
@@ -32,9 +34,14 @@ struct DevBuf {
     if (n) CMX_HIP(hipMalloc(&p, n));
     bytes = n;
   }
+  // refers to o's memory without owning it (o must outlive this buffer)
+  void alias(const DevBuf& o) {
+    release();
+    p = o.p, bytes = o.bytes, own = false;
+  }
   void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr, bytes = 0;
+    if (p && own) (void)hipFree(p);
+    p = nullptr, bytes = 0, own = true;
   }
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
@@ -86,6 +93,7 @@ struct Op {
   double logdet_even = 0;
   long nneg_even = 0;
   hipStream_t stream = nullptr;
+  bool shared_stream = false;  // a clone: the stream and operator fields belong to another Op
   DevBuf partials;
   int npartials = 0;
   DevBuf state_S, state_F;
@@ -145,6 +153,10 @@ struct Op {
 
 std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);
 Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
+// A second solver context on the operator o: shares o's links, clover fields, staging buffer and
+// stream (o must outlive it) and owns its solver state and workspace.  o's fields for precision
+// prec are created first.
+Op* clone_op(Op& o, int prec);
 
 // Solvers (solvers.cpp)
 // Learned preconditioner (learned.cpp)
@@ -156,5 +168,10 @@ void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
 
 // mdagm: solve M^dag M psi = chi instead of M psi = chi
 void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
+// The same for n problems at once (batch.h): problem i solves on ops[i] (distinct solver contexts,
+// one stream); solver RELIABLE_CG or CG
+void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const cmx_batch_params& bp,
+                 double* const* psi, const double* const* chi, bool mdagm, cmx_solver_result* res,
+                 cmx_batch_result& bres);
 
 }  // namespace cmx

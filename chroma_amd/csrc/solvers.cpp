@@ -18,6 +18,7 @@
 #include <utility>
 #include <vector>
 
+#include "batch.h"
 #include "op.h"
 
 namespace cmx {
@@ -163,43 +164,63 @@ double flops_per_apply(const Op& op) {
 
 // ---- CG_INVERTER: CGNE on M^dag M in FP64 ---------------------------------------------------------
 // mdagm: chi is already the source of M^dag M psi = chi (MdagMSysSolverCG, syssolver_mdagm_cg.h:70)
-void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
-  const int n = op.subset_sites();
-  Vec &chi_tmp = op.work(10, P64, n), &r = op.work(11, P64, n), &pv = op.work(12, P64, n),
-      &mp = op.work(13, P64, n), &mmp = op.work(14, P64, n);
-  const int nb = blas_blocks(n);
-  // chi_tmp = M^dag chi  (syssolver_linop_cg.h:65-66)
-  if (mdagm)
-    op.blas(B_COPY, {&chi_tmp, &chi});
-  else
-    op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
-  const double chi_sq = norm2(op, chi_tmp);
-  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
-  Timer tm(op.stream);
-  tm.start();
-  // r = chi_tmp - M^dag M psi ; cp = |r|^2 ; p = r   (invcg2.cc:109-131)
-  op.apply(P64, mp.view(), psi.view(), +1, nullptr);
-  op.apply(P64, mmp.view(), mp.view(), -1, nullptr);
-  op.blas(B_XMY_NORM, {&r, &chi_tmp, &mmp});
-  op.finalize(nb, 1, PG_CG_INIT);
-  op.blas(B_COPY, {&pv, &r});
-  int* F = op.state_F.as<int>();
-  auto iter = [&]() {
+// The pieces (setup, one iteration, finish) serve the single solve and the batched solves.
+struct CGState {
+  Op& op;
+  const cmx_solver_params& p;
+  Vec& psi;
+  const Vec& chi;
+  bool mdagm;
+  int n, nb;
+  Vec &chi_tmp, &r, &pv, &mp, &mmp;
+  CGState(Op& o, const cmx_solver_params& p_, Vec& psi_, const Vec& chi_, bool mdagm_)
+      : op(o), p(p_), psi(psi_), chi(chi_), mdagm(mdagm_), n(o.subset_sites()), nb(blas_blocks(n)),
+        chi_tmp(o.work(10, P64, n)), r(o.work(11, P64, n)), pv(o.work(12, P64, n)), mp(o.work(13, P64, n)),
+        mmp(o.work(14, P64, n)) {}
+  // chi_tmp = M^dag chi, the tolerance, and the start r = chi_tmp - M^dag M psi ; p = r
+  void setup(Timer* tm) {
+    if (mdagm)  // syssolver_linop_cg.h:65-66
+      op.blas(B_COPY, {&chi_tmp, &chi});
+    else
+      op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+    const double chi_sq = norm2(op, chi_tmp);
+    reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+    if (tm) tm->start();
+    // invcg2.cc:109-131
+    op.apply(P64, mp.view(), psi.view(), +1, nullptr);
+    op.apply(P64, mmp.view(), mp.view(), -1, nullptr);
+    op.blas(B_XMY_NORM, {&r, &chi_tmp, &mmp});
+    op.finalize(nb, 1, PG_CG_INIT);
+    op.blas(B_COPY, {&pv, &r});
+  }
+  void iter() {
+    int* F = op.state_F.as<int>();
     const int np = op.apply(P64, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);  // Mp, |Mp|^2
     op.finalize(np, 1, PG_CG_D);                                                                 // a = c/d
     op.apply(P64, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);                               // M^dag M p
     op.blas(B_AXPY_NORM, {&r, &mmp}, S_NEGA, 0, F_DONE);                                         // r -= a M^dag M p
     op.finalize(nb, 1, PG_CG_R);                                                                 // cp, b, test
     op.blas(B_CG_UPDATE, {&psi, &pv, &r}, S_A, S_B, F_DONE);                                     // psi += a p; p = r + b p
-  };
-  res.n_count = run_batches(op, p, 1 + (mdagm ? 100 : 0), psi, chi, iter);
+  }
+  // results once the device has stopped (state read back)
+  void finish(double seconds, cmx_solver_result& res) {
+    res.n_count = op.hF[F_NCOUNT];
+    res.solve_seconds = seconds;
+    const int err = op.hF[F_ERR];
+    res.gflops = (res.n_count * (2 * flops_per_apply(op) + 10 * 12.0 * n) + 2 * flops_per_apply(op)) /
+                 std::max(seconds, 1e-12) * 1e-9;
+    if (err == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "CG: breakdown (|M p| = 0 or not finite)");
+    if (err == CMX_ERR_NOCONV) res.n_count = p.max_iter;  // InvCG2 returns MaxCG with a warning
+  }
+};
+
+void solve_cg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
+  CGState st(op, p, psi, chi, mdagm);
+  Timer tm(op.stream);
+  st.setup(&tm);
+  run_batches(op, p, 1 + (mdagm ? 100 : 0), psi, chi, [&] { st.iter(); });
   read_state(op);
-  res.solve_seconds = tm.stop();
-  const int err = op.hF[F_ERR];
-  res.gflops = (res.n_count * (2 * flops_per_apply(op) + 10 * 12.0 * n) + 2 * flops_per_apply(op)) /
-               std::max(res.solve_seconds, 1e-12) * 1e-9;
-  if (err == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "CG: breakdown (|M p| = 0 or not finite)");
-  if (err == CMX_ERR_NOCONV) res.n_count = p.max_iter;  // InvCG2 returns MaxCG with a warning
+  st.finish(tm.stop(), res);
 }
 
 // ---- BICGSTAB_INVERTER ----------------------------------------------------------------------------
@@ -242,46 +263,88 @@ void solve_bicgstab(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
 }
 
 // ---- reliable-update mixed-precision CG on M^dag M ----------------------------------------------------
-void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
-  const int n = op.subset_sites();
-  const int ps = p.sloppy_prec;
-  CMX_REQUIRE(ps == P32 || ps == P16, CMX_ERR_ARG, "reliable CG: sloppy precision must be single or half");
-  const int nb = blas_blocks(n);
-  Vec &chi_tmp = op.work(30, P64, n), &b = op.work(31, P64, n), &r_dble = op.work(32, P64, n),
-      &x_dble = op.work(33, P64, n), &tmp1 = op.work(34, P64, n), &tmp2 = op.work(35, P64, n);
-  Vec &r = op.work(36, P32, n), &x = op.work(37, P32, n), &pv = op.work(38, ps, n), &mp = op.work(39, ps, n),
-      &mmp = op.work(40, ps, n);
-  // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125); the MdagM solver takes chi as is
-  // (syssolver_mdagm_rel_cg_clover.h:126-133)
-  if (mdagm)
-    op.blas(B_COPY, {&chi_tmp, &chi});
-  else
-    op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
-  const double chi_sq = norm2(op, chi_tmp);
-  reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
-  Timer tm(op.stream);
-  tm.start();
-  int* F = op.state_F.as<int>();
-  // b = chi - A^dag A psi ; r = b ; x = 0   (reliable_cg.cc:44-64)
-  op.apply(P64, tmp1.view(), psi.view(), +1, nullptr);
-  op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr);
-  op.blas(B_XMY_NORM_COPY, {&b, &chi_tmp, &tmp2, &r});
-  op.finalize(nb, 1, PG_RCG_INIT);
-  op.blas(B_ZERO, {&x});
-  op.blas(B_ZERO, {&pv});
-  // One host synchronisation per iteration: the flags of the updateR / updateX decision are read
-  // back, and the FP64 reliable-update kernels are launched only when the device asks for them
-  // (an early-exit launch of a full-lattice grid costs tens of microseconds, more than the read-back).
-  long launched = 0;
-  GraphCache gc(op, p.graph != 0, 3 + 10 * ps + (mdagm ? 100 : 0), psi, chi);
-  auto inner = [&]() {
+struct RelCGState {
+  Op& op;
+  const cmx_solver_params& p;
+  Vec& psi;
+  const Vec& chi;
+  bool mdagm;
+  int n, ps, nb;
+  Vec &chi_tmp, &b, &r_dble, &x_dble, &tmp1, &tmp2;
+  Vec &r, &x, &pv, &mp, &mmp;
+  static int check_prec(int ps) {
+    CMX_REQUIRE(ps == P32 || ps == P16, CMX_ERR_ARG, "reliable CG: sloppy precision must be single or half");
+    return ps;
+  }
+  RelCGState(Op& o, const cmx_solver_params& p_, Vec& psi_, const Vec& chi_, bool mdagm_)
+      : op(o), p(p_), psi(psi_), chi(chi_), mdagm(mdagm_), n(o.subset_sites()), ps(check_prec(p_.sloppy_prec)),
+        nb(blas_blocks(n)), chi_tmp(o.work(30, P64, n)), b(o.work(31, P64, n)), r_dble(o.work(32, P64, n)),
+        x_dble(o.work(33, P64, n)), tmp1(o.work(34, P64, n)), tmp2(o.work(35, P64, n)), r(o.work(36, P32, n)),
+        x(o.work(37, P32, n)), pv(o.work(38, ps, n)), mp(o.work(39, ps, n)), mmp(o.work(40, ps, n)) {}
+  void setup(Timer* tm) {
+    // chi_tmp = M^dag chi (syssolver_linop_rel_cg_clover.h:121-125); the MdagM solver takes chi as is
+    // (syssolver_mdagm_rel_cg_clover.h:126-133)
+    if (mdagm)
+      op.blas(B_COPY, {&chi_tmp, &chi});
+    else
+      op.apply(P64, chi_tmp.view(), chi.view(), -1, nullptr);
+    const double chi_sq = norm2(op, chi_tmp);
+    reset_state(op, p.rsd_target * p.rsd_target * chi_sq, p.max_iter, p.delta);
+    op.ensure_prec(ps);
+    if (tm) tm->start();
+    // b = chi - A^dag A psi ; r = b ; x = 0   (reliable_cg.cc:44-64)
+    op.apply(P64, tmp1.view(), psi.view(), +1, nullptr);
+    op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr);
+    op.blas(B_XMY_NORM_COPY, {&b, &chi_tmp, &tmp2, &r});
+    op.finalize(nb, 1, PG_RCG_INIT);
+    op.blas(B_ZERO, {&x});
+    op.blas(B_ZERO, {&pv});
+  }
+  void inner() {
+    int* F = op.state_F.as<int>();
     op.blas(B_XPAY, {&pv, &r}, S_BETA, 0, F_DONE);                                          // p = r + beta p
     const int np = op.apply(ps, mp.view(), pv.view(), +1, op.partials.as<double>(), F + F_DONE);
     op.finalize(np, 1, PG_RCG_D);                                                           // c = r_sq; a = c/d
     op.apply(ps, mmp.view(), mp.view(), -1, nullptr, F + F_DONE);
     op.blas(B_AXPY2_NORM, {&x, &r, &pv, &mmp}, S_A, 0, F_DONE);                             // x += a p; r -= a mmp
     op.finalize(nb, 1, PG_RCG_R);                                                           // updateX / updateR
-  };
+  }
+  // r = b - A^dag A x in FP64 (reliable_cg.cc:120-136)
+  void update_r(int skip, int run) {
+    int* F = op.state_F.as<int>();
+    op.blas(B_COPY, {&x_dble, &x}, 0, 0, skip, run);
+    op.apply(P64, tmp1.view(), x_dble.view(), +1, nullptr, skip >= 0 ? F + skip : nullptr, run >= 0 ? F + run : nullptr);
+    op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr, skip >= 0 ? F + skip : nullptr, run >= 0 ? F + run : nullptr);
+    op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp2, &r}, 0, 0, skip, run);
+  }
+  // The update step with the decisions taken on the device: the FP64 kernels run when the flags ask
+  // for them and return at once otherwise (same arithmetic as the host-decided step)
+  void update_gated() {
+    update_r(F_DONE, F_UPDR);
+    op.finalize(nb, 1, PG_RCG_UPD);                                                         // norms, convergence
+    op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble}, 0, 0, F_DONE, F_UPDX);                // psi += x; x = 0; b = r
+  }
+  void finish(double seconds, cmx_solver_result& res) {
+    res.n_count = op.hF[F_NCOUNT];
+    res.solve_seconds = seconds;
+    res.n_reliable = op.hF[F_NUPD];
+    res.gflops = (res.n_count + 1) * (2 * flops_per_apply(op) + 16 * 12.0 * n) / std::max(seconds, 1e-12) * 1e-9;
+    if (op.hF[F_ERR] == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "reliable CG: breakdown");
+    if (op.hF[F_ERR] == CMX_ERR_NOCONV)
+      throw Error(CMX_ERR_NOCONV, "reliable CG: no convergence in " + std::to_string(p.max_iter) + " iterations");
+  }
+};
+
+void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
+  RelCGState st(op, p, psi, chi, mdagm);
+  Timer tm(op.stream);
+  st.setup(&tm);
+  // One host synchronisation per iteration: the flags of the updateR / updateX decision are read
+  // back, and the FP64 reliable-update kernels are launched only when the device asks for them
+  // (an early-exit launch of a full-lattice grid costs tens of microseconds, more than the read-back).
+  long launched = 0;
+  GraphCache gc(op, p.graph != 0, 3 + 10 * st.ps + (mdagm ? 100 : 0), psi, chi);
+  auto inner = [&]() { st.inner(); };
   while (true) {
     if (launched == 0)
       inner();  // eager: performs the lazy allocations of the operator's scratch vectors
@@ -290,27 +353,15 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, b
     CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
     op.sync();
     if (op.hF[F_DONE]) break;  // converged (or stopped) in the previous iteration
-    if (op.hF[F_UPDR]) {       // r = b - A^dag A x in FP64 (reliable_cg.cc:120-136)
-      op.blas(B_COPY, {&x_dble, &x});
-      op.apply(P64, tmp1.view(), x_dble.view(), +1, nullptr);
-      op.apply(P64, tmp2.view(), tmp1.view(), -1, nullptr);
-      op.blas(B_XMY_NORM_COPY, {&r_dble, &b, &tmp2, &r});
-    }
-    op.finalize(nb, 1, PG_RCG_UPD);                                                         // norms, convergence
-    if (op.hF[F_UPDX]) op.blas(B_ADD_ZERO_COPY, {&psi, &x, &b, &r_dble});                   // psi += x; x = 0; b = r
+    if (op.hF[F_UPDR]) st.update_r(-1, -1);
+    op.finalize(st.nb, 1, PG_RCG_UPD);                                                      // norms, convergence
+    if (op.hF[F_UPDX]) op.blas(B_ADD_ZERO_COPY, {&psi, &st.x, &st.b, &st.r_dble});          // psi += x; x = 0; b = r
     if (++launched > (long)p.max_iter + 4) break;                                           // device stops at MaxIter
   }
-  op.blas(B_ADD, {&psi, &x});                                                               // psi += x (reliable_cg.cc:158-163)
-  res.n_count = op.hF[F_NCOUNT];
+  op.blas(B_ADD, {&psi, &st.x});                                                            // psi += x (reliable_cg.cc:158-163)
   read_state(op);
-  res.solve_seconds = tm.stop();
-  res.n_reliable = op.hF[F_NUPD];
-  res.gflops = (res.n_count + 1) * (2 * flops_per_apply(op) + 16 * 12.0 * n) / std::max(res.solve_seconds, 1e-12) * 1e-9;
-  if (op.hF[F_ERR] == CMX_ERR_BREAKDOWN) throw Error(CMX_ERR_BREAKDOWN, "reliable CG: breakdown");
-  if (op.hF[F_ERR] == CMX_ERR_NOCONV)
-    throw Error(CMX_ERR_NOCONV, "reliable CG: no convergence in " + std::to_string(p.max_iter) + " iterations");
+  st.finish(tm.stop(), res);
 }
-
 
 // ---- reliable-update mixed-precision BiCGStab on M (isign) ------------------------------------------
 // Vectors r, r0, p, v, t are stored in the sloppy precision (the operator's input and output), the
@@ -418,7 +469,286 @@ void solve_pass(Op& op, const cmx_solver_params& q, Vec& psi, const Vec& chi, bo
                     : 0.0;
 }
 
+// resid = |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87), or |chi - M^dag M psi|
+// (syssolver_mdagm_cg.h:77-85), and resid / |chi|; workspace slots 2, 3
+void true_residual(Op& op, const Vec& psi, const Vec& chi, bool mdagm, cmx_solver_result& res) {
+  const int n = op.subset_sites();
+  Vec &t = op.work(2, P64, n), &rr = op.work(3, P64, n);
+  op.apply(P64, t.view(), psi.view(), +1, nullptr);
+  if (mdagm) {
+    op.apply(P64, rr.view(), t.view(), -1, nullptr);
+    op.blas(B_COPY, {&t, &rr});
+  }
+  op.blas(B_XMY_NORM, {&rr, &chi, &t});
+  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
+  op.blas(B_NORM2, {&chi});
+  op.finalize(blas_blocks(n), 1, PG_STORE, S_TR + 1);
+  CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, 2 * sizeof(double),
+                         hipMemcpyDeviceToHost, op.stream));
+  op.sync();
+  res.resid = std::sqrt(op.hS[S_TR]);
+  res.rel_resid = op.hS[S_TR + 1] > 0 ? res.resid / std::sqrt(op.hS[S_TR + 1]) : res.resid;
+}
+
 }  // namespace
+
+// ---- batched solves (batch.h) ---------------------------------------------------------------------
+// Each problem's iteration is recorded once (the reliable-update step in its flag-gated form, so that
+// the sequence is fixed), the recordings are merged into batched stages, and the host replays the
+// stages until every problem has stopped.  Problems that stop early return at the first instruction of
+// each later kernel.
+namespace {
+
+// candidate launch shapes for a batch: the plane-grid kernel needs it admitted on every lattice
+std::vector<HopConfig> batch_candidates(const std::vector<Op*>& ops, int ps) {
+  bool fast_ok = ps != P64;
+  for (Op* o : ops) {
+    HopLaunch h = o->hop_base(ps);
+    HopConfig c;
+    c.fast = 1;
+    hop_apply_config(h, c);
+    if (!h.fast) fast_ok = false;
+  }
+  std::vector<HopConfig> c;
+  for (int block : {64, 128, 256}) {
+    c.push_back(HopConfig{block, 0, 1, 0, 0});
+    if (fast_ok) c.push_back(HopConfig{block, 0, 1, 0, 1 | HOP_FAST_STREAM});
+  }
+  return c;
+}
+
+}  // namespace
+
+void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const cmx_batch_params& bp,
+                 double* const* psi_h, const double* const* chi_h, bool mdagm, cmx_solver_result* res,
+                 cmx_batch_result& bres) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int np = (int)ops.size();
+  CMX_REQUIRE(np >= 1, CMX_ERR_ARG, "solve_batch: no problems");
+  CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
+  CMX_REQUIRE(p.solver == CMX_SOLVER_CG || p.solver == CMX_SOLVER_RELIABLE_CG, CMX_ERR_UNSUPPORTED,
+              "solve_batch: CMX_SOLVER_CG or CMX_SOLVER_RELIABLE_CG");
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < i; ++j)
+      CMX_REQUIRE(ops[i] != ops[j], CMX_ERR_ARG, "solve_batch: each problem needs its own solver context");
+  std::memset(&bres, 0, sizeof(bres));
+  const bool rel = p.solver == CMX_SOLVER_RELIABLE_CG;
+  const int ps = rel ? p.sloppy_prec : P64;
+  if (rel) RelCGState::check_prec(ps);
+  hipStream_t s0 = ops[0]->stream;
+  // the operators' launch shapes are restored on exit
+  std::vector<HopConfig> saved(np);
+  for (int i = 0; i < np; ++i) saved[i] = ops[i]->hcfg[ps][0];
+  struct Restore {
+    const std::vector<Op*>& ops;
+    const std::vector<HopConfig>& saved;
+    int ps;
+    ~Restore() {
+      for (size_t i = 0; i < ops.size(); ++i) ops[i]->hcfg[ps][0] = saved[i];
+    }
+  } restore{ops, saved, ps};
+
+  std::vector<Vec*> psi(np), chi(np);
+  std::vector<cmx_solver_params> q(np, p);
+  for (int i = 0; i < np; ++i) {
+    Op& op = *ops[i];
+    const int n = op.subset_sites();
+    psi[i] = &op.work(0, P64, n);
+    chi[i] = &op.work(1, P64, n);
+    op.work(2, P64, n);
+    op.work(3, P64, n);
+    op.pack(chi_h[i], n, *chi[i]);
+    op.pack(psi_h[i], n, *psi[i]);
+    std::memset(&res[i], 0, sizeof(res[i]));
+    // true-residual mode aims the first pass a little below the target, as the single solve does
+    if (p.true_residual) q[i].rsd_target = 0.5 * p.rsd_target;
+  }
+
+  // launch shape: timed candidates (one batched operator application of every problem), or the
+  // operators' own
+  if (bp.tune) {
+    std::vector<std::unique_ptr<Vec>> va, vb, vc;
+    for (Op* o : ops) {
+      const int n = o->subset_sites();
+      o->ensure_prec(ps);
+      va.emplace_back(new Vec(ps, n));
+      vb.emplace_back(new Vec(ps, n));
+      vc.emplace_back(new Vec(ps, n));
+      o->blas(B_ZERO, {va.back().get()});
+    }
+    for (Op* o : ops) o->sync();
+    float best = 1e30f;
+    HopConfig bestc{};
+    hipEvent_t e0, e1;
+    CMX_HIP(hipEventCreate(&e0));
+    CMX_HIP(hipEventCreate(&e1));
+    for (const HopConfig& c : batch_candidates(ops, ps)) {
+      for (Op* o : ops) o->hcfg[ps][0] = c;
+      std::vector<Recorder> recs(np);
+      for (int i = 0; i < np; ++i) {
+        RecordScope rs(recs[i]);
+        Op& op = *ops[i];
+        op.apply(ps, vb[i]->view(), va[i]->view(), +1, op.partials.as<double>());
+        op.apply(ps, vc[i]->view(), vb[i]->view(), -1, nullptr);
+      }
+      BatchPlan plan;
+      plan.build(recs);
+      plan.run(s0);
+      CMX_HIP(hipEventRecord(e0, s0));
+      for (int k = 0; k < 4; ++k) plan.run(s0);
+      CMX_HIP(hipEventRecord(e1, s0));
+      CMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) best = ms, bestc = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (Op* o : ops) o->hcfg[ps][0] = bestc;
+    CMX_HIP(hipStreamSynchronize(s0));
+  }
+  {
+    const HopLaunch h = ops[0]->hop_base(ps);
+    bres.hop_block = h.block;
+    bres.hop_fast = h.fast;
+  }
+
+  // passes: every problem, then (true-residual mode) the problems whose FP64 residual is above target,
+  // restarted from their current solution with a tightened inner tolerance
+  std::vector<int> act(np);
+  for (int i = 0; i < np; ++i) act[i] = i;
+  for (int restart = 0; !act.empty(); ++restart) {
+    const int na = (int)act.size();
+    std::vector<std::unique_ptr<CGState>> cg(na);
+    std::vector<std::unique_ptr<RelCGState>> rcg(na);
+    for (int k = 0; k < na; ++k) {
+      const int i = act[k];
+      if (rel) {
+        rcg[k].reset(new RelCGState(*ops[i], q[i], *psi[i], *chi[i], mdagm));
+        rcg[k]->setup(nullptr);
+      } else {
+        cg[k].reset(new CGState(*ops[i], q[i], *psi[i], *chi[i], mdagm));
+        cg[k]->setup(nullptr);
+      }
+    }
+    for (int i : act) ops[i]->sync();
+    // one iteration per problem, recorded (the reliable update in its flag-gated form)
+    std::vector<Recorder> recs(na);
+    for (int k = 0; k < na; ++k) {
+      RecordScope rs(recs[k]);
+      if (rel) {
+        rcg[k]->inner();
+        rcg[k]->update_gated();
+      } else {
+        cg[k]->iter();
+      }
+    }
+    // replay until every problem has stopped (the device stops each at its MaxIter); the problems
+    // that have stopped leave the batched grid at the next check
+    std::vector<int> live(na);
+    for (int k = 0; k < na; ++k) live[k] = k;
+    BatchPlan plan;
+    auto rebuild = [&]() {
+      std::vector<Recorder> sub;
+      sub.reserve(live.size());
+      for (int k : live) sub.push_back(recs[k]);
+      plan.build(sub);
+    };
+    rebuild();
+    if (restart == 0) {
+      bres.n_stages = (int)plan.stages();
+      bres.blocks_per_iter = plan.blocks();
+    }
+    int maxit = 0;
+    for (int i : act) maxit = std::max(maxit, q[i].max_iter);
+    Timer tm(s0);
+    tm.start();
+    int batch = p.check_every > 0 ? p.check_every : 4;
+    long launched = 0;
+    hipGraphExec_t gx = nullptr;
+    int gx_len = 0;
+    auto drop_graph = [&]() {
+      if (gx) (void)hipGraphExecDestroy(gx);
+      gx = nullptr;
+    };
+    try {
+      while (true) {
+        if (bp.graph && launched > 0) {
+          if (!gx || gx_len != batch) {
+            drop_graph();
+            hipGraph_t gr = nullptr;
+            CMX_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < batch; ++j) plan.run(s0);
+            CMX_HIP(hipStreamEndCapture(s0, &gr));
+            const hipError_t e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            CMX_HIP(e);
+            gx_len = batch;
+          }
+          CMX_HIP(hipGraphLaunch(gx, s0));
+        } else {
+          for (int j = 0; j < batch; ++j) plan.run(s0);
+        }
+        launched += batch;
+        for (int k : live) {
+          Op* o = ops[act[k]];
+          CMX_HIP(hipMemcpyAsync(o->hF, o->state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, s0));
+        }
+        CMX_HIP(hipStreamSynchronize(s0));
+        std::vector<int> still;
+        for (int k : live)
+          if (!ops[act[k]]->hF[F_CONV]) still.push_back(k);
+        if (still.empty() || launched > (long)maxit + 8) break;
+        if (still.size() < live.size()) {
+          live = still;
+          drop_graph();
+          rebuild();
+        }
+        if (p.check_every <= 0 && batch < 32) batch *= 2;
+      }
+    } catch (...) {
+      drop_graph();
+      throw;
+    }
+    drop_graph();
+    const double secs = tm.stop();
+    bres.solve_seconds += secs;
+    bres.n_iter += (int)launched;
+    // per problem: final update, results, true residual
+    std::vector<int> again;
+    for (int k = 0; k < na; ++k) {
+      const int i = act[k];
+      Op& op = *ops[i];
+      if (rel) op.blas(B_ADD, {psi[i], &rcg[k]->x});  // psi += x (reliable_cg.cc:158-163)
+      read_state(op);
+      cmx_solver_result part;
+      std::memset(&part, 0, sizeof(part));
+      if (rel)
+        rcg[k]->finish(secs, part);
+      else
+        cg[k]->finish(secs, part);
+      res[i].n_count += part.n_count;
+      res[i].n_reliable += part.n_reliable;
+      res[i].gflops += part.gflops * secs;
+      true_residual(op, *psi[i], *chi[i], mdagm, res[i]);
+      if (p.true_residual && res[i].rel_resid > p.rsd_target && restart < 8 && res[i].n_count < p.max_iter) {
+        q[i].rsd_target = q[i].rsd_target * std::max(0.02, 0.5 * p.rsd_target / res[i].rel_resid);
+        q[i].max_iter = std::max(1, p.max_iter - res[i].n_count);
+        res[i].n_restarts++;
+        again.push_back(i);
+      }
+    }
+    act = again;
+  }
+  for (int i = 0; i < np; ++i) {
+    Op& op = *ops[i];
+    op.unpack(psi_h[i], op.subset_sites(), *psi[i]);
+    res[i].solve_seconds = bres.solve_seconds;
+    res[i].gflops = bres.solve_seconds > 0 ? res[i].gflops / bres.solve_seconds : 0.0;
+  }
+  bres.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < np; ++i) res[i].total_seconds = bres.total_seconds;
+}
 
 void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_h, bool mdagm, cmx_solver_result& res) {
   CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
@@ -431,7 +761,9 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
   cmx_solver_params q = p;
   // true-residual mode aims the first pass a little below the target, so that a restart is rare
   if (p.true_residual) q.rsd_target = 0.5 * p.rsd_target;
-  Vec &t = op.work(2, P64, n), &rr = op.work(3, P64, n);
+  // the true-residual workspace is allocated before the solver's graphs are captured
+  op.work(2, P64, n);
+  op.work(3, P64, n);
   for (int restart = 0;; ++restart) {
     cmx_solver_result part;
     std::memset(&part, 0, sizeof(part));
@@ -440,22 +772,7 @@ void solve(Op& op, const cmx_solver_params& p, double* psi_h, const double* chi_
     res.n_reliable += part.n_reliable;
     res.solve_seconds += part.solve_seconds;
     res.gflops += part.gflops * part.solve_seconds;
-    // true residual |chi - A psi| in FP64 (syssolver_linop_cg.h:80-87), or |chi - M^dag M psi|
-    // (syssolver_mdagm_cg.h:77-85)
-    op.apply(P64, t.view(), psi.view(), +1, nullptr);
-    if (mdagm) {
-      op.apply(P64, rr.view(), t.view(), -1, nullptr);
-      op.blas(B_COPY, {&t, &rr});
-    }
-    op.blas(B_XMY_NORM, {&rr, &chi, &t});
-    op.finalize(blas_blocks(n), 1, PG_STORE, S_TR);
-    op.blas(B_NORM2, {&chi});
-    op.finalize(blas_blocks(n), 1, PG_STORE, S_TR + 1);
-    CMX_HIP(hipMemcpyAsync(op.hS + S_TR, static_cast<double*>(op.state_S.p) + S_TR, 2 * sizeof(double),
-                           hipMemcpyDeviceToHost, op.stream));
-    op.sync();
-    res.resid = std::sqrt(op.hS[S_TR]);
-    res.rel_resid = op.hS[S_TR + 1] > 0 ? res.resid / std::sqrt(op.hS[S_TR + 1]) : res.resid;
+    true_residual(op, psi, chi, mdagm, res);
     if (!p.true_residual || res.rel_resid <= p.rsd_target || restart >= 8 || res.n_count >= p.max_iter) break;
     // restart from psi with the inner tolerance scaled by the observed true/inner ratio
     q.rsd_target = q.rsd_target * std::max(0.02, 0.5 * p.rsd_target / res.rel_resid);

@@ -60,6 +60,16 @@ class SolverResult(ctypes.Structure):
                 ("total_seconds", ctypes.c_double), ("gflops", ctypes.c_double), ("n_restarts", ctypes.c_int)]
 
 
+class BatchParams(ctypes.Structure):
+    _fields_ = [("tune", ctypes.c_int), ("graph", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("solve_seconds", ctypes.c_double), ("total_seconds", ctypes.c_double), ("n_iter", ctypes.c_int),
+                ("n_stages", ctypes.c_int), ("blocks_per_iter", ctypes.c_long), ("hop_block", ctypes.c_int),
+                ("hop_fast", ctypes.c_int)]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_long)
@@ -96,6 +106,9 @@ def lib():
                                      ctypes.POINTER(SolverResult)]),
         "cmx_solve_mdagm": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
                                            ctypes.POINTER(SolverResult)]),
+        "cmx_solve_batch": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(SolverParams),
+                                           ctypes.POINTER(BatchParams), ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                           ctypes.c_int, ctypes.POINTER(SolverResult), ctypes.POINTER(BatchResult)]),
         "cmx_net_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
         "cmx_net_destroy": (ctypes.c_int, [_vp]),
         "cmx_net_init_neumann": (ctypes.c_int, [_vp, ctypes.c_double]),
@@ -281,6 +294,8 @@ class Operator:
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
 
     def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0, fast=0, target=0):
+        """fast: 0 generic kernel, 1 plane-grid kernel, 3 plane-grid kernel with the read-once
+        loads (links, clover, xpay input) streamed non-temporally."""
         cfg = (ctypes.c_int * 6)(int(block), int(tiled), int(zb), int(prefetch), int(fast), int(target))
         check(lib().cmx_op_set_hop_config(self.h, int(prec), cfg))
 
@@ -315,6 +330,35 @@ class Operator:
                     solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
                     n_restarts=r.n_restarts) for r in res]
         return (psis[0], out[0]) if single else (psis, out)
+
+
+def solve_batch(ops, chis, psis=None, solver=RELIABLE_CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=HALF,
+                check_every=0, mdagm=False, true_residual=False, tune=True, graph=False):
+    """n independent solves advanced together, every kernel launched once for all of them
+    (cmx_solve_batch).  ops: Operators (an Operator may repeat: the sources of one propagator);
+    chis: subset arrays, one per problem.  Returns (psis, per-problem results, batch summary)."""
+    n = len(ops)
+    assert len(chis) == n
+    chis = [np.ascontiguousarray(c, np.complex128) for c in chis]
+    psis = [np.zeros_like(c) for c in chis] if psis is None else \
+        [np.ascontiguousarray(x, np.complex128).copy() for x in psis]
+    p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
+                     0, int(bool(true_residual)), 16, 0, PRECOND_NONE, 4, 1e-12, 1.0, 1, -1, 0.0, 0)
+    bp = BatchParams(int(bool(tune)), int(bool(graph)))
+    hs = (_vp * n)(*[o.h for o in ops])
+    pp = (_dp * n)(*[_d(x) for x in psis])
+    cp = (_dp * n)(*[_d(c) for c in chis])
+    res = (SolverResult * n)()
+    br = BatchResult()
+    check(lib().cmx_solve_batch(hs, n, ctypes.byref(p), ctypes.byref(bp), pp, cp, int(bool(mdagm)), res,
+                                ctypes.byref(br)))
+    out = [dict(n_count=r.n_count, resid=r.resid, rel_resid=r.rel_resid, n_reliable=r.n_reliable,
+                solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
+                n_restarts=r.n_restarts) for r in res]
+    summary = dict(solve_seconds=br.solve_seconds, total_seconds=br.total_seconds, n_iter=br.n_iter,
+                   n_stages=br.n_stages, blocks_per_iter=br.blocks_per_iter, hop_block=br.hop_block,
+                   hop_fast=br.hop_fast)
+    return psis, out, summary
 
 
 class Net:

@@ -185,6 +185,34 @@ int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* 
 int cmx_solve_mdagm(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
                     cmx_solver_result* res);
 
+/* ---- batched solves: no single counterpart in the reference -------------------------------------
+ * n independent solves advance together: every kernel of an iteration is launched once over a grid
+ * that covers all n problems (work-group -> problem, work-group of that problem), so small volumes
+ * share launches and fill the device.  Problem i solves A_i psi_i = chi_i (mdagm: M_i^dag M_i psi_i =
+ * chi_i) on ops[i]; operators may differ in lattice, gauge field and mass, and may repeat (the 12
+ * sources of quarkProp4_a, quarkprop4_w.cc:70-117, on one operator).  Solver CMX_SOLVER_CG or
+ * CMX_SOLVER_RELIABLE_CG; each problem's iterations, stopping rule and result are those of cmx_solve
+ * with the same launch shapes (true_residual restarts are not done).  psi[i] is the initial guess on
+ * entry; res[i].solve_seconds is the device time of the whole batch. */
+typedef struct {
+  int tune;   /* 1: time candidate launch shapes for the batch (plane-grid or site-linear hopping
+                 kernel, 64/128/256 threads) and keep the fastest; 0: the operators' own shapes, which
+                 must select the same kernel on every problem */
+  int graph;  /* 1: replay the batched iterations from a captured HIP graph */
+} cmx_batch_params;
+typedef struct {
+  double solve_seconds;  /* device time of the batched iterations (HIP events)                   */
+  double total_seconds;  /* wall time of the call                                                 */
+  int n_iter;            /* batched iterations launched                                           */
+  int n_stages;          /* kernels per batched iteration                                         */
+  long blocks_per_iter;  /* work-groups per batched iteration                                     */
+  int hop_block;         /* hopping-kernel launch shape used: threads per work-group              */
+  int hop_fast;          /* 0 site-linear, 1 plane-grid, 3 plane-grid with streamed loads          */
+} cmx_batch_result;
+int cmx_solve_batch(const cmx_op* ops, int n, const cmx_solver_params* p, const cmx_batch_params* bp,
+                    double* const* psi, const double* const* chi, int mdagm, cmx_solver_result* res,
+                    cmx_batch_result* bres);
+
 /* ---- learned preconditioner (CMX_PRECOND_LEARNED): no counterpart in the reference ------------------
  * A gauge-equivariant parallel-transport network on the odd checkerboard,
  *   P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),  g_q(y) = sum_c W2[q][c] sum_p W1[c][p] U_p(y) v(y+p),
@@ -235,8 +263,8 @@ int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
 
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
- * double-buffered direction loads (0/1), plane-grid kernel (0/1; FP16 and FP32 with 12-real links and
- * real boundary phases), target (0 both, 1 the EO operator's fused kernels, 2 the plain hopping
+ * double-buffered direction loads (0/1), plane-grid kernel (0 off, 1 on, 3 on with the loads of data read
+ * once per launch -- links, clover, xpay input -- streamed non-temporally; FP16 and FP32 with 12-real links and real boundary phases), target (0 both, 1 the EO operator's fused kernels, 2 the plain hopping
  * term; for get, 2 reads the plain-hop shape and 0/1 the operator's)}.  No counterpart in the
  * reference beyond QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
 int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[6]);

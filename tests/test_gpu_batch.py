@@ -1,0 +1,116 @@
+"""GPU: batched solves (cmx_solve_batch) -- several problems advancing through one sequence of batched
+kernels -- against the same problems solved one at a time.
+
+With the operators' own launch shapes (tune=0) a batched problem runs the kernels of its single solve
+with the same work-group decomposition and reduction order, so its solution must equal the single
+solve's bit for bit, with the same iteration count.  With tuned shapes the solutions are judged like
+every solve: FP64 true residual <= 1e-10 and agreement with an FP64 CG solution to 1e-8.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from chroma_amd import lib as cmx
+from tests.util import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_problem(L, seed, kappa, prec=cmx.HALF):
+    U = cmx.gauge_generate(L, kind=1, seed=seed, eps=0.3)
+    U = cmx.gauge_apply_bc(L, U, (1, 1, 1, -1))
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), 1.0, sloppy_prec=prec)
+    Vh = int(np.prod(L)) // 2
+    chi = cmx.z2_source((Vh, 4, 3), seed + 1)
+    return op, chi
+
+
+UNEVEN = [((4, 4, 4, 4), 1000, 0.120), ((6, 6, 6, 12), 1001, 0.128), ((8, 8, 8, 16), 1002, 0.134),
+          ((4, 4, 4, 4), 1003, 0.131), ((8, 8, 8, 16), 1004, 0.122)]
+
+
+@pytest.mark.parametrize("solver,sloppy", [(cmx.RELIABLE_CG, cmx.HALF), (cmx.RELIABLE_CG, cmx.SINGLE),
+                                           (cmx.CG, cmx.DOUBLE)])
+def test_batch_uneven_matches_single(solver, sloppy):
+    say("batch uneven", solver, sloppy)
+    probs = [make_problem(L, s, k, cmx.HALF if sloppy == cmx.HALF else cmx.SINGLE) for L, s, k in UNEVEN]
+    ops = [p[0] for p in probs]
+    chis = [p[1] for p in probs]
+    kw = dict(solver=solver, rsd=1e-10, maxit=5000, delta=0.1, sloppy_prec=sloppy, true_residual=True)
+    single = [op.solve(chi, **kw) for op, chi in probs]
+    psis, res, summ = cmx.solve_batch(ops, chis, tune=False, **kw)
+    say("  summary", summ)
+    for k in range(len(probs)):
+        say("  problem", k, "single", single[k][1]["n_count"], "batch", res[k]["n_count"], res[k]["rel_resid"],
+            "restarts", res[k]["n_restarts"])
+        assert res[k]["n_count"] == single[k][1]["n_count"]
+        assert res[k]["n_restarts"] == single[k][1]["n_restarts"]
+        assert np.array_equal(psis[k].view(np.uint64), single[k][0].view(np.uint64))
+        assert res[k]["rel_resid"] <= 1e-10
+
+
+def test_batch_graph_matches_eager():
+    say("batch graph")
+    probs = [make_problem(L, s, k) for L, s, k in UNEVEN[:3]]
+    ops = [p[0] for p in probs]
+    chis = [p[1] for p in probs]
+    kw = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, tune=False, true_residual=True)
+    p0, r0, _ = cmx.solve_batch(ops, chis, graph=False, **kw)
+    for _ in range(2):
+        p1, r1, _ = cmx.solve_batch(ops, chis, graph=True, **kw)
+        for k in range(len(probs)):
+            assert r1[k]["n_count"] == r0[k]["n_count"]
+            assert np.array_equal(p1[k].view(np.uint64), p0[k].view(np.uint64))
+
+
+def test_batch_tuned_uneven():
+    """Tuned launch shape over the batch: every solution meets the FP64 true residual and agrees with
+    an FP64 CG solution."""
+    say("batch tuned")
+    probs = [make_problem(L, s, k) for L, s, k in UNEVEN]
+    ops = [p[0] for p in probs]
+    chis = [p[1] for p in probs]
+    psis, res, summ = cmx.solve_batch(ops, chis, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=5000,
+                                      sloppy_prec=cmx.HALF, true_residual=True, tune=True)
+    say("  summary", summ)
+    assert summ["hop_block"] in (64, 128, 256)
+    for k, (op, chi) in enumerate(probs):
+        ref, _ = op.solve(chi, solver=cmx.CG, rsd=1e-12, maxit=20000)
+        say("  problem", k, res[k]["n_count"], res[k]["rel_resid"], rel(psis[k], ref))
+        assert res[k]["rel_resid"] <= 1e-10
+        assert rel(psis[k], ref) < 1e-8
+
+
+def test_batch_sources_on_one_operator():
+    """The 12 spin-colour point sources of a propagator (quarkProp4_a) on one operator: the repeated
+    operator gets one solver context per source; each solution equals its single solve."""
+    L = (8, 8, 8, 16)
+    say("batch sources on one operator")
+    op, _ = make_problem(L, 1010, 0.115)
+    Vh = int(np.prod(L)) // 2
+    chis = []
+    for s in range(4):
+        for c in range(3):
+            chi = np.zeros((Vh, 4, 3), complex)
+            chi[0, s, c] = 1.0  # the first odd site
+            chis.append(chi)
+    kw = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, true_residual=True)
+    psis, res, summ = cmx.solve_batch([op] * 12, chis, tune=False, **kw)
+    say("  summary", summ)
+    for k in range(12):
+        ref, rr = op.solve(chis[k], **kw)
+        assert res[k]["rel_resid"] <= 1e-10
+        assert res[k]["n_count"] == rr["n_count"]
+        assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
+
+
+def test_batch_rejects_bad_input():
+    op, chi = make_problem((4, 4, 4, 4), 1020, 0.12)
+    with pytest.raises(cmx.CmxError):
+        cmx.solve_batch([op], [chi], solver=cmx.BICGSTAB)

@@ -349,13 +349,15 @@ def test_dslash_multi_source(L, nrhs):
     U = gauge_field(L, 21)
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1, sloppy_prec=cmx.HALF, sloppy_recon=12)
     psis = np.stack([random_fermion(V, 100 + k) for k in range(nrhs)])
-    for cb in (0, 1):
-        for isign in (+1, -1):
-            src = np.ascontiguousarray(psis[:, (1 - cb) * Vh:(2 - cb) * Vh])
-            got = op.dslash_multi(src, isign, cb, cmx.HALF)
-            for k in range(nrhs):
-                ref = oracle.dslash(L, U, psis[k], isign, cb)[cb * Vh:(cb + 1) * Vh]
-                assert rel(got[k], ref) < TOL[cmx.HALF], (cb, isign, k)
+    for fast in (1, 3):  # link loads default / streamed
+        op.set_hop_config(cmx.HALF, 256, 0, 1, 0, fast)
+        for cb in (0, 1):
+            for isign in (+1, -1):
+                src = np.ascontiguousarray(psis[:, (1 - cb) * Vh:(2 - cb) * Vh])
+                got = op.dslash_multi(src, isign, cb, cmx.HALF)
+                for k in range(nrhs):
+                    ref = oracle.dslash(L, U, psis[k], isign, cb)[cb * Vh:(cb + 1) * Vh]
+                    assert rel(got[k], ref) < TOL[cmx.HALF], (fast, cb, isign, k)
 
 
 @pytest.mark.parametrize("solver,sloppy", [(cmx.CG, cmx.DOUBLE), (cmx.BICGSTAB, cmx.DOUBLE),
@@ -402,10 +404,10 @@ def test_twisted_golden_through_gpu():
 @pytest.mark.parametrize("cfg", [(64, 0, 1, 0, 0), (64, 1, 1, 0, 0), (64, 1, 2, 1, 0), (128, 1, 4, 1, 0),
                                  (128, 0, 1, 1, 0), (128, 1, 8, 0, 0), (256, 1, 2, 1, 0), (64, 0, 1, 0, 1),
                                  (128, 0, 1, 0, 1), (256, 0, 1, 0, 1), (64, 1, 4, 0, 1), (256, 1, 2, 0, 1),
-                                 (128, 1, 8, 0, 1)])
+                                 (128, 1, 8, 0, 1), (256, 0, 1, 0, 3), (128, 1, 8, 0, 3)])
 def test_hop_launch_shapes(prec, cfg):
     """Every launch shape the tuner may pick (block size, tiled work-group order, prefetch, plane-grid
-    kernel) gives the same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and
+    kernel with or without streamed link loads) gives the same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and
     128, and its antiperiodic links admit the plane-grid kernel."""
     L = (16, 16, 8, 16)
     say("launch shape", prec, cfg)
@@ -417,7 +419,7 @@ def test_hop_launch_shapes(prec, cfg):
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sp)
     op.set_hop_config(prec, *cfg)
     if cfg[4] and prec != cmx.DOUBLE:
-        assert op.hop_config(prec)["fast"] == 1
+        assert op.hop_config(prec)["fast"] == cfg[4]
     psi = random_fermion(V, 17)
     for isign in (+1, -1):
         ref = ref_op(psi, isign)[V // 2:]

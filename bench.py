@@ -90,6 +90,79 @@ def small_volume_solves(cmx):
     return out
 
 
+def batched_uneven_solves(cmx):
+    """64 independent reliable-CG solves (FP16 storage, FP64 reliable updates, FP64 true residual 1e-10)
+    on uneven volumes cycled 8^4 / 12^3x24 / 16^3x32, each with its own field (exp(0.3 i H) recipe,
+    seed 1000+i) and kappa drawn uniformly in [0.120, 0.134] (seed 999): one solve after another vs
+    all 64 advanced together through batched launches (one launch per kernel for every problem)."""
+    import numpy as np
+    rng = np.random.default_rng(999)
+    kap = rng.uniform(0.120, 0.134, 64)
+    lats = [(8, 8, 8, 8), (12, 12, 12, 24), (16, 16, 16, 32)]
+    ops, chis = [], []
+    for i in range(64):
+        L = lats[i % 3]
+        U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=1, seed=1000 + i, eps=0.3), (1, 1, 1, -1))
+        ops.append(cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, cmx.kappa_to_mass(kap[i]), sloppy_prec=cmx.HALF))
+        chi = np.zeros((int(np.prod(L)) // 2, 4, 3), complex)
+        chi[0, 0, 0] = 1.0  # point source on the first odd site
+        chis.append(chi)
+    kw = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, true_residual=True)
+    seq = None
+    for _ in range(2):  # the first pass allocates the solvers' vectors; the second is reported
+        t0 = time.time()
+        res = [op.solve(chi, **kw)[1] for op, chi in zip(ops, chis)]
+        seq = dict(seconds=round(sum(r["solve_seconds"] for r in res), 4), wall_seconds=round(time.time() - t0, 4),
+                   iterations=[min(r["n_count"] for r in res), max(r["n_count"] for r in res)],
+                   worst_true_rel_resid=max(r["rel_resid"] for r in res))
+    bat = None
+    for _ in range(2):
+        t0 = time.time()
+        _, res, summ = cmx.solve_batch(ops, chis, tune=True, **kw)
+        bat = dict(seconds=round(summ["solve_seconds"], 4), wall_seconds=round(time.time() - t0, 4),
+                   batched_iterations=summ["n_iter"], kernels_per_iteration=summ["n_stages"],
+                   launch_shape={"block": summ["hop_block"], "fast": summ["hop_fast"]},
+                   worst_true_rel_resid=max(r["rel_resid"] for r in res))
+    out = {"problems": 64, "volumes": [list(v) for v in lats], "operator": "EO Wilson, FP16 storage",
+           "solver": "reliable CG, FP64 true residual 1e-10", "sequential": seq, "batched": bat,
+           "speedup": round(seq["seconds"] / bat["seconds"], 3)}
+    print(f"# batched uneven: sequential {seq['seconds']:.4f} s, batched {bat['seconds']:.4f} s", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def batched_sources(cmx, L):
+    """The 12 spin-colour point sources of a propagator (quarkProp4_a) on one EO Wilson operator
+    (exp(0.3 i H) field, seed 1, kappa 0.115), reliable CG with FP16 storage to an FP64 true residual of
+    1e-10: 12 solves one after another vs the 12 advanced together (one launch per kernel for all 12)."""
+    import numpy as np
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=1, seed=1, eps=0.3), (1, 1, 1, -1))
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, cmx.kappa_to_mass(0.115), sloppy_prec=cmx.HALF)
+    del U
+    Vh = int(np.prod(L)) // 2
+    chis = []
+    for s in range(4):
+        for c in range(3):
+            chi = np.zeros((Vh, 4, 3), complex)
+            chi[0, s, c] = 1.0  # point source on the first odd site
+            chis.append(chi)
+    kw = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, true_residual=True)
+    op.solve(chis[0], **kw)  # allocations
+    res = [op.solve(chi, **kw)[1] for chi in chis]
+    seq = dict(seconds=round(sum(r["solve_seconds"] for r in res), 4),
+               iterations=[min(r["n_count"] for r in res), max(r["n_count"] for r in res)],
+               worst_true_rel_resid=max(r["rel_resid"] for r in res))
+    _, res, summ = cmx.solve_batch([op] * 12, chis, tune=True, **kw)
+    bat = dict(seconds=round(summ["solve_seconds"], 4), batched_iterations=summ["n_iter"],
+               launch_shape={"block": summ["hop_block"], "fast": summ["hop_fast"]},
+               worst_true_rel_resid=max(r["rel_resid"] for r in res))
+    out = {"lattice": list(L), "sources": 12, "operator": "EO Wilson, kappa 0.115, FP16 storage",
+           "sequential": seq, "batched": bat, "speedup": round(seq["seconds"] / bat["seconds"], 3)}
+    print(f"# batched sources {L}: sequential {seq['seconds']:.4f} s, batched {bat['seconds']:.4f} s",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def learned_solve(cmx, op, L, kappa, csw, solves, a):
     """Learned preconditioner: trained on separate fields of the same recipe (Haar + 6 stout steps,
     seeds 10, 11, ...) on a smaller lattice, then used as the right preconditioner of FGMRES-DR on the
@@ -143,6 +216,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-lattice", type=str, default="16,16,16,32")
     ap.add_argument("--train-fields", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--hop-shape", type=str, default="",
+                    help="block,tiled,zb,prefetch,fast: fixed launch shape of the FP16 kernels instead of the "
+                         "tuned one (profiling runs: the counter passes serialise kernels and skew the tuner)")
+    ap.add_argument("--sources-lattice", type=str, default="48,48,48,96")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -167,6 +245,8 @@ def main():
     t0 = time.time()
     op = cmx.Operator(gauge, cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), csw, sloppy_prec=cmx.HALF,
                       sloppy_recon=cmx.RECONS_12, autotune=True)
+    if a.hop_shape:
+        op.set_hop_config(cmx.HALF, *[int(v) for v in a.hop_shape.split(",")])
     print(f"# operator built and launch shapes tuned ({time.time() - t0:.1f} s): "
           f"{ {p: (op.hop_config(p, 1), op.hop_config(p, 2)) for p in (0, 1, 2)} }", file=sys.stderr, flush=True)
     # warm-up, then exactly `steps` timed applications (HIP events on the operator's stream)
@@ -217,6 +297,11 @@ def main():
     if not a.no_solve and not a.no_learned:
         learned = learned_solve(cmx, op, L, kappa, csw, solves, a)
 
+    batched = None
+    if not a.no_solve and not a.no_batch:
+        batched = {"uneven_volumes": batched_uneven_solves(cmx),
+                   "propagator_sources": batched_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")))}
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(L, U)
@@ -248,6 +333,7 @@ def main():
         "solve_time_learned_s": (learned or {}).get("seconds"),
         "learned": learned,
         "small_volume": small,
+        "batched": batched,
         "solves": solves,
         "kernels": extra,
         "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}

@@ -168,8 +168,12 @@ template <> struct SpinorIO<P16> {
     float m = 0.f;
 #pragma unroll
     for (int k = 0; k < 24; ++k) m = fmaxf(m, fabsf(s[k]));
-    const float inv = m > 0.f ? 1.0f / m : 0.f;
-    nrm[i] = m;
+    // a site whose largest component is below the FP32 normal range is stored as zero: 1/m would
+    // overflow to infinity (the front of an exponentially decaying propagator reaches such values on
+    // large lattices, and inf * 0 would turn the site into NaN)
+    const bool keep = m >= __FLT_MIN__;
+    const float inv = keep ? 1.0f / m : 0.f;
+    nrm[i] = keep ? m : 0.f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       H8 v;

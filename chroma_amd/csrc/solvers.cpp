@@ -351,7 +351,12 @@ void solve_relcg(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, b
     else
       gc.run(0, inner);
     CMX_HIP(hipMemcpyAsync(op.hF, op.state_F.p, sizeof(int) * F_NFLAGS, hipMemcpyDeviceToHost, op.stream));
+    if (p.verbose)
+      CMX_HIP(hipMemcpyAsync(op.hS, op.state_S.p, sizeof(double) * 16, hipMemcpyDeviceToHost, op.stream));
     op.sync();
+    if (p.verbose)  // reliable_cg.cc:108-110
+      std::printf("RelCG: k = %d  r_sq = %.6e  d = %.6e  a = %.6e  updateR %d updateX %d err %d\n", op.hF[F_K],
+                  op.hS[S_RSQ], op.hS[S_D], op.hS[S_A], op.hF[F_UPDR], op.hF[F_UPDX], op.hF[F_ERR]);
     if (op.hF[F_DONE]) break;  // converged (or stopped) in the previous iteration
     if (op.hF[F_UPDR]) st.update_r(-1, -1);
     op.finalize(st.nb, 1, PG_RCG_UPD);                                                      // norms, convergence

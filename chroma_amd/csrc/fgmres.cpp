@@ -70,8 +70,12 @@ struct Givens {
 struct KryDev {
   Op& op;
   int n;
-  DevBuf part;
-  explicit KryDev(Op& o, int n_) : op(o), n(n_) { part.alloc(sizeof(double) * KRY_BLOCKS * 2 * KRY_MAXV); }
+  DevBuf& part;  // reduction partials, kept on the operator across solves
+  static DevBuf& partials_of(Op& o) {
+    if (!o.kry_part.p) o.kry_part.alloc(sizeof(double) * KRY_BLOCKS * 2 * KRY_MAXV);
+    return o.kry_part;
+  }
+  explicit KryDev(Op& o, int n_) : op(o), n(n_), part(partials_of(o)) {}
   MultiLaunch base(const Vec& w, const std::vector<Vec*>& vs, int m) const {
     MultiLaunch L;
     std::memset(&L, 0, sizeof(L));
@@ -135,9 +139,9 @@ struct MRPrecond {
   Op& op;
   int ps, n, maxit;
   double rsd, omega;
-  Vec r, mr;
-  MRPrecond(Op& o, int ps_, int n_, int maxit_, double rsd_, double omega_)
-      : op(o), ps(ps_), n(n_), maxit(maxit_), rsd(rsd_), omega(omega_), r(ps_, n_), mr(ps_, n_) {}
+  Vec &r, &mr;
+  MRPrecond(Op& o, int ps_, int n_, int maxit_, double rsd_, double omega_, Vec& r_, Vec& mr_)
+      : op(o), ps(ps_), n(n_), maxit(maxit_), rsd(rsd_), omega(omega_), r(r_), mr(mr_) {}
   void apply(Vec& z, const Vec& v) {
     const int nb = blas_blocks(n);
     int* F = op.state_F.as<int>();
@@ -176,21 +180,26 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   const int pb = p.basis_prec < 0 ? ps : p.basis_prec;
   CMX_REQUIRE(pb == P64 || pb == P32 || pb == P16, CMX_ERR_ARG, "FGMRES-DR: bad basis precision");
   const int total = nk + nd_max;
-  std::vector<std::unique_ptr<Vec>> V(total + 1), Z(total);
-  for (auto& v : V) v.reset(new Vec(pb, n));
-  for (auto& z : Z) z.reset(new Vec(pb, n));
-  for (auto& z : Z) op.blas(B_ZERO, {z.get()});
-  Vec r(P64, n), tmp(P64, n);
-  std::unique_ptr<Vec> xs, ys;  // operator-precision staging when pb != ps
+  // the bases, the spare vectors of the deflated restart and the scratch vectors live in the
+  // operator's workspace (slots 100-305), so repeated solves allocate nothing; V and Z map basis
+  // indices to those vectors and are permuted, not copied, when the deflated basis is formed
+  std::vector<Vec*> V(total + 1), Z(total), spareV(nd_max + 1), spareZ(nd_max);
+  for (int i = 0; i <= total; ++i) V[i] = &op.work(100 + i, pb, n);
+  for (int i = 0; i < total; ++i) Z[i] = &op.work(150 + i, pb, n);
+  for (int i = 0; i <= nd_max; ++i) spareV[i] = &op.work(200 + i, pb, n);
+  for (int i = 0; i < nd_max; ++i) spareZ[i] = &op.work(250 + i, pb, n);
+  for (Vec* z : Z) op.blas(B_ZERO, {z});
+  Vec &r = op.work(300, P64, n), &tmp = op.work(301, P64, n);
+  Vec *xs = nullptr, *ys = nullptr;  // operator-precision staging when pb != ps
   if (pb != ps) {
-    xs.reset(new Vec(ps, n));
-    ys.reset(new Vec(ps, n));
+    xs = &op.work(302, ps, n);
+    ys = &op.work(303, ps, n);
   }
   KryDev kd(op, n);
   std::unique_ptr<MRPrecond> mr;
   if (p.precond == CMX_PRECOND_MR)
     mr.reset(new MRPrecond(op, ps, n, std::max(0, p.precond_iters), p.precond_rsd,
-                           p.precond_omega != 0 ? p.precond_omega : 1.0));
+                           p.precond_omega != 0 ? p.precond_omega : 1.0, op.work(304, ps, n), op.work(305, ps, n)));
   const int reorth = p.reorth >= 2 ? 2 : 1;
   const int nb = blas_blocks(n);
 
@@ -237,19 +246,19 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       else op.blas(B_COPY, {&z, &v});
       return;
     }
-    op.blas(B_COPY, {xs.get(), &v});
-    if (p.precond == CMX_PRECOND_MR) op.blas(B_COPY, {ys.get(), &z});  // MR starts from z
+    op.blas(B_COPY, {xs, &v});
+    if (p.precond == CMX_PRECOND_MR) op.blas(B_COPY, {ys, &z});  // MR starts from z
     precondition_s(*ys, *xs);
-    op.blas(B_COPY, {&z, ys.get()});
+    op.blas(B_COPY, {&z, ys});
   };
   auto apply_a = [&](Vec& w, const Vec& z) {  // w = A z at the operator precision
     if (pb == ps) {
       op.apply(ps, w.view(), z.view(), +1, nullptr);
       return;
     }
-    op.blas(B_COPY, {xs.get(), &z});
+    op.blas(B_COPY, {xs, &z});
     op.apply(ps, ys->view(), xs->view(), +1, nullptr);
-    op.blas(B_COPY, {&w, ys.get()});
+    op.blas(B_COPY, {&w, ys});
   };
 
   bool plain_next = true;  // the first cycle, and any cycle after a deflated one that stalled
@@ -265,7 +274,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       H = Mat(total + 1, total);
       R = Mat(total + 1, total);
       kd.put1(K_WN + 1, 1.0 / r_norm);
-      op.blas(B_SCALE, {V[0].get(), &r}, K_WN + 1);
+      op.blas(B_SCALE, {V[0], &r}, K_WN + 1);
       nd = 0;
     } else {
       // deflated restart (:514-688): harmonic Ritz vectors of the previous cycle
@@ -304,27 +313,25 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       for (int i = 0; i < m; ++i)
         for (int j = 0; j < nd; ++j) Qk_(i, j) = Q(i, j);
       Mat Hn = dense::matmul(dense::adjoint(Q), dense::matmul(Hfull, Qk_));
-      // new V_i = sum_j V_j Q(j, i) (i <= nd); new Z_i = sum_j Z_j Q(j, i) (i < nd)
-      std::vector<std::unique_ptr<Vec>> nV(nd + 1), nZ(nd);
-      for (int j = 0; j <= m; ++j) vs[j] = V[j].get();
+      // new V_i = sum_j V_j Q(j, i) (i <= nd); new Z_i = sum_j Z_j Q(j, i) (i < nd), formed in the
+      // spare vectors (basis precision), which then take the places of V_i, Z_i
+      for (int j = 0; j <= m; ++j) vs[j] = V[j];
       for (int i = 0; i <= nd; ++i) {
-        nV[i].reset(new Vec(ps, n));
         std::vector<cd> col(m + 1);
         for (int j = 0; j <= m; ++j) col[j] = Q(j, i);
         kd.put(K_C, col);
-        kd.maxpy(*nV[i], vs, m + 1, K_C, 1.0, true, -1);
+        kd.maxpy(*spareV[i], vs, m + 1, K_C, 1.0, true, -1);
       }
-      for (int j = 0; j < m; ++j) vs[j] = Z[j].get();
+      for (int j = 0; j < m; ++j) vs[j] = Z[j];
       for (int i = 0; i < nd; ++i) {
-        nZ[i].reset(new Vec(ps, n));
         std::vector<cd> col(m);
         for (int j = 0; j < m; ++j) col[j] = Q(j, i);
         kd.put(K_C, col);
-        kd.maxpy(*nZ[i], vs, m, K_C, 1.0, true, -1);
+        kd.maxpy(*spareZ[i], vs, m, K_C, 1.0, true, -1);
       }
-      for (int i = 0; i <= nd; ++i) V[i].swap(nV[i]);
-      for (int i = 0; i < nd; ++i) Z[i].swap(nZ[i]);
-      for (int i = nd; i < total; ++i) op.blas(B_ZERO, {Z[i].get()});
+      for (int i = 0; i <= nd; ++i) std::swap(V[i], spareV[i]);
+      for (int i = 0; i < nd; ++i) std::swap(Z[i], spareZ[i]);
+      for (int i = nd; i < total; ++i) op.blas(B_ZERO, {Z[i]});
       // reset H, R, c, g; c = V_{k+1}^H r (:640-650)
       H = Mat(total + 1, total);
       R = Mat(total + 1, total);
@@ -334,7 +341,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
         // <V_i, r> with r in FP64: convert r once to the basis precision
         Vec rs(pb, n);
         op.blas(B_COPY, {&rs, &r});
-        for (int i = 0; i <= nd; ++i) vs[i] = V[i].get();
+        for (int i = 0; i <= nd; ++i) vs[i] = V[i];
         kd.mdot(vs, nd + 1, rs, K_H);
         kd.get(K_H, 2 * (nd + 1));
         for (int i = 0; i <= nd; ++i) c[i] = g[i] = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
@@ -362,7 +369,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     for (int j = nd; j < nd + nk; ++j) {
       precondition(*Z[j], *V[j]);                                     // z_j = M v_j
       apply_a(*V[j + 1], *Z[j]);                                     // w = A z_j
-      for (int i = 0; i <= j; ++i) vs[i] = V[i].get();
+      for (int i = 0; i <= j; ++i) vs[i] = V[i];
       kd.mdot(vs, j + 1, *V[j + 1], K_H);                             // h_i = <v_i, w>
       kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
       if (reorth == 2) {
@@ -380,7 +387,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
         dim = j;
         break;
       }
-      op.blas(B_SCALE, {V[j + 1].get(), V[j + 1].get()}, K_WN + 1);  // v_{j+1} = w / |w|
+      op.blas(B_SCALE, {V[j + 1], V[j + 1]}, K_WN + 1);  // v_{j+1} = w / |w|
       std::vector<cd> col(j + 2);
       for (int i = 0; i <= j + 1; ++i) col[i] = H(i, j);
       if (nd > 0) {  // R_col <- Q_k^H R_col on the deflated rows (:133-152)
@@ -407,7 +414,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
         for (int col = row + 1; col < dim; ++col) s -= R(row, col) * eta[col];
         eta[row] = s / R(row, row);
       }
-      for (int j = 0; j < dim; ++j) vs[j] = Z[j].get();
+      for (int j = 0; j < dim; ++j) vs[j] = Z[j];
       kd.put(K_C, std::vector<cd>(eta.begin(), eta.begin() + dim));
       kd.maxpy(psi, vs, dim, K_C, 1.0, false, -1);
     }

@@ -112,6 +112,7 @@ struct Op {
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs;
+  DevBuf kry_part;  // reduction partials of the Krylov-basis kernels (FGMRES-DR)
   Vec& work(int slot, int prec, int n);
   // learned preconditioner: the attached network, the weight version last uploaded, device buffers
   std::shared_ptr<LearnedNet> net;

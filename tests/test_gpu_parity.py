@@ -359,6 +359,29 @@ def test_fgmres_dr_sloppy_basis(sloppy, n_defl):
     assert rel(psi, psi_cg) < 1e-8
 
 
+@pytest.mark.parametrize("n_defl", [0, 4])
+def test_fgmres_dr_wide_basis_and_repeat(n_defl):
+    """FP16 operator with an FP32 Krylov basis, plain and deflated restarts (the deflated basis is
+    formed at the basis precision), solved twice on the operator's persistent workspace with identical
+    results."""
+    L = (8, 8, 8, 16)
+    say("fgmres-dr wide basis", n_defl)
+    V = int(np.prod(L))
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=98, nstout=6, rho=0.1), (1, 1, 1, -1))
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, cmx.kappa_to_mass(0.1350), 1.0, sloppy_prec=cmx.HALF)
+    chi = cmx.z2_source((V // 2, 4, 3), 7)
+    psi_cg, _ = op.solve(chi, solver=cmx.CG, rsd=1e-12, maxit=20000)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, basis_prec=cmx.SINGLE,
+              n_krylov=16, n_defl=n_defl, cycle_reduction=1e-3)
+    psi1, r1 = op.solve(chi, **kw)
+    psi2, r2 = op.solve(chi, **kw)
+    say("  n_count", r1["n_count"], "cycles", r1["n_reliable"], "rel resid", r1["rel_resid"])
+    assert r1["rel_resid"] <= 1e-10
+    assert rel(psi1, psi_cg) < 1e-8
+    assert r2["n_count"] == r1["n_count"]
+    assert np.array_equal(psi2.view(np.uint64), psi1.view(np.uint64))
+
+
 @pytest.mark.parametrize("L", [(8, 8, 8, 16), (16, 16, 8, 16), (6, 4, 8, 10)])
 @pytest.mark.parametrize("nrhs", [1, 5, 12])
 def test_dslash_multi_source(L, nrhs):

@@ -22,12 +22,12 @@ struct PtcLaunch {
   void* out;             // odd-checkerboard output (scatter) / delta (w0 grad)
   float* out_nrm;
   int out_stride;
-  unsigned short* T;     // transported inputs, BF16 MFMA tiles (Vh/32 x 3 x 4 KiB)
+  unsigned short* T;     // transported half spinors, BF16 MFMA tiles (Vh/32 x 3 x 2 KiB)
   unsigned short* dT;    // transported loss gradient, same layout
-  unsigned short* G;     // mixed field g_q, BF16 (8 x 3 x Vh x 16 B)
-  const unsigned short* Afr;  // real embedding of Wc in MFMA A-operand order (2 x 8 x 64 x 4 BF16)
+  unsigned short* G;     // mixed field gh_q (half spinors), BF16 (8 x 3 x Vh x 8 B)
+  const unsigned short* Afr;  // real embedding of Wh in MFMA A-operand order (4 x 64 x 4 BF16)
   const float* W0;       // 4x4 complex, row-major (re, im)
-  float* gpart;          // gradient partials (waves x 4096 floats)
+  float* gpart;          // gradient partials (waves x 1024 floats)
   double* w0part;        // W0 gradient partials (blocks x 32 doubles)
 };
 
@@ -37,7 +37,7 @@ void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
 // inference: variant 0 operands in registers, 1 transports staged in LDS
 void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s);
 void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s);
-void launch_ptc_grad(const PtcLaunch& L, double* out4096, hipStream_t s);
+void launch_ptc_grad(const PtcLaunch& L, double* out1024, hipStream_t s);
 int launch_w0_grad(const PtcLaunch& L, hipStream_t s);
 
 // Network weights (host, FP32 complex) with Adam state.  Component indices: path/channel-major,
@@ -49,7 +49,8 @@ struct LearnedNet {
   long step = 0;
   long version = 0;  // bumped on every weight change; operators re-upload when it differs
   LearnedNet();
-  // Wc = W2 W1 (32 x 32 complex), row-major over (q*4+s, p*4+t)
+  // Wc = W2 W1 (32 x 32 complex), row-major over (q*4+s, p*4+t); the network applies it through the
+  // half-spinor projectors of the paths (Pi_q Wc Pi_p)
   std::vector<cf> contracted() const;
 };
 

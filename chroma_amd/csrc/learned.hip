@@ -1,24 +1,31 @@
 // Gauge-equivariant parallel-transport network used as the right preconditioner of the flexible
 // solver.  On odd-checkerboard vectors v:
 //
-//   P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),                       x odd,  q in {+-mu}
-//   g_q(y)  = sum_{c<16} W2[q][c] h_c(y),  h_c(y) = sum_p W1[c][p] U_p(y) v(y+p),   y even
+//   P(v)(x) = W0 v(x) + sum_q U_q(x) Pi_q g_q(x+q),                       x odd,  q in {+-mu}
+//   g_q(y)  = sum_p Wc[q][p] U_p(y) (Pi_p v)(y+p),   Wc = W2 W1,             y even
 //
-// U_{+mu}(y) = U_mu(y), U_{-mu}(y) = U_mu(y-mu)^dag (parallel transport to y); W0, W1, W2 are 4x4 complex
-// spin matrices acting identically on colour, so P commutes with gauge transformations.  The two
-// linear channel layers contract to one 32x32 complex matrix Wc = W2 W1 over (path, spin), applied on
-// MFMA in BF16 with FP32 accumulation; the parallel transports run on the vector units.
+// U_{+mu}(y) = U_mu(y), U_{-mu}(y) = U_mu(y-mu)^dag; Pi_p = (1 + sigma_p gamma_mu)/2 (sigma = -1 forward,
+// +1 backward) is the hopping term's rank-2 spin projector, so every transport moves a half spinor, as
+// in the hopping term.  W0 and the 4x4 spin blocks of Wc act identically on colour, so P commutes with
+// gauge transformations.  In half-spinor coordinates (Pi = R Phat, Phat = (1/2)[1, sigma B_mu],
+// R = [1; sigma B_mu^dag], B_mu the upper-right block of gamma_mu in the DeGrand-Rossi basis) the network
+// is one 16x16 complex matrix Wh = Phat_q Wc R_p over (path, half-spin):
+//
+//   th_p(y) = U_p(y) (Phat_p v)(y+p)          gh = Wh th          P(v)(x) = W0 v(x) + sum_q U_q(x) R_q gh_q(x+q)
+//
+// Wh is applied on MFMA (v_mfma_f32_32x32x8bf16_1k, its real embedding is one 32x32 A operand) in BF16
+// with FP32 accumulation; the transports run on the vector units.
 //
 // Kernels:
-//   k_ptc_gather   (even sites)  t_p(y) = U_p(y) v(y+p) for the 8 paths, stored in BF16 tiles laid out as
-//                  the B operand of the 32x32x8 MFMA (or, with ADJ, the transport of the loss gradient
-//                  dg_q(y) = t_{-q}[delta](y), the adjoint of the scatter)
-//   k_ptc_mix      (MFMA)        g = Wc t per (site, colour) column
+//   k_ptc_gather   (even sites)  th_p(y) for the 8 paths, stored in BF16 tiles laid out as the B operand
+//                  of the MFMA (or, with ADJ, the loss gradient dgh_q(y) = R_q^dag U_q(y-q)^dag delta(y-q),
+//                  the adjoint of the scatter, gathered along -q)
+//   k_ptc_mix      (MFMA)        gh = Wh th per (site, colour) column
 //   k_ptc_gather_mix             both of the above in one kernel, transports packed straight into the
 //                                MFMA operands (inference path)
 //   k_ptc_gather_mix_lds         the same with the transports staged in LDS tiles
-//   k_ptc_scatter  (odd sites)   P(v) = W0 v + sum_q U_q g_q
-//   k_ptc_grad     (MFMA)        S = sum_{y,c} dg(y,c) t(y,c)^dag, the gradient of |P(v) - e|^2 w.r.t. conj(Wc)
+//   k_ptc_scatter  (odd sites)   P(v) = W0 v + sum_q U_q R_q gh_q
+//   k_ptc_grad     (MFMA)        Sh = sum_{y,c} dgh(y,c) th(y,c)^dag, the gradient w.r.t. conj(Wh)
 //   k_w0_grad                    sum_x delta(x) v(x)^dag, the gradient w.r.t. conj(W0)
 #include "hop_kernel.h"
 #include "learned.h"
@@ -52,32 +59,6 @@ __device__ __forceinline__ int ptc_index(const Geom& g, const int (&x)[4]) {
   return ((x[3] * g.L[2] + x[2]) * g.L[1] + x[1]) * g.Lxh + (x[0] >> 1);
 }
 
-// o[s] = U v[s] (DAG: U^dag v[s]) for 4 spins; v, o as [spin][colour][re,im]
-template <bool DAG, typename R>
-__device__ __forceinline__ void transport4(const R (&u)[18], const R (&v)[24], R (&o)[24]) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      R re = 0, im = 0;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const R vr = v[2 * (3 * s + j)], vi = v[2 * (3 * s + j) + 1];
-        if constexpr (!DAG) {
-          const R ur = u[2 * (3 * i + j)], ui = u[2 * (3 * i + j) + 1];
-          re += ur * vr - ui * vi;
-          im += ur * vi + ui * vr;
-        } else {
-          const R ur = u[2 * (3 * j + i)], ui = u[2 * (3 * j + i) + 1];
-          re += ur * vr + ui * vi;
-          im += ur * vi - ui * vr;
-        }
-      }
-      o[2 * (3 * s + i)] = re;
-      o[2 * (3 * s + i) + 1] = im;
-    }
-}
-
 // neighbour y +- mu of site x (coordinates), its checkerboard index, and whether the link crosses
 // the last slice (for the 12-real reconstruction phase)
 __device__ __forceinline__ int ptc_neighbour(const Geom& g, const int (&x)[4], int mu, bool fwd, bool& last) {
@@ -93,12 +74,136 @@ __device__ __forceinline__ int ptc_neighbour(const Geom& g, const int (&x)[4], i
   return ptc_index(g, y);
 }
 
+// h[s'] = scale (v_up[s'] + sigma (B_mu v_lo)[s']), s' = 0, 1; v as [spin][colour][re,im], h as
+// [s'][colour][re,im].  B_0 = [[0, i], [i, 0]], B_1 = [[0, -1], [1, 0]], B_2 = [[i, 0], [0, -i]], B_3 = 1.
+template <int MU, typename R>
+__device__ __forceinline__ void half_project(const R (&v)[24], float sigma, float scale, float (&h)[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float u0r = (float)v[2 * c], u0i = (float)v[2 * c + 1];
+    const float u1r = (float)v[2 * (3 + c)], u1i = (float)v[2 * (3 + c) + 1];
+    const float l0r = (float)v[2 * (6 + c)], l0i = (float)v[2 * (6 + c) + 1];
+    const float l1r = (float)v[2 * (9 + c)], l1i = (float)v[2 * (9 + c) + 1];
+    float b0r, b0i, b1r, b1i;  // B v_lo
+    if constexpr (MU == 0) {
+      b0r = -l1i, b0i = l1r, b1r = -l0i, b1i = l0r;
+    } else if constexpr (MU == 1) {
+      b0r = -l1r, b0i = -l1i, b1r = l0r, b1i = l0i;
+    } else if constexpr (MU == 2) {
+      b0r = -l0i, b0i = l0r, b1r = l1i, b1i = -l1r;
+    } else {
+      b0r = l0r, b0i = l0i, b1r = l1r, b1i = l1i;
+    }
+    h[2 * c] = scale * (u0r + sigma * b0r);
+    h[2 * c + 1] = scale * (u0i + sigma * b0i);
+    h[2 * (3 + c)] = scale * (u1r + sigma * b1r);
+    h[2 * (3 + c) + 1] = scale * (u1i + sigma * b1i);
+  }
+}
+
+// acc += R o = (o, sigma B_mu^dag o) for a half spinor o [s'][colour][re,im]
+template <int MU>
+__device__ __forceinline__ void reconstruct_add(const float (&o)[12], float sigma, float (&acc)[24]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float o0r = o[2 * c], o0i = o[2 * c + 1], o1r = o[2 * (3 + c)], o1i = o[2 * (3 + c) + 1];
+    float l0r, l0i, l1r, l1i;  // B^dag o
+    if constexpr (MU == 0) {  // B_0^dag = [[0, -i], [-i, 0]]
+      l0r = o1i, l0i = -o1r, l1r = o0i, l1i = -o0r;
+    } else if constexpr (MU == 1) {  // [[0, 1], [-1, 0]]
+      l0r = o1r, l0i = o1i, l1r = -o0r, l1i = -o0i;
+    } else if constexpr (MU == 2) {  // [[-i, 0], [0, i]]
+      l0r = o0i, l0i = -o0r, l1r = -o1i, l1i = o1r;
+    } else {
+      l0r = o0r, l0i = o0i, l1r = o1r, l1i = o1i;
+    }
+    acc[2 * c] += o0r;
+    acc[2 * c + 1] += o0i;
+    acc[2 * (3 + c)] += o1r;
+    acc[2 * (3 + c) + 1] += o1i;
+    acc[2 * (6 + c)] += sigma * l0r;
+    acc[2 * (6 + c) + 1] += sigma * l0i;
+    acc[2 * (9 + c)] += sigma * l1r;
+    acc[2 * (9 + c) + 1] += sigma * l1i;
+  }
+}
+
+// o[s'] = U h[s'] (DAG: U^dag h[s']) for the two half-spin components
+template <bool DAG, typename R>
+__device__ __forceinline__ void transport2(const R (&u)[18], const float (&h)[12], float (&o)[12]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      float re = 0, im = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float hr = h[2 * (3 * s + j)], hi = h[2 * (3 * s + j) + 1];
+        if constexpr (!DAG) {
+          const float ur = (float)u[2 * (3 * i + j)], ui = (float)u[2 * (3 * i + j) + 1];
+          re += ur * hr - ui * hi;
+          im += ur * hi + ui * hr;
+        } else {
+          const float ur = (float)u[2 * (3 * j + i)], ui = (float)u[2 * (3 * j + i) + 1];
+          re += ur * hr + ui * hi;
+          im += ur * hi - ui * hr;
+        }
+      }
+      o[2 * (3 * s + i)] = re;
+      o[2 * (3 * s + i) + 1] = im;
+    }
+}
+
+// The transported half spinors of the two paths 2 mu (forward) and 2 mu + 1 (backward) at even site y
+// (coordinates x): th[d][s'][c].  ADJ: the adjoint gather of the loss gradient, dgh_{q}(y) for q = p^1,
+// i.e. the projection of the opposite path with a factor 2 (R^dag = 2 Phat), stored under slot p^1.
+template <int P, int RECON, int MU, bool ADJ>
+__device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4], int y, float (&th)[2][12]) {
+  using R = typename SpinorIO<P>::R;
+  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
+  const Geom& g = L.g;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const bool fwd = d == 0;
+    bool last;
+    const int yn = ptc_neighbour(g, x, MU, fwd, last);
+    R v[24], u[18];
+    SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
+    const float sigma_p = fwd ? -1.f : 1.f;
+    float h[12];
+    half_project<MU>(v, ADJ ? -sigma_p : sigma_p, ADJ ? 1.f : 0.5f, h);
+    const int slot = ADJ ? (d ^ 1) : d;
+    if (fwd) {  // U_mu(y) from the even-parity links
+      const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)MU * LCH * LCB * g.Vh;
+      LinkIO<P, RECON>::load(gb, g.Vh, y, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      transport2<false>(u, h, th[slot]);
+    } else {    // U_mu(y-mu)^dag from the odd-parity links
+      const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)MU * LCH * LCB * g.Vh;
+      LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      transport2<true>(u, h, th[slot]);
+    }
+  }
+}
+
+// B-operand components of direction mu for colour c: real parts (rows 2p + s', p = 2 mu, 2 mu + 1) and
+// imaginary parts (rows 16 + 2p + s'); rows 8 ks + 4 h + j with ks = mu / 2 (+2 for imaginary),
+// h = mu % 2, j = 2 (p % 2) + s'
+__device__ __forceinline__ void pack_mu(const float (&th)[2][12], int c, bf4& re, bf4& im) {
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      re[2 * d + s] = (short)f2bf(th[d][2 * (3 * s + c)]);
+      im[2 * d + s] = (short)f2bf(th[d][2 * (3 * s + c) + 1]);
+    }
+}
+
 // T tile layout (the B operand of v_mfma_f32_32x32x8bf16_1k for 32 sites x one colour):
-//   T[((blk * 3 + c) * 8 + ks) * 256 + (h * 32 + n) * 4 + j]  (bf16)
-// holds real-embedding row k = 8 ks + 4 h + j of column n (site blk*32+n, colour c); rows 0..31 are the
-// real parts of the components p*4+s (path p, spin s), rows 32..63 their imaginary parts.
+//   T[((blk * 3 + c) * 4 + ks) * 256 + (h * 32 + n) * 4 + j]  (bf16)
+// holds real-embedding row k = 8 ks + 4 h + j of column n (site blk*32+n, colour c); rows 0..15 are the
+// real parts of the half-spinor components p*2+s' (path p, half-spin s'), rows 16..31 their imaginary parts.
 __device__ __forceinline__ size_t tile_off(int blk, int c, int ks, int h, int n) {
-  return ((size_t)((blk * 3 + c) * 8 + ks) * 64 + h * 32 + n) * 4;
+  return ((size_t)((blk * 3 + c) * 4 + ks) * 64 + h * 32 + n) * 4;
 }
 
 template <int P, int RECON, bool ADJ>
@@ -106,272 +211,202 @@ __global__ void __launch_bounds__(256) k_ptc_gather(PtcLaunch L) {
   const int yi = blockIdx.x * blockDim.x + threadIdx.x;
   const Geom& g = L.g;
   if (yi >= g.Vh) return;
-  using R = typename SpinorIO<P>::R;
-  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   int x[4];
   ptc_coords(g, 0, yi, x);
   const int blk = yi >> 5, n = yi & 31;
   unsigned short* T = L.T;
+  auto one = [&](auto MUc) {
+    constexpr int MU = decltype(MUc)::value;
+    float th[2][12];
+    gather_mu<P, RECON, MU, ADJ>(L, x, yi, th);
 #pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const bool fwd = d == 0;
-      bool last;
-      const int yn = ptc_neighbour(g, x, mu, fwd, last);
-      R v[24], u[18], t[24];
-      SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
-      if (fwd) {  // U_mu(y) from the even-parity links
-        const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, yi, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-        transport4<false>(u, v, t);
-      } else {    // U_mu(y-mu)^dag from the odd-parity links
-        const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-        transport4<true>(u, v, t);
-      }
-      const int p = 2 * mu + d;
-      const int slot = ADJ ? (p ^ 1) : p;  // dg_q = t_{-q}[delta]
-      const int ks = slot >> 1, h = slot & 1;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        ushort4 re, im;
-        re.x = f2bf((float)t[2 * (0 + c)]);
-        re.y = f2bf((float)t[2 * (3 + c)]);
-        re.z = f2bf((float)t[2 * (6 + c)]);
-        re.w = f2bf((float)t[2 * (9 + c)]);
-        im.x = f2bf((float)t[2 * (0 + c) + 1]);
-        im.y = f2bf((float)t[2 * (3 + c) + 1]);
-        im.z = f2bf((float)t[2 * (6 + c) + 1]);
-        im.w = f2bf((float)t[2 * (9 + c) + 1]);
-        *reinterpret_cast<ushort4*>(T + tile_off(blk, c, ks, h, n)) = re;
-        *reinterpret_cast<ushort4*>(T + tile_off(blk, c, 4 + ks, h, n)) = im;
-      }
+    for (int c = 0; c < 3; ++c) {
+      bf4 re, im;
+      pack_mu(th, c, re, im);
+      *reinterpret_cast<bf4*>(T + tile_off(blk, c, MU >> 1, MU & 1, n)) = re;
+      *reinterpret_cast<bf4*>(T + tile_off(blk, c, 2 + (MU >> 1), MU & 1, n)) = im;
     }
-  }
+  };
+  one(std::integral_constant<int, 0>());
+  one(std::integral_constant<int, 1>());
+  one(std::integral_constant<int, 2>());
+  one(std::integral_constant<int, 3>());
 }
 
-// g = Wc t: one wave per (32-site block, colour) tile, grid-stride over tiles.  A operand (the real
-// embedding [[Wr, -Wi], [Wi, Wr]] of Wc, 64x64) is read once per wave into registers.
-// G layout: G[((q * 3 + c) * Vh + y) * 8 + 2 s + ri]  (bf16), one 16-byte group per (q, c, y).
+// gh rows of one 32-site x colour MFMA result written to G: lane l (site l % 32, h = l / 32) holds rows
+// 8 g + 4 h + i (g = 0, 1 real; 2, 3 imaginary), i.e. the half-spinor components of the paths
+// q = 4 g + 2 h + m (m = 0, 1), both half-spins.
+// G layout: G[((q * 3 + c) * Vh + y) * 4 + (2 s' + ri)]  (bf16), one 8-byte group per (q, c, y).
+__device__ __forceinline__ void store_gh(unsigned short* G, int Vh, int y, int c, int h, const f16v& acc) {
+#pragma unroll
+  for (int gq = 0; gq < 2; ++gq)
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int q = 4 * gq + 2 * h + m;
+      ushort4 w;
+      w.x = f2bf(acc[4 * gq + 2 * m + 0]);
+      w.y = f2bf(acc[8 + 4 * gq + 2 * m + 0]);
+      w.z = f2bf(acc[4 * gq + 2 * m + 1]);
+      w.w = f2bf(acc[8 + 4 * gq + 2 * m + 1]);
+      *reinterpret_cast<ushort4*>(G + ((size_t)(q * 3 + c) * Vh + y) * 4) = w;
+    }
+}
+
+// gh = Wh th: one wave per (32-site block, colour) tile, grid-stride over tiles.  The A operand (the
+// real embedding [[Wr, -Wi], [Wi, Wr]] of Wh, 32x32) is read once per wave into registers.
 __global__ void __launch_bounds__(256) k_ptc_mix(PtcLaunch L) {
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int nblk = (L.g.Vh + 31) >> 5;
   const int ntiles = nblk * 3;
-  bf4 a[2][8];
+  bf4 a[4];
   const bf4* A = reinterpret_cast<const bf4*>(L.Afr);
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) a[mt][ks] = A[(mt * 8 + ks) * 64 + lane];
+  for (int ks = 0; ks < 4; ++ks) a[ks] = A[ks * 64 + lane];
   for (int tile = wave; tile < ntiles; tile += nwaves) {
     const int blk = tile / 3, c = tile - 3 * blk;
     const bf4* B = reinterpret_cast<const bf4*>(L.T + tile_off(blk, c, 0, 0, 0));
-    f16v acc0 = {0}, acc1 = {0};
+    f16v acc = {0};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const bf4 b = B[ks * 64 + lane];
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[0][ks], b, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[1][ks], b, acc1, 0, 0, 0);
-    }
+    for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], B[ks * 64 + lane], acc, 0, 0, 0);
     const int y = blk * 32 + (lane & 31);
-    if (y < L.g.Vh) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int q = 2 * j + (lane >> 5);
-        ushort4 lo, hi;  // (s0 re, s0 im, s1 re, s1 im), (s2 re, s2 im, s3 re, s3 im)
-        lo.x = f2bf(acc0[4 * j + 0]);
-        lo.y = f2bf(acc1[4 * j + 0]);
-        lo.z = f2bf(acc0[4 * j + 1]);
-        lo.w = f2bf(acc1[4 * j + 1]);
-        hi.x = f2bf(acc0[4 * j + 2]);
-        hi.y = f2bf(acc1[4 * j + 2]);
-        hi.z = f2bf(acc0[4 * j + 3]);
-        hi.w = f2bf(acc1[4 * j + 3]);
-        ushort4* dst = reinterpret_cast<ushort4*>(L.G + ((size_t)(q * 3 + c) * L.g.Vh + y) * 8);
-        dst[0] = lo;
-        dst[1] = hi;
-      }
-    }
+    if (y < L.g.Vh) store_gh(L.G, L.g.Vh, y, c, lane >> 5, acc);
   }
 }
 
 // Fused gather + mix for inference, no LDS.  A wave owns 32 even sites; lane l takes site l % 32 and
-// the forward (l < 32) or backward (l >= 32) paths of it.  In the B-operand layout of the 32x32x8 MFMA
-// lane l holds rows 8 ks + 4 (l / 32) + j of column l % 32, which for the real-embedded components
-// p*4 + s are exactly path p = 2 mu + (l / 32) (mu = ks for the real rows ks < 4, mu = ks - 4 for the
-// imaginary rows): each lane computes its 4 transports and packs them straight into its B fragments.
+// the directions mu = h, 2 + h (h = l / 32), both paths of each: in the B-operand layout these are
+// exactly the rows lane l holds, so each lane packs its four transports straight into its fragments.
 template <int P, int RECON>
 __global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
   const Geom& g = L.g;
-  using R = typename SpinorIO<P>::R;
-  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   const int lane = threadIdx.x & 63;
-  const int h = lane >> 5;  // 0: forward paths, 1: backward paths
+  const int h = lane >> 5;
   const int y = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 32 + (lane & 31);
   const bool live = y < g.Vh;
-  bf4 b[3][8];
+  bf4 b[3][4];
   if (live) {
     int x[4];
     ptc_coords(g, 0, y, x);
+    float th[2][12];
+    auto put = [&](int ksr) {
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-      bool last;
-      const int yn = ptc_neighbour(g, x, mu, h == 0, last);
-      R v[24], u[18], t[24];
-      SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
-      if (h == 0) {
-        const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, y, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-        transport4<false>(u, v, t);
-      } else {
-        const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-        transport4<true>(u, v, t);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          b[c][mu][s] = (short)f2bf((float)t[2 * (3 * s + c)]);
-          b[c][4 + mu][s] = (short)f2bf((float)t[2 * (3 * s + c) + 1]);
-        }
-      }
+      for (int c = 0; c < 3; ++c) pack_mu(th, c, b[c][ksr], b[c][2 + ksr]);
+    };
+    if (h == 0) {
+      gather_mu<P, RECON, 0, false>(L, x, y, th);
+      put(0);
+      gather_mu<P, RECON, 2, false>(L, x, y, th);
+      put(1);
+    } else {
+      gather_mu<P, RECON, 1, false>(L, x, y, th);
+      put(0);
+      gather_mu<P, RECON, 3, false>(L, x, y, th);
+      put(1);
     }
   } else {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) b[c][ks] = (bf4){0, 0, 0, 0};
+      for (int ks = 0; ks < 4; ++ks) b[c][ks] = (bf4){0, 0, 0, 0};
   }
   const bf4* A = reinterpret_cast<const bf4*>(L.Afr);
+  bf4 a[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) a[ks] = A[ks * 64 + lane];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    f16v acc0 = {0}, acc1 = {0};
+    f16v acc = {0};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A[ks * 64 + lane], b[c][ks], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A[(8 + ks) * 64 + lane], b[c][ks], acc1, 0, 0, 0);
-    }
-    if (live) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int q = 2 * j + h;
-        ushort4 lo, hi;
-        lo.x = f2bf(acc0[4 * j + 0]);
-        lo.y = f2bf(acc1[4 * j + 0]);
-        lo.z = f2bf(acc0[4 * j + 1]);
-        lo.w = f2bf(acc1[4 * j + 1]);
-        hi.x = f2bf(acc0[4 * j + 2]);
-        hi.y = f2bf(acc1[4 * j + 2]);
-        hi.z = f2bf(acc0[4 * j + 3]);
-        hi.w = f2bf(acc1[4 * j + 3]);
-        ushort4* dst = reinterpret_cast<ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + y) * 8);
-        dst[0] = lo;
-        dst[1] = hi;
-      }
-    }
+    for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], b[c][ks], acc, 0, 0, 0);
+    if (live) store_gh(L.G, g.Vh, y, c, h, acc);
   }
 }
 
 // Fused gather + mix for inference: a work-group of 128 threads transports its 128 even sites into an
-// LDS copy of their four T tiles (48 KiB), then its two waves run the MFMA mix of those 12 (block,
+// LDS copy of their four T tiles (24 KiB), then its two waves run the MFMA mix of those 12 (block,
 // colour) tiles and write G.  T never goes to HBM.
 template <int P, int RECON>
 __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
-  __shared__ bf4 lds[4 * 3 * 8 * 64];
+  __shared__ bf4 lds[4 * 3 * 4 * 64];
   const Geom& g = L.g;
-  using R = typename SpinorIO<P>::R;
-  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   const int yi = blockIdx.x * 128 + threadIdx.x;
   const int lb = threadIdx.x >> 5, n = threadIdx.x & 31;  // local block, site within the block
-  auto lds_at = [&](int c, int ks, int h) -> bf4& { return lds[((lb * 3 + c) * 8 + ks) * 64 + h * 32 + n]; };
+  auto lds_at = [&](int c, int ks, int h) -> bf4& { return lds[((lb * 3 + c) * 4 + ks) * 64 + h * 32 + n]; };
   if (yi < g.Vh) {
     int x[4];
     ptc_coords(g, 0, yi, x);
+    auto one = [&](auto MUc) {
+      constexpr int MU = decltype(MUc)::value;
+      float th[2][12];
+      gather_mu<P, RECON, MU, false>(L, x, yi, th);
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-#pragma unroll
-      for (int d = 0; d < 2; ++d) {
-        const bool fwd = d == 0;
-        bool last;
-        const int yn = ptc_neighbour(g, x, mu, fwd, last);
-        R v[24], u[18], t[24];
-        SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
-        if (fwd) {
-          const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)mu * LCH * LCB * g.Vh;
-          LinkIO<P, RECON>::load(gb, g.Vh, yi, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-          transport4<false>(u, v, t);
-        } else {
-          const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)mu * LCH * LCB * g.Vh;
-          LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-          transport4<true>(u, v, t);
-        }
-        const int p = 2 * mu + d;
-        const int ks = p >> 1, h = p & 1;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          bf4 re, im;
-          re[0] = (short)f2bf((float)t[2 * (0 + c)]);
-          re[1] = (short)f2bf((float)t[2 * (3 + c)]);
-          re[2] = (short)f2bf((float)t[2 * (6 + c)]);
-          re[3] = (short)f2bf((float)t[2 * (9 + c)]);
-          im[0] = (short)f2bf((float)t[2 * (0 + c) + 1]);
-          im[1] = (short)f2bf((float)t[2 * (3 + c) + 1]);
-          im[2] = (short)f2bf((float)t[2 * (6 + c) + 1]);
-          im[3] = (short)f2bf((float)t[2 * (9 + c) + 1]);
-          lds_at(c, ks, h) = re;
-          lds_at(c, 4 + ks, h) = im;
-        }
-      }
-    }
+      for (int c = 0; c < 3; ++c) pack_mu(th, c, lds_at(c, MU >> 1, MU & 1), lds_at(c, 2 + (MU >> 1), MU & 1));
+    };
+    one(std::integral_constant<int, 0>());
+    one(std::integral_constant<int, 1>());
+    one(std::integral_constant<int, 2>());
+    one(std::integral_constant<int, 3>());
   } else {  // sites past the end of the lattice contribute zero columns
     const bf4 z = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks)
+      for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
         for (int h = 0; h < 2; ++h) lds_at(c, ks, h) = z;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  bf4 a[2][8];
+  bf4 a[4];
   const bf4* A = reinterpret_cast<const bf4*>(L.Afr);
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) a[mt][ks] = A[(mt * 8 + ks) * 64 + lane];
+  for (int ks = 0; ks < 4; ++ks) a[ks] = A[ks * 64 + lane];
   for (int tile = wave; tile < 12; tile += 2) {  // (local block, colour)
     const int tb = tile / 3, c = tile - 3 * tb;
-    f16v acc0 = {0}, acc1 = {0};
+    f16v acc = {0};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const bf4 b = lds[((tb * 3 + c) * 8 + ks) * 64 + lane];
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[0][ks], b, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[1][ks], b, acc1, 0, 0, 0);
-    }
+    for (int ks = 0; ks < 4; ++ks)
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], lds[((tb * 3 + c) * 4 + ks) * 64 + lane], acc, 0, 0, 0);
     const int y = blockIdx.x * 128 + tb * 32 + (lane & 31);
-    if (y < g.Vh) {
+    if (y < g.Vh) store_gh(L.G, g.Vh, y, c, lane >> 5, acc);
+  }
+}
+
+// P(v)(x) = W0 v(x) + sum_q U_q(x) R_q gh_q(x+q) at odd sites x
+template <int P, int RECON, int MU>
+__device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4], int xi, float (&acc)[24]) {
+  using R = typename SpinorIO<P>::R;
+  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
+  const Geom& g = L.g;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int q = 2 * j + (lane >> 5);
-        ushort4 lo, hi;
-        lo.x = f2bf(acc0[4 * j + 0]);
-        lo.y = f2bf(acc1[4 * j + 0]);
-        lo.z = f2bf(acc0[4 * j + 1]);
-        lo.w = f2bf(acc1[4 * j + 1]);
-        hi.x = f2bf(acc0[4 * j + 2]);
-        hi.y = f2bf(acc1[4 * j + 2]);
-        hi.z = f2bf(acc0[4 * j + 3]);
-        hi.w = f2bf(acc1[4 * j + 3]);
-        ushort4* dst = reinterpret_cast<ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + y) * 8);
-        dst[0] = lo;
-        dst[1] = hi;
-      }
+  for (int d = 0; d < 2; ++d) {
+    const bool fwd = d == 0;
+    bool last;
+    const int yn = ptc_neighbour(g, x, MU, fwd, last);
+    const int q = 2 * MU + d;
+    float gh[12];  // gh_q(y) as [s'][colour][re,im]
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const ushort4 w = *reinterpret_cast<const ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + yn) * 4);
+      gh[2 * c] = bf2f(w.x);
+      gh[2 * c + 1] = bf2f(w.y);
+      gh[2 * (3 + c)] = bf2f(w.z);
+      gh[2 * (3 + c) + 1] = bf2f(w.w);
     }
+    R u[18];
+    float o[12];
+    if (fwd) {  // U_mu(x), odd-parity links
+      const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)MU * LCH * LCB * g.Vh;
+      LinkIO<P, RECON>::load(gb, g.Vh, xi, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      transport2<false>(u, gh, o);
+    } else {    // U_mu(x-mu)^dag, even-parity links
+      const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)MU * LCH * LCB * g.Vh;
+      LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      transport2<true>(u, gh, o);
+    }
+    reconstruct_add<MU>(o, fwd ? -1.f : 1.f, acc);
   }
 }
 
@@ -381,7 +416,6 @@ __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
   const Geom& g = L.g;
   if (xi >= g.Vh) return;
   using R = typename SpinorIO<P>::R;
-  constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   int x[4];
   ptc_coords(g, 1, xi, x);
   float acc[24];
@@ -404,67 +438,28 @@ __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
         acc[2 * (3 * s + c) + 1] = im;
       }
   }
-#pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const bool fwd = d == 0;
-      bool last;
-      const int yn = ptc_neighbour(g, x, mu, fwd, last);
-      const int q = 2 * mu + d;
-      float gv[24];  // g_q(y) as [spin][colour][re,im]
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const ushort4* src = reinterpret_cast<const ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + yn) * 8);
-        const ushort4 lo = src[0], hi = src[1];
-        gv[2 * (0 + c)] = bf2f(lo.x);
-        gv[2 * (0 + c) + 1] = bf2f(lo.y);
-        gv[2 * (3 + c)] = bf2f(lo.z);
-        gv[2 * (3 + c) + 1] = bf2f(lo.w);
-        gv[2 * (6 + c)] = bf2f(hi.x);
-        gv[2 * (6 + c) + 1] = bf2f(hi.y);
-        gv[2 * (9 + c)] = bf2f(hi.z);
-        gv[2 * (9 + c) + 1] = bf2f(hi.w);
-      }
-      R u[18];
-      float uf[18], o[24];
-      if (fwd) {  // U_mu(x), odd-parity links
-        const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, xi, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-#pragma unroll
-        for (int k = 0; k < 18; ++k) uf[k] = (float)u[k];
-        transport4<false>(uf, gv, o);
-      } else {    // U_mu(x-mu)^dag, even-parity links
-        const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)mu * LCH * LCB * g.Vh;
-        LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[mu][last][0]), R(L.ph[mu][last][1]));
-#pragma unroll
-        for (int k = 0; k < 18; ++k) uf[k] = (float)u[k];
-        transport4<true>(uf, gv, o);
-      }
-#pragma unroll
-      for (int k = 0; k < 24; ++k) acc[k] += o[k];
-    }
-  }
+  scatter_mu<P, RECON, 0>(L, x, xi, acc);
+  scatter_mu<P, RECON, 1>(L, x, xi, acc);
+  scatter_mu<P, RECON, 2>(L, x, xi, acc);
+  scatter_mu<P, RECON, 3>(L, x, xi, acc);
   typename SpinorIO<P>::R out[24];
 #pragma unroll
   for (int k = 0; k < 24; ++k) out[k] = (typename SpinorIO<P>::R)acc[k];
   SpinorIO<P>::store(L.out, L.out_nrm, L.out_stride, xi, out);
 }
 
-// S_real (64x64) = sum over tiles of dG_tile (64 x 32 sites) T_tile^T (32 sites x 64), per wave into
-// partials[wave][4][64][16] (four 32x32 output blocks (mt, nt) in the MFMA accumulator layout).
+// Sh_real (32x32) = sum over tiles of dG_tile (32 x 32 sites) T_tile^T (32 sites x 32), per wave into
+// partials[wave][64][16] (the MFMA accumulator layout).
 __global__ void __launch_bounds__(256) k_ptc_grad(PtcLaunch L) {
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int nblk = (L.g.Vh + 31) >> 5;
   const int ntiles = nblk * 3;
-  f16v acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (f16v){0};
+  f16v acc = {0};
   const int row = lane & 31, half = lane >> 5;
+  // real-embedding row -> (ks, h, j) = (row / 8, (row % 8) / 4, row % 4)
+  const size_t rbase = ((size_t)(row >> 3) * 64 + ((row >> 2) & 1) * 32) * 4 + (row & 3);
   for (int tile = wave; tile < ntiles; tile += nwaves) {
     const int blk = tile / 3, c = tile - 3 * blk;
     const unsigned short* dG = L.dT + tile_off(blk, c, 0, 0, 0);
@@ -472,39 +467,26 @@ __global__ void __launch_bounds__(256) k_ptc_grad(PtcLaunch L) {
 #pragma unroll
     for (int kq = 0; kq < 4; ++kq) {  // 8 sites per step: lanes 0-31 sites 8kq..+3, lanes 32-63 8kq+4..+7
       const int s0 = 8 * kq + 4 * half;
-      bf4 a[2], b[2];
+      bf4 a, b;
 #pragma unroll
-      for (int m = 0; m < 2; ++m) {
-        const int r = 32 * m + row;  // real-embedding row: (ks, h, j) = (r / 8, (r % 8) / 4, r % 4)
-        const size_t base = ((size_t)(r >> 3) * 64 + ((r >> 2) & 1) * 32) * 4 + (r & 3);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          a[m][j] = (short)dG[base + (size_t)(s0 + j) * 4];
-          b[m][j] = (short)T[base + (size_t)(s0 + j) * 4];
-        }
+      for (int j = 0; j < 4; ++j) {
+        a[j] = (short)dG[rbase + (size_t)(s0 + j) * 4];
+        b[j] = (short)T[rbase + (size_t)(s0 + j) * 4];
       }
-#pragma unroll
-      for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
   }
-  float* out = L.gpart + (size_t)wave * 4 * 64 * 16;
+  float* out = L.gpart + (size_t)wave * 64 * 16;
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) out[((mt * 2 + nt) * 64 + lane) * 16 + r] = acc[mt][nt][r];
+  for (int r = 0; r < 16; ++r) out[lane * 16 + r] = acc[r];
 }
 
-// out[e] = sum over waves of partials[wave][e], e < 4*64*16 (FP64 accumulation, fixed order)
+// out[e] = sum over waves of partials[wave][e], e < 64*16 (FP64 accumulation, fixed order)
 __global__ void __launch_bounds__(256) k_ptc_grad_sum(const float* part, int nwaves, double* out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 4 * 64 * 16) return;
+  if (e >= 64 * 16) return;
   double s = 0;
-  for (int w = 0; w < nwaves; ++w) s += part[(size_t)w * 4 * 64 * 16 + e];
+  for (int w = 0; w < nwaves; ++w) s += part[(size_t)w * 64 * 16 + e];
   out[e] = s;
 }
 
@@ -588,7 +570,7 @@ void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s) {
 
 void launch_ptc_grad(const PtcLaunch& L, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_ptc_grad, dim3(ptc_waves() / 4), dim3(256), 0, s, L);
-  hipLaunchKernelGGL(k_ptc_grad_sum, dim3(4 * 64 * 16 / 256), dim3(256), 0, s, L.gpart, ptc_waves(), out);
+  hipLaunchKernelGGL(k_ptc_grad_sum, dim3(64 * 16 / 256), dim3(256), 0, s, L.gpart, ptc_waves(), out);
 }
 
 int launch_w0_grad(const PtcLaunch& L, hipStream_t s) {

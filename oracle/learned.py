@@ -3,17 +3,35 @@
 numpy restatement of the learned preconditioner (the library's gauge-equivariant parallel-transport
 network) and of its loss gradient, in FP64, on odd-checkerboard vectors:
 
-    P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),        x odd,  q in {+-mu}
-    g_q(y)  = sum_c W2[q][c] h_c(y),   h_c(y) = sum_p W1[c][p] U_p(y) v(y+p),     y even
+    P(v)(x) = W0 v(x) + sum_q U_q(x) Pi_q g_q(x+q),        x odd,  q in {+-mu}
+    g_q(y)  = sum_c W2[q][c] h_c(y),   h_c(y) = sum_p W1[c][p] U_p(y) (Pi_p v)(y+p),     y even
 
-U_{+mu}(y) = U_mu(y), U_{-mu}(y) = U_mu(y-mu)^dag.  Weights: W1 complex[64, 32] indexed
-((c*4 + s), (p*4 + t)), W2 complex[32, 64] indexed ((q*4 + s), (c*4 + t)), W0 complex[4, 4]; path
-p = 2 mu + (0 forward, 1 backward).  Spin matrices act on spin, the identity on colour.
+U_{+mu}(y) = U_mu(y), U_{-mu}(y) = U_mu(y-mu)^dag; Pi_p = (1 + sigma_p gamma_mu) / 2 with sigma = -1 on the
+forward and +1 on the backward path: the rank-2 spin projector of the hopping term's direction
+(lwldslash_w.h:252-295), so every transport moves a half spinor, as the hopping term's does.  Weights:
+W1 complex[64, 32] indexed ((c*4 + s), (p*4 + t)), W2 complex[32, 64] indexed ((q*4 + s), (c*4 + t)),
+W0 complex[4, 4]; path p = 2 mu + (0 forward, 1 backward).  Spin matrices act on spin, the identity on
+colour.  The network depends on Wc = W2 W1 only through Pi_q Wc Pi_p.
 
 Host layout as in oracle/__init__.py: full lattice checkerboard-major, x-fastest; gauge
 complex128[4, V, 3, 3] with the boundary conditions applied.
 """
 import numpy as np
+
+# DeGrand-Rossi gamma matrices (the spin basis of the fermion layout), as in oracle/oracle.cpp
+_I = 1j
+GAMMA = np.array([
+    [[0, 0, 0, _I], [0, 0, _I, 0], [0, -_I, 0, 0], [-_I, 0, 0, 0]],
+    [[0, 0, 0, -1], [0, 0, 1, 0], [0, 1, 0, 0], [-1, 0, 0, 0]],
+    [[0, 0, _I, 0], [0, 0, 0, -_I], [-_I, 0, 0, 0], [0, _I, 0, 0]],
+    [[0, 0, 1, 0], [0, 0, 0, 1], [1, 0, 0, 0], [0, 1, 0, 0]],
+], dtype=complex)
+
+
+def projector(p):
+    """Pi_p = (1 + sigma_p gamma_mu) / 2, sigma = -1 forward (p even), +1 backward."""
+    mu, sg = p // 2, (-1.0 if p % 2 == 0 else 1.0)
+    return (np.eye(4) + sg * GAMMA[mu]) / 2
 
 
 def neighbour_tables(L):
@@ -50,14 +68,16 @@ def neighbour_tables(L):
 
 
 def _transports(L, U, v_full, sites):
-    """t[p][k] = U_p(y) v(y+p) for y = sites[k]: array [8, len(sites), 4, 3]."""
+    """t[p][k] = U_p(y) (Pi_p v)(y+p) for y = sites[k]: array [8, len(sites), 4, 3]."""
     fwd, bwd = neighbour_tables(L)
     out = np.zeros((8, len(sites), 4, 3), complex)
     for mu in range(4):
         yf = fwd[mu][sites]
-        out[2 * mu] = np.einsum("kab,ksb->ksa", U[mu, sites], v_full[yf])
+        pv = np.einsum("st,ktc->ksc", projector(2 * mu), v_full[yf])
+        out[2 * mu] = np.einsum("kab,ksb->ksa", U[mu, sites], pv)
         yb = bwd[mu][sites]
-        out[2 * mu + 1] = np.einsum("kba,ksb->ksa", U[mu, yb].conj(), v_full[yb])
+        pv = np.einsum("st,ktc->ksc", projector(2 * mu + 1), v_full[yb])
+        out[2 * mu + 1] = np.einsum("kba,ksb->ksa", U[mu, yb].conj(), pv)
     return out
 
 
@@ -73,7 +93,8 @@ def ptc_apply(L, U, W1, W2, W0, v_odd, return_parts=False):
     Wc = W2 @ W1
     g = np.einsum("qp,kpc->kqc", Wc, tk).reshape(Vh, 8, 4, 3)  # g_q(y)
     gfull = np.zeros((8, V, 4, 3), complex)
-    gfull[:, :Vh] = g.transpose(1, 0, 2, 3)
+    for q in range(8):  # Pi_q g_q
+        gfull[q, :Vh] = np.einsum("st,ktc->ksc", projector(q), g[:, q])
     fwd, bwd = neighbour_tables(L)
     odd = np.arange(Vh, V)
     out = np.einsum("st,kta->ksa", W0, v_odd)
@@ -103,6 +124,8 @@ def ptc_grad(L, U, W1, W2, W0, v_odd, e_odd):
         np.add.at(dg[2 * mu], yf, np.einsum("kba,ksb->ksa", U[mu, odd].conj(), delta))
         yb = bwd[mu][odd]
         np.add.at(dg[2 * mu + 1], yb, np.einsum("kab,ksb->ksa", U[mu, yb], delta))
+    for q in range(8):  # the adjoint of Pi_q (Hermitian)
+        dg[q] = np.einsum("st,ktc->ksc", projector(q), dg[q])
     dgk = dg[:, :Vh].transpose(1, 0, 2, 3).reshape(Vh, 32, 3)
     S = np.einsum("kqc,kpc->qp", dgk, tk.conj())
     S0 = np.einsum("ksc,ktc->st", delta, v_odd.conj())

@@ -1,6 +1,8 @@
 """The oracle's learned-preconditioner network, pinned independently of its own formula:
-the Neumann-series weights reproduce 1/a + (1/(4a^3)) D_oe D_eo built from the oracle's hopping term;
-the gradient agrees with central finite differences of the loss; the network is gauge covariant."""
+the Neumann-series weights reproduce 1/a + (1/(4a^3)) D_oe D_eo built from the oracle's hopping term
+(which fixes the signs of the half-spinor projectors); the gradient agrees with central finite
+differences of the loss; the network is gauge covariant; the projectors are the rank-2 spin projectors
+of the hopping directions, and the output depends on Wc only through Pi_q Wc Pi_p."""
 import numpy as np
 
 import oracle
@@ -89,3 +91,27 @@ def test_gauge_covariance():
     out = learned.ptc_apply(L, U, W1, W2, W0, v)
     outg = learned.ptc_apply(L, Ug, W1, W2, W0, vg)
     assert rel(outg, np.einsum("kab,ksb->ksa", g[Vh:], out)) < 1e-12
+
+
+def test_projectors_and_projected_dependence():
+    for p in range(8):
+        P = learned.projector(p)
+        assert np.allclose(P @ P, P) and np.allclose(P, P.conj().T)
+        assert abs(np.trace(P).real - 2) < 1e-14
+        mu, sg = p // 2, (-1.0 if p % 2 == 0 else 1.0)
+        assert np.allclose(2 * P, np.eye(4) + sg * GAMMA[mu])  # the tests' own gammas
+    rng = np.random.default_rng(70)
+    U = numpy_gauge(L, 71)
+    Wc = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    Pblk = np.zeros((32, 32), complex)
+    for p in range(8):
+        Pblk[4 * p:4 * p + 4, 4 * p:4 * p + 4] = learned.projector(p)
+    W1 = np.vstack([np.eye(32), np.zeros((32, 32))])
+    W0 = np.eye(4) * 0.2
+    v = random_fermion(Vh, 72)
+    a = learned.ptc_apply(L, U, W1, np.hstack([Wc, np.zeros((32, 32))]), W0, v)
+    b = learned.ptc_apply(L, U, W1, np.hstack([Pblk @ Wc @ Pblk, np.zeros((32, 32))]), W0, v)
+    assert rel(a, b) < 1e-13
+    # and the complement does nothing
+    c = learned.ptc_apply(L, U, W1, np.hstack([Wc - Pblk @ Wc @ Pblk, np.zeros((32, 32))]), W0 * 0, v)
+    assert np.abs(c).max() < 1e-12

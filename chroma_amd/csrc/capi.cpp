@@ -432,7 +432,7 @@ int cmx_op_set_hop_config(cmx_op h, int prec, const int cfg[6]) {
     cmx::Op& op = O(h);
     CMX_REQUIRE(cfg && prec >= 0 && prec <= 2, CMX_ERR_ARG, "set_hop_config: bad argument");
     CMX_REQUIRE(cfg[0] == 64 || cfg[0] == 128 || cfg[0] == 256, CMX_ERR_ARG, "set_hop_config: block must be 64, 128 or 256");
-    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1) && (cfg[4] == 0 || cfg[4] == 1 || cfg[4] == 3),
+    CMX_REQUIRE((cfg[1] == 0 || cfg[1] == 1) && cfg[2] >= 1 && (cfg[3] == 0 || cfg[3] == 1) && cfg[4] >= 0 && cfg[4] <= 3,
                 CMX_ERR_ARG,
                 "set_hop_config: bad tiling / prefetch flags");
     CMX_REQUIRE(cfg[5] >= 0 && cfg[5] <= 2, CMX_ERR_ARG, "set_hop_config: target must be 0, 1 or 2");

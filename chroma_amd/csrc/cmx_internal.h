@@ -105,7 +105,9 @@ template <> struct SpinorIO<P64> {
   }
 };
 
-// Load with a cache policy: CP = 0 an ordinary load, otherwise non-temporal (data read once)
+// Cache policy of a load: 0 an ordinary load, CMX_NT non-temporal (streamed through L2 with early
+// eviction: for data read once per launch, so that it displaces less of the data that is re-read)
+#define CMX_NT 2
 template <int CP, typename T> __device__ __forceinline__ T ld_cp(const T* p) {
   static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "ld_cp: 4, 8 or 16 bytes");
   if constexpr (CP == 0) {
@@ -209,11 +211,12 @@ __device__ __forceinline__ void recon_row2(R (&u)[18], R dre, R dim) {
 
 template <> struct LinkIO<P64, 18> {
   using R = double;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R, R) {
     const double2* b = reinterpret_cast<const double2*>(base);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      double2 v = b[k * Vh + i];
+      double2 v = ld_cp<CP>(&b[k * Vh + i]);
       u[2 * k] = v.x;
       u[2 * k + 1] = v.y;
     }
@@ -221,11 +224,12 @@ template <> struct LinkIO<P64, 18> {
 };
 template <> struct LinkIO<P32, 18> {
   using R = float;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R, R) {
     const float2* b = reinterpret_cast<const float2*>(base);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      float2 v = b[k * Vh + i];
+      float2 v = ld_cp<CP>(&b[k * Vh + i]);
       u[2 * k] = v.x;
       u[2 * k + 1] = v.y;
     }
@@ -233,11 +237,12 @@ template <> struct LinkIO<P32, 18> {
 };
 template <> struct LinkIO<P32, 12> {
   using R = float;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R dre, R dim) {
     const float4* b = reinterpret_cast<const float4*>(base);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      float4 v = b[k * Vh + i];
+      float4 v = ld_cp<CP>(&b[k * Vh + i]);
       u[4 * k] = v.x;
       u[4 * k + 1] = v.y;
       u[4 * k + 2] = v.z;
@@ -252,12 +257,13 @@ union H4 {
 };
 template <> struct LinkIO<P16, 12> {
   using R = float;
+  template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R dre, R dim) {
     const uint2* b = reinterpret_cast<const uint2*>(base);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       H4 v;
-      v.u = b[k * Vh + i];
+      v.u = ld_cp<CP>(&b[k * Vh + i]);
       float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
       u[4 * k] = f0.x;
       u[4 * k + 1] = f0.y;
@@ -356,9 +362,10 @@ struct HopLaunch {
   int tile_zb;       // z planes per tile
   int tile_t8;       // t slices per XCD slab = Lt/8
   int prefetch;      // 1: double-buffered raw loads across the 8 directions
-  int fast;          // bit 0: 2-D grid kernel of hop_fast.h (FP16/FP32, 12-real links, real phases);
-                     // bit 1 (HOP_FAST_STREAM): its loads of data read once per launch (links,
-                     // clover, the xpay input) carry the non-temporal hint
+  int fast;          // bit 0: 2-D grid kernel of hop_fast.h (FP16/FP32, 12-real links, real phases),
+                     // else the site-linear kernel of hop_kernel.h; bit 1 (HOP_FAST_STREAM): the
+                     // loads of data read once per launch (links, clover, the xpay input) carry the
+                     // non-temporal hint
   float inv_lxh;     // 1 / (Lx/2), for the fast kernel's plane decomposition
   float sgn[4][2];   // real det(U) phase per direction: [bulk | last slice]
 };

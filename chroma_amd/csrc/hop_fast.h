@@ -20,9 +20,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsig
   // raw buffer, stride 0; word 3 = 0x00020000 for the gfx9 family (loads past `bytes` return 0)
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
-// CP: cache-policy operand of the buffer loads; 0 = default, CMX_NT = non-temporal (streamed through
-// L2 with early eviction: for data read once, so that it displaces less of the reused data)
-#define CMX_NT 2
+// CP: cache-policy operand of the buffer loads; 0 = default, CMX_NT = non-temporal
 template <int CP = 0>
 __device__ __forceinline__ uint4 bload128(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
   const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CP);

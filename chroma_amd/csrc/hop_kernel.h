@@ -103,7 +103,7 @@ template <int P, int RECON> struct HopTypes {
   static constexpr int LCB = link_chunk_bytes(P, RECON);
 };
 
-template <int P, int RECON, int MU, int ISIGN>
+template <int P, int RECON, int MU, int ISIGN, int CP = 0>
 __device__ __forceinline__ void hop_dir(const HopLaunch& h, typename SpinorIO<P>::R (&acc)[24], int i, const int (&x)[4],
                                         int rowbase_unused) {
   using R = typename SpinorIO<P>::R;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void hop_dir(const HopLaunch& h, typename SpinorIO<P>
   {
     SpinorIO<P>::load(h.in, h.in_nrm, h.in_stride, jf, p);
     const int last = (x[MU] == L - 1);
-    LinkIO<P, RECON>::load(gout, Vh, i, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
+    LinkIO<P, RECON>::template load<CP>(gout, Vh, i, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
     project<MU, -ISIGN>(p, hs);
     link_mul<false>(u, hs, o);
     accumulate<MU, -ISIGN>(acc, o);
@@ -134,7 +134,7 @@ __device__ __forceinline__ void hop_dir(const HopLaunch& h, typename SpinorIO<P>
   {
     SpinorIO<P>::load(h.in, h.in_nrm, h.in_stride, jb, p);
     const int last = (x[MU] == 0);
-    LinkIO<P, RECON>::load(gin, Vh, jb, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
+    LinkIO<P, RECON>::template load<CP>(gin, Vh, jb, u, R(h.ph[MU][last][0]), R(h.ph[MU][last][1]));
     project<MU, ISIGN>(p, hs);
     link_mul<true>(u, hs, o);
     accumulate<MU, ISIGN>(acc, o);
@@ -412,7 +412,9 @@ __device__ __forceinline__ int hop_site_index(const HopLaunch& h, int blk, int n
 }
 
 // work-group blk of nblk of the launch h
-template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
+// LCP: cache policy of the loads of data read once per launch (links, clover, xpay input); the
+// double-buffered (PF) path loads its links with the default policy
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>
 __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) {
   using R = typename SpinorIO<P>::R;
   if (h.skip && *h.skip) return;
@@ -450,10 +452,10 @@ __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) 
       }
       hop_prefetched<P, RECON, ISIGN>(h, st, acc);
     } else {
-      hop_dir<P, RECON, 0, ISIGN>(h, acc, i, x, 0);
-      hop_dir<P, RECON, 1, ISIGN>(h, acc, i, x, 0);
-      hop_dir<P, RECON, 2, ISIGN>(h, acc, i, x, 0);
-      hop_dir<P, RECON, 3, ISIGN>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 0, ISIGN, LCP>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 1, ISIGN, LCP>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 2, ISIGN, LCP>(h, acc, i, x, 0);
+      hop_dir<P, RECON, 3, ISIGN, LCP>(h, acc, i, x, 0);
     }
 
     R out[24];
@@ -464,17 +466,17 @@ __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) 
     } else if constexpr (EPI == EPI_CLOV) {
 #pragma unroll
       for (int k = 0; k < 24; ++k) acc[k] *= b;
-      clover_apply_site<P>(h.clov, h.g.Vh, i, acc, out);
+      clover_apply_site<P, LCP>(h.clov, h.g.Vh, i, acc, out);
     } else if constexpr (EPI == EPI_XPAY) {
       R xs[24];
-      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
+      SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
       const R a = R(h.a);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = a * xs[k] + b * acc[k];
     } else {
       R xs[24];
-      SpinorIO<P>::load(h.x, h.x_nrm, h.x_stride, i, xs);
-      clover_apply_site<P>(h.clov, h.g.Vh, i, xs, out);
+      SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
+      clover_apply_site<P, LCP>(h.clov, h.g.Vh, i, xs, out);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] += b * acc[k];
     }
@@ -492,25 +494,32 @@ __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) 
   }
 }
 
-template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>
 __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
-  hop_body<P, RECON, ISIGN, EPI, NORM, PF>(h, blockIdx.x, gridDim.x);
+  hop_body<P, RECON, ISIGN, EPI, NORM, PF, LCP>(h, blockIdx.x, gridDim.x);
 }
-template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF>
+template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>
 __global__ void __launch_bounds__(256) k_hop_b(const HopLaunch* __restrict__ tab, const int4* __restrict__ map) {
   const int4 m = map[blockIdx.x];
-  hop_body<P, RECON, ISIGN, EPI, NORM, PF>(tab[m.x], m.y, m.z);
+  hop_body<P, RECON, ISIGN, EPI, NORM, PF, LCP>(tab[m.x], m.y, m.z);
 }
 
 template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
   const int nb = (h.g.Vh + h.block - 1) / h.block;
   const bool norm = h.partials != nullptr;
-#define CMX_HOP_GO(IS, EP, NM)                                                                            \
+  const bool lnt = (h.fast & HOP_FAST_STREAM) != 0 && !PF;
+#define CMX_HOP_GO2(IS, EP, NM, CP)                                                                       \
   if (bg)                                                                                                 \
-    hipLaunchKernelGGL((k_hop_b<P, RECON, IS, EP, NM, PF>), dim3(bg->nblocks), dim3(h.block), 0, s,      \
+    hipLaunchKernelGGL((k_hop_b<P, RECON, IS, EP, NM, PF, CP>), dim3(bg->nblocks), dim3(h.block), 0, s,  \
                        static_cast<const HopLaunch*>(bg->tab), bg->map);                                  \
   else                                                                                                    \
-    hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, NM, PF>), dim3(nb), dim3(h.block), 0, s, h);
+    hipLaunchKernelGGL((k_hop<P, RECON, IS, EP, NM, PF, CP>), dim3(nb), dim3(h.block), 0, s, h);
+#define CMX_HOP_GO(IS, EP, NM)                                                                            \
+  if (lnt) {                                                                                              \
+    CMX_HOP_GO2(IS, EP, NM, CMX_NT)                                                                       \
+  } else {                                                                                                \
+    CMX_HOP_GO2(IS, EP, NM, 0)                                                                            \
+  }
 #define CMX_HOP_CASE(IS, EP)                                                                              \
   if (h.isign == IS && h.epi == EP) {                                                                     \
     if (norm) {                                                                                           \
@@ -530,6 +539,7 @@ template <int P, int RECON, bool PF> void launch_hop_t(const HopLaunch& h, hipSt
   CMX_HOP_CASE(-1, EPI_CLOV_XPAY)
 #undef CMX_HOP_CASE
 #undef CMX_HOP_GO
+#undef CMX_HOP_GO2
   throw Error(CMX_ERR_ARG, "launch_hop: bad isign/epilogue");
 }
 

@@ -137,7 +137,7 @@ template <int NB> void launch_nb(const MultiHopLaunch& M, int r0, hipStream_t s)
 
 void launch_hop_multi(const MultiHopLaunch& M, hipStream_t s) {
   const HopLaunch& h = M.h;
-  CMX_REQUIRE(h.prec == P16 && h.recon == 12 && h.fast, CMX_ERR_UNSUPPORTED,
+  CMX_REQUIRE(h.prec == P16 && h.recon == 12 && (h.fast & 1), CMX_ERR_UNSUPPORTED,
               "multi-source hop: FP16 storage with 12-real links and real boundary phases only");
   CMX_REQUIRE(M.nrhs >= 1 && M.nrhs <= MULTI_MAXR, CMX_ERR_ARG, "multi-source hop: 1..12 sources");
   CMX_REQUIRE(h.epi == EPI_NONE && !h.partials, CMX_ERR_ARG, "multi-source hop: plain hopping term only");

@@ -3,7 +3,7 @@
 
 namespace cmx {
 void launch_hop_p16(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
-  if (h.fast) {
+  if (h.fast & 1) {
     launch_hop_fast_t<P16>(h, s, bg);
     return;
   }

@@ -29,7 +29,7 @@ void launch_hop(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
   }
 }
 int hop_num_blocks(const HopLaunch& h) {
-  if (h.fast) return ((h.g.Lxh * h.g.L[1] + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3];
+  if (h.fast & 1) return ((h.g.Lxh * h.g.L[1] + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3];
   return (h.g.Vh + h.block - 1) / h.block;
 }
 
@@ -44,12 +44,11 @@ int hop_apply_config(HopLaunch& h, const HopConfig& c) {
       if (std::fabs(h.ph[mu][k][1]) > 1e-12 || std::fabs(std::fabs(h.ph[mu][k][0]) - 1.0) > 1e-6) real_phases = false;
       h.sgn[mu][k] = (float)(h.ph[mu][k][0] < 0 ? -1.0 : 1.0);
     }
-  h.fast = ((c.fast & 1) && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
-            h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536)
-               ? c.fast
-               : 0;
+  const bool plane = (c.fast & 1) && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
+                     h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536;
+  h.fast = plane ? c.fast : (c.fast & HOP_FAST_STREAM);  // the stream bit applies to both kernels
   h.inv_lxh = 1.0f / (float)h.g.Lxh;
-  if (h.fast) {  // the plane-grid kernel tiles whole planes: any block size, Lt % 8 and Lz % zb only
+  if (h.fast & 1) {  // the plane-grid kernel tiles whole planes: any block size, Lt % 8 and Lz % zb only
     if (c.tiled && h.g.L[3] % 8 == 0 && c.zb > 0 && h.g.L[2] % c.zb == 0) {
       h.tiled = 1;
       h.tile_zb = c.zb;
@@ -391,7 +390,7 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
   // link cache policy of that choice
   c.fast = 1 | (c.fast & HOP_FAST_STREAM);
   hop_apply_config(M.h, c);
-  if (!M.h.fast) return false;
+  if (!(M.h.fast & 1)) return false;
   M.h.isign = isign;
   M.h.cb = cb;
   M.h.epi = EPI_NONE;
@@ -546,10 +545,13 @@ void Op::autotune() {
             for (int zb : {1, 4, 8}) cands.push_back(HopConfig{block, 1, zb, 0, fm});
           }
         }
-        for (int pf : {0, 1}) {
+        for (int pf : {0, 1}) {  // site-linear kernel; streamed loads on the single-buffered path
           if (pf && prec == P64) continue;
-          cands.push_back(HopConfig{block, 0, 1, pf, 0});
-          for (int zb : {2, 8}) cands.push_back(HopConfig{block, 1, zb, pf, 0});
+          for (int st : {0, HOP_FAST_STREAM}) {
+            if (pf && st) continue;
+            cands.push_back(HopConfig{block, 0, 1, pf, st});
+            for (int zb : {2, 8}) cands.push_back(HopConfig{block, 1, zb, pf, st});
+          }
         }
       }
       float best = 1e30f;
@@ -558,7 +560,7 @@ void Op::autotune() {
       for (const HopConfig& c : cands) {
         HopLaunch probe = hop_base(prec, which);
         hop_apply_config(probe, c);
-        if ((c.tiled && !probe.tiled) || (c.fast && !probe.fast)) continue;  // not admitted on this lattice
+        if ((c.tiled && !probe.tiled) || ((c.fast & 1) && !(probe.fast & 1))) continue;  // not admitted here
         hcfg[prec][which] = c;
         auto run = [&]() {
           if (which == 0)

@@ -512,11 +512,12 @@ std::vector<HopConfig> batch_candidates(const std::vector<Op*>& ops, int ps) {
     HopConfig c;
     c.fast = 1;
     hop_apply_config(h, c);
-    if (!h.fast) fast_ok = false;
+    if (!(h.fast & 1)) fast_ok = false;
   }
   std::vector<HopConfig> c;
   for (int block : {64, 128, 256}) {
     c.push_back(HopConfig{block, 0, 1, 0, 0});
+    c.push_back(HopConfig{block, 0, 1, 0, HOP_FAST_STREAM});
     if (fast_ok) c.push_back(HopConfig{block, 0, 1, 0, 1 | HOP_FAST_STREAM});
   }
   return c;

@@ -294,8 +294,8 @@ class Operator:
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
 
     def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0, fast=0, target=0):
-        """fast: 0 generic kernel, 1 plane-grid kernel, 3 plane-grid kernel with the read-once
-        loads (links, clover, xpay input) streamed non-temporally."""
+        """fast: 0 site-linear kernel, 1 plane-grid kernel; +2 with the read-once loads (links, clover,
+        xpay input) streamed non-temporally."""
         cfg = (ctypes.c_int * 6)(int(block), int(tiled), int(zb), int(prefetch), int(fast), int(target))
         check(lib().cmx_op_set_hop_config(self.h, int(prec), cfg))
 

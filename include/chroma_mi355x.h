@@ -207,7 +207,7 @@ typedef struct {
   int n_stages;          /* kernels per batched iteration                                         */
   long blocks_per_iter;  /* work-groups per batched iteration                                     */
   int hop_block;         /* hopping-kernel launch shape used: threads per work-group              */
-  int hop_fast;          /* 0 site-linear, 1 plane-grid, 3 plane-grid with streamed loads          */
+  int hop_fast;          /* 0 site-linear, 1 plane-grid; +2 with streamed read-once loads        */
 } cmx_batch_result;
 int cmx_solve_batch(const cmx_op* ops, int n, const cmx_solver_params* p, const cmx_batch_params* bp,
                     double* const* psi, const double* const* chi, int mdagm, cmx_solver_result* res,
@@ -263,9 +263,10 @@ int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
 
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
- * double-buffered direction loads (0/1), plane-grid kernel (0 off, 1 on, 3 on with the loads of data read
- * once per launch -- links, clover, xpay input -- streamed non-temporally; FP16 and FP32 with 12-real links and real boundary phases), target (0 both, 1 the EO operator's fused kernels, 2 the plain hopping
- * term; for get, 2 reads the plain-hop shape and 0/1 the operator's)}.  No counterpart in the
+ * double-buffered direction loads (0/1), kernel (0 site-linear, 1 plane-grid -- FP16 and FP32 with
+ * 12-real links and real boundary phases; +2 with the loads of data read once per launch -- links,
+ * clover, xpay input -- streamed non-temporally), target (0 both, 1 the EO operator's fused kernels,
+ * 2 the plain hopping term; for get, 2 reads the plain-hop shape and 0/1 the operator's)}.  No counterpart in the
  * reference beyond QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
 int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[6]);
 int cmx_op_set_hop_config(cmx_op op, int prec, const int cfg[6]);

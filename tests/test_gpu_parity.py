@@ -447,7 +447,8 @@ def test_twisted_golden_through_gpu():
 @pytest.mark.parametrize("cfg", [(64, 0, 1, 0, 0), (64, 1, 1, 0, 0), (64, 1, 2, 1, 0), (128, 1, 4, 1, 0),
                                  (128, 0, 1, 1, 0), (128, 1, 8, 0, 0), (256, 1, 2, 1, 0), (64, 0, 1, 0, 1),
                                  (128, 0, 1, 0, 1), (256, 0, 1, 0, 1), (64, 1, 4, 0, 1), (256, 1, 2, 0, 1),
-                                 (128, 1, 8, 0, 1), (256, 0, 1, 0, 3), (128, 1, 8, 0, 3)])
+                                 (128, 1, 8, 0, 1), (256, 0, 1, 0, 3), (128, 1, 8, 0, 3), (64, 1, 2, 0, 2),
+                                 (128, 0, 1, 0, 2)])
 def test_hop_launch_shapes(prec, cfg):
     """Every launch shape the tuner may pick (block size, tiled work-group order, prefetch, plane-grid
     kernel with or without streamed link loads) gives the same operator as the oracle; 16x16x8x16 admits the tiling for blocks of 64 and

@@ -422,8 +422,11 @@ int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[6]) {
   return guarded([&] {
     cmx::Op& op = O(h);
     CMX_REQUIRE(cfg && prec >= 0 && prec <= 2, CMX_ERR_ARG, "get_hop_config: bad argument");
-    const cmx::HopConfig& c = op.hcfg[prec][cfg[5] == 2 ? 1 : 0];
-    cfg[0] = c.block, cfg[1] = c.tiled, cfg[2] = c.zb, cfg[3] = c.prefetch, cfg[4] = c.fast;
+    const int which = cfg[5] == 2 ? 1 : 0;
+    const cmx::HopConfig& c = op.hcfg[prec][which];
+    // the kernel actually admitted for this operator (the plane-grid kernel needs 12-real links,
+    // real boundary phases and a field below 2 GiB; otherwise the site-linear kernel runs)
+    cfg[0] = c.block, cfg[1] = c.tiled, cfg[2] = c.zb, cfg[3] = c.prefetch, cfg[4] = op.hop_base(prec, which).fast;
   });
 }
 

@@ -54,7 +54,7 @@ __host__ __device__ constexpr int spinor_chunks(int p) { return p == P64 ? 12 : 
 inline size_t spinor_bytes(int p, long Vh) {
   return (size_t)spinor_chunks(p) * 16 * Vh + (p == P16 ? sizeof(float) * Vh : 0);
 }
-__host__ __device__ constexpr int link_chunks(int p, int recon) { return recon == 18 ? 9 : 3; }
+__host__ __device__ constexpr int link_chunks(int p, int recon) { return recon == 18 ? 9 : (p == P64 ? 6 : 3); }
 __host__ __device__ constexpr int link_chunk_bytes(int p, int recon) {
   return p == P64 ? 16 : (p == P32 ? (recon == 18 ? 8 : 16) : 8);
 }
@@ -220,6 +220,20 @@ template <> struct LinkIO<P64, 18> {
       u[2 * k] = v.x;
       u[2 * k + 1] = v.y;
     }
+  }
+};
+template <> struct LinkIO<P64, 12> {
+  using R = double;
+  template <int CP = 0>
+  __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R dre, R dim) {
+    const double2* b = reinterpret_cast<const double2*>(base);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double2 v = ld_cp<CP>(&b[k * Vh + i]);
+      u[2 * k] = v.x;
+      u[2 * k + 1] = v.y;
+    }
+    recon_row2<R>(u, dre, dim);
   }
 };
 template <> struct LinkIO<P32, 18> {

@@ -49,6 +49,12 @@ template <> struct FastSpinor<P32> {
   __device__ __forceinline__ float re(int k) const { return c[k].x; }
   __device__ __forceinline__ float im(int k) const { return c[k].y; }
 };
+template <> struct FastSpinor<P64> {
+  double2 c[12];
+  static constexpr double n = 1.0;
+  __device__ __forceinline__ double re(int k) const { return c[k].x; }
+  __device__ __forceinline__ double im(int k) const { return c[k].y; }
+};
 template <int P> struct FastLink;
 template <> struct FastLink<P16> {
   __half2 c[6];
@@ -60,6 +66,21 @@ template <> struct FastLink<P32> {
   __device__ __forceinline__ float re(int k) const { return c[k].x; }
   __device__ __forceinline__ float im(int k) const { return c[k].y; }
 };
+template <> struct FastLink<P64> {
+  double2 c[6];
+  __device__ __forceinline__ double re(int k) const { return c[k].x; }
+  __device__ __forceinline__ double im(int k) const { return c[k].y; }
+};
+// arithmetic type of the plane kernel: FP32 for FP16/FP32 storage, FP64 for FP64 storage
+template <int P> struct FastReal { typedef float T; };
+template <> struct FastReal<P64> { typedef double T; };
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ __forceinline__ double2 as_double2(uint4 v) {
+  double2 d;
+  __builtin_memcpy(&d, &v, 16);
+  return d;
+}
 
 union H8x {
   uint4 u;
@@ -82,13 +103,16 @@ __device__ __forceinline__ void fast_load_spinor(__amdgpu_buffer_rsrc_t rs, __am
       for (int q = 0; q < 4; ++q) s.c[4 * k + q] = v.h[q];
     }
     s.n = bload32f(rn, j * 4, 0);
-  } else {
+  } else if constexpr (P == P32) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const uint4 v = bload128(rs, j * 16, k * stride * 16);
       s.c[2 * k] = make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
       s.c[2 * k + 1] = make_float2(__uint_as_float(v.z), __uint_as_float(v.w));
     }
+  } else {  // 12 chunks: the chunk offset (up to 11 x stride x 16 B) is formed in 32-bit unsigned
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s.c[k] = as_double2(bload128(rs, j * 16, (int)((unsigned)k * (unsigned)stride * 16u)));
   }
 }
 
@@ -102,13 +126,16 @@ __device__ __forceinline__ void fast_load_link(__amdgpu_buffer_rsrc_t rl, int Vh
       u.c[2 * k] = v.h[0];
       u.c[2 * k + 1] = v.h[1];
     }
-  } else {
+  } else if constexpr (P == P32) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const uint4 v = bload128<CP>(rl, j * 16, k * Vh * 16);
+      const uint4 v = bload128<CP>(rl, j * 16, (int)((unsigned)k * (unsigned)Vh * 16u));
       u.c[2 * k] = make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
       u.c[2 * k + 1] = make_float2(__uint_as_float(v.z), __uint_as_float(v.w));
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) u.c[k] = as_double2(bload128<CP>(rl, j * 16, (int)((unsigned)k * (unsigned)Vh * 16u)));
   }
 }
 
@@ -280,145 +307,147 @@ __device__ __forceinline__ void fast_step_h_body(const FastSpinor<P16>& ps, cons
 // One hopping step (direction MU, forward or backward) accumulated into acc with factor `nf`
 // (neighbour norm) and boundary sign `sg` of the rebuilt third link row.
 template <int P, int MU, bool FWD, int ISIGN>
-__device__ __forceinline__ void fast_step(const FastSpinor<P>& ps, const FastLink<P>& lk, float sg, float (&acc)[24]) {
+__device__ __forceinline__ void fast_step(const FastSpinor<P>& ps, const FastLink<P>& lk, typename FastReal<P>::T sg,
+                                          typename FastReal<P>::T (&acc)[24]) {
+  using R = typename FastReal<P>::T;
   constexpr int S = FWD ? -ISIGN : ISIGN;  // (1 + S gamma_mu)
   // half spinor h[2][3] (complex), unscaled by the neighbour norm
-  float hr[2][3], hi[2][3];
+  R hr[2][3], hi[2][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int k0 = c, k1 = 3 + c, k2 = 6 + c, k3 = 9 + c;
     if constexpr (MU == 0) {  // h0 = p0 + S i p3, h1 = p1 + S i p2
-      hr[0][c] = __fmaf_rn(ps.im(k3), (float)-S, ps.re(k0));
-      hi[0][c] = __fmaf_rn(ps.re(k3), (float)S, ps.im(k0));
-      hr[1][c] = __fmaf_rn(ps.im(k2), (float)-S, ps.re(k1));
-      hi[1][c] = __fmaf_rn(ps.re(k2), (float)S, ps.im(k1));
+      hr[0][c] = fmar(ps.im(k3), R(-S), ps.re(k0));
+      hi[0][c] = fmar(ps.re(k3), R(S), ps.im(k0));
+      hr[1][c] = fmar(ps.im(k2), R(-S), ps.re(k1));
+      hi[1][c] = fmar(ps.re(k2), R(S), ps.im(k1));
     } else if constexpr (MU == 1) {  // h0 = p0 - S p3, h1 = p1 + S p2
-      hr[0][c] = __fmaf_rn(ps.re(k3), (float)-S, ps.re(k0));
-      hi[0][c] = __fmaf_rn(ps.im(k3), (float)-S, ps.im(k0));
-      hr[1][c] = __fmaf_rn(ps.re(k2), (float)S, ps.re(k1));
-      hi[1][c] = __fmaf_rn(ps.im(k2), (float)S, ps.im(k1));
+      hr[0][c] = fmar(ps.re(k3), R(-S), ps.re(k0));
+      hi[0][c] = fmar(ps.im(k3), R(-S), ps.im(k0));
+      hr[1][c] = fmar(ps.re(k2), R(S), ps.re(k1));
+      hi[1][c] = fmar(ps.im(k2), R(S), ps.im(k1));
     } else if constexpr (MU == 2) {  // h0 = p0 + S i p2, h1 = p1 - S i p3
-      hr[0][c] = __fmaf_rn(ps.im(k2), (float)-S, ps.re(k0));
-      hi[0][c] = __fmaf_rn(ps.re(k2), (float)S, ps.im(k0));
-      hr[1][c] = __fmaf_rn(ps.im(k3), (float)S, ps.re(k1));
-      hi[1][c] = __fmaf_rn(ps.re(k3), (float)-S, ps.im(k1));
+      hr[0][c] = fmar(ps.im(k2), R(-S), ps.re(k0));
+      hi[0][c] = fmar(ps.re(k2), R(S), ps.im(k0));
+      hr[1][c] = fmar(ps.im(k3), R(S), ps.re(k1));
+      hi[1][c] = fmar(ps.re(k3), R(-S), ps.im(k1));
     } else {  // h0 = p0 + S p2, h1 = p1 + S p3
-      hr[0][c] = __fmaf_rn(ps.re(k2), (float)S, ps.re(k0));
-      hi[0][c] = __fmaf_rn(ps.im(k2), (float)S, ps.im(k0));
-      hr[1][c] = __fmaf_rn(ps.re(k3), (float)S, ps.re(k1));
-      hi[1][c] = __fmaf_rn(ps.im(k3), (float)S, ps.im(k1));
+      hr[0][c] = fmar(ps.re(k2), R(S), ps.re(k0));
+      hi[0][c] = fmar(ps.im(k2), R(S), ps.im(k0));
+      hr[1][c] = fmar(ps.re(k3), R(S), ps.re(k1));
+      hi[1][c] = fmar(ps.im(k3), R(S), ps.im(k1));
     }
   }
   // third row of the link: c = conj(u0 x u1) (the boundary sign is applied through sg)
-  float cr[3], ci[3];
+  R cr[3], ci[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int a = (k + 1) % 3, b = (k + 2) % 3;
     // u0_a u1_b - u0_b u1_a
-    float xr = lk.re(a) * lk.re(3 + b);
-    xr = __fmaf_rn(-lk.im(a), lk.im(3 + b), xr);
-    xr = __fmaf_rn(-lk.re(b), lk.re(3 + a), xr);
-    xr = __fmaf_rn(lk.im(b), lk.im(3 + a), xr);
-    float xi = lk.re(a) * lk.im(3 + b);
-    xi = __fmaf_rn(lk.im(a), lk.re(3 + b), xi);
-    xi = __fmaf_rn(-lk.re(b), lk.im(3 + a), xi);
-    xi = __fmaf_rn(-lk.im(b), lk.re(3 + a), xi);
+    R xr = lk.re(a) * lk.re(3 + b);
+    xr = fmar(-lk.im(a), lk.im(3 + b), xr);
+    xr = fmar(-lk.re(b), lk.re(3 + a), xr);
+    xr = fmar(lk.im(b), lk.im(3 + a), xr);
+    R xi = lk.re(a) * lk.im(3 + b);
+    xi = fmar(lk.im(a), lk.re(3 + b), xi);
+    xi = fmar(-lk.re(b), lk.im(3 + a), xi);
+    xi = fmar(-lk.im(b), lk.re(3 + a), xi);
     cr[k] = xr;
     ci[k] = -xi;
   }
   // o = U h (forward) or U^dag h (backward); row/column 2 carries the boundary sign
-  float orr[2][3], oi[2][3];
+  R orr[2][3], oi[2][3];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      float re, im;
+      R re, im;
       if constexpr (FWD) {
         if (i < 2) {  // rows 0,1 from storage
           re = lk.re(3 * i) * hr[s][0];
-          re = __fmaf_rn(-lk.im(3 * i), hi[s][0], re);
+          re = fmar(-lk.im(3 * i), hi[s][0], re);
           im = lk.re(3 * i) * hi[s][0];
-          im = __fmaf_rn(lk.im(3 * i), hr[s][0], im);
+          im = fmar(lk.im(3 * i), hr[s][0], im);
 #pragma unroll
           for (int j = 1; j < 3; ++j) {
-            re = __fmaf_rn(lk.re(3 * i + j), hr[s][j], re);
-            re = __fmaf_rn(-lk.im(3 * i + j), hi[s][j], re);
-            im = __fmaf_rn(lk.re(3 * i + j), hi[s][j], im);
-            im = __fmaf_rn(lk.im(3 * i + j), hr[s][j], im);
+            re = fmar(lk.re(3 * i + j), hr[s][j], re);
+            re = fmar(-lk.im(3 * i + j), hi[s][j], re);
+            im = fmar(lk.re(3 * i + j), hi[s][j], im);
+            im = fmar(lk.im(3 * i + j), hr[s][j], im);
           }
         } else {  // rebuilt row: sg * c . h
           re = cr[0] * hr[s][0];
-          re = __fmaf_rn(-ci[0], hi[s][0], re);
+          re = fmar(-ci[0], hi[s][0], re);
           im = cr[0] * hi[s][0];
-          im = __fmaf_rn(ci[0], hr[s][0], im);
+          im = fmar(ci[0], hr[s][0], im);
 #pragma unroll
           for (int j = 1; j < 3; ++j) {
-            re = __fmaf_rn(cr[j], hr[s][j], re);
-            re = __fmaf_rn(-ci[j], hi[s][j], re);
-            im = __fmaf_rn(cr[j], hi[s][j], im);
-            im = __fmaf_rn(ci[j], hr[s][j], im);
+            re = fmar(cr[j], hr[s][j], re);
+            re = fmar(-ci[j], hi[s][j], re);
+            im = fmar(cr[j], hi[s][j], im);
+            im = fmar(ci[j], hr[s][j], im);
           }
           re *= sg;
           im *= sg;
         }
       } else {  // o_i = sum_j conj(U_ji) h_j, U_2i = sg * c_i
         re = lk.re(i) * hr[s][0];
-        re = __fmaf_rn(lk.im(i), hi[s][0], re);
+        re = fmar(lk.im(i), hi[s][0], re);
         im = lk.re(i) * hi[s][0];
-        im = __fmaf_rn(-lk.im(i), hr[s][0], im);
-        re = __fmaf_rn(lk.re(3 + i), hr[s][1], re);
-        re = __fmaf_rn(lk.im(3 + i), hi[s][1], re);
-        im = __fmaf_rn(lk.re(3 + i), hi[s][1], im);
-        im = __fmaf_rn(-lk.im(3 + i), hr[s][1], im);
-        const float h2r = sg * hr[s][2], h2i = sg * hi[s][2];
-        re = __fmaf_rn(cr[i], h2r, re);
-        re = __fmaf_rn(ci[i], h2i, re);
-        im = __fmaf_rn(cr[i], h2i, im);
-        im = __fmaf_rn(-ci[i], h2r, im);
+        im = fmar(-lk.im(i), hr[s][0], im);
+        re = fmar(lk.re(3 + i), hr[s][1], re);
+        re = fmar(lk.im(3 + i), hi[s][1], re);
+        im = fmar(lk.re(3 + i), hi[s][1], im);
+        im = fmar(-lk.im(3 + i), hr[s][1], im);
+        const R h2r = sg * hr[s][2], h2i = sg * hi[s][2];
+        re = fmar(cr[i], h2r, re);
+        re = fmar(ci[i], h2i, re);
+        im = fmar(cr[i], h2i, im);
+        im = fmar(-ci[i], h2r, im);
       }
       orr[s][i] = re;
       oi[s][i] = im;
     }
   }
   // accumulate n * o into the upper spins and n * S B^dag o into the lower spins
-  const float n = ps.n;
+  const R n = ps.n;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float o0r = orr[0][c], o0i = oi[0][c], o1r = orr[1][c], o1i = oi[1][c];
-    acc[2 * c] = __fmaf_rn(n, o0r, acc[2 * c]);
-    acc[2 * c + 1] = __fmaf_rn(n, o0i, acc[2 * c + 1]);
-    acc[2 * (3 + c)] = __fmaf_rn(n, o1r, acc[2 * (3 + c)]);
-    acc[2 * (3 + c) + 1] = __fmaf_rn(n, o1i, acc[2 * (3 + c) + 1]);
-    const float ns = n * (float)S;
-    float* a2 = &acc[2 * (6 + c)];
-    float* a3 = &acc[2 * (9 + c)];
+    const R o0r = orr[0][c], o0i = oi[0][c], o1r = orr[1][c], o1i = oi[1][c];
+    acc[2 * c] = fmar(n, o0r, acc[2 * c]);
+    acc[2 * c + 1] = fmar(n, o0i, acc[2 * c + 1]);
+    acc[2 * (3 + c)] = fmar(n, o1r, acc[2 * (3 + c)]);
+    acc[2 * (3 + c) + 1] = fmar(n, o1i, acc[2 * (3 + c) + 1]);
+    const R ns = n * R(S);
+    R* a2 = &acc[2 * (6 + c)];
+    R* a3 = &acc[2 * (9 + c)];
     if constexpr (MU == 0) {  // lo2 = -S i o1, lo3 = -S i o0
-      a2[0] = __fmaf_rn(ns, o1i, a2[0]);
-      a2[1] = __fmaf_rn(-ns, o1r, a2[1]);
-      a3[0] = __fmaf_rn(ns, o0i, a3[0]);
-      a3[1] = __fmaf_rn(-ns, o0r, a3[1]);
+      a2[0] = fmar(ns, o1i, a2[0]);
+      a2[1] = fmar(-ns, o1r, a2[1]);
+      a3[0] = fmar(ns, o0i, a3[0]);
+      a3[1] = fmar(-ns, o0r, a3[1]);
     } else if constexpr (MU == 1) {  // lo2 = S o1, lo3 = -S o0
-      a2[0] = __fmaf_rn(ns, o1r, a2[0]);
-      a2[1] = __fmaf_rn(ns, o1i, a2[1]);
-      a3[0] = __fmaf_rn(-ns, o0r, a3[0]);
-      a3[1] = __fmaf_rn(-ns, o0i, a3[1]);
+      a2[0] = fmar(ns, o1r, a2[0]);
+      a2[1] = fmar(ns, o1i, a2[1]);
+      a3[0] = fmar(-ns, o0r, a3[0]);
+      a3[1] = fmar(-ns, o0i, a3[1]);
     } else if constexpr (MU == 2) {  // lo2 = -S i o0, lo3 = S i o1
-      a2[0] = __fmaf_rn(ns, o0i, a2[0]);
-      a2[1] = __fmaf_rn(-ns, o0r, a2[1]);
-      a3[0] = __fmaf_rn(-ns, o1i, a3[0]);
-      a3[1] = __fmaf_rn(ns, o1r, a3[1]);
+      a2[0] = fmar(ns, o0i, a2[0]);
+      a2[1] = fmar(-ns, o0r, a2[1]);
+      a3[0] = fmar(-ns, o1i, a3[0]);
+      a3[1] = fmar(ns, o1r, a3[1]);
     } else {  // lo2 = S o0, lo3 = S o1
-      a2[0] = __fmaf_rn(ns, o0r, a2[0]);
-      a2[1] = __fmaf_rn(ns, o0i, a2[1]);
-      a3[0] = __fmaf_rn(ns, o1r, a3[0]);
-      a3[1] = __fmaf_rn(ns, o1i, a3[1]);
+      a2[0] = fmar(ns, o0r, a2[0]);
+      a2[1] = fmar(ns, o0i, a2[1]);
+      a3[0] = fmar(ns, o1r, a3[0]);
+      a3[1] = fmar(ns, o1i, a3[1]);
     }
   }
 }
 
 // Materialises the accumulator after each step: without it the compiler defers the products of
 // earlier steps to the end of the kernel and the register count doubles.
-__device__ __forceinline__ void pin_acc(float (&acc)[24]) {
+template <typename T> __device__ __forceinline__ void pin_acc(T (&acc)[24]) {
 #pragma unroll
   for (int k = 0; k < 24; ++k) asm volatile("" : "+v"(acc[k]));
 }
@@ -477,11 +506,12 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
                        i + tp_off,                 i + tm_off};
     const bool lastf[4] = {x == 2 * Lxh - 1, y == Ly - 1, z == Lz - 1, t == Lt - 1};
     const bool lastb[4] = {x == 0, y == 0, z == 0, t == 0};
-    constexpr int SB = P == P16 ? 48 : 96;  // bytes per fermion site (chunks)
+    using R = typename FastReal<P>::T;
+    constexpr int SB = P == P16 ? 48 : (P == P32 ? 96 : 192);  // bytes per fermion site (chunks)
     const __amdgpu_buffer_rsrc_t rin = make_rsrc(h.in, (unsigned)(SB * h.in_stride));
     const __amdgpu_buffer_rsrc_t rnin = make_rsrc(h.in_nrm, (unsigned)(4 * h.in_stride));
-    constexpr int LB = P == P16 ? 24 : 48;  // bytes per 12-real link
-    float acc[24];
+    constexpr int LB = P == P16 ? 24 : (P == P32 ? 48 : 96);  // bytes per 12-real link
+    R acc[24];
 #pragma unroll
     for (int k = 0; k < 24; ++k) acc[k] = 0.f;
     // 8 steps (mu forward / backward) with a one-step lookahead of the raw loads, so that two steps
@@ -519,8 +549,8 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
     CMX_FAST_STEP(6)
     CMX_FAST_STEP(7)
 #undef CMX_FAST_STEP
-    float out[24];
-    const float b = (float)h.b;
+    R out[24];
+    const R b = (R)h.b;
     if constexpr (EPI == EPI_NONE) {
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = b * acc[k];
@@ -529,17 +559,17 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
       for (int k = 0; k < 24; ++k) acc[k] *= b;
       clover_apply_site<P, LCP>(h.clov, Vh, i, acc, out);
     } else if constexpr (EPI == EPI_XPAY) {
-      float xs[24];
+      R xs[24];
       SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
-      const float a = (float)h.a;
+      const R a = (R)h.a;
 #pragma unroll
-      for (int k = 0; k < 24; ++k) out[k] = __fmaf_rn(a, xs[k], b * acc[k]);
+      for (int k = 0; k < 24; ++k) out[k] = fmar(a, xs[k], b * acc[k]);
     } else {
-      float xs[24];
+      R xs[24];
       SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
       clover_apply_site<P, LCP>(h.clov, Vh, i, xs, out);
 #pragma unroll
-      for (int k = 0; k < 24; ++k) out[k] = __fmaf_rn(b, acc[k], out[k]);
+      for (int k = 0; k < 24; ++k) out[k] = fmar(b, acc[k], out[k]);
     }
     SpinorIO<P>::store(h.out, h.out_nrm, h.out_stride, i, out);
     if constexpr (NORM) {
@@ -565,6 +595,9 @@ __global__ void __launch_bounds__(256) k_hop_fast_b(const HopLaunch* __restrict_
 }
 
 template <int P> void launch_hop_fast_t(const HopLaunch& h, hipStream_t s, const BatchGrid* bg) {
+  constexpr size_t SB = P == P16 ? 48 : (P == P32 ? 96 : 192);
+  if (!bg && SB * std::max((size_t)h.in_stride, (size_t)h.g.Vh) >= ((size_t)1 << 31))
+    throw Error(CMX_ERR_UNSUPPORTED, "plane-grid hopping kernel: field above 2 GiB (32-bit buffer offsets)");
   const int Pl = h.g.Lxh * h.g.L[1];
   const dim3 grid(((Pl + h.block - 1) / h.block) * h.g.L[2] * h.g.L[3]);
   const bool norm = h.partials != nullptr;

@@ -520,7 +520,8 @@ __global__ void __launch_bounds__(256) k_w0_grad(PtcLaunch L) {
 }
 
 template <typename F> void dispatch_pr(int prec, int recon, F&& f) {
-  if (prec == P64) f(std::integral_constant<int, P64>(), std::integral_constant<int, 18>());
+  if (prec == P64 && recon == 12) f(std::integral_constant<int, P64>(), std::integral_constant<int, 12>());
+  else if (prec == P64) f(std::integral_constant<int, P64>(), std::integral_constant<int, 18>());
   else if (prec == P32 && recon == 18) f(std::integral_constant<int, P32>(), std::integral_constant<int, 18>());
   else if (prec == P32) f(std::integral_constant<int, P32>(), std::integral_constant<int, 12>());
   else if (prec == P16 && recon == 12) f(std::integral_constant<int, P16>(), std::integral_constant<int, 12>());

@@ -44,8 +44,11 @@ int hop_apply_config(HopLaunch& h, const HopConfig& c) {
       if (std::fabs(h.ph[mu][k][1]) > 1e-12 || std::fabs(std::fabs(h.ph[mu][k][0]) - 1.0) > 1e-6) real_phases = false;
       h.sgn[mu][k] = (float)(h.ph[mu][k][0] < 0 ? -1.0 : 1.0);
     }
-  const bool plane = (c.fast & 1) && real_phases && h.recon == 12 && (h.prec == P16 || h.prec == P32) &&
-                     h.g.Lxh * h.g.L[1] < 65536 && h.g.L[2] * h.g.L[3] < 65536;
+  // buffer offsets of the plane kernel are 32-bit: a whole field (spinor chunks or one direction's
+  // links) stays below 2 GiB
+  const size_t site_bytes = h.prec == P16 ? 48 : (h.prec == P32 ? 96 : 192);
+  const bool plane = (c.fast & 1) && real_phases && h.recon == 12 && h.g.Lxh * h.g.L[1] < 65536 &&
+                     h.g.L[2] * h.g.L[3] < 65536 && site_bytes * (size_t)h.g.Vh < ((size_t)1 << 31);
   h.fast = plane ? c.fast : (c.fast & HOP_FAST_STREAM);  // the stream bit applies to both kernels
   h.inv_lxh = 1.0f / (float)h.g.Lxh;
   if (h.fast & 1) {  // the plane-grid kernel tiles whole planes: any block size, Lt % 8 and Lz % zb only
@@ -174,6 +177,9 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
   op->fact = 4.0 + p.mass;
   op->recon[P32] = p.sloppy_prec == CMX_SINGLE ? p.sloppy_recon : 12;
   op->recon[P16] = 12;
+  // FP64 links as 12 reals (third row rebuilt in FP64 on use) when they are SU(3) up to a boundary
+  // phase to rounding, which makes the rebuilt row as exact as a stored one
+  op->recon[P64] = gd->su3_dev <= 1e-14 ? 12 : 18;
   if (gd->su3_dev > 1e-6)
     CMX_REQUIRE(p.sloppy_recon == 18 && p.sloppy_prec == CMX_SINGLE, CMX_ERR_ARG,
                 "op_create: links are not SU(3) up to a per-direction boundary phase (deviation " +
@@ -217,6 +223,7 @@ Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p) {
     op->logdet_even = op->hS[0];
     op->nneg_even = (long)std::llround(op->hS[1]);
   }
+  op->ensure_prec(P64);
   op->ensure_prec(p.sloppy_prec);
   if (p.autotune_dslash) op->autotune();
   CMX_HIP(hipStreamSynchronize(op->stream));
@@ -256,7 +263,7 @@ Op* clone_op(Op& o, int prec) {
 }
 
 void Op::ensure_prec(int prec) {
-  if (prec == P64) return;
+  if (prec == P64 && recon[P64] == 18) return;  // the gauge field's own FP64 links
   if (gauge[prec][0].p) return;
   const int rc = recon[prec];
   const size_t gb = (size_t)4 * link_chunks(prec, rc) * link_chunk_bytes(prec, rc) * Vh;
@@ -264,7 +271,7 @@ void Op::ensure_prec(int prec) {
     gauge[prec][par_].alloc(gb);
     gauge_pack_sloppy(g, prec, rc, gd->g64[par_].as<double2>(), gauge[prec][par_].p, stream);
   }
-  if (clover()) {
+  if (clover() && prec != P64) {
     clov_odd[prec].alloc(clover_bytes(prec, Vh));
     clovinv_even[prec].alloc(clover_bytes(prec, Vh));
     clover_convert((int)Vh, prec, clov_odd[P64].as<double2>(), clov_odd[prec].p, stream);
@@ -295,7 +302,7 @@ HopLaunch Op::hop_base(int prec, int which) const {
   h.g = g;
   h.prec = prec;
   h.recon = recon[prec];
-  for (int p_ = 0; p_ < 2; ++p_) h.gauge[p_] = prec == P64 ? gd->g64[p_].p : gauge[prec][p_].p;
+  for (int p_ = 0; p_ < 2; ++p_) h.gauge[p_] = (prec == P64 && recon[P64] == 18) ? gd->g64[p_].p : gauge[prec][p_].p;
   std::memcpy(h.ph, gd->ph, sizeof(h.ph));
   hop_apply_config(h, hcfg[prec][which]);
   return h;
@@ -539,7 +546,7 @@ void Op::autotune() {
       blas(B_ZERO, {&in});
       std::vector<HopConfig> cands;
       for (int block : {64, 128, 256}) {
-        if (prec != P64) {  // plane-grid kernel, linear or XCD-slab order, links streamed or not
+        {  // plane-grid kernel, linear or XCD-slab order, links streamed or not (where admitted)
           for (int fm : {1, 1 | HOP_FAST_STREAM}) {
             cands.push_back(HopConfig{block, 0, 1, 0, fm});
             for (int zb : {1, 4, 8}) cands.push_back(HopConfig{block, 1, zb, 0, fm});

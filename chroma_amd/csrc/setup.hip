@@ -104,7 +104,11 @@ __global__ void k_gauge_pack_sloppy(Geom g, const double2* src, void* dst) {
     u[2 * k] = v.x;
     u[2 * k + 1] = v.y;
   }
-  if constexpr (P == P32 && RECON == 18) {
+  if constexpr (P == P64) {  // rows 0, 1 in FP64 (12-real; the third row is rebuilt on use)
+    double2* d = reinterpret_cast<double2*>(dst) + (size_t)mu * 6 * g.Vh;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[(size_t)k * g.Vh + i] = make_double2(u[2 * k], u[2 * k + 1]);
+  } else if constexpr (P == P32 && RECON == 18) {
     float2* d = reinterpret_cast<float2*>(dst) + (size_t)mu * 9 * g.Vh;
 #pragma unroll
     for (int k = 0; k < 9; ++k) d[(size_t)k * g.Vh + i] = make_float2((float)u[2 * k], (float)u[2 * k + 1]);
@@ -441,6 +445,8 @@ void gauge_pack_sloppy(const Geom& g, int prec, int recon, const double2* src, v
     hipLaunchKernelGGL((k_gauge_pack_sloppy<P32, 12>), grid, dim3(256), 0, s, g, src, dst);
   else if (prec == P16 && recon == 12)
     hipLaunchKernelGGL((k_gauge_pack_sloppy<P16, 12>), grid, dim3(256), 0, s, g, src, dst);
+  else if (prec == P64 && recon == 12)
+    hipLaunchKernelGGL((k_gauge_pack_sloppy<P64, 12>), grid, dim3(256), 0, s, g, src, dst);
   else
     throw Error(CMX_ERR_UNSUPPORTED, "gauge: unsupported precision/reconstruct combination");
 }

@@ -288,7 +288,8 @@ class Operator:
         return out
 
     def hop_config(self, prec, target=1):
-        """Launch shape of the EO operator's kernels (target 1) or of the plain hopping term (2)."""
+        """Launch shape of the EO operator's kernels (target 1) or of the plain hopping term (2);
+        `fast` is the kernel admitted for this operator (bit 0 plane-grid, bit 1 streamed loads)."""
         cfg = (ctypes.c_int * 6)(0, 0, 0, 0, 0, int(target))
         check(lib().cmx_op_get_hop_config(self.h, int(prec), cfg))
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])

@@ -462,14 +462,31 @@ def test_hop_launch_shapes(prec, cfg):
     sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sp)
     op.set_hop_config(prec, *cfg)
-    if cfg[4] and prec != cmx.DOUBLE:
-        assert op.hop_config(prec)["fast"] == cfg[4]
+    assert op.hop_config(prec)["fast"] == cfg[4]  # SU(3) links: every precision admits the plane kernel
     psi = random_fermion(V, 17)
     for isign in (+1, -1):
         ref = ref_op(psi, isign)[V // 2:]
         assert rel(op(psi[V // 2:], isign, prec), ref) < TOL[prec]
     ref = oracle.dslash(L, U, psi, +1, 1)[V // 2:]
     assert rel(op.dslash(psi[: V // 2], +1, 1, prec), ref) < TOL[prec]
+
+
+def test_fp64_links_not_su3_keep_18_reals():
+    """FP64 links that are not SU(3) to rounding (here perturbed by 1e-9) are read as 18 reals by the
+    site-linear kernel, also when the plane-grid kernel is asked for; the operator matches the oracle."""
+    L = (8, 8, 8, 16)
+    say("fp64 non-su3 links")
+    V = int(np.prod(L))
+    U = gauge_field(L, 103)
+    U = U + 1e-9 * np.random.default_rng(5).standard_normal(U.shape)
+    mass = cmx.kappa_to_mass(0.13)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.SINGLE)
+    op.set_hop_config(cmx.DOUBLE, 128, 1, 8, 0, 3)
+    assert op.hop_config(cmx.DOUBLE)["fast"] == 2
+    psi = random_fermion(V, 19)
+    for isign in (+1, -1):
+        assert rel(op(psi[V // 2:], isign, cmx.DOUBLE), ref_op(psi, isign)[V // 2:]) < TOL[cmx.DOUBLE]
 
 
 def test_autotune_keeps_results():

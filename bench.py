@@ -270,11 +270,13 @@ def main():
     print(f"# hop FP16/12: {ms * 1e3:.1f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     extra = {}
+    r64 = op.link_recon(cmx.DOUBLE)  # 12 when the links are SU(3) to rounding
     for name, what, prec, nbytes in [
         ("hop_fp32_recon12", 0, cmx.SINGLE, hop_bytes(1, 12) * Vh),
-        ("hop_fp64_recon18", 0, cmx.DOUBLE, hop_bytes(0, 18) * Vh),
+        (f"hop_fp64_recon{r64}", 0, cmx.DOUBLE, hop_bytes(0, r64) * Vh),
         ("eo_clover_op_fp16", 1, cmx.HALF, eo_clover_bytes(2, 12) * Vh),
         ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
+        ("eo_clover_op_fp64", 1, cmx.DOUBLE, eo_clover_bytes(0, r64) * Vh),
         ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
         # 12 sources: each spinor read and written once per source, each link once
         ("hop_fp16_recon12_12src", 4, cmx.HALF, (12 * (48 + 4 + 48 + 4) + 8 * 12 * 2) * Vh),

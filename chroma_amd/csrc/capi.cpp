@@ -430,6 +430,14 @@ int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[6]) {
   });
 }
 
+int cmx_op_link_recon(cmx_op h, int prec, int* recon) {
+  return guarded([&] {
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(recon && prec >= 0 && prec <= 2, CMX_ERR_ARG, "link_recon: bad argument");
+    *recon = op.recon[prec];
+  });
+}
+
 int cmx_op_set_hop_config(cmx_op h, int prec, const int cfg[6]) {
   return guarded([&] {
     cmx::Op& op = O(h);

@@ -16,6 +16,8 @@ void clover_invert(const Geom& g, const double2* src, double2* dst, double* logd
 void clover_convert(int Vh, int prec, const double2* src, void* dst, hipStream_t s);
 void clover_unpack(int Vh, const double2* src, double* out, hipStream_t s);
 void spinor_pack(int prec, int n, const double* host_dev, void* v, float* nrm, int stride, hipStream_t s);
+// hash noise in [-1/2, 1/2) per component (the launch-shape tuner's input: timings depend on the data)
+void spinor_noise(int prec, int n, unsigned seed, void* v, float* nrm, int stride, hipStream_t s);
 void spinor_unpack(int prec, int n, const void* v, const float* nrm, int stride, double* host_dev, hipStream_t s);
 void stream_copy(long n_float4, const void* a, void* b, hipStream_t s);
 int sum_partials(int n, const double* a, double* partials, hipStream_t s);

@@ -406,6 +406,23 @@ template <int P> __global__ void k_spinor_pack(int n, const double* __restrict__
   for (int k = 0; k < 24; ++k) s[k] = (typename SpinorIO<P>::R)host[(size_t)i * 24 + k];
   SpinorIO<P>::store(v, nrm, stride, i, s);
 }
+// uniform noise in [-1/2, 1/2) from a hash of (seed, site, component): input of the launch-shape tuner
+template <int P> __global__ void k_spinor_noise(int n, unsigned seed, void* v, float* nrm, int stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typename SpinorIO<P>::R s[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) {
+    unsigned x = (unsigned)i * 24u + (unsigned)k + seed * 0x9E3779B9u;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    s[k] = (typename SpinorIO<P>::R)((float)(x >> 8) * (1.0f / 16777216.0f) - 0.5f);
+  }
+  SpinorIO<P>::store(v, nrm, stride, i, s);
+}
 template <int P> __global__ void k_spinor_unpack(int n, const void* v, const float* nrm, int stride, double* __restrict__ host) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -481,6 +498,15 @@ void spinor_pack(int prec, int n, const double* host_dev, void* v, float* nrm, i
     case P32: hipLaunchKernelGGL(k_spinor_pack<P32>, grid_for(n), dim3(256), 0, s, n, host_dev, v, nrm, stride); break;
     case P16: hipLaunchKernelGGL(k_spinor_pack<P16>, grid_for(n), dim3(256), 0, s, n, host_dev, v, nrm, stride); break;
     default: throw Error(CMX_ERR_ARG, "spinor_pack: bad precision");
+  }
+}
+
+void spinor_noise(int prec, int n, unsigned seed, void* v, float* nrm, int stride, hipStream_t s) {
+  switch (prec) {
+    case P64: hipLaunchKernelGGL(k_spinor_noise<P64>, grid_for(n), dim3(256), 0, s, n, seed, v, nrm, stride); break;
+    case P32: hipLaunchKernelGGL(k_spinor_noise<P32>, grid_for(n), dim3(256), 0, s, n, seed, v, nrm, stride); break;
+    case P16: hipLaunchKernelGGL(k_spinor_noise<P16>, grid_for(n), dim3(256), 0, s, n, seed, v, nrm, stride); break;
+    default: throw Error(CMX_ERR_ARG, "spinor_noise: bad precision");
   }
 }
 

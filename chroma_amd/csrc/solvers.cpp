@@ -599,13 +599,22 @@ void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const 
       }
       BatchPlan plan;
       plan.build(recs);
-      plan.run(s0);
+      // sustained rate, as in Op::autotune: about 4 ms untimed, then about 4 ms timed
       CMX_HIP(hipEventRecord(e0, s0));
-      for (int k = 0; k < 4; ++k) plan.run(s0);
+      plan.run(s0);
+      CMX_HIP(hipEventRecord(e1, s0));
+      CMX_HIP(hipEventSynchronize(e1));
+      float t1 = 0;
+      CMX_HIP(hipEventElapsedTime(&t1, e0, e1));
+      const int nrep = std::max(4, std::min(200, (int)(4.0f / std::max(t1, 1e-3f))));
+      for (int k = 0; k < nrep; ++k) plan.run(s0);
+      CMX_HIP(hipEventRecord(e0, s0));
+      for (int k = 0; k < nrep; ++k) plan.run(s0);
       CMX_HIP(hipEventRecord(e1, s0));
       CMX_HIP(hipEventSynchronize(e1));
       float ms = 0;
       CMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= (float)nrep;
       if (ms < best) best = ms, bestc = c;
     }
     (void)hipEventDestroy(e0);

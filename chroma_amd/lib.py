@@ -122,6 +122,7 @@ def lib():
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "cmx_op_link_recon": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
         "cmx_op_set_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
         "cmx_op_autotune": (ctypes.c_int, [_vp]),
         "cmx_gauge_generate": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int,
@@ -293,6 +294,12 @@ class Operator:
         cfg = (ctypes.c_int * 6)(0, 0, 0, 0, 0, int(target))
         check(lib().cmx_op_get_hop_config(self.h, int(prec), cfg))
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
+
+    def link_recon(self, prec):
+        """Reals stored per link at `prec` (12 or 18)."""
+        r = ctypes.c_int()
+        check(lib().cmx_op_link_recon(self.h, int(prec), ctypes.byref(r)))
+        return r.value
 
     def set_hop_config(self, prec, block=256, tiled=0, zb=1, prefetch=0, fast=0, target=0):
         """fast: 0 site-linear kernel, 1 plane-grid kernel; +2 with the read-once loads (links, clover,

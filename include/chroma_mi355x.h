@@ -263,14 +263,18 @@ int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
 
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
- * double-buffered direction loads (0/1), kernel (0 site-linear, 1 plane-grid -- FP16 and FP32 with
- * 12-real links and real boundary phases; +2 with the loads of data read once per launch -- links,
+ * double-buffered direction loads (0/1), kernel (0 site-linear, 1 plane-grid -- 12-real links, real
+ * boundary phases, fields below 2 GiB; +2 with the loads of data read once per launch -- links,
  * clover, xpay input -- streamed non-temporally), target (0 both, 1 the EO operator's fused kernels,
  * 2 the plain hopping term; for get, 2 reads the plain-hop shape and 0/1 the operator's)}.  No counterpart in the
  * reference beyond QUDA's AutotuneDslash flag (syssolver_quda_clover_params.cc:93-99). */
 int cmx_op_get_hop_config(cmx_op op, int prec, int cfg[6]);
 int cmx_op_set_hop_config(cmx_op op, int prec, const int cfg[6]);
 int cmx_op_autotune(cmx_op op);
+/* Reals stored per link at precision prec (12 or 18).  FP64 links are stored as 12 reals when every
+ * link is SU(3) times its boundary phase to 1e-14, else as 18; FP32 follows cmx_op_params.sloppy_recon;
+ * FP16 is always 12.  Like QUDA's cuda_reconstruct (syssolver_quda_clover_params.cc:71-88). */
+int cmx_op_link_recon(cmx_op op, int prec, int* recon);
 
 /* ---- measurement -------------------------------------------------------------------------------
  * Average device milliseconds per call over n back-to-back launches on the operator's stream,

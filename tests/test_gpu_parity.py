@@ -462,6 +462,7 @@ def test_hop_launch_shapes(prec, cfg):
     sp = cmx.HALF if prec == cmx.HALF else cmx.SINGLE
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=sp)
     op.set_hop_config(prec, *cfg)
+    assert op.link_recon(cmx.DOUBLE) == 12
     assert op.hop_config(prec)["fast"] == cfg[4]  # SU(3) links: every precision admits the plane kernel
     psi = random_fermion(V, 17)
     for isign in (+1, -1):
@@ -482,6 +483,7 @@ def test_fp64_links_not_su3_keep_18_reals():
     mass = cmx.kappa_to_mass(0.13)
     ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.SINGLE)
+    assert op.link_recon(cmx.DOUBLE) == 18
     op.set_hop_config(cmx.DOUBLE, 128, 1, 8, 0, 3)
     assert op.hop_config(cmx.DOUBLE)["fast"] == 2
     psi = random_fermion(V, 19)

@@ -1,6 +1,7 @@
 // Launch recording and batched replay (see batch.h).
 #include "batch.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -59,6 +60,7 @@ void BatchPlan::build(const std::vector<Recorder>& recs) {
     std::vector<unsigned char> tab;
     tab.reserve(r0.bytes.size() * np);
     std::vector<int4> map;
+    int maxnb = 0;
     for (int p = 0; p < np; ++p) {
       const LaunchRec& r = recs[p].recs[j];
       CMX_REQUIRE(r.kind == r0.kind && std::memcmp(r.key, r0.key, sizeof(r.key)) == 0 && r.bytes.size() == r0.bytes.size(),
@@ -66,8 +68,26 @@ void BatchPlan::build(const std::vector<Recorder>& recs) {
                   "batch: stage " + std::to_string(j) + " selects a different kernel for problem " + std::to_string(p) +
                       " (launch shape, precision or epilogue)");
       tab.insert(tab.end(), r.bytes.begin(), r.bytes.end());
-      if (s.kind != REC_FIN)
+      if (s.kind == REC_BLAS)
         for (int b = 0; b < r.nblocks; ++b) map.push_back(make_int4(p, b, r.nblocks, 0));
+      maxnb = std::max(maxnb, r.nblocks);
+    }
+    if (s.kind == REC_HOP) {
+      // Hopping stages interleave the problems in runs of 8 work-groups: the work-groups of every
+      // problem over the same 8 local blocks follow each other, and global work-group g lands on XCD
+      // g mod 8 = local block mod 8.  Problems on one gauge field (the sources of a propagator) then
+      // read the links and clover of a block from their XCD's L2 after the first problem brought
+      // them in, and the plane kernel's XCD-aware tiling keeps its meaning.  Runs shorter than 8
+      // are padded with idle entries (problem -1) to keep the alignment.
+      for (int j8 = 0; j8 < maxnb; j8 += 8)
+        for (int p = 0; p < np; ++p) {
+          const int nbp = recs[p].recs[j].nblocks;
+          if (j8 >= nbp) continue;
+          for (int x = 0; x < 8; ++x) {
+            const int b = j8 + x;
+            map.push_back(b < nbp ? make_int4(p, b, nbp, 0) : make_int4(-1, 0, 0, 0));
+          }
+        }
     }
     s.nblocks = s.kind == REC_FIN ? np : (int)map.size();
     s.tab.alloc(tab.size());

@@ -591,6 +591,7 @@ __global__ void __launch_bounds__(256) k_hop_fast(HopLaunch h) {
 template <int P, int ISIGN, int EPI, bool NORM, int LCP>
 __global__ void __launch_bounds__(256) k_hop_fast_b(const HopLaunch* __restrict__ tab, const int4* __restrict__ map) {
   const int4 m = map[blockIdx.x];
+  if (m.x < 0) return;  // alignment padding of the batched grid
   hop_fast_body<P, ISIGN, EPI, NORM, LCP>(tab[m.x], m.y);
 }
 

@@ -501,6 +501,7 @@ __global__ void __launch_bounds__(256) k_hop(HopLaunch h) {
 template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>
 __global__ void __launch_bounds__(256) k_hop_b(const HopLaunch* __restrict__ tab, const int4* __restrict__ map) {
   const int4 m = map[blockIdx.x];
+  if (m.x < 0) return;  // alignment padding of the batched grid
   hop_body<P, RECON, ISIGN, EPI, NORM, PF, LCP>(tab[m.x], m.y, m.z);
 }
 

@@ -518,7 +518,10 @@ std::vector<HopConfig> batch_candidates(const std::vector<Op*>& ops, int ps) {
   for (int block : {64, 128, 256}) {
     c.push_back(HopConfig{block, 0, 1, 0, 0});
     c.push_back(HopConfig{block, 0, 1, 0, HOP_FAST_STREAM});
-    if (fast_ok) c.push_back(HopConfig{block, 0, 1, 0, 1 | HOP_FAST_STREAM});
+    if (fast_ok) {
+      c.push_back(HopConfig{block, 0, 1, 0, 1});  // links kept in L2 (shared by problems on one gauge field)
+      c.push_back(HopConfig{block, 0, 1, 0, 1 | HOP_FAST_STREAM});
+    }
   }
   return c;
 }

@@ -430,6 +430,13 @@ int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[6]) {
   });
 }
 
+int cmx_op_set_multi_group(cmx_op h, int group) {
+  return guarded([&] {
+    CMX_REQUIRE(group >= 0 && group <= 4, CMX_ERR_ARG, "set_multi_group: group must be 0..4");
+    O(h).multi_group = group;
+  });
+}
+
 int cmx_op_link_recon(cmx_op h, int prec, int* recon) {
   return guarded([&] {
     cmx::Op& op = O(h);

@@ -4,8 +4,9 @@
 //   fermion  : chunk k of site i at base[k*Vh + i]; FP64 12 x double2 (1 complex each),
 //              FP32 6 x float4 (2 complex), FP16 3 x 16 B (4 complex as half2) + float norm[Vh]
 //              (value = half * norm, norm = max |component| of the site)
-//   link     : per (parity, mu) block of nchunk*Vh chunks; FP64/18 9 x double2, FP32/18 9 x float2,
-//              FP32/12 3 x float4 (rows 0,1), FP16/12 3 x 8 B (4 halves)
+//   link     : per (parity, mu) block of nchunk*Vh chunks; FP64/18 9 x double2, FP64/12 6 x double2,
+//              FP32/18 9 x float2, FP32/12 3 x float4 (rows 0,1), FP16/12 a 16 B chunk (8 halves)
+//              of every site followed by an 8 B chunk (4 halves) of every site: two loads per link
 //   clover   : per site 72 reals {diag[0][6], offd[0][15] c, diag[1][6], offd[1][15] c}, so each
 //              chiral block is 36 consecutive reals; FP64 36 x double2, FP32 18 x float4,
 //              FP16 18 x 8 B (4 halves) + float norm[Vh] (max |entry| of the site)
@@ -274,10 +275,14 @@ template <> struct LinkIO<P16, 12> {
   template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, R (&u)[18], R dre, R dim) {
     const uint2* b = reinterpret_cast<const uint2*>(base);
+    const uint4 w = ld_cp<CP>(&reinterpret_cast<const uint4*>(base)[i]);
+    H4 vv[3];
+    vv[0].u = make_uint2(w.x, w.y);
+    vv[1].u = make_uint2(w.z, w.w);
+    vv[2].u = ld_cp<CP>(&b[2 * Vh + i]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      H4 v;
-      v.u = ld_cp<CP>(&b[k * Vh + i]);
+      const H4 v = vv[k];
       float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
       u[4 * k] = f0.x;
       u[4 * k + 1] = f0.y;

@@ -119,13 +119,16 @@ __device__ __forceinline__ void fast_load_spinor(__amdgpu_buffer_rsrc_t rs, __am
 template <int P, int CP = 0>
 __device__ __forceinline__ void fast_load_link(__amdgpu_buffer_rsrc_t rl, int Vh, int j, FastLink<P>& u) {
   if constexpr (P == P16) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      H4x v;
-      v.u = bload64<CP>(rl, j * 8, k * Vh * 8);
-      u.c[2 * k] = v.h[0];
-      u.c[2 * k + 1] = v.h[1];
-    }
+    H8x w;
+    w.u = bload128<CP>(rl, j * 16, 0);
+    u.c[0] = w.h[0];
+    u.c[1] = w.h[1];
+    u.c[2] = w.h[2];
+    u.c[3] = w.h[3];
+    H4x v;
+    v.u = bload64<CP>(rl, j * 8, Vh * 16);
+    u.c[4] = v.h[0];
+    u.c[5] = v.h[1];
   } else if constexpr (P == P32) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {

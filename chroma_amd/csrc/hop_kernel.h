@@ -307,9 +307,11 @@ template <> struct RawLink<P32, 18> {
 template <> struct RawLink<P16, 12> {
   uint2 c[3];
   __device__ __forceinline__ void load(const void* base, int Vh, int j) {
-    const uint2* b = reinterpret_cast<const uint2*>(base);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) c[k] = b[(size_t)k * Vh + j];
+    // the 16 B chunk (reals 0-7) of every site, then the 8 B chunk (reals 8-11)
+    const uint4 w = reinterpret_cast<const uint4*>(base)[j];
+    c[0] = make_uint2(w.x, w.y);
+    c[1] = make_uint2(w.z, w.w);
+    c[2] = reinterpret_cast<const uint2*>(base)[(size_t)2 * Vh + j];
   }
   __device__ __forceinline__ void decode(float (&u)[18], float dre, float dim) const {
 #pragma unroll

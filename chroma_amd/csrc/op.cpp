@@ -423,7 +423,9 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
     for (int gsz = 1; gsz <= 4; ++gsz) {
       M.group = gsz;
       run();
+      run();
       CMX_HIP(hipEventRecord(e0, stream));
+      run();
       run();
       run();
       CMX_HIP(hipEventRecord(e1, stream));

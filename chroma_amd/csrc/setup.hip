@@ -118,14 +118,16 @@ __global__ void k_gauge_pack_sloppy(Geom g, const double2* src, void* dst) {
     for (int k = 0; k < 3; ++k)
       d[(size_t)k * g.Vh + i] = make_float4((float)u[4 * k], (float)u[4 * k + 1], (float)u[4 * k + 2], (float)u[4 * k + 3]);
   } else {
+    // per direction: 16 B (reals 0-7) of every site, then 8 B (reals 8-11) of every site
     uint2* d = reinterpret_cast<uint2*>(dst) + (size_t)mu * 3 * g.Vh;
+    H4 v[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      H4 v;
-      v.h[0] = __floats2half2_rn((float)u[4 * k], (float)u[4 * k + 1]);
-      v.h[1] = __floats2half2_rn((float)u[4 * k + 2], (float)u[4 * k + 3]);
-      d[(size_t)k * g.Vh + i] = v.u;
+      v[k].h[0] = __floats2half2_rn((float)u[4 * k], (float)u[4 * k + 1]);
+      v[k].h[1] = __floats2half2_rn((float)u[4 * k + 2], (float)u[4 * k + 3]);
     }
+    reinterpret_cast<uint4*>(d)[i] = make_uint4(v[0].u.x, v[0].u.y, v[1].u.x, v[1].u.y);
+    d[(size_t)2 * g.Vh + i] = v[2].u;
   }
 }
 

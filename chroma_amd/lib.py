@@ -123,6 +123,7 @@ def lib():
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
         "cmx_op_link_recon": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "cmx_op_set_multi_group": (ctypes.c_int, [_vp, ctypes.c_int]),
         "cmx_op_set_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
         "cmx_op_autotune": (ctypes.c_int, [_vp]),
         "cmx_gauge_generate": (ctypes.c_int, [_ip, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_int,
@@ -294,6 +295,10 @@ class Operator:
         cfg = (ctypes.c_int * 6)(0, 0, 0, 0, 0, int(target))
         check(lib().cmx_op_get_hop_config(self.h, int(prec), cfg))
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
+
+    def set_multi_group(self, group):
+        """Sources per lane of dslash_multi: 0 timed at first use, or 1..4."""
+        check(lib().cmx_op_set_multi_group(self.h, int(group)))
 
     def link_recon(self, prec):
         """Reals stored per link at `prec` (12 or 18)."""

@@ -275,6 +275,10 @@ int cmx_op_autotune(cmx_op op);
  * link is SU(3) times its boundary phase to 1e-14, else as 18; FP32 follows cmx_op_params.sloppy_recon;
  * FP16 is always 12.  Like QUDA's cuda_reconstruct (syssolver_quda_clover_params.cc:71-88). */
 int cmx_op_link_recon(cmx_op op, int prec, int* recon);
+/* Sources per lane of cmx_dslash_multi (the links are loaded once per lane for all of them): 0 timed
+ * at first use (default), or 1..4.  A tuning knob like cmx_op_set_hop_config; no counterpart in the
+ * reference. */
+int cmx_op_set_multi_group(cmx_op op, int group);
 
 /* ---- measurement -------------------------------------------------------------------------------
  * Average device milliseconds per call over n back-to-back launches on the operator's stream,

@@ -9,7 +9,8 @@
 //              of every site followed by an 8 B chunk (4 halves) of every site: two loads per link
 //   clover   : per site 72 reals {diag[0][6], offd[0][15] c, diag[1][6], offd[1][15] c}, so each
 //              chiral block is 36 consecutive reals; FP64 36 x double2, FP32 18 x float4,
-//              FP16 18 x 8 B (4 halves) + float norm[Vh] (max |entry| of the site)
+//              FP16 per block four 16 B chunks (reals 0-31) and one 8 B chunk (reals 32-35), the
+//              eight 16 B chunks first, + float norm[Vh] (max |entry| of the site)
 // Consecutive lanes own consecutive sites, so every chunk load of a wavefront is one contiguous
 // 64 x 16 B (or 64 x 8 B) segment.
 #pragma once

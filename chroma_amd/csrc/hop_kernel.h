@@ -169,17 +169,24 @@ template <> struct CloverBlockIO<P32> {
   }
 };
 template <> struct CloverBlockIO<P16> {
-  // 18 chunks of 4 halves, then one float norm per site
+  // per chiral block b: four 16 B chunks (reals 0-31, chunk 4b+k of every site), then one 8 B chunk
+  // (reals 32-35) after the eight 16 B chunks; then one float norm per site
   template <int CP = 0>
   __device__ __forceinline__ static void load(const void* base, long Vh, long i, int b, float (&c)[36]) {
     const uint2* q = reinterpret_cast<const uint2*>(base);
+    const uint4* q16 = reinterpret_cast<const uint4*>(base);
     const float n = ld_cp<CP>(&reinterpret_cast<const float*>(q + 18 * Vh)[i]);
-    q += (size_t)9 * b * Vh;
+    H4 v[9];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 w = ld_cp<CP>(&q16[(size_t)(4 * b + k) * Vh + i]);
+      v[2 * k].u = make_uint2(w.x, w.y);
+      v[2 * k + 1].u = make_uint2(w.z, w.w);
+    }
+    v[8].u = ld_cp<CP>(&q[(size_t)(16 + b) * Vh + i]);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      H4 v;
-      v.u = ld_cp<CP>(&q[k * Vh + i]);
-      float2 f0 = __half22float2(v.h[0]), f1 = __half22float2(v.h[1]);
+      float2 f0 = __half22float2(v[k].h[0]), f1 = __half22float2(v[k].h[1]);
       c[4 * k] = f0.x * n;
       c[4 * k + 1] = f0.y * n;
       c[4 * k + 2] = f1.x * n;

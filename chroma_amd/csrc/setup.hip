@@ -366,15 +366,23 @@ template <int P> __global__ void k_clover_convert(int Vh, const double2* src, vo
     for (int k = 0; k < 18; ++k)
       d[(size_t)k * Vh + i] = make_float4((float)r[4 * k], (float)r[4 * k + 1], (float)r[4 * k + 2], (float)r[4 * k + 3]);
   } else {
+    // per chiral block b: reals 0-31 in four 16 B chunks (chunk 4b+k of every site), reals 32-35 in
+    // one 8 B chunk after the eight 16 B chunks; then the norm
     uint2* d = reinterpret_cast<uint2*>(dst);
+    uint4* d16 = reinterpret_cast<uint4*>(dst);
     double m = 0;
     for (int k = 0; k < 72; ++k) m = fmax(m, fabs(r[k]));
     const double inv = m > 0 ? 1.0 / m : 0.0;
-    for (int k = 0; k < 18; ++k) {
-      H4 v;
-      v.h[0] = __floats2half2_rn((float)(r[4 * k] * inv), (float)(r[4 * k + 1] * inv));
-      v.h[1] = __floats2half2_rn((float)(r[4 * k + 2] * inv), (float)(r[4 * k + 3] * inv));
-      d[(size_t)k * Vh + i] = v.u;
+    for (int b = 0; b < 2; ++b) {
+      H4 v[9];
+      for (int k = 0; k < 9; ++k) {
+        const double* q = r + 36 * b + 4 * k;
+        v[k].h[0] = __floats2half2_rn((float)(q[0] * inv), (float)(q[1] * inv));
+        v[k].h[1] = __floats2half2_rn((float)(q[2] * inv), (float)(q[3] * inv));
+      }
+      for (int k = 0; k < 4; ++k)
+        d16[(size_t)(4 * b + k) * Vh + i] = make_uint4(v[2 * k].u.x, v[2 * k].u.y, v[2 * k + 1].u.x, v[2 * k + 1].u.y);
+      d[(size_t)(16 + b) * Vh + i] = v[8].u;
     }
     reinterpret_cast<float*>(d + (size_t)18 * Vh)[i] = (float)m;
   }

@@ -63,7 +63,26 @@ def build(incremental=False, jobs=None, verbose=False):
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+HOST_PROP = os.path.join(HOST, "mi355x_prop")
+
+
+def build_host():
+    """The C++ host layer (host/mi355x_solver.{h,cpp}) and its propagator driver, with g++ against the
+    library; the driver finds the library through its run path."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+           os.path.join(HOST, "mi355x_solver.cpp"), os.path.join(HOST, "mi355x_prop.cpp"),
+           "-L", HERE, "-lchroma_mi355x", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", HOST_PROP + ".tmp"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("host build failed:\n" + p.stdout + p.stderr)
+    os.replace(HOST_PROP + ".tmp", HOST_PROP)
+    return HOST_PROP
 
 
 if __name__ == "__main__":

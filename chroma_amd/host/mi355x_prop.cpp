@@ -1,0 +1,150 @@
+// Point-source propagator on the MI355X library through the C++ host layer: the path a Chroma
+// propagator task takes for an even-odd Wilson or clover action (quarkProp4_a with the qprop solver
+// object, quarkprop4_w.cc:31-117, eoprec_fermact_qprop.cc:41-80), with the propagator correlator of
+// inline_propagator_w.cc:383-395.  The gauge field is one of the library's synthetic fields.
+// Prints one JSON line: per-solve iteration counts and residuals, their sum (ncg_had) and prop_corr.
+//
+//   mi355x_prop --nrow 4 4 4 8 --cfg unit --fermact WILSON --kappa 0.1 --boundary 1 1 1 1
+//               --twist 0.2 0.4 0.0 --solver CG --rsd 1e-8 --maxit 1000
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mi355x_solver.h"
+
+using namespace cmx_chroma;
+
+namespace {
+void usage() {
+  std::printf(
+      "usage: mi355x_prop [--nrow Lx Ly Lz Lt] [--cfg unit|weak|haar] [--seed N] [--eps E] [--nstout N --rho R]\n"
+      "                   [--boundary bx by bz bt] [--twist px py pz] [--fermact WILSON|CLOVER] [--kappa K]\n"
+      "                   [--csw C] [--solver CG|BICGSTAB|RELIABLE_CG|RELIABLE_BICGSTAB] [--rsd R] [--maxit N]\n"
+      "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n");
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+  int nrow[4] = {4, 4, 4, 8}, boundary[4] = {1, 1, 1, -1}, nstout = 0, device = 0;
+  std::string cfg = "weak";
+  unsigned long long seed = 1;
+  double eps = 0.3, rho = 0.1, twist[3] = {0, 0, 0};
+  bool twisted = false;
+  FermActParams fp;
+  double kappa = 0.12, csw = 1.0;
+  SysSolverMI355XParams sp;
+  long src_site = 0;
+  for (int a = 1; a < argc; ++a) {
+    const std::string k = argv[a];
+    auto need = [&](int n) {
+      if (a + n >= argc) {
+        usage();
+        std::exit(2);
+      }
+    };
+    if (k == "--help" || k == "-h") {
+      usage();
+      return 0;
+    } else if (k == "--nrow") {
+      need(4);
+      for (int m = 0; m < 4; ++m) nrow[m] = std::atoi(argv[++a]);
+    } else if (k == "--cfg") {
+      need(1);
+      cfg = argv[++a];
+    } else if (k == "--seed") {
+      need(1);
+      seed = std::strtoull(argv[++a], nullptr, 10);
+    } else if (k == "--eps") {
+      need(1);
+      eps = std::atof(argv[++a]);
+    } else if (k == "--nstout") {
+      need(1);
+      nstout = std::atoi(argv[++a]);
+    } else if (k == "--rho") {
+      need(1);
+      rho = std::atof(argv[++a]);
+    } else if (k == "--boundary") {
+      need(4);
+      for (int m = 0; m < 4; ++m) boundary[m] = std::atoi(argv[++a]);
+    } else if (k == "--twist") {
+      need(3);
+      for (int m = 0; m < 3; ++m) twist[m] = std::atof(argv[++a]);
+      twisted = true;
+    } else if (k == "--fermact") {
+      need(1);
+      fp.FermAct = argv[++a];
+    } else if (k == "--kappa") {
+      need(1);
+      kappa = std::atof(argv[++a]);
+    } else if (k == "--csw") {
+      need(1);
+      csw = std::atof(argv[++a]);
+    } else if (k == "--solver") {
+      need(1);
+      sp.SolverType = argv[++a];
+    } else if (k == "--rsd") {
+      need(1);
+      sp.RsdTarget = std::atof(argv[++a]);
+    } else if (k == "--maxit") {
+      need(1);
+      sp.MaxIter = std::atoi(argv[++a]);
+    } else if (k == "--sloppy") {
+      need(1);
+      sp.SloppyPrecision = argv[++a];
+    } else if (k == "--delta") {
+      need(1);
+      sp.Delta = std::atof(argv[++a]);
+    } else if (k == "--true-residual") {
+      sp.TrueResidual = true;
+    } else if (k == "--src-site") {
+      need(1);
+      src_site = std::atol(argv[++a]);
+    } else if (k == "--device") {
+      need(1);
+      device = std::atoi(argv[++a]);
+    } else {
+      std::fprintf(stderr, "unknown option %s\n", k.c_str());
+      usage();
+      return 2;
+    }
+  }
+  const int kind = cfg == "unit" ? 0 : (cfg == "weak" ? 1 : (cfg == "haar" ? 2 : -1));
+  if (kind < 0) {
+    std::fprintf(stderr, "unknown --cfg %s\n", cfg.c_str());
+    return 2;
+  }
+  const long V = (long)nrow[0] * nrow[1] * nrow[2] * nrow[3];
+  try {
+    check(cmx_init(device), "cmx_init");
+    std::vector<double> links((size_t)4 * V * 18);
+    check(cmx_gauge_generate(nrow, kind, seed, eps, nstout, rho, links.data()), "cmx_gauge_generate");
+    const int tdir[3] = {0, 1, 2};
+    check(cmx_gauge_apply_bc(nrow, links.data(), boundary, twisted ? twist : nullptr, twisted ? tdir : nullptr),
+          "cmx_gauge_apply_bc");
+    GaugeField u(nrow, links);
+    fp.Mass = kappaToMass(kappa);
+    if (fp.FermAct == "CLOVER") fp.clovCoeffR = fp.clovCoeffT = csw;
+    if (sp.SloppyPrecision != "DOUBLE") fp.SloppyPrecision = sp.SloppyPrecision;
+    EvenOddPrecLinOp A(u, fp);
+    LinOpSysSolverMI355X invA(A, sp);
+    PrecFermActQprop qprop(A, invA);
+    std::vector<Complex> prop;
+    std::vector<SystemSolverResults_t> res;
+    const int ncg_had = quarkProp4(prop, qprop, V, src_site, &res);
+    const std::vector<double> corr = propCorr(nrow, prop);
+    std::printf("{\"nrow\": [%d, %d, %d, %d], \"n_count\": [", nrow[0], nrow[1], nrow[2], nrow[3]);
+    for (size_t k = 0; k < res.size(); ++k) std::printf("%s%d", k ? ", " : "", res[k].n_count);
+    std::printf("], \"resid\": [");
+    for (size_t k = 0; k < res.size(); ++k) std::printf("%s%.17g", k ? ", " : "", res[k].resid);
+    std::printf("], \"ncg_had\": %d, \"prop_corr\": [", ncg_had);
+    for (size_t t = 0; t < corr.size(); ++t) std::printf("%s%.17g", t ? ", " : "", corr[t]);
+    std::printf("]}\n");
+    cmx_finalize();
+  } catch (const SolverAbort& e) {
+    std::fprintf(stderr, "mi355x_prop: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

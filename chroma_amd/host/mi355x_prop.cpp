@@ -5,7 +5,7 @@
 // Prints one JSON line: per-solve iteration counts and residuals, their sum (ncg_had) and prop_corr.
 //
 //   mi355x_prop --nrow 4 4 4 8 --cfg unit --fermact WILSON --kappa 0.1 --boundary 1 1 1 1
-//               --twist 0.2 0.4 0.0 --solver CG --rsd 1e-8 --maxit 1000
+//               --twist 0.2 0.4 0.0 --solver CG --rsd 1e-8 --maxit 1000 [--batch]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,7 +22,8 @@ void usage() {
       "usage: mi355x_prop [--nrow Lx Ly Lz Lt] [--cfg unit|weak|haar] [--seed N] [--eps E] [--nstout N --rho R]\n"
       "                   [--boundary bx by bz bt] [--twist px py pz] [--fermact WILSON|CLOVER] [--kappa K]\n"
       "                   [--csw C] [--solver CG|BICGSTAB|RELIABLE_CG|RELIABLE_BICGSTAB] [--rsd R] [--maxit N]\n"
-      "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n");
+      "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n"
+      "                   [--batch]   (the 12 solves advanced together: one launch per kernel for all)\n");
 }
 }  // namespace
 
@@ -36,6 +37,7 @@ int main(int argc, char** argv) {
   double kappa = 0.12, csw = 1.0;
   SysSolverMI355XParams sp;
   long src_site = 0;
+  bool batched = false;
   for (int a = 1; a < argc; ++a) {
     const std::string k = argv[a];
     auto need = [&](int n) {
@@ -96,6 +98,8 @@ int main(int argc, char** argv) {
     } else if (k == "--delta") {
       need(1);
       sp.Delta = std::atof(argv[++a]);
+    } else if (k == "--batch") {
+      batched = true;
     } else if (k == "--true-residual") {
       sp.TrueResidual = true;
     } else if (k == "--src-site") {
@@ -132,7 +136,7 @@ int main(int argc, char** argv) {
     PrecFermActQprop qprop(A, invA);
     std::vector<Complex> prop;
     std::vector<SystemSolverResults_t> res;
-    const int ncg_had = quarkProp4(prop, qprop, V, src_site, &res);
+    const int ncg_had = quarkProp4(prop, qprop, V, src_site, &res, batched);
     const std::vector<double> corr = propCorr(nrow, prop);
     std::printf("{\"nrow\": [%d, %d, %d, %d], \"n_count\": [", nrow[0], nrow[1], nrow[2], nrow[3]);
     for (size_t k = 0; k < res.size(); ++k) std::printf("%s%d", k ? ", " : "", res[k].n_count);

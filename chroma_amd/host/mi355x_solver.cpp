@@ -74,21 +74,66 @@ void EvenOddPrecLinOp::piece(int which, LatticeFermion& chi, const LatticeFermio
   check(cmx_op_eo_piece(op_, which, chi.data(), psi.data(), isign), "cmx_op_eo_piece");
 }
 
+namespace {
+cmx_solver_params solver_params(const SysSolverMI355XParams& p) {
+  cmx_solver_params sp{};
+  sp.solver = solver_code(p.SolverType);
+  sp.max_iter = p.MaxIter;
+  sp.rsd_target = p.RsdTarget;
+  sp.delta = p.Delta;
+  sp.sloppy_prec = precision_code(p.SloppyPrecision);
+  sp.check_every = 0;
+  sp.verbose = p.Verbose ? 1 : 0;
+  sp.true_residual = p.TrueResidual ? 1 : 0;
+  sp.n_krylov = p.NKrylov;
+  sp.n_defl = p.NDefl;
+  sp.precond = CMX_PRECOND_NONE;
+  sp.reorth = 1;
+  sp.basis_prec = -1;
+  return sp;
+}
+}  // namespace
+
 LinOpSysSolverMI355X::LinOpSysSolverMI355X(const EvenOddPrecLinOp& A, const SysSolverMI355XParams& p)
-    : A_(A), sp_{}, silent_(p.SilentFail) {
-  sp_.solver = solver_code(p.SolverType);
-  sp_.max_iter = p.MaxIter;
-  sp_.rsd_target = p.RsdTarget;
-  sp_.delta = p.Delta;
-  sp_.sloppy_prec = precision_code(p.SloppyPrecision);
-  sp_.check_every = 0;
-  sp_.verbose = p.Verbose ? 1 : 0;
-  sp_.true_residual = p.TrueResidual ? 1 : 0;
-  sp_.n_krylov = p.NKrylov;
-  sp_.n_defl = p.NDefl;
-  sp_.precond = CMX_PRECOND_NONE;
-  sp_.reorth = 1;
-  sp_.basis_prec = -1;
+    : A_(A), sp_(solver_params(p)), silent_(p.SilentFail) {}
+
+MdagMSysSolverMI355X::MdagMSysSolverMI355X(const EvenOddPrecLinOp& A, const SysSolverMI355XParams& p)
+    : A_(A), sp_(solver_params(p)), silent_(p.SilentFail) {}
+
+SystemSolverResults_t MdagMSysSolverMI355X::operator()(LatticeFermion& psi, const LatticeFermion& chi) const {
+  const long V = A_.V(), off = 24 * A_.Vh();
+  if ((long)chi.size() != 24 * V || (long)psi.size() != 24 * V) throw SolverAbort("solver: field size");
+  cmx_solver_result r{};
+  const int st = cmx_solve_mdagm(A_.handle(), &sp_, psi.data() + off, chi.data() + off, 1, &r);
+  if (st != CMX_OK && !(silent_ && st == CMX_ERR_NOCONV)) check(st, "cmx_solve_mdagm");
+  SystemSolverResults_t res;
+  res.n_count = r.n_count;
+  res.resid = r.resid;
+  return res;
+}
+
+std::vector<SystemSolverResults_t> LinOpSysSolverMI355X::operator()(std::vector<LatticeFermion>& psi,
+                                                                    const std::vector<LatticeFermion>& chi) const {
+  const long V = A_.V(), off = 24 * A_.Vh();
+  const int n = (int)chi.size();
+  if ((int)psi.size() != n || n < 1) throw SolverAbort("batched solver: psi and chi counts differ");
+  std::vector<cmx_op> ops(n, A_.handle());
+  std::vector<double*> pp(n);
+  std::vector<const double*> cp(n);
+  for (int k = 0; k < n; ++k) {
+    if ((long)chi[k].size() != 24 * V || (long)psi[k].size() != 24 * V) throw SolverAbort("solver: field size");
+    pp[k] = psi[k].data() + off;
+    cp[k] = chi[k].data() + off;
+  }
+  std::vector<cmx_solver_result> r(n);
+  cmx_batch_params bp{/*tune*/ 0, /*graph*/ 0};
+  cmx_batch_result br{};
+  const int st = cmx_solve_batch(ops.data(), n, &sp_, &bp, pp.data(), cp.data(), 0, r.data(), &br);
+  if (st != CMX_OK && !(silent_ && st == CMX_ERR_NOCONV)) check(st, "cmx_solve_batch");
+  std::vector<SystemSolverResults_t> res(n);
+  for (int k = 0; k < n; ++k) res[k].n_count = r[k].n_count, res[k].resid = r[k].resid;
+  last_ = r.back();
+  return res;
 }
 
 SystemSolverResults_t LinOpSysSolverMI355X::operator()(LatticeFermion& psi, const LatticeFermion& chi) const {
@@ -103,17 +148,37 @@ SystemSolverResults_t LinOpSysSolverMI355X::operator()(LatticeFermion& psi, cons
   return res;
 }
 
-SystemSolverResults_t PrecFermActQprop::operator()(LatticeFermion& psi, const LatticeFermion& chi) const {
+// (i) chi_tmp_o = chi_o - D_oe A_ee^-1 chi_e
+void PrecFermActQprop::prepare(LatticeFermion& chi_tmp, const LatticeFermion& chi) const {
   const long V = A_.V(), Vh = A_.Vh();
-  // (i) chi_tmp_o = chi_o - D_oe A_ee^-1 chi_e
-  LatticeFermion tmp1(24 * V, 0.0), tmp2(24 * V, 0.0), chi_tmp(24 * V, 0.0);
+  LatticeFermion tmp1(24 * V, 0.0), tmp2(24 * V, 0.0);
   A_.evenEvenInvLinOp(tmp1, chi, CMX_PLUS);
   A_.oddEvenLinOp(tmp2, tmp1, CMX_PLUS);
+  chi_tmp.assign(24 * V, 0.0);
   for (long k = 24 * Vh; k < 24 * V; ++k) chi_tmp[k] = chi[k] - tmp2[k];
+}
+
+SystemSolverResults_t PrecFermActQprop::operator()(LatticeFermion& psi, const LatticeFermion& chi) const {
+  LatticeFermion chi_tmp;
+  prepare(chi_tmp, chi);
   SystemSolverResults_t res = invA_(psi, chi_tmp);
-  // (ii) psi_e = A_ee^-1 (chi_e - D_eo psi_o)
-  std::fill(tmp1.begin(), tmp1.end(), 0.0);
-  std::fill(tmp2.begin(), tmp2.end(), 0.0);
+  reconstruct(psi, chi, res);
+  return res;
+}
+
+std::vector<SystemSolverResults_t> PrecFermActQprop::operator()(std::vector<LatticeFermion>& psi,
+                                                                const std::vector<LatticeFermion>& chi) const {
+  std::vector<LatticeFermion> chi_tmp(chi.size());
+  for (size_t k = 0; k < chi.size(); ++k) prepare(chi_tmp[k], chi[k]);
+  std::vector<SystemSolverResults_t> res = invA_(psi, chi_tmp);
+  for (size_t k = 0; k < chi.size(); ++k) reconstruct(psi[k], chi[k], res[k]);
+  return res;
+}
+
+// (ii) psi_e = A_ee^-1 (chi_e - D_eo psi_o), then the residual of the unpreconditioned system
+void PrecFermActQprop::reconstruct(LatticeFermion& psi, const LatticeFermion& chi, SystemSolverResults_t& res) const {
+  const long V = A_.V(), Vh = A_.Vh();
+  LatticeFermion tmp1(24 * V, 0.0), tmp2(24 * V, 0.0);
   A_.evenOddLinOp(tmp1, psi, CMX_PLUS);
   for (long k = 0; k < 24 * Vh; ++k) tmp2[k] = chi[k] - tmp1[k];
   LatticeFermion pe(24 * V, 0.0);
@@ -124,28 +189,39 @@ SystemSolverResults_t PrecFermActQprop::operator()(LatticeFermion& psi, const La
   A_.unprecLinOp(r, psi, CMX_PLUS);
   for (long k = 0; k < 24 * V; ++k) r[k] -= chi[k];
   res.resid = std::sqrt(norm2(r.data(), 24 * V));
-  return res;
 }
 
 int quarkProp4(std::vector<Complex>& prop, const PrecFermActQprop& qprop, long V, long src_site,
-               std::vector<SystemSolverResults_t>* results) {
+               std::vector<SystemSolverResults_t>* results, bool batched) {
   prop.assign((size_t)V * 144, Complex(0, 0));
+  // the 12 sources (colour outer, spin inner, as quarkprop4_w.cc:70-117 loops), normalised
+  // (quarkprop4_w.cc:86-92, 107-109)
+  std::vector<LatticeFermion> chi(12, LatticeFermion(24 * V, 0.0)), psi(12, LatticeFermion(24 * V, 0.0));
+  std::vector<double> fact(12);
+  for (int color_source = 0; color_source < 3; ++color_source)
+    for (int spin_source = 0; spin_source < 4; ++spin_source) {
+      const int j = color_source * 4 + spin_source;
+      chi[j][(size_t)src_site * 24 + (spin_source * 3 + color_source) * 2] = 1.0;
+      const double nrm = std::sqrt(norm2(chi[j].data(), 24 * V));
+      fact[j] = nrm != 0 ? 1.0 / nrm : 1.0;
+      for (double& v : chi[j]) v *= fact[j];
+    }
+  std::vector<SystemSolverResults_t> res;
+  if (batched) {
+    res = qprop(psi, chi);
+  } else {
+    for (int j = 0; j < 12; ++j) res.push_back(qprop(psi[j], chi[j]));
+  }
   int ncg_had = 0;
   for (int color_source = 0; color_source < 3; ++color_source)
     for (int spin_source = 0; spin_source < 4; ++spin_source) {
-      LatticeFermion chi(24 * V, 0.0), psi(24 * V, 0.0);
-      chi[(size_t)src_site * 24 + (spin_source * 3 + color_source) * 2] = 1.0;
-      // source normalisation (quarkprop4_w.cc:86-92, 107-109)
-      const double nrm = std::sqrt(norm2(chi.data(), 24 * V));
-      const double fact = nrm != 0 ? 1.0 / nrm : 1.0;
-      for (double& v : chi) v *= fact;
-      const SystemSolverResults_t res = qprop(psi, chi);
-      ncg_had += res.n_count;
-      if (results) results->push_back(res);
+      const int j = color_source * 4 + spin_source;
+      ncg_had += res[j].n_count;
+      if (results) results->push_back(res[j]);
       for (long s = 0; s < V; ++s)
         for (int k = 0; k < 12; ++k)
           prop[((size_t)s * 12 + k) * 12 + spin_source * 3 + color_source] =
-              Complex(psi[(size_t)s * 24 + 2 * k], psi[(size_t)s * 24 + 2 * k + 1]) / fact;
+              Complex(psi[j][(size_t)s * 24 + 2 * k], psi[j][(size_t)s * 24 + 2 * k + 1]) / fact[j];
     }
   return ncg_had;
 }

@@ -8,6 +8,7 @@
 //   LinOpSysSolverMI355X: the inverter a Chroma build registers in place of QUDA_CLOVER_INVERTER
 //     (syssolver_linop_clover_quda_w.h:41-168; parameters as SysSolverQUDACloverParams,
 //     syssolver_quda_clover_params.cc:12-131, and SysSolverCGParams)
+//   MdagMSysSolverMI355X: MdagMSystemSolver<T>::operator()           lib/syssolver.h:72-84
 //   PrecFermActQprop::operator()                                     eoprec_fermact_qprop.cc:41-80
 //   quarkProp4 (QUARK_SPIN_TYPE_FULL point sources)                  quarkprop4_w.cc:31-117
 //   the propagator correlator of the propagator task                 inline_propagator_w.cc:383-395
@@ -124,6 +125,10 @@ class LinOpSysSolverMI355X {
  public:
   LinOpSysSolverMI355X(const EvenOddPrecLinOp& A, const SysSolverMI355XParams& p);
   SystemSolverResults_t operator()(LatticeFermion& psi, const LatticeFermion& chi) const;
+  // Several right-hand sides on this operator advanced together (cmx_solve_batch: one launch per
+  // kernel for all of them); each solution equals its single solve's.  CG and RELIABLE_CG.
+  std::vector<SystemSolverResults_t> operator()(std::vector<LatticeFermion>& psi,
+                                                const std::vector<LatticeFermion>& chi) const;
   // the device solver's own figures of the last solve (iterations, FP64 residual, device seconds)
   const cmx_solver_result& lastResult() const { return last_; }
 
@@ -134,14 +139,33 @@ class LinOpSysSolverMI355X {
   mutable cmx_solver_result last_{};
 };
 
+// MdagMSystemSolver<LatticeFermion> (lib/syssolver.h:72-84): solves M^dag M psi = chi on rb[1]
+// (MdagMSysSolverCG, syssolver_mdagm_cg.h:56-90, and the reliable and two-step BiCGStab forms);
+// resid = |chi - M^dag M psi|.
+class MdagMSysSolverMI355X {
+ public:
+  MdagMSysSolverMI355X(const EvenOddPrecLinOp& A, const SysSolverMI355XParams& p);
+  SystemSolverResults_t operator()(LatticeFermion& psi, const LatticeFermion& chi) const;
+
+ private:
+  const EvenOddPrecLinOp& A_;
+  cmx_solver_params sp_;
+  bool silent_;
+};
+
 // EvenOddPrecWilsonTypeFermAct::qprop's solver object (eoprec_fermact_qprop.cc:16-80): even-odd
 // source preparation, the odd solve, the even reconstruction and the unpreconditioned residual.
 class PrecFermActQprop {
  public:
   PrecFermActQprop(const EvenOddPrecLinOp& A, const LinOpSysSolverMI355X& invA) : A_(A), invA_(invA) {}
   SystemSolverResults_t operator()(LatticeFermion& psi, const LatticeFermion& chi) const;
+  // the same on several sources, with one batched odd solve
+  std::vector<SystemSolverResults_t> operator()(std::vector<LatticeFermion>& psi,
+                                                const std::vector<LatticeFermion>& chi) const;
 
  private:
+  void prepare(LatticeFermion& chi_tmp, const LatticeFermion& chi) const;
+  void reconstruct(LatticeFermion& psi, const LatticeFermion& chi, SystemSolverResults_t& res) const;
   const EvenOddPrecLinOp& A_;
   const LinOpSysSolverMI355X& invA_;
 };
@@ -149,8 +173,9 @@ class PrecFermActQprop {
 // quarkProp4_a (quarkprop4_w.cc:31-117) for point sources at site src_site: the 12 spin x colour
 // solves with source normalisation.  prop[(site*12 + sink)*12 + source] with index spin*3 + colour.
 // Returns the summed iteration count (ncg_had).
+// batched = true solves the 12 sources together (one batched odd solve).
 int quarkProp4(std::vector<Complex>& prop, const PrecFermActQprop& qprop, long V, long src_site,
-               std::vector<SystemSolverResults_t>* results = nullptr);
+               std::vector<SystemSolverResults_t>* results = nullptr, bool batched = false);
 
 // sum over each time slice of localNorm2(prop) (inline_propagator_w.cc:383-395)
 std::vector<double> propCorr(const int nrow[4], const std::vector<Complex>& prop);

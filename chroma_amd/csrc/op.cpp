@@ -1,18 +1,17 @@
 // Operator and gauge handles: device storage, precision copies and the fused even-odd operator.
 #include "op.h"
 
-#include <cstdio>
-#include <cstdlib>
-#include "hop_multi.h"
-
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
 
 #include "batch.h"
+#include "hop_multi.h"
 #include "parallel.h"
 #include "setup.h"
 

@@ -106,6 +106,7 @@ enum BlasOp {
                     //                                     (x += omega s ; r = s - omega t ; |r|^2, <r0|r>)
   B_SCALE,          // v0 = S[ia] v1 (real)
   B_AXPY_CNORM,     // v0 += S[ia] v1 (complex) ; red0 = |v0|^2
+  B_AXPBY,          // v0 = S[ia] v1 + S[ib] v0 (real)
   B_NOPS
 };
 

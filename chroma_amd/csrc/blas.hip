@@ -84,6 +84,12 @@ CMX_F(FXpay, 0, 3, 1) {
   for (int k = 0; k < 24; ++k) v0[k] = v1[k] + a * v0[k];
 }
 };
+CMX_F(FAxpby, 0, 3, 1) {
+  const R a = (R)S[ia], b = (R)S[ib];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) v0[k] = a * v1[k] + b * v0[k];
+}
+};
 CMX_F(FCgUpdate, 0, 7, 3) {
   const R a = (R)S[ia], b = (R)S[ib];
 #pragma unroll
@@ -646,6 +652,7 @@ void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s, 
     case B_AXPY_NORM: dispatch<FAxpyNorm>(b, prec, s, bg); break;
     case B_AXPY: dispatch<FAxpy>(b, prec, s, bg); break;
     case B_XPAY: dispatch<FXpay>(b, prec, s, bg); break;
+    case B_AXPBY: dispatch<FAxpby>(b, prec, s, bg); break;
     case B_CG_UPDATE: dispatch<FCgUpdate>(b, prec, s, bg); break;
     case B_AXPY2_NORM: dispatch<FAxpy2Norm>(b, prec, s, bg); break;
     case B_ADD_ZERO_COPY: dispatch<FAddZeroCopy>(b, prec, s, bg); break;

@@ -319,6 +319,16 @@ int cmx_solve_mdagm(cmx_op h, const cmx_solver_params* p, double* psi, const dou
   });
 }
 
+int cmx_solve_multishift(cmx_op h, const cmx_solver_params* p, int nshift, const double* shifts, const double* rsd,
+                         double* psi, const double* chi, cmx_solver_result* res) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(p && shifts && rsd && psi && chi && res && nshift >= 1, CMX_ERR_ARG, "solve_multishift: bad argument");
+    cmx::solve_multishift(op, *p, nshift, shifts, rsd, psi, chi, res);
+  });
+}
+
 int cmx_solve_batch(const cmx_op* hs, int n, const cmx_solver_params* p, const cmx_batch_params* bp,
                     double* const* psi, const double* const* chi, int mdagm, cmx_solver_result* res,
                     cmx_batch_result* bres) {

@@ -169,6 +169,9 @@ void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
 
 // mdagm: solve M^dag M psi = chi instead of M psi = chi
 void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
+// MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
+void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
+                      double* psi, const double* chi, cmx_solver_result* res);
 // The same for n problems at once (batch.h): problem i solves on ops[i] (distinct solver contexts,
 // one stream); solver RELIABLE_CG or CG
 void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const cmx_batch_params& bp,

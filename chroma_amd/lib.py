@@ -106,6 +106,8 @@ def lib():
                                      ctypes.POINTER(SolverResult)]),
         "cmx_solve_mdagm": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), _dp, _dp, ctypes.c_int,
                                            ctypes.POINTER(SolverResult)]),
+        "cmx_solve_multishift": (ctypes.c_int, [_vp, ctypes.POINTER(SolverParams), ctypes.c_int, _dp, _dp, _dp, _dp,
+                                                ctypes.POINTER(SolverResult)]),
         "cmx_solve_batch": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(SolverParams),
                                            ctypes.POINTER(BatchParams), ctypes.POINTER(_dp), ctypes.POINTER(_dp),
                                            ctypes.c_int, ctypes.POINTER(SolverResult), ctypes.POINTER(BatchResult)]),
@@ -343,6 +345,21 @@ class Operator:
                     solve_seconds=r.solve_seconds, total_seconds=r.total_seconds, gflops=r.gflops,
                     n_restarts=r.n_restarts) for r in res]
         return (psis[0], out[0]) if single else (psis, out)
+
+    def solve_multishift(self, chi, shifts, rsd=1e-8, maxit=1000, verbose=False):
+        """(M^dag M + shifts[s])^-1 chi for every shift with one Krylov space (MInvCG2, minvcg2.cc:72-355),
+        FP64.  rsd: one relative target or one per shift.  Returns (psi[nshift, ...], [result per shift])."""
+        chi = np.ascontiguousarray(chi, np.complex128)
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        rsd = np.ascontiguousarray(np.broadcast_to(np.asarray(rsd, np.float64), shifts.shape))
+        ns = len(shifts)
+        psis = np.zeros((ns,) + chi.shape, np.complex128)
+        p = SolverParams(CG, int(maxit), float(rsd.max()), 0.1, DOUBLE, 0, int(bool(verbose)), 0, 16, 0, 0, 4,
+                         1e-12, 1.0, 1, -1, 0.0, 0)
+        res = (SolverResult * ns)()
+        check(lib().cmx_solve_multishift(self.h, ctypes.byref(p), ns, _d(shifts), _d(rsd), _d(psis), _d(chi), res))
+        return psis, [dict(n_count=r.n_count, resid=r.resid, rel_resid=r.rel_resid, solve_seconds=r.solve_seconds,
+                           total_seconds=r.total_seconds) for r in res]
 
 
 def solve_batch(ops, chis, psis=None, solver=RELIABLE_CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=HALF,

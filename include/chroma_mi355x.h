@@ -185,6 +185,18 @@ int cmx_solve(cmx_op op, const cmx_solver_params* p, double* psi, const double* 
 int cmx_solve_mdagm(cmx_op op, const cmx_solver_params* p, double* psi, const double* chi, int nrhs,
                     cmx_solver_result* res);
 
+/* MdagMMultiSystemSolver with CG (multi_syssolver_mdagm_cg.h:58-100 -> MInvCG2, minvcg2.cc:72-355):
+ * solves (M^dag M + shifts[s]) psi_s = chi for s < nshift (1..64) with one Krylov space, in FP64 --
+ * the rational-approximation solves of RHMC.  Every psi_s starts at zero, as MInvCG2 requires (the
+ * entry values of psi are ignored); psi holds nshift consecutive subset vectors.  rsd[s] is the
+ * relative target of shift s (RsdCG per shift); shift s stops updating once its shifted residual
+ * estimate is below it, and the solve ends when all have (the smallest shift decides the first test).
+ * p: max_iter and verbose are used.  res[s]: n_count the common iteration count, resid =
+ * |chi - (M^dag M + shift_s) psi_s| recomputed in FP64.  CMX_ERR_NOCONV at MaxCG iterations
+ * (QDP_error_exit in the reference). */
+int cmx_solve_multishift(cmx_op op, const cmx_solver_params* p, int nshift, const double* shifts, const double* rsd,
+                         double* psi, const double* chi, cmx_solver_result* res);
+
 /* ---- batched solves: no single counterpart in the reference -------------------------------------
  * n independent solves advance together: every kernel of an iteration is launched once over a grid
  * that covers all n problems (work-group -> problem, work-group of that problem), so small volumes

@@ -52,6 +52,8 @@ def lib():
                                 ctypes.c_double, _ip, _dp, _dp, ctypes.c_int, _ip]
         L.orc_solve_mdagm.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                       ctypes.c_double, _ip, _dp]
+        L.orc_minvcg2.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                  ctypes.POINTER(ctypes.c_int), _dp]
         L.orc_qprop.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_double, _ip, _dp]
         L.orc_quarkprop4_point.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double,
@@ -192,6 +194,18 @@ class Op:
         _check(lib().orc_solve_mdagm(self.h, int(solver), _d(psi), _d(chi), float(rsd), int(maxit), float(delta),
                                      ctypes.byref(nc), ctypes.byref(rs)))
         return psi, nc.value, rs.value
+
+    def solve_multishift(self, chi, shifts, rsd, maxit=1000):
+        """MInvCG2 (minvcg2.cc:72-355): psi[s] = (M^dag M + shifts[s])^-1 chi; returns
+        (psi[nshift, ...], n_count, resid[nshift])."""
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        rsd = np.ascontiguousarray(np.broadcast_to(rsd, shifts.shape), np.float64)
+        psi = np.zeros((len(shifts),) + chi.shape, np.complex128)
+        nc = ctypes.c_int()
+        res = np.zeros(len(shifts))
+        _check(lib().orc_minvcg2(self.h, len(shifts), _d(shifts), _d(rsd), int(maxit), _d(psi),
+                                 _d(np.ascontiguousarray(chi, np.complex128)), ctypes.byref(nc), _d(res)))
+        return psi, nc.value, res
 
     def qprop(self, chi, solver=CG, rsd=1e-8, maxit=1000, delta=0.1):
         psi = np.zeros_like(chi)

@@ -575,6 +575,82 @@ Result invcg2(const LinOp<R>& M, const cplx<R>* chi, cplx<R>* psi, double RsdCG,
   return res;
 }
 
+// MInvCG2_a, minvcg2.cc:72-355: multishift CG for (M^dag M + shifts[s]) psi[s] = chi with one Krylov
+// space (Jegerlehner, hep-lat/9708029).  Every psi[s] starts at zero (:118-121); a zero source returns
+// at once (:133-153, fuzz 1e-10 in double); the first test uses the smallest shift (:99-104, :247);
+// shift s stops updating once c z_s^2 < RsdCG[s]^2 |chi|^2 (:326-339).  Returns the iteration count.
+template <typename R>
+int minvcg2(const LinOp<R>& M, const cplx<R>* chi, std::vector<std::vector<cplx<R>>>& psi,
+            const std::vector<double>& shifts, const std::vector<double>& RsdCG, int MaxCG) {
+  const Range sub = M.subset();
+  const long n = M.g->V * 12;
+  const int ns = (int)shifts.size();
+  int isz = 0;
+  for (int s = 1; s < ns; ++s)
+    if (shifts[s] < shifts[isz]) isz = s;
+  psi.assign(ns, std::vector<cplx<R>>(n, cplx<R>(0)));
+  const double chi_norm_sq = norm2<R>(chi, sub);
+  if (std::sqrt(chi_norm_sq) < 1e-10) return 0;
+  std::vector<double> rsd_sq(ns);
+  double cp = chi_norm_sq;
+  for (int s = 0; s < ns; ++s) rsd_sq[s] = cp * RsdCG[s] * RsdCG[s];
+  std::vector<cplx<R>> r(n), p0(n), Mp(n), MMp(n);
+  std::vector<std::vector<cplx<R>>> p(ns, std::vector<cplx<R>>(n));
+  copy<R>(r.data(), chi, sub);
+  copy<R>(p0.data(), chi, sub);
+  for (int s = 0; s < ns; ++s) copy<R>(p[s].data(), chi, sub);
+  M.apply(Mp.data(), p0.data(), +1);
+  double d = norm2<R>(Mp.data(), sub);
+  M.apply(MMp.data(), Mp.data(), -1);
+  double b = -cp / d;
+  axpby<R>(r.data(), R(b), MMp.data(), R(1), sub);  // r += b M^dag M p0
+  std::vector<double> bs(ns), z[2] = {std::vector<double>(ns), std::vector<double>(ns)};
+  int iz = 1;
+  for (int s = 0; s < ns; ++s) {
+    z[1 - iz][s] = 1.0;
+    z[iz][s] = 1.0 / (1.0 - shifts[s] * b);
+    bs[s] = b * z[iz][s];
+  }
+  for (int s = 0; s < ns; ++s) axpby<R>(psi[s].data(), R(-bs[s]), chi, R(0), sub);  // psi = -bs chi
+  double c = norm2<R>(r.data(), sub);
+  std::vector<char> conv(ns, 0);
+  bool convP = c < rsd_sq[isz];
+  int k = 1, n_count = 0;
+  for (; k <= MaxCG && !convP; ++k) {
+    const double a = c / cp;
+    axpby<R>(p0.data(), R(1), r.data(), R(a), sub);  // p0 = r + a p0
+    for (int s = 0; s < ns; ++s)
+      if (!conv[s]) {
+        const double as = a * z[iz][s] * bs[s] / (z[1 - iz][s] * b);
+        axpby<R>(p[s].data(), R(z[iz][s]), r.data(), R(as), sub);  // p_s = z_s r + as p_s
+      }
+    cp = c;
+    M.apply(Mp.data(), p0.data(), +1);
+    d = norm2<R>(Mp.data(), sub);
+    M.apply(MMp.data(), Mp.data(), -1);
+    const double bp = b;
+    b = -cp / d;
+    axpby<R>(r.data(), R(b), MMp.data(), R(1), sub);
+    c = norm2<R>(r.data(), sub);
+    iz = 1 - iz;
+    for (int s = 0; s < ns; ++s)
+      if (!conv[s]) {
+        const double z0 = z[1 - iz][s], z1 = z[iz][s];
+        z[iz][s] = z0 * z1 * bp / (b * a * (z1 - z0) + z1 * bp * (1.0 - shifts[s] * b));
+        bs[s] = b * z[iz][s] / z0;
+      }
+    for (int s = 0; s < ns; ++s)
+      if (!conv[s]) axpby<R>(psi[s].data(), R(-bs[s]), p[s].data(), R(1), sub);  // psi_s -= bs p_s
+    convP = true;
+    for (int s = 0; s < ns; ++s) {
+      if (!conv[s]) conv[s] = c * z[iz][s] * z[iz][s] < rsd_sq[s];
+      convP = convP && conv[s];
+    }
+    n_count = k;
+  }
+  return n_count;
+}
+
 // LinOpSysSolverCG::operator(), syssolver_linop_cg.h:57-96: chi_tmp = M^dag chi; CG2; resid = |chi - M psi|
 template <typename R>
 Result syssolver_cg(const LinOp<R>& A, cplx<R>* psi, const cplx<R>* chi, double RsdCG, int MaxCG, std::vector<double>* trace) {
@@ -1054,6 +1130,29 @@ int orc_solve_mdagm(void* h, int solver, double* psi, const double* chi, double 
     }
     *n_count = r.n_count;
     *resid = true_resid(*o->op, p, c, true);
+  });
+}
+
+// MdagMMultiSysSolverCG (multi_syssolver_mdagm_cg.h:58-100) -> MInvCG2: psi holds nshift subset-sized
+// fields back to back (full-lattice arrays of the operator); resid[s] = |chi - (A^dag A + shift_s) psi_s|.
+int orc_minvcg2(void* h, int nshift, const double* shifts, const double* rsd, int maxit, double* psi,
+                const double* chi, int* n_count, double* resid) {
+  return guard([&] {
+    auto* o = static_cast<OpHandle*>(h);
+    const long n = o->g->V * 12;
+    std::vector<std::vector<cd>> ps;
+    const std::vector<double> sh(shifts, shifts + nshift), rs(rsd, rsd + nshift);
+    *n_count = minvcg2<double>(*o->op, reinterpret_cast<const cd*>(chi), ps, sh, rs, maxit);
+    const Range sub = o->op->subset();
+    std::vector<cd> mp(n), mmp(n);
+    for (int s = 0; s < nshift; ++s) {
+      std::memcpy(psi + (size_t)s * 2 * n, ps[s].data(), sizeof(cd) * n);
+      o->op->apply(mp.data(), ps[s].data(), +1);
+      o->op->apply(mmp.data(), mp.data(), -1);
+      for (long q = sub.lo * 12; q < sub.hi * 12; ++q)
+        mmp[q] = reinterpret_cast<const cd*>(chi)[q] - (mmp[q] + shifts[s] * ps[s][q]);
+      resid[s] = std::sqrt(norm2<double>(mmp.data(), sub));
+    }
   });
 }
 

@@ -399,3 +399,37 @@ def test_planted_mistakes_are_caught():
         for (mu, nu), f in Fn.items():
             A -= c_bad * np.einsum("ab,vij->vaibj", GAMMA[mu] @ GAMMA[nu], f).reshape(V, 12, 12)
         assert np.max(np.abs(tri_to_dense(tri) - A)) > 1e-3
+
+
+def test_minvcg2_against_dense_solves():
+    """MInvCG2 (minvcg2.cc:72-355): every shifted solution equals the dense solution of
+    (M^dag M + shift) psi = chi; the shift-0 iteration count is CG's on M^dag M; the returned residual
+    is the shifted system's."""
+    L = (2, 2, 2, 4)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 40)
+    mass = 0.2
+    op = oracle.Op(oracle.UNPREC_WILSON, L, U, mass)
+    M = (4 + mass) * np.eye(12 * V) - 0.5 * dense_dslash(L, U, +1)
+    A = M.conj().T @ M
+    chi = random_fermion(V, 41)
+    shifts = [0.0, 0.01, 0.25, 1.5]
+    psi, nc, resid = op.solve_multishift(chi, shifts, 1e-12, maxit=2000)
+    for s, sh in enumerate(shifts):
+        exact = np.linalg.solve(A + sh * np.eye(12 * V), chi.reshape(-1)).reshape(V, 4, 3)
+        assert rel(psi[s], exact) < 1e-9, (s, rel(psi[s], exact))
+        r = chi.reshape(-1) - (A + sh * np.eye(12 * V)) @ psi[s].reshape(-1)
+        assert abs(resid[s] - np.linalg.norm(r)) < 1e-12 * np.linalg.norm(chi)
+    _, nc_cg, _ = op.solve_mdagm(chi, solver=oracle.CG, rsd=1e-12, maxit=2000)
+    assert abs(nc - nc_cg) <= 1
+    # shifts in any order, with per-shift targets: the loose shift stops early and meets only its own target
+    psi2, nc2, resid2 = op.solve_multishift(chi, [0.25, 0.0], [1e-4, 1e-12], maxit=2000)
+    assert nc2 == nc
+    assert resid2[0] <= 1e-4 * np.linalg.norm(chi) and rel(psi2[1], psi[0]) < 1e-12
+
+
+def test_minvcg2_zero_source():
+    L = (2, 2, 2, 4)
+    op = oracle.Op(oracle.UNPREC_WILSON, L, numpy_gauge(L, 42), 0.2)
+    psi, nc, _ = op.solve_multishift(np.zeros((int(np.prod(L)), 4, 3), complex), [0.0, 0.1], 1e-10)
+    assert nc == 0 and not np.any(psi)

@@ -23,7 +23,8 @@ void usage() {
       "                   [--boundary bx by bz bt] [--twist px py pz] [--fermact WILSON|CLOVER] [--kappa K]\n"
       "                   [--csw C] [--solver CG|BICGSTAB|RELIABLE_CG|RELIABLE_BICGSTAB] [--rsd R] [--maxit N]\n"
       "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n"
-      "                   [--batch]   (the 12 solves advanced together: one launch per kernel for all)\n");
+      "                   [--batch]   (the 12 solves advanced together: one launch per kernel for all)\n"
+      "                   [--multishift s0,s1,...]   (instead: (M^dag M + s) psi = chi_o for a point source)\n");
 }
 }  // namespace
 
@@ -38,6 +39,7 @@ int main(int argc, char** argv) {
   SysSolverMI355XParams sp;
   long src_site = 0;
   bool batched = false;
+  std::vector<double> shifts;  // --multishift: the RHMC solves on the odd part of the first point source
   for (int a = 1; a < argc; ++a) {
     const std::string k = argv[a];
     auto need = [&](int n) {
@@ -100,6 +102,9 @@ int main(int argc, char** argv) {
       sp.Delta = std::atof(argv[++a]);
     } else if (k == "--batch") {
       batched = true;
+    } else if (k == "--multishift") {
+      need(1);
+      for (char* t = std::strtok(argv[++a], ","); t; t = std::strtok(nullptr, ",")) shifts.push_back(std::atof(t));
     } else if (k == "--true-residual") {
       sp.TrueResidual = true;
     } else if (k == "--src-site") {
@@ -132,6 +137,24 @@ int main(int argc, char** argv) {
     if (fp.FermAct == "CLOVER") fp.clovCoeffR = fp.clovCoeffT = csw;
     if (sp.SloppyPrecision != "DOUBLE") fp.SloppyPrecision = sp.SloppyPrecision;
     EvenOddPrecLinOp A(u, fp);
+    if (!shifts.empty()) {  // MdagMMultiSysSolverCG on the odd checkerboard
+      LatticeFermion chi(24 * V, 0.0);
+      chi[(size_t)(V / 2 + src_site) * 24] = 1.0;
+      MdagMMultiSysSolverCGMI355X msolve(A, {sp.RsdTarget}, sp.MaxIter);
+      std::vector<LatticeFermion> psi;
+      const SystemSolverResults_t r = msolve(psi, shifts, chi);
+      std::printf("{\"n_count\": %d, \"resid\": [", r.n_count);
+      for (size_t s = 0; s < shifts.size(); ++s) std::printf("%s%.17g", s ? ", " : "", msolve.shiftResiduals()[s]);
+      std::printf("], \"psi_norm2\": [");
+      for (size_t s = 0; s < shifts.size(); ++s) {
+        double n2 = 0;
+        for (double v : psi[s]) n2 += v * v;
+        std::printf("%s%.17g", s ? ", " : "", n2);
+      }
+      std::printf("]}\n");
+      cmx_finalize();
+      return 0;
+    }
     LinOpSysSolverMI355X invA(A, sp);
     PrecFermActQprop qprop(A, invA);
     std::vector<Complex> prop;

@@ -1,6 +1,7 @@
 // C++ host side over the C-ABI: see mi355x_solver.h.
 #include "mi355x_solver.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace cmx_chroma {
@@ -109,6 +110,37 @@ SystemSolverResults_t MdagMSysSolverMI355X::operator()(LatticeFermion& psi, cons
   SystemSolverResults_t res;
   res.n_count = r.n_count;
   res.resid = r.resid;
+  return res;
+}
+
+SystemSolverResults_t MdagMMultiSysSolverCGMI355X::operator()(std::vector<LatticeFermion>& psi,
+                                                             const std::vector<double>& shifts,
+                                                             const LatticeFermion& chi) const {
+  const long V = A_.V(), Vh = A_.Vh(), off = 24 * Vh;
+  const int ns = (int)shifts.size();
+  if ((long)chi.size() != 24 * V) throw SolverAbort("multishift: field size");
+  std::vector<double> rsd(ns);
+  if (rsd_.size() == 1)
+    rsd.assign(ns, rsd_[0]);
+  else if ((int)rsd_.size() == ns)
+    rsd = rsd_;
+  else
+    throw SolverAbort("MdagMMultiSysSolverCG: shifts incompatible");  // :62-75
+  std::vector<double> buf((size_t)ns * 24 * Vh);
+  std::vector<cmx_solver_result> r(ns);
+  cmx_solver_params sp = solver_params(SysSolverMI355XParams{});
+  sp.max_iter = maxcg_;
+  check(cmx_solve_multishift(A_.handle(), &sp, ns, shifts.data(), rsd.data(), buf.data(), chi.data() + off, r.data()),
+        "cmx_solve_multishift");
+  psi.assign(ns, LatticeFermion(24 * V, 0.0));
+  resid_.assign(ns, 0.0);
+  for (int s = 0; s < ns; ++s) {
+    std::copy(buf.begin() + (size_t)s * 24 * Vh, buf.begin() + (size_t)(s + 1) * 24 * Vh, psi[s].begin() + off);
+    resid_[s] = r[s].resid;
+  }
+  SystemSolverResults_t res;
+  res.n_count = r[0].n_count;
+  for (int s = 0; s < ns; ++s) res.resid = std::max(res.resid, r[s].resid);
   return res;
 }
 

@@ -9,6 +9,8 @@
 //     (syssolver_linop_clover_quda_w.h:41-168; parameters as SysSolverQUDACloverParams,
 //     syssolver_quda_clover_params.cc:12-131, and SysSolverCGParams)
 //   MdagMSysSolverMI355X: MdagMSystemSolver<T>::operator()           lib/syssolver.h:72-84
+//   MdagMMultiSysSolverCGMI355X: MdagMMultiSystemSolver<T>::operator()(psi, shifts, chi), the RHMC
+//     multishift solve                                               multi_syssolver_mdagm_cg.h:58-100
 //   PrecFermActQprop::operator()                                     eoprec_fermact_qprop.cc:41-80
 //   quarkProp4 (QUARK_SPIN_TYPE_FULL point sources)                  quarkprop4_w.cc:31-117
 //   the propagator correlator of the propagator task                 inline_propagator_w.cc:383-395
@@ -151,6 +153,25 @@ class MdagMSysSolverMI355X {
   const EvenOddPrecLinOp& A_;
   cmx_solver_params sp_;
   bool silent_;
+};
+
+// MdagMMultiSysSolverCG (multi_syssolver_mdagm_cg.h:58-100 -> MInvCG2, minvcg2.cc:72-355): psi[s] =
+// (M^dag M + shifts[s])^-1 chi on rb[1] for every shift with one Krylov space, FP64.  RsdCG holds one
+// target or one per shift, as the reference's parameter does (:62-75); psi is resized to the shifts.
+class MdagMMultiSysSolverCGMI355X {
+ public:
+  MdagMMultiSysSolverCGMI355X(const EvenOddPrecLinOp& A, std::vector<double> RsdCG, int MaxCG)
+      : A_(A), rsd_(std::move(RsdCG)), maxcg_(MaxCG) {}
+  SystemSolverResults_t operator()(std::vector<LatticeFermion>& psi, const std::vector<double>& shifts,
+                                   const LatticeFermion& chi) const;
+  // |chi - (M^dag M + shift_s) psi_s| of the last solve, per shift
+  const std::vector<double>& shiftResiduals() const { return resid_; }
+
+ private:
+  const EvenOddPrecLinOp& A_;
+  std::vector<double> rsd_;
+  int maxcg_;
+  mutable std::vector<double> resid_;
 };
 
 // EvenOddPrecWilsonTypeFermAct::qprop's solver object (eoprec_fermact_qprop.cc:16-80): even-odd

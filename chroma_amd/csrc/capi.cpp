@@ -167,6 +167,33 @@ int cmx_dslash_multi(cmx_op h, int nrhs, double* chi, const double* psi, int isi
   });
 }
 
+int cmx_dslash_deriv(cmx_op h, double* ds_u, const double* chi, const double* psi, int isign, int cb) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(ds_u && chi && psi, CMX_ERR_ARG, "dslash_deriv: null array");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "dslash_deriv: isign must be +1 or -1");
+    CMX_REQUIRE(cb >= -1 && cb <= 1, CMX_ERR_ARG, "dslash_deriv: cb must be 0, 1 or -1");
+    const long Vh = op.Vh, V = op.V;
+    cmx::Vec a(cmx::P64, (int)Vh), b(cmx::P64, (int)Vh);
+    cmx::DevBuf ds((size_t)4 * 9 * 16 * Vh);
+    std::vector<double> host((size_t)4 * 18 * Vh);
+    std::memset(ds_u, 0, sizeof(double) * 4 * 18 * V);
+    for (int c = 0; c < 2; ++c) {
+      if (cb >= 0 && c != cb) continue;
+      op.pack(chi + (size_t)c * 24 * Vh, (int)Vh, a);
+      op.pack(psi + (size_t)(1 - c) * 24 * Vh, (int)Vh, b);
+      cmx::hop_deriv(op.g, c, isign, a.view(), b.view(), ds.p, op.stream);
+      CMX_HIP(hipGetLastError());
+      CMX_HIP(hipMemcpyAsync(host.data(), ds.p, sizeof(double) * host.size(), hipMemcpyDeviceToHost, op.stream));
+      op.sync();
+      for (int mu = 0; mu < 4; ++mu)
+        std::memcpy(ds_u + ((size_t)mu * V + (size_t)c * Vh) * 18, host.data() + (size_t)mu * Vh * 18,
+                    sizeof(double) * 18 * Vh);
+    }
+  });
+}
+
 int cmx_op_eo_piece(cmx_op h, int which, double* chi, const double* psi, int isign) {
   return guarded([&] {
     require_init();

@@ -169,6 +169,9 @@ void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
 
 // mdagm: solve M^dag M psi = chi instead of M psi = chi
 void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
+// WilsonDslashBase::deriv (lwldslash_base_w.h:132-227) for the sites of cb, FP64 (deriv.hip): chi on cb,
+// psi on 1 - cb, ds = 4 * Vh * 9 double2 (direction, site of cb, 3x3 row-major)
+void hop_deriv(const Geom& g, int cb, int isign, VRef chi, VRef psi, void* ds, hipStream_t s);
 // MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
 void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
                       double* psi, const double* chi, cmx_solver_result* res);

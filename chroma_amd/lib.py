@@ -99,6 +99,7 @@ def lib():
         "cmx_dslash_apply": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "cmx_op_eo_piece": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
         "cmx_dslash_multi": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "cmx_dslash_deriv": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
         "cmx_clover_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "cmx_clover_logdet": (ctypes.c_int, [_vp, _dp, _lp]),
         "cmx_clover_pack_quda": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
@@ -261,6 +262,14 @@ class Operator:
         check(lib().cmx_dslash_multi(self.h, int(psis_cb.shape[0]), _d(chi), _d(psis_cb), int(isign), int(cb),
                                      int(prec)))
         return chi
+
+    def dslash_deriv(self, chi_full, psi_full, isign=PLUS, cb=-1):
+        """WilsonDslashBase::deriv (lwldslash_base_w.h:132-227): ds_u complex[4, V, 3, 3] from full-lattice
+        chi, psi; cb 0/1 one checkerboard, -1 both."""
+        ds = np.zeros((4, self.V, 3, 3), np.complex128)
+        check(lib().cmx_dslash_deriv(self.h, _d(ds), _d(np.ascontiguousarray(chi_full, np.complex128)),
+                                     _d(np.ascontiguousarray(psi_full, np.complex128)), int(isign), int(cb)))
+        return ds
 
     def piece(self, which, psi_full, isign=PLUS):
         names = {"evenEven": 0, "evenEvenInv": 1, "evenOdd": 2, "oddEven": 3, "oddOdd": 4, "unprec": 5}

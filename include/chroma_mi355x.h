@@ -116,6 +116,14 @@ int cmx_dslash_multi(cmx_op op, int nrhs, double* chi, const double* psi, int is
  * 2 evenOdd, 3 oddEven, 4 oddOdd, 5 unprecLinOp.  Only the target checkerboard of chi is written. */
 int cmx_op_eo_piece(cmx_op op, int which, double* chi, const double* psi, int isign);
 
+/* Link derivative of the hopping term (WilsonDslashBase::deriv, lwldslash_base_w.h:132-227), FP64, the
+ * fermion-force building block of HMC: for x on checkerboard cb and every mu,
+ *   ds_u[mu](x) = traceSpin( [(1 - isign gamma_mu) psi](x+mu) outer chi(x)^dag ),
+ * zero on the other checkerboard; cb = -1 gives the sum over both (:110-124).  chi, psi: full-lattice
+ * fermions (V*24 doubles); ds_u: 4*V*18 doubles in the link layout (no link factor: the force is
+ * taproj(U ds_u) as the reference forms it). */
+int cmx_dslash_deriv(cmx_op op, double* ds_u, const double* chi, const double* psi, int isign, int cb);
+
 /* Clover term download (FP64): which 0 = A on all sites, 1 = A_ee^-1 on the even sites.
  * tri: V*72 doubles in PrimitiveClovTriang layout; sites not covered are left untouched. */
 int cmx_clover_download(cmx_op op, int which, double* tri);

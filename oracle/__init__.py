@@ -42,6 +42,7 @@ def lib():
         L.orc_op_apply_f32.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_int]
         L.orc_op_eo_piece.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.orc_dslash.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_dslash_deriv.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_dslash_f32.argtypes = [_ip, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.orc_mesfield.argtypes = [_ip, _dp, _dp]
         L.orc_make_clover.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
@@ -90,6 +91,14 @@ def set_threads(n=0):
 
 def fermion_zeros(L):
     return np.zeros((int(np.prod(L)), 4, 3), np.complex128)
+
+
+def dslash_deriv(L, chi, psi, isign, cb=-1):
+    """WilsonDslashBase::deriv (lwldslash_base_w.h:132-227): ds_u[4, V, 3, 3]; cb -1 sums both."""
+    ds = np.zeros((4, int(np.prod(L)), 3, 3), np.complex128)
+    _check(lib().orc_dslash_deriv(_dims(L), _d(ds), _d(np.ascontiguousarray(chi, np.complex128)),
+                                  _d(np.ascontiguousarray(psi, np.complex128)), int(isign), int(cb)))
+    return ds
 
 
 def dslash(L, U, psi, isign, cb, out=None):

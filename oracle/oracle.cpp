@@ -134,6 +134,32 @@ void dslash(const Geom& g, const cplx<R>* U, cplx<R>* chi, const cplx<R>* psi, i
   }
 }
 
+// WilsonDslashBase::deriv, lwldslash_base_w.h:132-227: for x on cb and every mu,
+//   ds_u[mu](x)_{ab} = sum_spin [(1 - isign gamma_mu) psi](x+mu)_{spin,a} conj(chi(x)_{spin,b})
+// (traceSpin(outerProduct(shift(temp, FORWARD, mu), chi)); isotropic weights 1), zero on rb[1-cb];
+// the simple and twisted fermion BCs leave it unchanged (SimpleFermBC::zero, simple_fermbc.h:123).
+// ds_u is [4][V][3][3].
+template <typename R>
+void dslash_deriv(const Geom& g, cplx<R>* ds, const cplx<R>* chi, const cplx<R>* psi, int isign, int cb) {
+  for (int mu = 0; mu < 4; ++mu)
+    for (long s = (1 - cb) * g.Vh; s < (2 - cb) * g.Vh; ++s)
+      for (int k = 0; k < 9; ++k) ds[((long)mu * g.V + s) * 9 + k] = 0;
+  const long lo = cb * g.Vh, hi = lo + g.Vh;
+#pragma omp parallel for schedule(static)
+  for (long s = lo; s < hi; ++s)
+    for (int mu = 0; mu < 4; ++mu) {
+      cplx<R> h[4][3];
+      spin_proj(h, psi + g.fwd(s, mu) * 12, mu, -double(isign));
+      cplx<R>* d = ds + ((long)mu * g.V + s) * 9;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          cplx<R> t = 0;
+          for (int sp = 0; sp < 4; ++sp) t += h[sp][a] * std::conj(chi[s * 12 + sp * 3 + b]);
+          d[a * 3 + b] = t;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Field strength, mesfield.cc:30-78 (f[offset] for (mu,nu) = (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)).
 // F is [6][V][3][3].  shift(f, FORWARD, mu)(x) = f(x+mu); shift(f, BACKWARD, mu)(x) = f(x-mu).
@@ -1130,6 +1156,25 @@ int orc_solve_mdagm(void* h, int solver, double* psi, const double* chi, double 
     }
     *n_count = r.n_count;
     *resid = true_resid(*o->op, p, c, true);
+  });
+}
+
+// WilsonDslashBase::deriv (lwldslash_base_w.h:132-227): cb = 0 or 1, or -1 for both checkerboards
+// (the sum, :110-124).  chi, psi full-lattice fermions; ds_u [4][V][3][3].
+int orc_dslash_deriv(const int dims[4], double* ds_u, const double* chi, const double* psi, int isign, int cb) {
+  return guard([&] {
+    Geom g(dims);
+    auto* ds = reinterpret_cast<cd*>(ds_u);
+    auto* c = reinterpret_cast<const cd*>(chi);
+    auto* p = reinterpret_cast<const cd*>(psi);
+    if (cb >= 0) {
+      dslash_deriv<double>(g, ds, c, p, isign, cb);
+    } else {
+      std::vector<cd> t((size_t)4 * g.V * 9);
+      dslash_deriv<double>(g, ds, c, p, isign, 0);
+      dslash_deriv<double>(g, t.data(), c, p, isign, 1);
+      for (size_t k = 0; k < t.size(); ++k) ds[k] += t[k];
+    }
   });
 }
 

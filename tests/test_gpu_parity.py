@@ -509,3 +509,21 @@ def test_autotune_keeps_results():
     psi = random_fermion(V, 18)
     for prec in (cmx.DOUBLE, cmx.HALF):
         assert rel(op(psi[V // 2:], +1, prec), ref(psi)[V // 2:]) < TOL[prec]
+
+
+@pytest.mark.parametrize("isign", [+1, -1])
+@pytest.mark.parametrize("cb", [0, 1, -1])
+def test_dslash_deriv(isign, cb):
+    """WilsonDslashBase::deriv (lwldslash_base_w.h:132-227) on the device against the oracle (FP64)."""
+    L = (4, 4, 6, 8)
+    say("dslash deriv", isign, cb)
+    V = int(np.prod(L))
+    U = gauge_field(L, 70)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1)
+    chi = random_fermion(V, 71)
+    psi = random_fermion(V, 72)
+    got = op.dslash_deriv(chi, psi, isign, cb)
+    ref = oracle.dslash_deriv(L, chi, psi, isign, cb)
+    assert rel(got, ref) < 1e-13
+    if cb >= 0:
+        assert not np.any(got[:, (1 - cb) * (V // 2):(2 - cb) * (V // 2)])

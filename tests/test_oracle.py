@@ -433,3 +433,30 @@ def test_minvcg2_zero_source():
     op = oracle.Op(oracle.UNPREC_WILSON, L, numpy_gauge(L, 42), 0.2)
     psi, nc, _ = op.solve_multishift(np.zeros((int(np.prod(L)), 4, 3), complex), [0.0, 0.1], 1e-10)
     assert nc == 0 and not np.any(psi)
+
+
+@pytest.mark.parametrize("isign", [+1, -1])
+@pytest.mark.parametrize("cb", [0, 1])
+def test_dslash_deriv_is_the_link_derivative(isign, cb):
+    """WilsonDslashBase::deriv (lwldslash_base_w.h:132-227): with chi supported on cb, the forward hops
+    into cb are the only terms of <chi, D psi> that contain the links U_mu(x), x on cb, and they are
+    linear in them, so sum_{x in cb, mu} tr(U_mu(x) ds_mu(x)) = <chi, D(U) psi> - <chi, D(U') psi>
+    with U' = U with those links set to zero; ds is zero off cb."""
+    L = (4, 4, 4, 8)
+    V = int(np.prod(L))
+    Vh = V // 2
+    U = numpy_gauge(L, 50)
+    chi = random_fermion(V, 51)
+    chi[(1 - cb) * Vh:(2 - cb) * Vh] = 0
+    psi = random_fermion(V, 52)
+    ds = oracle.dslash_deriv(L, chi, psi, isign, cb)
+    assert not np.any(ds[:, (1 - cb) * Vh:(2 - cb) * Vh])
+    U0 = U.copy()
+    U0[:, cb * Vh:(cb + 1) * Vh] = 0
+    full = np.vdot(chi, oracle.dslash(L, U, psi, isign, cb))
+    rest = np.vdot(chi, oracle.dslash(L, U0, psi, isign, cb))
+    lin = np.einsum("mvab,mvba->", U, ds)
+    assert abs(lin - (full - rest)) < 1e-12 * abs(full)
+    # both checkerboards at once is the sum (lwldslash_base_w.h:110-124)
+    both = oracle.dslash_deriv(L, chi, psi, isign, -1)
+    assert np.allclose(both, ds + oracle.dslash_deriv(L, chi, psi, isign, 1 - cb), atol=0, rtol=0)

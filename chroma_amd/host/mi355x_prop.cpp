@@ -24,7 +24,8 @@ void usage() {
       "                   [--csw C] [--solver CG|BICGSTAB|RELIABLE_CG|RELIABLE_BICGSTAB] [--rsd R] [--maxit N]\n"
       "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n"
       "                   [--batch]   (the 12 solves advanced together: one launch per kernel for all)\n"
-      "                   [--multishift s0,s1,...]   (instead: (M^dag M + s) psi = chi_o for a point source)\n");
+      "                   [--multishift s0,s1,...]   (instead: (M^dag M + s) psi = chi_o for a point source)\n"
+      "                   [--chrono N]   (instead: 5 M^dag M solves on chi_k = e0 + 0.1 k e1, MRE predictor of N)\n");
 }
 }  // namespace
 
@@ -40,6 +41,7 @@ int main(int argc, char** argv) {
   long src_site = 0;
   bool batched = false;
   std::vector<double> shifts;  // --multishift: the RHMC solves on the odd part of the first point source
+  int chrono = 0;              // --chrono N: a sequence of M^dag M solves with the MRE predictor
   for (int a = 1; a < argc; ++a) {
     const std::string k = argv[a];
     auto need = [&](int n) {
@@ -102,6 +104,9 @@ int main(int argc, char** argv) {
       sp.Delta = std::atof(argv[++a]);
     } else if (k == "--batch") {
       batched = true;
+    } else if (k == "--chrono") {
+      need(1);
+      chrono = std::atoi(argv[++a]);
     } else if (k == "--multishift") {
       need(1);
       for (char* t = std::strtok(argv[++a], ","); t; t = std::strtok(nullptr, ",")) shifts.push_back(std::atof(t));
@@ -137,6 +142,35 @@ int main(int argc, char** argv) {
     if (fp.FermAct == "CLOVER") fp.clovCoeffR = fp.clovCoeffT = csw;
     if (sp.SloppyPrecision != "DOUBLE") fp.SloppyPrecision = sp.SloppyPrecision;
     EvenOddPrecLinOp A(u, fp);
+    if (chrono > 0) {  // HMC-like sequence of M^dag M solves with and without the chronological predictor
+      MdagMSysSolverMI355X solve(A, sp);
+      MREChronoPredictor pred(chrono);
+      std::vector<int> nc_chrono, nc_zero;
+      std::vector<double> guess_rel;
+      for (int k = 0; k < 5; ++k) {
+        LatticeFermion chi(24 * V, 0.0), psi(24 * V, 0.0), guess, mg(24 * V, 0.0), mmg(24 * V, 0.0);
+        chi[(size_t)(V / 2) * 24] = 1.0;
+        chi[(size_t)(V / 2 + 1) * 24] = 0.1 * k;
+        pred(guess, A, chi);
+        A(mg, guess, CMX_PLUS);
+        A(mmg, mg, CMX_MINUS);
+        double rr = 0, cc = 0;
+        for (long q = 24 * (V / 2); q < 24 * V; ++q) rr += (chi[q] - mmg[q]) * (chi[q] - mmg[q]), cc += chi[q] * chi[q];
+        guess_rel.push_back(std::sqrt(rr / cc));
+        nc_chrono.push_back(solve(psi, chi, pred).n_count);
+        LatticeFermion psi0(24 * V, 0.0);
+        nc_zero.push_back(solve(psi0, chi).n_count);
+      }
+      std::printf("{\"n_count_chrono\": [");
+      for (size_t q = 0; q < nc_chrono.size(); ++q) std::printf("%s%d", q ? ", " : "", nc_chrono[q]);
+      std::printf("], \"n_count_zero\": [");
+      for (size_t q = 0; q < nc_zero.size(); ++q) std::printf("%s%d", q ? ", " : "", nc_zero[q]);
+      std::printf("], \"guess_rel_resid\": [");
+      for (size_t q = 0; q < guess_rel.size(); ++q) std::printf("%s%.17g", q ? ", " : "", guess_rel[q]);
+      std::printf("]}\n");
+      cmx_finalize();
+      return 0;
+    }
     if (!shifts.empty()) {  // MdagMMultiSysSolverCG on the odd checkerboard
       LatticeFermion chi(24 * V, 0.0);
       chi[(size_t)(V / 2 + src_site) * 24] = 1.0;

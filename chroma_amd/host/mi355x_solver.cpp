@@ -144,6 +144,92 @@ SystemSolverResults_t MdagMMultiSysSolverCGMI355X::operator()(std::vector<Lattic
   return res;
 }
 
+namespace {
+// <a, b> on rb[1] (innerProduct over the odd subset)
+Complex odd_dot(const LatticeFermion& a, const LatticeFermion& b, long Vh) {
+  double re = 0, im = 0;
+  for (long k = 24 * Vh; k < 48 * Vh; k += 2) {
+    re += a[k] * b[k] + a[k + 1] * b[k + 1];
+    im += a[k] * b[k + 1] - a[k + 1] * b[k];
+  }
+  return Complex(re, im);
+}
+// x_s += c y_s on rb[1]
+void odd_axpy(LatticeFermion& x, Complex c, const LatticeFermion& y, long Vh) {
+  for (long k = 24 * Vh; k < 48 * Vh; k += 2) {
+    x[k] += c.real() * y[k] - c.imag() * y[k + 1];
+    x[k + 1] += c.real() * y[k + 1] + c.imag() * y[k];
+  }
+}
+// LUSolve (lu_solve.cc): a = G^-1 b, Gaussian elimination with partial pivoting
+std::vector<Complex> lu_solve(std::vector<Complex> G, std::vector<Complex> b, int n) {
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(G[r * n + c]) > std::abs(G[piv * n + c])) piv = r;
+    if (std::abs(G[piv * n + c]) == 0) throw SolverAbort("MRE predictor: singular G");
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(G[c * n + k], G[piv * n + k]);
+      std::swap(b[c], b[piv]);
+    }
+    for (int r = c + 1; r < n; ++r) {
+      const Complex f = G[r * n + c] / G[c * n + c];
+      for (int k = c; k < n; ++k) G[r * n + k] -= f * G[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  std::vector<Complex> a(n);
+  for (int r = n - 1; r >= 0; --r) {
+    Complex s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= G[r * n + k] * a[k];
+    a[r] = s / G[r * n + r];
+  }
+  return a;
+}
+}  // namespace
+
+void MREChronoPredictor::newVector(const LatticeFermion& psi) {
+  buf_.push_front(psi);
+  while ((int)buf_.size() > max_) buf_.pop_back();
+}
+
+void MREChronoPredictor::operator()(LatticeFermion& psi, const EvenOddPrecLinOp& M, const LatticeFermion& chi) const {
+  const long V = M.V(), Vh = M.Vh();
+  psi.assign(24 * V, 0.0);
+  const int n = (int)buf_.size();
+  if (n == 0) return;
+  // orthonormal basis of the buffer (GramSchm, then normalise)
+  std::vector<LatticeFermion> v;
+  for (int i = 0; i < n; ++i) {
+    LatticeFermion t(24 * V, 0.0);
+    std::copy(buf_[i].begin() + 24 * Vh, buf_[i].end(), t.begin() + 24 * Vh);
+    for (int j = 0; j < i; ++j) odd_axpy(t, -odd_dot(v[j], t, Vh), v[j], Vh);
+    const double nrm = std::sqrt(odd_dot(t, t, Vh).real());
+    if (nrm == 0) throw SolverAbort("MRE predictor: linearly dependent buffer");
+    for (long k = 24 * Vh; k < 48 * Vh; ++k) t[k] /= nrm;
+    v.push_back(std::move(t));
+  }
+  // G_nm = <v_n, M^dag M v_m>, b_n = <v_n, chi>
+  std::vector<Complex> G((size_t)n * n), b(n);
+  LatticeFermion mv(24 * V, 0.0), mmv(24 * V, 0.0);
+  for (int m = 0; m < n; ++m) {
+    M(mv, v[m], CMX_PLUS);
+    M(mmv, mv, CMX_MINUS);
+    for (int r = 0; r < n; ++r) G[(size_t)r * n + m] = odd_dot(v[r], mmv, Vh);
+  }
+  for (int r = 0; r < n; ++r) b[r] = odd_dot(v[r], chi, Vh);
+  const std::vector<Complex> a = lu_solve(G, b, n);
+  for (int r = 0; r < n; ++r) odd_axpy(psi, a[r], v[r], Vh);
+}
+
+SystemSolverResults_t MdagMSysSolverMI355X::operator()(LatticeFermion& psi, const LatticeFermion& chi,
+                                                       MREChronoPredictor& chrono) const {
+  chrono(psi, A_, chi);
+  const SystemSolverResults_t res = (*this)(psi, chi);
+  chrono.newVector(psi);
+  return res;
+}
+
 std::vector<SystemSolverResults_t> LinOpSysSolverMI355X::operator()(std::vector<LatticeFermion>& psi,
                                                                     const std::vector<LatticeFermion>& chi) const {
   const long V = A_.V(), off = 24 * A_.Vh();

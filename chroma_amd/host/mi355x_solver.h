@@ -11,6 +11,8 @@
 //   MdagMSysSolverMI355X: MdagMSystemSolver<T>::operator()           lib/syssolver.h:72-84
 //   MdagMMultiSysSolverCGMI355X: MdagMMultiSystemSolver<T>::operator()(psi, shifts, chi), the RHMC
 //     multishift solve                                               multi_syssolver_mdagm_cg.h:58-100
+//   MREChronoPredictor: MinimalResidualExtrapolation4DChronoPredictor, the HMC initial guess of the
+//     M^dag M solves                                                 mre_extrap_predictor.h:104-230
 //   PrecFermActQprop::operator()                                     eoprec_fermact_qprop.cc:41-80
 //   quarkProp4 (QUARK_SPIN_TYPE_FULL point sources)                  quarkprop4_w.cc:31-117
 //   the propagator correlator of the propagator task                 inline_propagator_w.cc:383-395
@@ -21,6 +23,7 @@
 #pragma once
 
 #include <complex>
+#include <deque>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -144,15 +147,36 @@ class LinOpSysSolverMI355X {
 // MdagMSystemSolver<LatticeFermion> (lib/syssolver.h:72-84): solves M^dag M psi = chi on rb[1]
 // (MdagMSysSolverCG, syssolver_mdagm_cg.h:56-90, and the reliable and two-step BiCGStab forms);
 // resid = |chi - M^dag M psi|.
+class MREChronoPredictor;
 class MdagMSysSolverMI355X {
  public:
   MdagMSysSolverMI355X(const EvenOddPrecLinOp& A, const SysSolverMI355XParams& p);
   SystemSolverResults_t operator()(LatticeFermion& psi, const LatticeFermion& chi) const;
+  // with a chronological predictor (the MdagM solvers' predictor argument, syssolver_mdagm.h:24-36):
+  // psi starts from the predictor's guess and the solution is added to its buffer
+  SystemSolverResults_t operator()(LatticeFermion& psi, const LatticeFermion& chi, MREChronoPredictor& chrono) const;
 
  private:
   const EvenOddPrecLinOp& A_;
   cmx_solver_params sp_;
   bool silent_;
+};
+
+// MinimalResidualExtrapolation4DChronoPredictor (mre_extrap_predictor.h:104-230): keeps the last
+// MaxChrono solutions; the guess for A psi = chi (A = M^dag M on rb[1]) is sum_n a_n v_n, with v the
+// Gram-Schmidt-orthonormalised buffer (newest first) and G a = b, G_nm = <v_n, A v_m>, b_n = <v_n, chi>,
+// solved by LU.  An empty buffer predicts zero.
+class MREChronoPredictor {
+ public:
+  explicit MREChronoPredictor(int MaxChrono) : max_(MaxChrono) {}
+  void operator()(LatticeFermion& psi, const EvenOddPrecLinOp& M, const LatticeFermion& chi) const;
+  void newVector(const LatticeFermion& psi);
+  void reset() { buf_.clear(); }
+  int size() const { return (int)buf_.size(); }
+
+ private:
+  int max_;
+  std::deque<LatticeFermion> buf_;  // newest first (CircularBuffer order)
 };
 
 // MdagMMultiSysSolverCG (multi_syssolver_mdagm_cg.h:58-100 -> MInvCG2, minvcg2.cc:72-355): psi[s] =

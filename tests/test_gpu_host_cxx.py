@@ -98,3 +98,19 @@ def test_cxx_host_multishift_matches_python():
         assert out["resid"][s] == res[s]["resid"]
         assert abs(out["psi_norm2"][s] - np.vdot(psi[s], psi[s]).real) <= 1e-12 * out["psi_norm2"][s]
         assert res[s]["rel_resid"] < 1e-9
+
+
+def test_cxx_host_mre_chrono_predictor():
+    """MinimalResidualExtrapolation4DChronoPredictor (mre_extrap_predictor.h:104-230) in the C++ host layer:
+    on sources chi_k = e0 + 0.1 k e1 every chi_k (k >= 2) lies in the span of the earlier sources, so the
+    predicted guess is already the solution (to the earlier solves' accuracy) and the M^dag M CG that
+    starts from it needs almost no iterations, against the full count from zero."""
+    out = run_driver(["--nrow", "8", "8", "8", "16", "--cfg", "weak", "--seed", "6", "--fermact", "CLOVER",
+                      "--kappa", "0.12", "--csw", "1.0", "--solver", "CG", "--rsd", "1e-11", "--maxit", "5000",
+                      "--sloppy", "DOUBLE", "--chrono", "4"])
+    say("  chrono", out)
+    assert out["guess_rel_resid"][0] == 1.0                 # empty buffer: zero guess
+    assert out["n_count_chrono"][0] == out["n_count_zero"][0]
+    for k in range(2, 5):
+        assert out["guess_rel_resid"][k] < 1e-8
+        assert out["n_count_chrono"][k] <= out["n_count_zero"][k] // 3

@@ -216,14 +216,21 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   kd.get(K_WN, 1);
   const double norm_rhs = std::sqrt(op.hS[K_WN]);
   const double target = norm_rhs * p.rsd_target;
-  auto true_residual = [&]() {
-    op.apply(P64, tmp.view(), psi.view(), +1, nullptr);
+  auto true_residual = [&](bool psi_zero = false) {
+    if (psi_zero)  // A 0 = 0 exactly: the same r without the FP64 operator
+      op.blas(B_ZERO, {&tmp});
+    else
+      op.apply(P64, tmp.view(), psi.view(), +1, nullptr);
     op.blas(B_XMY_NORM, {&r, &chi, &tmp});
     op.finalize(nb, 1, PG_STORE, K_WN);
     kd.get(K_WN, 1);
     return std::sqrt(op.hS[K_WN]);
   };
-  double r_norm = true_residual();
+  // a zero initial guess (the usual case) needs no operator application for the first residual
+  op.blas(B_NORM2, {&psi});
+  op.finalize(nb, 1, PG_STORE, K_WN);
+  kd.get(K_WN, 1);
+  double r_norm = true_residual(op.hS[K_WN] == 0.0);
 
   Mat H(total + 1, total), R(total + 1, total);
   std::vector<cd> g(total + 1), c(total + 1), eta(total);

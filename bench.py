@@ -163,6 +163,62 @@ def batched_sources(cmx, L):
     return out
 
 
+def train_net(cmx, mass, csw, a):
+    """The learned preconditioner's network, trained on separate fields of the benchmark recipe."""
+    net = cmx.Net()
+    net.init_neumann(4.0 + mass)
+    TL = tuple(int(v) for v in a.train_lattice.split(","))
+    train = []
+    for k in range(a.train_fields):
+        U = cmx.gauge_apply_bc(TL, cmx.gauge_generate(TL, kind=2, seed=10 + k, nstout=6, rho=0.1), (1, 1, 1, -1))
+        top = cmx.Operator(cmx.Gauge(TL, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF)
+        cmx.net_attach(top, net)
+        train.append(cmx.net_train(top, n_sources=8, snapshot_every=1, pairs_per_source=4, normalize=False,
+                                   epochs=4, lr=1e-3, seed=100 + k, prec=cmx.HALF))
+    return net, train
+
+
+def north_star_sources(cmx, L, kappa, csw, a):
+    """The 12 spin-colour point sources of a propagator on a near-critical clover operator (Haar + 6 stout
+    steps, seed 1), all to the FP64 true residual 1e-10: plain mixed-precision CG one after another and
+    batched, against the learned FGMRES-DR one after another."""
+    import numpy as np
+    mass = cmx.kappa_to_mass(kappa)
+    t0 = time.time()
+    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=1, nstout=6, rho=0.1), (1, 1, 1, -1))
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF, autotune=True)
+    del U
+    Vh = int(np.prod(L)) // 2
+    chis = []
+    for s in range(4):
+        for c in range(3):
+            chi = np.zeros((Vh, 4, 3), complex)
+            chi[0, s, c] = 1.0
+            chis.append(chi)
+    kwp = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=cmx.HALF, true_residual=True)
+    op.solve(chis[0], **kwp)
+    plain = [op.solve(c, **kwp)[1] for c in chis]
+    _, rb, summ = cmx.solve_batch([op] * 12, chis, tune=True, **kwp)
+    net, _ = train_net(cmx, mass, csw, a)
+    cmx.net_attach(op, net)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, basis_prec=cmx.HALF, n_krylov=16,
+              cycle_reduction=3e-3, precond=cmx.PRECOND_LEARNED)
+    op.solve(chis[0], **kw)
+    learned = [op.solve(c, **kw)[1] for c in chis]
+    ps, ls = sum(r["solve_seconds"] for r in plain), sum(r["solve_seconds"] for r in learned)
+    out = {"lattice": list(L), "kappa": kappa, "sources": 12,
+           "plain_mixed_cg_sequential": {"seconds": round(ps, 4), "iterations": [r["n_count"] for r in plain],
+                                         "worst_true_rel_resid": max(r["rel_resid"] for r in plain)},
+           "plain_mixed_cg_batched": {"seconds": round(summ["solve_seconds"], 4),
+                                      "worst_true_rel_resid": max(r["rel_resid"] for r in rb)},
+           "learned_fgmres_sequential": {"seconds": round(ls, 4), "iterations": [r["n_count"] for r in learned],
+                                         "worst_true_rel_resid": max(r["rel_resid"] for r in learned)},
+           "speedup_vs_plain": round(ps / ls, 3), "speedup_vs_plain_batched": round(summ["solve_seconds"] / ls, 3),
+           "wall_seconds": round(time.time() - t0, 1)}
+    print(f"# north star {L}: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def learned_solve(cmx, gauge, L, kappa, csw, a):
     """Learned preconditioner at kappa 0.1355 (near critical) on the benchmark field: trained on separate
     fields of the same recipe (Haar + 6 stout steps, seeds 10, 11, ...) on a smaller lattice, then used
@@ -183,18 +239,11 @@ def learned_solve(cmx, gauge, L, kappa, csw, a):
              "true_rel_resid": rc["rel_resid"], "reliable_updates": rc["n_reliable"]}
     print(f"# solve plain_mixed_cg_fp16 kappa {kappa}: {rc['n_count']} it, {rc['solve_seconds']:.4f} s",
           file=sys.stderr, flush=True)
-    net = cmx.Net()
-    net.init_neumann(4.0 + mass)
     TL = tuple(int(v) for v in a.train_lattice.split(","))
     t0 = time.time()
-    train = []
-    for k in range(a.train_fields):
-        U = cmx.gauge_apply_bc(TL, cmx.gauge_generate(TL, kind=2, seed=10 + k, nstout=6, rho=0.1), (1, 1, 1, -1))
-        top = cmx.Operator(cmx.Gauge(TL, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF)
-        cmx.net_attach(top, net)
-        train.append(cmx.net_train(top, n_sources=8, snapshot_every=1, pairs_per_source=4, normalize=False,
-                                   epochs=4, lr=1e-3, seed=100 + k, prec=cmx.HALF))
-        print(f"# train field {k}: {train[-1]}", file=sys.stderr, flush=True)
+    net, train = train_net(cmx, mass, csw, a)
+    for k, t in enumerate(train):
+        print(f"# train field {k}: {t}", file=sys.stderr, flush=True)
     t_train = time.time() - t0
     cmx.net_attach(op, net)
     kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, basis_prec=cmx.HALF, n_krylov=16,
@@ -234,6 +283,9 @@ def main():
                     help="block,tiled,zb,prefetch,fast: fixed launch shape of the FP16 kernels instead of the "
                          "tuned one (profiling runs: the counter passes serialise kernels and skew the tuner)")
     ap.add_argument("--sources-lattice", type=str, default="48,48,48,96")
+    ap.add_argument("--north-star", action="store_true",
+                    help="also run the 12 point sources at --sources-lattice, kappa 0.1355: plain mixed CG "
+                         "(sequential, batched) vs learned FGMRES-DR (about a minute more)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -317,6 +369,10 @@ def main():
         batched = {"uneven_volumes": batched_uneven_solves(cmx),
                    "propagator_sources": batched_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")))}
 
+    north = None
+    if a.north_star:
+        north = north_star_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")), 0.1355, csw, a)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(L, U)
@@ -349,6 +405,7 @@ def main():
         "learned": learned,
         "small_volume": small,
         "batched": batched,
+        "north_star_12_sources": north,
         "solves": solves,
         "kernels": extra,
         "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}

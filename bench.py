@@ -163,6 +163,11 @@ def batched_sources(cmx, L):
     return out
 
 
+def stout_field(cmx, L, seed):
+    """Haar SU(3) start (host generator), then 6 isotropic stout steps with rho = 0.1 on the device."""
+    return cmx.gauge_stout_smear(L, cmx.gauge_generate(L, kind=2, seed=seed), 6, 0.1)
+
+
 def train_net(cmx, mass, csw, a):
     """The learned preconditioner's network, trained on separate fields of the benchmark recipe."""
     net = cmx.Net()
@@ -170,7 +175,7 @@ def train_net(cmx, mass, csw, a):
     TL = tuple(int(v) for v in a.train_lattice.split(","))
     train = []
     for k in range(a.train_fields):
-        U = cmx.gauge_apply_bc(TL, cmx.gauge_generate(TL, kind=2, seed=10 + k, nstout=6, rho=0.1), (1, 1, 1, -1))
+        U = cmx.gauge_apply_bc(TL, stout_field(cmx, TL, 10 + k), (1, 1, 1, -1))
         top = cmx.Operator(cmx.Gauge(TL, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF)
         cmx.net_attach(top, net)
         train.append(cmx.net_train(top, n_sources=8, snapshot_every=1, pairs_per_source=4, normalize=False,
@@ -185,7 +190,7 @@ def north_star_sources(cmx, L, kappa, csw, a):
     import numpy as np
     mass = cmx.kappa_to_mass(kappa)
     t0 = time.time()
-    U = cmx.gauge_apply_bc(L, cmx.gauge_generate(L, kind=2, seed=1, nstout=6, rho=0.1), (1, 1, 1, -1))
+    U = cmx.gauge_apply_bc(L, stout_field(cmx, L, 1), (1, 1, 1, -1))
     op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF, autotune=True)
     del U
     Vh = int(np.prod(L)) // 2
@@ -300,7 +305,7 @@ def main():
     Vh = V // 2
     kappa, csw = 0.1350, 1.0
     t0 = time.time()
-    U = cmx.gauge_generate(L, kind=2, seed=1, nstout=6, rho=0.1)
+    U = stout_field(cmx, L, 1)
     plaq = cmx.plaquette(L, U)
     U = cmx.gauge_apply_bc(L, U, (1, 1, 1, -1))
     t_gen = time.time() - t0

@@ -194,6 +194,13 @@ int cmx_dslash_deriv(cmx_op h, double* ds_u, const double* chi, const double* ps
   });
 }
 
+int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho) {
+  return guarded([&] {
+    require_init();
+    cmx::stout_smear(dims, links, nstep, rho);
+  });
+}
+
 int cmx_op_eo_piece(cmx_op h, int which, double* chi, const double* psi, int isign) {
   return guarded([&] {
     require_init();

@@ -172,6 +172,8 @@ void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, b
 // WilsonDslashBase::deriv (lwldslash_base_w.h:132-227) for the sites of cb, FP64 (deriv.hip): chi on cb,
 // psi on 1 - cb, ds = 4 * Vh * 9 double2 (direction, site of cb, 3x3 row-major)
 void hop_deriv(const Geom& g, int cb, int isign, VRef chi, VRef psi, void* ds, hipStream_t s);
+// nstep isotropic stout steps on the device, in place on host-layout links (stout.hip)
+void stout_smear(const int dims[4], double* links, int nstep, double rho);
 // MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
 void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
                       double* psi, const double* chi, cmx_solver_result* res);

@@ -133,6 +133,7 @@ def lib():
                                               ctypes.c_double, _dp]),
         "cmx_gauge_apply_bc": (ctypes.c_int, [_ip, _dp, _ip, _dp, _ip]),
         "cmx_plaquette": (ctypes.c_double, [_ip, _dp]),
+        "cmx_gauge_stout_smear": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double]),
         "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
     }
     for name, (res, args) in sig.items():
@@ -186,6 +187,15 @@ def gauge_apply_bc(L, U, boundary=(1, 1, 1, 1), twist_by_pi=None, twist_dir=(0, 
         td = (ctypes.c_int * 3)(*twist_dir)
         check(lib().cmx_gauge_apply_bc(_dims(L), _d(U), b, _d(tw), td))
     return U
+
+
+def gauge_stout_smear(L, U, nstep, rho):
+    """nstep isotropic stout steps on the device (Stouting::smear_links, stout_utils.cc:936-962); returns a
+    new complex128[4, V, 3, 3]."""
+    init()
+    out = np.array(U, np.complex128, order="C", copy=True)
+    check(lib().cmx_gauge_stout_smear(_dims(L), _d(out), int(nstep), float(rho)))
+    return out
 
 
 def plaquette(L, U):

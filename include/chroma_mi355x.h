@@ -323,6 +323,11 @@ int cmx_gauge_generate(const int dims[4], int kind, unsigned long long seed, dou
  * links in direction twist_dir[i] times exp(i pi twist_by_pi[i] / L[dir]). */
 int cmx_gauge_apply_bc(const int dims[4], double* links, const int boundary[4], const double* twist_by_pi,
                        const int* twist_dir);
+/* Isotropic stout smearing on the device, in place: nstep applications of Stouting::smear_links
+ * (stout_utils.cc:936-962) with every direction smeared and rho(mu,nu) = rho; staple sum and Q as in
+ * getQsandCs (:105-180), exp(iQ) to rounding, each new link re-projected onto SU(3) at rounding level.
+ * links: 4*V*18 doubles (host layout).  Needs cmx_init; FP64 throughout. */
+int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho);
 /* Average plaquette Re tr(U_p)/3 over all sites and planes (MesPlq normalisation). */
 double cmx_plaquette(const int dims[4], const double* links);
 /* Z2 noise source: each real and imaginary part +-1/sqrt(2) from (seed, site, component). */

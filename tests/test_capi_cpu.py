@@ -108,3 +108,13 @@ def test_learned_net_neumann_initialisation():
     net.set_weights(r1, e2, e0)
     g1, _, _ = net.get_weights()
     assert np.allclose(g1, r1.astype(np.complex64), rtol=0, atol=0)
+
+
+def test_generator_stout_matches_oracle():
+    """The host generator's stout steps (Taylor exponential with scaling and squaring, re-projected onto
+    SU(3)) against the oracle's (plain Taylor series, no re-projection)."""
+    import oracle
+    L = (4, 4, 6, 4)
+    U0 = cmx.gauge_generate(L, kind=2, seed=21)
+    U2 = cmx.gauge_generate(L, kind=2, seed=21, nstout=2, rho=0.12)
+    assert rel(U2, oracle.stout(L, U0, 0.12, 2)) < 1e-13

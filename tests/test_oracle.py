@@ -460,3 +460,137 @@ def test_dslash_deriv_is_the_link_derivative(isign, cb):
     # both checkerboards at once is the sum (lwldslash_base_w.h:110-124)
     both = oracle.dslash_deriv(L, chi, psi, isign, -1)
     assert np.allclose(both, ds + oracle.dslash_deriv(L, chi, psi, isign, 1 - cb), atol=0, rtol=0)
+
+
+# ---------------------------------------------------------------------------------------------------
+# Stout smearing (Stouting::smear_links, stout_utils.cc:936-962; staples and Q of getQsandCs, :105-180)
+STOUT_X0, STOUT_MU0, STOUT_EPS, STOUT_H = (1, 2, 1, 2), 1, 0.4, np.array([1.0, 1.0, -2.0])
+
+
+def _one_link_field(L):
+    """Unit field except U_mu0(x0) = diag(exp(i eps h)), h traceless but with sin(eps h) not traceless."""
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:, :] = np.eye(3)
+    U[STOUT_MU0, index_of(L, STOUT_X0)] = np.diag(np.exp(1j * STOUT_EPS * STOUT_H))
+    return U
+
+
+def _one_link_closed_form(L, rho):
+    """One stout step of _one_link_field by hand.  Every link stays diagonal, so exp(iQ) is a phase per
+    colour.  For U_mu0(x0) all six staples are 1: C = 6 rho, Omega = 6 rho D^dag, and
+    U' = exp(i (eps h - 6 rho (sin(eps h) - mean)));  a link with one staple equal to D (or D^dag) and
+    five equal to 1 gets exp(+i rho (sin(eps h) - mean)) (or exp(-i ...)).  That is U_nu(x0) and
+    U_nu(x0-nu+mu0), U_mu0(x0+nu) and U_mu0(x0-nu) with D; U_nu(x0+mu0) and U_nu(x0-nu) with D^dag."""
+    U = _one_link_field(L)
+    sn = np.sin(STOUT_EPS * STOUT_H)
+    ph = rho * (sn - sn.mean())
+    x0, mu0 = np.array(STOUT_X0), STOUT_MU0
+
+    def at(mu, shifts, sign):
+        x = x0.copy()
+        for m, d in shifts:
+            x[m] = (x[m] + d) % L[m]
+        U[mu, index_of(L, x)] = np.diag(np.exp(1j * sign * ph))
+
+    U[mu0, index_of(L, x0)] = np.diag(np.exp(1j * (STOUT_EPS * STOUT_H - 6 * ph)))
+    for nu in range(4):
+        if nu == mu0:
+            continue
+        at(nu, [], +1)
+        at(nu, [(mu0, 1), (nu, -1)], +1)
+        at(mu0, [(nu, 1)], +1)
+        at(mu0, [(nu, -1)], +1)
+        at(nu, [(mu0, 1)], -1)
+        at(nu, [(nu, -1)], -1)
+    return U
+
+
+@pytest.mark.parametrize("rho", [0.1, 0.22])
+def test_stout_one_link_closed_form(rho):
+    L = (4, 4, 4, 4)
+    out = oracle.stout(L, _one_link_field(L), rho)
+    assert np.max(np.abs(out - _one_link_closed_form(L, rho))) < 1e-14
+
+
+def test_stout_gauge_covariance_unitarity_and_rho_zero():
+    """stout(g U g^dag(x+mu)) = g stout(U) g^dag(x+mu) for a random gauge transform g (two steps on a Haar
+    field): a staple on the wrong sites or with a missing dagger breaks it.  The result stays in SU(3),
+    rho = 0 returns the field unchanged, and smearing raises the plaquette of a rough field."""
+    L = (4, 4, 4, 6)
+    V = int(np.prod(L))
+    U = numpy_gauge(L, 60)
+    from tests.util import random_su3_numpy
+    g = random_su3_numpy(V, 61)
+    X = coords(L)
+
+    def transform(W):
+        out = np.empty_like(W)
+        for mu in range(4):
+            xp = X.copy(); xp[:, mu] = (xp[:, mu] + 1) % L[mu]
+            idx = np.array([index_of(L, x) for x in xp])
+            out[mu] = g @ W[mu] @ g[idx].conj().transpose(0, 2, 1)
+        return out
+
+    S = oracle.stout(L, U, 0.1, 2)
+    assert rel(oracle.stout(L, transform(U), 0.1, 2), transform(S)) < 1e-13
+    eye = np.eye(3)
+    assert np.max(np.abs(S @ S.conj().transpose(0, 1, 3, 2) - eye)) < 1e-13
+    assert np.max(np.abs(np.linalg.det(S) - 1)) < 1e-13
+    assert np.array_equal(oracle.stout(L, U, 0.0), U)
+
+    def plaq(W):
+        acc = 0.0
+        for mu in range(3):
+            for nu in range(mu + 1, 4):
+                xm = X.copy(); xm[:, mu] = (xm[:, mu] + 1) % L[mu]
+                xn = X.copy(); xn[:, nu] = (xn[:, nu] + 1) % L[nu]
+                im = np.array([index_of(L, x) for x in xm])
+                inn = np.array([index_of(L, x) for x in xn])
+                P = W[mu] @ W[nu][im] @ W[mu][inn].conj().transpose(0, 2, 1) @ W[nu].conj().transpose(0, 2, 1)
+                acc += np.trace(P, axis1=1, axis2=2).real.sum() / 3
+        return acc / (6 * V)
+
+    p = [plaq(oracle.stout(L, U, 0.1, n)) for n in (0, 1, 3)]
+    assert abs(p[0]) < 0.1 and p[0] < p[1] < p[2]
+
+
+def test_stout_planted_mistakes_are_caught():
+    """A numpy copy of one stout step with a planted mistake fails the closed form each time: the
+    backward staple built at x+nu, no traceless projection, Q without the factor 1/2, Q of the wrong sign."""
+    from scipy.linalg import expm
+    L = (4, 4, 4, 4)
+    V = int(np.prod(L))
+    U = _one_link_field(L)
+    X = coords(L)
+    want = _one_link_closed_form(L, 0.1)
+
+    def nb(s, mu, d):
+        x = X[s].copy(); x[mu] = (x[mu] + d) % L[mu]
+        return index_of(L, x)
+
+    def step(mistake):
+        out = np.empty_like(U)
+        dg = lambda m: m.conj().T
+        for mu in range(4):
+            for s in range(V):
+                C = np.zeros((3, 3), complex)
+                for nu in range(4):
+                    if nu == mu:
+                        continue
+                    C += U[nu, s] @ U[mu, nb(s, nu, 1)] @ dg(U[nu, nb(s, mu, 1)])
+                    y = nb(s, nu, 1 if mistake == "backward_site" else -1)
+                    C += dg(U[nu, y]) @ U[mu, y] @ U[nu, nb(y, mu, 1)]
+                Om = 0.1 * C @ dg(U[mu, s])
+                T = dg(Om) - Om
+                if mistake != "no_traceless":
+                    T -= np.trace(T) / 3 * np.eye(3)
+                Q = (1.0 if mistake == "no_half" else 0.5) * 1j * T
+                if mistake == "sign":
+                    Q = -Q
+                out[mu, s] = expm(1j * Q) @ U[mu, s]
+        return out
+
+    assert np.max(np.abs(step(None) - want)) < 1e-13
+    for mistake in ("backward_site", "no_traceless", "no_half", "sign"):
+        assert np.max(np.abs(step(mistake) - want)) > 1e-3, mistake

@@ -201,6 +201,14 @@ int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rh
   });
 }
 
+int cmx_meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+                  const int* moms, double* corr) {
+  return guarded([&] {
+    require_init();
+    cmx::meson_2pt(dims, prop1, prop2, origin, nmom, moms, corr);
+  });
+}
+
 int cmx_op_eo_piece(cmx_op h, int which, double* chi, const double* psi, int isign) {
   return guarded([&] {
     require_init();

@@ -1,0 +1,199 @@
+// Meson two-point functions on the device, in FP64: mesons() (mesons_w.cc:97-180) for the 16 gamma
+// insertions Gamma(n) = gamma_1^n0 gamma_2^n1 gamma_3^n2 gamma_4^n3 (docs/notes/chroma_gamma_matrices.tex),
+//   c_n(x) = tr[ adj(gamma_5 S_2(x) gamma_5) Gamma(n) S_1(x) Gamma(n) ]      (trace over spin and colour),
+// then the sink-momentum projection of SftMom::sft (sftmom.cc:405-460, 520-530) over the spatial sites of
+// each time slice, phase exp(i 2 pi sum_k n_k (x_k - origin_k) / L_k), with the time slice counted from
+// the source, t_eff = (t - origin_t) mod L_t (mesons_w.cc:160-165).
+//
+// Every Gamma(n) and gamma_5 of the DeGrand-Rossi basis (docs/notes/gamma_conventions.tex:74-79) has one
+// non-zero entry, a power of i, per row, so each of the 16 spin blocks (a, b) of the trace reads one 3x3
+// colour block of S_2 and one of S_1 with a fixed coefficient; the host builds that table per Gamma.
+// Propagators: [site][sink spin*3 + colour][source spin*3 + colour] complex, sites in the host order.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "op.h"
+
+namespace cmx {
+
+namespace {
+
+struct Term {  // one spin block (a, b) of the trace for one Gamma
+  double2 coef;
+  int r2, c2, r1, c1;  // spin row / column of the S_2 and S_1 colour blocks
+};
+
+// signed permutation: row a has its one non-zero entry ph[a] in column p[a]
+struct SPerm {
+  int p[4];
+  double ph_re[4], ph_im[4];
+};
+
+SPerm sp_mul(const SPerm& A, const SPerm& B) {
+  SPerm C;
+  for (int a = 0; a < 4; ++a) {
+    const int k = A.p[a];
+    C.p[a] = B.p[k];
+    C.ph_re[a] = A.ph_re[a] * B.ph_re[k] - A.ph_im[a] * B.ph_im[k];
+    C.ph_im[a] = A.ph_re[a] * B.ph_im[k] + A.ph_im[a] * B.ph_re[k];
+  }
+  return C;
+}
+
+SPerm gamma_dr(int mu) {  // gamma_1..gamma_4 (mu = 0..3) of the DeGrand-Rossi basis
+  static const int perm[4][4] = {{3, 2, 1, 0}, {3, 2, 1, 0}, {2, 3, 0, 1}, {2, 3, 0, 1}};
+  static const double re[4][4] = {{0, 0, 0, 0}, {-1, 1, 1, -1}, {0, 0, 0, 0}, {1, 1, 1, 1}};
+  static const double im[4][4] = {{1, 1, -1, -1}, {0, 0, 0, 0}, {1, -1, -1, 1}, {0, 0, 0, 0}};
+  SPerm g;
+  for (int a = 0; a < 4; ++a) g.p[a] = perm[mu][a], g.ph_re[a] = re[mu][a], g.ph_im[a] = im[mu][a];
+  return g;
+}
+
+SPerm gamma_n(int n) {
+  SPerm g;
+  for (int a = 0; a < 4; ++a) g.p[a] = a, g.ph_re[a] = 1, g.ph_im[a] = 0;
+  for (int mu = 0; mu < 4; ++mu)
+    if (n >> mu & 1) g = sp_mul(g, gamma_dr(mu));
+  return g;
+}
+
+// c_n = sum_{a,b} sum_{ij} conj(anti_{(a,i),(b,j)}) (G S_1 G)_{(a,i),(b,j)} with anti = g5 S_2 g5:
+// anti_{(a,i),(b,j)} = g5ph[a] S_2[(p5[a],i),(q,j)] g5ph[q] where p5[q] = b, and
+// (G S_1 G)_{(a,i),(b,j)} = Gph[a] S_1[(pG[a],i),(r,j)] Gph[r] where pG[r] = b.
+std::vector<Term> meson_terms() {
+  const SPerm g5 = gamma_n(15);
+  std::vector<Term> t;
+  for (int n = 0; n < 16; ++n) {
+    const SPerm G = gamma_n(n);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        int q = -1, r = -1;
+        for (int k = 0; k < 4; ++k) {
+          if (g5.p[k] == b) q = k;
+          if (G.p[k] == b) r = k;
+        }
+        // conj(g5ph[a] g5ph[q]) Gph[a] Gph[r]
+        const double ar = g5.ph_re[a] * g5.ph_re[q] - g5.ph_im[a] * g5.ph_im[q];
+        const double ai = -(g5.ph_re[a] * g5.ph_im[q] + g5.ph_im[a] * g5.ph_re[q]);
+        const double br = G.ph_re[a] * G.ph_re[r] - G.ph_im[a] * G.ph_im[r];
+        const double bi = G.ph_re[a] * G.ph_im[r] + G.ph_im[a] * G.ph_re[r];
+        t.push_back(Term{make_double2(ar * br - ai * bi, ar * bi + ai * br), g5.p[a], q, G.p[a], r});
+      }
+  }
+  return t;
+}
+
+// one lane per (site, Gamma): c[n][site]
+__global__ void __launch_bounds__(256) k_meson_sites(long V, const double2* __restrict__ s1,
+                                                     const double2* __restrict__ s2, const Term* __restrict__ terms,
+                                                     double2* __restrict__ c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 16 * V) return;
+  const long site = t >> 4;
+  const int n = (int)(t & 15);
+  const double2* p1 = s1 + site * 144;
+  const double2* p2 = s2 + site * 144;
+  double cr = 0, ci = 0;
+  for (int k = 0; k < 16; ++k) {
+    const Term tm = terms[n * 16 + k];
+    double sr = 0, si = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double2 x = p2[(tm.r2 * 3 + i) * 12 + tm.c2 * 3 + j];
+        const double2 y = p1[(tm.r1 * 3 + i) * 12 + tm.c1 * 3 + j];
+        sr += x.x * y.x + x.y * y.y;  // conj(x) y
+        si += x.x * y.y - x.y * y.x;
+      }
+    cr += tm.coef.x * sr - tm.coef.y * si;
+    ci += tm.coef.x * si + tm.coef.y * sr;
+  }
+  c[(size_t)n * V + site] = make_double2(cr, ci);
+}
+
+struct ProjGeom {
+  int L[4];
+  long V, Vh;
+  int org[4];
+};
+
+// one block per (time slice, momentum): out[n][m][t_eff] = sum over the slice of phase(x) c[n][x], summed in
+// a fixed order (strided per lane, then a tree over the block), so the result is deterministic
+__global__ void __launch_bounds__(256) k_meson_project(ProjGeom g, const double2* __restrict__ c,
+                                                       const int* __restrict__ moms, int nmom,
+                                                       double2* __restrict__ out) {
+  __shared__ double2 red[256];
+  const int t = blockIdx.x, m = blockIdx.y;
+  const long Vs = (long)g.L[0] * g.L[1] * g.L[2];
+  const double k0 = 6.283185307179586 * moms[3 * m] / g.L[0], k1 = 6.283185307179586 * moms[3 * m + 1] / g.L[1],
+               k2 = 6.283185307179586 * moms[3 * m + 2] / g.L[2];
+  double2 acc[16];
+#pragma unroll
+  for (int n = 0; n < 16; ++n) acc[n] = make_double2(0, 0);
+  for (long v = threadIdx.x; v < Vs; v += blockDim.x) {
+    const int x0 = (int)(v % g.L[0]), x1 = (int)(v / g.L[0] % g.L[1]), x2 = (int)(v / ((long)g.L[0] * g.L[1]));
+    const long lex = v + Vs * t;
+    const long site = ((x0 + x1 + x2 + t) & 1) * g.Vh + lex / 2;
+    double sn, cs;
+    sincos(k0 * (x0 - g.org[0]) + k1 * (x1 - g.org[1]) + k2 * (x2 - g.org[2]), &sn, &cs);
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const double2 z = c[(size_t)n * g.V + site];
+      acc[n].x += cs * z.x - sn * z.y;
+      acc[n].y += cs * z.y + sn * z.x;
+    }
+  }
+  const int teff = (t - g.org[3] % g.L[3] + g.L[3]) % g.L[3];
+  for (int n = 0; n < 16; ++n) {
+    red[threadIdx.x] = acc[n];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        red[threadIdx.x] = make_double2(red[threadIdx.x].x + red[threadIdx.x + w].x, red[threadIdx.x].y + red[threadIdx.x + w].y);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[((size_t)n * nmom + m) * g.L[3] + teff] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+               const int* moms, double* corr) {
+  CMX_REQUIRE(dims && prop1 && prop2 && corr, CMX_ERR_ARG, "meson_2pt: null array");
+  CMX_REQUIRE(nmom >= 1 && nmom <= 4096 && moms, CMX_ERR_ARG, "meson_2pt: 1..4096 momenta");
+  ProjGeom g;
+  g.V = 1;
+  for (int m = 0; m < 4; ++m) {
+    CMX_REQUIRE(dims[m] > 0 && dims[m] % 2 == 0, CMX_ERR_ARG, "meson_2pt: extents must be even and positive");
+    g.L[m] = dims[m];
+    g.V *= dims[m];
+    g.org[m] = origin ? origin[m] : 0;
+    CMX_REQUIRE(g.org[m] >= 0 && g.org[m] < dims[m], CMX_ERR_ARG, "meson_2pt: origin outside the lattice");
+  }
+  g.Vh = g.V / 2;
+  CMX_REQUIRE(16 * g.V <= (long)INT32_MAX * 256, CMX_ERR_ARG, "meson_2pt: lattice too large for one grid");
+  const std::vector<Term> terms = meson_terms();
+  const size_t pbytes = sizeof(double) * 288 * (size_t)g.V;
+  DevBuf d1(pbytes), d2(pbytes), dc(sizeof(double2) * 16 * (size_t)g.V), dt(sizeof(Term) * terms.size()),
+      dm(sizeof(int) * 3 * (size_t)nmom), dout(sizeof(double2) * 16 * (size_t)nmom * g.L[3]);
+  CMX_HIP(hipMemcpy(d1.p, prop1, pbytes, hipMemcpyHostToDevice));
+  if (prop2 == prop1)
+    CMX_HIP(hipMemcpy(d2.p, d1.p, pbytes, hipMemcpyDeviceToDevice));
+  else
+    CMX_HIP(hipMemcpy(d2.p, prop2, pbytes, hipMemcpyHostToDevice));
+  CMX_HIP(hipMemcpy(dt.p, terms.data(), dt.bytes, hipMemcpyHostToDevice));
+  CMX_HIP(hipMemcpy(dm.p, moms, dm.bytes, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_meson_sites, dim3((unsigned)((16 * g.V + 255) / 256)), dim3(256), 0, 0, g.V, d1.as<double2>(),
+                     d2.as<double2>(), dt.as<Term>(), dc.as<double2>());
+  CMX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_meson_project, dim3(g.L[3], nmom), dim3(256), 0, 0, g, dc.as<double2>(), dm.as<int>(), nmom,
+                     dout.as<double2>());
+  CMX_HIP(hipGetLastError());
+  CMX_HIP(hipMemcpy(corr, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+}
+
+}  // namespace cmx

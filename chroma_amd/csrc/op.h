@@ -174,6 +174,9 @@ void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, b
 void hop_deriv(const Geom& g, int cb, int isign, VRef chi, VRef psi, void* ds, hipStream_t s);
 // nstep isotropic stout steps on the device, in place on host-layout links (stout.hip)
 void stout_smear(const int dims[4], double* links, int nstep, double rho);
+// meson two-point functions of two host-layout propagators for the 16 gamma insertions (meson.hip)
+void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+               const int* moms, double* corr);
 // MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
 void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
                       double* psi, const double* chi, cmx_solver_result* res);

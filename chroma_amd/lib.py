@@ -134,6 +134,7 @@ def lib():
         "cmx_gauge_apply_bc": (ctypes.c_int, [_ip, _dp, _ip, _dp, _ip]),
         "cmx_plaquette": (ctypes.c_double, [_ip, _dp]),
         "cmx_gauge_stout_smear": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double]),
+        "cmx_meson_2pt": (ctypes.c_int, [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]),
         "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
     }
     for name, (res, args) in sig.items():
@@ -195,6 +196,24 @@ def gauge_stout_smear(L, U, nstep, rho):
     init()
     out = np.array(U, np.complex128, order="C", copy=True)
     check(lib().cmx_gauge_stout_smear(_dims(L), _d(out), int(nstep), float(rho)))
+    return out
+
+
+def meson_2pt(L, prop1, prop2=None, origin=(0, 0, 0, 0), moms=((0, 0, 0),)):
+    """mesons() (mesons_w.cc:97-180) on the device: complex128[16, len(moms), L_t], gamma insertion n,
+    sink momentum m, time counted from origin[3]; mesons() writes the real part.  Propagators are
+    complex128[V, 12, 12] (sink spin*3+colour, source spin*3+colour)."""
+    init()
+    p1 = np.ascontiguousarray(prop1, np.complex128)
+    p2 = p1 if prop2 is None else np.ascontiguousarray(prop2, np.complex128)
+    V = int(np.prod(L))
+    if p1.shape != (V, 12, 12) or p2.shape != (V, 12, 12):
+        raise ValueError("meson_2pt: propagators must be complex[V, 12, 12]")
+    mom = np.ascontiguousarray(moms, np.int32).reshape(-1, 3)
+    out = np.zeros((16, mom.shape[0], int(L[3])), np.complex128)
+    org = (ctypes.c_int * 4)(*origin)
+    check(lib().cmx_meson_2pt(_dims(L), _d(p1), _d(p2), org, int(mom.shape[0]),
+                              mom.ctypes.data_as(_ip), _d(out)))
     return out
 
 

@@ -328,6 +328,16 @@ int cmx_gauge_apply_bc(const int dims[4], double* links, const int boundary[4], 
  * getQsandCs (:105-180), exp(iQ) to rounding, each new link re-projected onto SU(3) at rounding level.
  * links: 4*V*18 doubles (host layout).  Needs cmx_init; FP64 throughout. */
 int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho);
+/* Meson two-point functions on the device: mesons() (mesons_w.cc:97-180) for the 16 insertions
+ * Gamma(n) = gamma_1^n0 gamma_2^n1 gamma_3^n2 gamma_4^n3 with the sink-momentum projection of
+ * SftMom::sft (sftmom.cc:405-460), decay direction 3:
+ *   corr[n][m][t_eff] = sum_{x in slice t} exp(i 2 pi sum_k moms[3m+k] (x_k - origin[k]) / L_k)
+ *                       tr[adj(gamma_5 prop2(x) gamma_5) Gamma(n) prop1(x) Gamma(n)],
+ * t_eff = (t - origin[3]) mod L_t.  prop1, prop2: V*144 complex (2*V*144 doubles), site in the host
+ * order, then sink spin*3+colour, then source spin*3+colour; prop2 may equal prop1.  corr: 16*nmom*L_t
+ * complex (mesons() writes the real part).  Needs cmx_init; FP64 throughout. */
+int cmx_meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+                  const int* moms, double* corr);
 /* Average plaquette Re tr(U_p)/3 over all sites and planes (MesPlq normalisation). */
 double cmx_plaquette(const int dims[4], const double* links);
 /* Z2 noise source: each real and imaginary part +-1/sqrt(2) from (seed, site, component). */

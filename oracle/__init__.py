@@ -46,6 +46,7 @@ def lib():
         L.orc_dslash_f32.argtypes = [_ip, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.orc_mesfield.argtypes = [_ip, _dp, _dp]
         L.orc_stout.argtypes = [_ip, _dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.orc_mesons.argtypes = [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]
         L.orc_make_clover.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_clover_invert.argtypes = [_ip, _dp, ctypes.c_int, _dp, _lp]
         L.orc_clover_apply.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int]
@@ -118,6 +119,17 @@ def stout(L, U, rho, nstep=1):
     U = np.ascontiguousarray(U, np.complex128)
     out = np.empty_like(U)
     _check(lib().orc_stout(_dims(L), _d(U), _d(out), float(rho), int(nstep)))
+    return out
+
+
+def mesons(L, prop1, prop2=None, origin=(0, 0, 0, 0), moms=((0, 0, 0),)):
+    """mesons() (mesons_w.cc:97-180): complex128[16, nmom, L_t] (gamma insertion, sink momentum, t_eff)."""
+    p1 = np.ascontiguousarray(prop1, np.complex128)
+    p2 = p1 if prop2 is None else np.ascontiguousarray(prop2, np.complex128)
+    mom = np.ascontiguousarray(moms, np.int32).reshape(-1, 3)
+    out = np.zeros((16, mom.shape[0], int(L[3])), np.complex128)
+    _check(lib().orc_mesons(_dims(L), _d(p1), _d(p2), (ctypes.c_int * 4)(*origin), int(mom.shape[0]),
+                            mom.ctypes.data_as(_ip), _d(out)))
     return out
 
 

@@ -291,6 +291,77 @@ void stout_step(const Geom& g, const cd* U, cd* out, double rho) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Meson two-point functions, mesons() (mesons_w.cc:97-180), for gamma_value n = 0..15 with
+//   Gamma(n) = gamma_1^n0 gamma_2^n1 gamma_3^n2 gamma_4^n3 (docs/notes/chroma_gamma_matrices.tex),
+//   anti_quark_prop = Gamma(15) S2 Gamma(15),  corr_fn = trace(adj(anti_quark_prop) Gamma(n) S1 Gamma(n)),
+//   hsum[m][t] = sum_{x in slice t} exp(i p_m.(x - origin)) corr_fn(x),  p_k = 2 pi mom_k / L_k
+//   (SftMom phases, sftmom.cc:405-460; decay direction 3), stored at t_eff = (t - t0) mod L_t
+//   (mesons_w.cc:160-165).  Dense 12x12 spin-colour matrices per site; S[s] is [12 sink][12 source].
+static void gamma_product(cd G[4][4], int n) {
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) G[a][b] = a == b ? 1.0 : 0.0;
+  for (int mu = 0; mu < 4; ++mu) {
+    if (!(n >> mu & 1)) continue;
+    cd T[4][4];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        T[a][b] = 0;
+        for (int k = 0; k < 4; ++k) T[a][b] += G[a][k] * kGamma[mu][k][b];
+      }
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) G[a][b] = T[a][b];
+  }
+}
+
+// (G x 1_colour) S (G x 1_colour) for a 12x12 S
+static void sandwich(cd* out, const cd G[4][4], const cd* S) {
+  cd T[144];
+  for (int r = 0; r < 12; ++r)
+    for (int c = 0; c < 12; ++c) {
+      cd t = 0;
+      for (int k = 0; k < 4; ++k) t += G[r / 3][k] * S[(k * 3 + r % 3) * 12 + c];
+      T[r * 12 + c] = t;
+    }
+  for (int r = 0; r < 12; ++r)
+    for (int c = 0; c < 12; ++c) {
+      cd t = 0;
+      for (int k = 0; k < 4; ++k) t += T[r * 12 + k * 3 + c % 3] * G[k][c / 3];
+      out[r * 12 + c] = t;
+    }
+}
+
+void mesons(const Geom& g, const cd* S1, const cd* S2, const int org[4], int nmom, const int* moms, cd* out) {
+  const int Lt = g.L[3];
+  std::vector<cd> corr((size_t)16 * g.V);
+  cd G5[4][4];
+  gamma_product(G5, 15);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < g.V; ++s) {
+    cd A[144], B[144], G[4][4];
+    sandwich(A, G5, S2 + s * 144);
+    for (int n = 0; n < 16; ++n) {
+      gamma_product(G, n);
+      sandwich(B, G, S1 + s * 144);
+      cd t = 0;
+      for (int k = 0; k < 144; ++k) t += std::conj(A[k]) * B[k];
+      corr[(size_t)n * g.V + s] = t;
+    }
+  }
+  for (size_t k = 0; k < (size_t)16 * nmom * Lt; ++k) out[k] = 0;
+  for (long s = 0; s < g.V; ++s) {
+    int x[4];
+    g.coords(s, x);
+    const int teff = ((x[3] - org[3]) % Lt + Lt) % Lt;
+    for (int m = 0; m < nmom; ++m) {
+      double pdx = 0;
+      for (int k = 0; k < 3; ++k) pdx += 6.283185307179586 * moms[3 * m + k] * (x[k] - org[k]) / g.L[k];
+      const cd ph(std::cos(pdx), std::sin(pdx));
+      for (int n = 0; n < 16; ++n) out[((size_t)n * nmom + m) * Lt + teff] += ph * corr[(size_t)n * g.V + s];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Clover build: QDPCloverTermT::create (clover_term_qdp_w.h:244-294) + makeClov (:525-553)
 // + makeClovSiteLoop (:398-522).  Isotropic only: clovCoeffR = clovCoeffT = csw (clover_fermact_
 // params_w.cc:65-77), then both scaled by 1/2 (:263-267); diag_mass = Nd + Mass (:274-278).
@@ -1108,6 +1179,16 @@ int orc_mesfield(const int L[4], const double* U, double* F) {
   return guard([&] {
     Geom g(L);
     mesfield<double>(g, reinterpret_cast<const cd*>(U), reinterpret_cast<cd*>(F));
+  });
+}
+
+// S1, S2: [V][12][12] complex; out: [16][nmom][L_t] complex
+int orc_mesons(const int L[4], const double* S1, const double* S2, const int origin[4], int nmom, const int* moms,
+               double* out) {
+  return guard([&] {
+    Geom g(L);
+    mesons(g, reinterpret_cast<const cd*>(S1), reinterpret_cast<const cd*>(S2), origin, nmom, moms,
+           reinterpret_cast<cd*>(out));
   });
 }
 

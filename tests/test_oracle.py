@@ -594,3 +594,86 @@ def test_stout_planted_mistakes_are_caught():
     assert np.max(np.abs(step(None) - want)) < 1e-13
     for mistake in ("backward_site", "no_traceless", "no_half", "sign"):
         assert np.max(np.abs(step(mistake) - want)) > 1e-3, mistake
+
+
+# ---------------------------------------------------------------------------------------------------
+# Meson two-point functions (mesons_w.cc:97-180 with SftMom phases, sftmom.cc:405-460)
+def _gamma_n(n):
+    """Gamma(n) = gamma_1^n0 gamma_2^n1 gamma_3^n2 gamma_4^n3 (docs/notes/chroma_gamma_matrices.tex)."""
+    G = np.eye(4, dtype=complex)
+    for mu in range(4):
+        if n >> mu & 1:
+            G = G @ GAMMA[mu]
+    return G
+
+
+def _mesons_numpy(L, S1, S2, origin, moms, mistake=None):
+    V = int(np.prod(L))
+    X = coords(L)
+    g5 = np.kron(G5, np.eye(3))
+    anti = S2 if mistake == "no_g5" else g5 @ S2 @ g5
+    out = np.zeros((16, len(moms), L[3]), complex)
+    teff = (X[:, 3] - origin[3]) % L[3]
+    for n in range(16):
+        G = np.kron(_gamma_n(n), np.eye(3))
+        B = G @ S1 @ G
+        c = np.einsum("vij,vij->v", anti if mistake == "no_conj" else anti.conj(), B)
+        for m, p in enumerate(moms):
+            pdx = sum(2 * np.pi * p[k] * (X[:, k] - origin[k]) / L[k] for k in range(3))
+            out[n, m] = np.bincount(teff, weights=(np.exp(1j * pdx) * c).real, minlength=L[3]) + \
+                1j * np.bincount(teff, weights=(np.exp(1j * pdx) * c).imag, minlength=L[3])
+    return out
+
+
+def test_mesons_against_numpy_contraction():
+    """Random propagators, an origin off zero and three sink momenta: the oracle equals a numpy
+    contraction written from mesons_w.cc:128-142 with the tests' own gamma matrices."""
+    L = (4, 4, 2, 6)
+    V = int(np.prod(L))
+    r = np.random.default_rng(70)
+    S1 = r.standard_normal((V, 12, 12)) + 1j * r.standard_normal((V, 12, 12))
+    S2 = r.standard_normal((V, 12, 12)) + 1j * r.standard_normal((V, 12, 12))
+    origin, moms = (1, 3, 0, 2), ((0, 0, 0), (1, 0, 0), (-1, 2, 1))
+    got = oracle.mesons(L, S1, S2, origin, moms)
+    assert rel(got, _mesons_numpy(L, S1, S2, origin, moms)) < 1e-13
+    # gamma_value 15 is gamma_5: with S1 = S2 the pion is sum |S|^2 per slice (prop_corr)
+    assert np.allclose(_gamma_n(15), G5)
+    pion = oracle.mesons(L, S1, None, origin)[15, 0]
+    X = coords(L)
+    want = np.bincount((X[:, 3] - origin[3]) % L[3], weights=np.sum(np.abs(S1) ** 2, axis=(1, 2)), minlength=L[3])
+    assert np.max(np.abs(pion - want)) < 1e-12 * np.max(want)
+    # summing the projection over every spatial momentum returns Vs times the correlator at the origin's
+    # spatial site (sum_p exp(i p.(x - o)) = Vs delta_{x,o}), which fixes 2 pi / L_k
+    allp = [(a, b, c) for a in range(L[0]) for b in range(L[1]) for c in range(L[2])]
+    tot = oracle.mesons(L, S1, S2, origin, allp).sum(axis=1)
+    site0 = np.zeros((16, L[3]), complex)
+    for t in range(L[3]):
+        x = [origin[0], origin[1], origin[2], (origin[3] + t) % L[3]]
+        s = index_of(L, x)
+        for n in range(16):
+            G = np.kron(_gamma_n(n), np.eye(3))
+            g5 = np.kron(G5, np.eye(3))
+            site0[n, t] = np.sum((g5 @ S2[s] @ g5).conj() * (G @ S1[s] @ G))
+    assert rel(tot, np.prod(L[:3]) * site0) < 1e-12
+
+
+def test_mesons_pion_is_the_golden_prop_corr():
+    """prec_wilson-twisted golden: the pion correlator of the oracle's propagator is the golden prop_corr
+    (t_srce = 0, so t_eff = t); planted mistakes (no gamma_5 sandwich on the anti-quark, no complex
+    conjugate) move it off."""
+    g = json.load(open(os.path.join(HERE, "golden", "prec_wilson_twisted.json")))
+    L = tuple(g["nrow"])
+    V = int(np.prod(L))
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:] = np.eye(3)
+    for ph, d in zip(g["phases_by_pi"], g["phases_dir"]):
+        U[d] *= np.exp(1j * np.pi * ph / L[d])
+    op = oracle.Op(oracle.EO_WILSON, L, U, 1 / (2 * g["kappa"]) - 4)
+    _, _, _, prop = op.quarkprop4_point(index_of(L, g["t_srce"]), oracle.CG, g["RsdCG"], g["MaxCG"], want_prop=True)
+    gold = np.array(g["prop_corr"])
+    m = oracle.mesons(L, prop, None, tuple(g["t_srce"]))
+    assert np.max(np.abs(m[15, 0].real - gold) / gold) < g["tolerance"]
+    assert np.max(np.abs(m[15, 0].imag)) < 1e-12 * np.max(gold)
+    for mistake in ("no_g5", "no_conj"):
+        bad = _mesons_numpy(L, prop, prop, tuple(g["t_srce"]), ((0, 0, 0),), mistake)[15, 0]
+        assert np.max(np.abs(bad - gold) / gold) > 1e-3, mistake

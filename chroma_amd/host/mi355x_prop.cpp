@@ -2,7 +2,8 @@
 // propagator task takes for an even-odd Wilson or clover action (quarkProp4_a with the qprop solver
 // object, quarkprop4_w.cc:31-117, eoprec_fermact_qprop.cc:41-80), with the propagator correlator of
 // inline_propagator_w.cc:383-395.  The gauge field is one of the library's synthetic fields.
-// Prints one JSON line: per-solve iteration counts and residuals, their sum (ncg_had) and prop_corr.
+// Prints one JSON line: per-solve iteration counts and residuals, their sum (ncg_had), prop_corr and the
+// zero-momentum point-sink mesons of the 16 gamma insertions (mesprop of mesons_w.cc:160-167).
 //
 //   mi355x_prop --nrow 4 4 4 8 --cfg unit --fermact WILSON --kappa 0.1 --boundary 1 1 1 1
 //               --twist 0.2 0.4 0.0 --solver CG --rsd 1e-8 --maxit 1000 [--batch]
@@ -195,12 +196,35 @@ int main(int argc, char** argv) {
     std::vector<SystemSolverResults_t> res;
     const int ncg_had = quarkProp4(prop, qprop, V, src_site, &res, batched);
     const std::vector<double> corr = propCorr(nrow, prop);
+    // point-sink mesons at zero momentum, time from the source (mesons_w.cc:97-180)
+    int origin[4];
+    {
+      const long Vh = V / 2, cb = src_site >= Vh ? 1 : 0;
+      long r = src_site - cb * Vh;
+      const int xh = (int)(r % (nrow[0] / 2));
+      r /= nrow[0] / 2;
+      origin[1] = (int)(r % nrow[1]);
+      r /= nrow[1];
+      origin[2] = (int)(r % nrow[2]);
+      origin[3] = (int)(r / nrow[2]);
+      origin[0] = 2 * xh + (int)((cb + origin[1] + origin[2] + origin[3]) & 1);
+    }
+    const int p0[3] = {0, 0, 0};
+    std::vector<double> mes((size_t)16 * nrow[3] * 2);
+    const double* pp = reinterpret_cast<const double*>(prop.data());
+    check(cmx_meson_2pt(nrow, pp, pp, origin, 1, p0, mes.data()), "cmx_meson_2pt");
     std::printf("{\"nrow\": [%d, %d, %d, %d], \"n_count\": [", nrow[0], nrow[1], nrow[2], nrow[3]);
     for (size_t k = 0; k < res.size(); ++k) std::printf("%s%d", k ? ", " : "", res[k].n_count);
     std::printf("], \"resid\": [");
     for (size_t k = 0; k < res.size(); ++k) std::printf("%s%.17g", k ? ", " : "", res[k].resid);
     std::printf("], \"ncg_had\": %d, \"prop_corr\": [", ncg_had);
     for (size_t t = 0; t < corr.size(); ++t) std::printf("%s%.17g", t ? ", " : "", corr[t]);
+    std::printf("], \"mesons\": [");
+    for (int n = 0; n < 16; ++n) {
+      std::printf("%s[", n ? ", " : "");
+      for (int t = 0; t < nrow[3]; ++t) std::printf("%s%.17g", t ? ", " : "", mes[((size_t)n * nrow[3] + t) * 2]);
+      std::printf("]");
+    }
     std::printf("]}\n");
     cmx_finalize();
   } catch (const SolverAbort& e) {

@@ -64,6 +64,10 @@ def test_cxx_host_matches_python_path():
     say("  cxx", out["n_count"], "python", [r["n_count"] for r in results])
     assert out["n_count"] == [r["n_count"] for r in results]
     assert np.max(np.abs(np.array(out["prop_corr"]) - corr) / corr) < 1e-12
+    # the driver's mesons: the pion is prop_corr (source at t = 0), all 16 equal the Python binding's
+    mes = np.array(out["mesons"])
+    assert np.max(np.abs(mes[15] - corr) / corr) < 1e-12
+    assert np.max(np.abs(mes - cmx.meson_2pt(L, prop)[:, 0].real)) < 1e-12 * np.max(np.abs(mes))
     # unpreconditioned residual of every solve, relative to the unit source
     assert max(out["resid"]) < 1e-8
 

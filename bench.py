@@ -51,21 +51,52 @@ def measured_traffic(key, lattice):
     return None, None
 
 
-def cpu_baseline(L, U, seconds_target=12.0):
-    """Oracle FP64 hopping term on the host cores (a bounded sample of the same workload)."""
+def cpu_baseline(L, U, mass, csw, gpu_cg_iterations=None, seconds_target=8.0):
+    """The oracle (FP64, OpenMP over the host cores) on bounded samples of the same workload: the hopping
+    term (the headline figure), the EO clover operator and one CG iteration on it, and the 8^3x16
+    unpreconditioned Wilson CGNE solved to 1e-10 in full."""
     import numpy as np
     import oracle
     ncores = len(os.sched_getaffinity(0))
     oracle.set_threads(ncores)
+    tag = "x".join(map(str, L))
+
+    def sample(op, what, target):
+        t1 = op.time(what, 1)  # one call to size the sample
+        n = max(1, int(target / max(t1, 1e-6)))
+        return n, op.time(what, n)
+
     op = oracle.Op(oracle.EO_WILSON, L, U, 0.0)  # the hop does not depend on the mass
-    t1 = op.time(0, 1)  # one call to size the sample
-    n = max(1, int(seconds_target / max(t1, 1e-6)))
-    t = op.time(0, n)
+    n, t = sample(op, 0, seconds_target)
+    del op
     Vh = int(np.prod(L)) // 2
     gbs = hop_bytes(0, 18) * Vh / t / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
-            "sample": f"{n} FP64 hopping-term applications at {'x'.join(map(str, L))} (oracle, {t * 1e3:.1f} ms each)",
-            "gflops": round(1320.0 * Vh / t / 1e9, 3)}
+    out = {"value": round(gbs, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
+           "sample": f"{n} FP64 hopping-term applications at {tag} (oracle, {t * 1e3:.1f} ms each)",
+           "gflops": round(1320.0 * Vh / t / 1e9, 3)}
+    t0 = time.time()
+    op = oracle.Op(oracle.EO_CLOVER, L, U, mass, csw, csw)
+    out["clover_build_seconds"] = round(time.time() - t0, 2)
+    n, t = sample(op, 1, seconds_target / 4)
+    out["eo_clover_op_ms"] = round(t * 1e3, 2)
+    n, t = sample(op, 2, seconds_target / 4)
+    out["cg_iteration_ms"] = round(t * 1e3, 2)
+    if gpu_cg_iterations:
+        out["plain_cg_solve_seconds_extrapolated"] = round(t * gpu_cg_iterations, 2)
+        out["extrapolation"] = (f"FP64 CG iteration time x the {gpu_cg_iterations} iterations of the GPU's plain "
+                                f"mixed CG on the same operator (not a measured CPU solve)")
+    del op
+    L1 = (8, 8, 8, 16)
+    from chroma_amd import lib as cmx
+    U1 = cmx.gauge_apply_bc(L1, cmx.gauge_generate(L1, kind=1, seed=1, eps=0.3), (1, 1, 1, -1))
+    op = oracle.Op(oracle.UNPREC_WILSON, L1, U1, cmx.kappa_to_mass(0.115))
+    chi = np.zeros((int(np.prod(L1)), 4, 3), np.complex128)
+    chi[0, 0, 0] = 1.0
+    t0 = time.time()
+    _, nc, resid, _ = op.solve(chi, solver=oracle.CG, rsd=1e-10, maxit=5000)
+    out["small_volume_cg"] = {"lattice": list(L1), "seconds": round(time.time() - t0, 4), "iterations": nc,
+                              "resid": resid}
+    return out
 
 
 def small_volume_solves(cmx):
@@ -383,7 +414,8 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(L, U)
+        cpu = cpu_baseline(L, U, cmx.kappa_to_mass(kappa), csw,
+                           (solves or {}).get("plain_mixed_cg_fp16", {}).get("iterations"))
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     traffic, traffic_src = measured_traffic("hop_fp16", L)

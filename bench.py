@@ -301,7 +301,36 @@ def learned_solve(cmx, gauge, L, kappa, csw, a):
     out["seconds"] = out["fgmres_learned"]["seconds"]
     out["speedup_vs_plain"] = round(plain["seconds"] / out["seconds"], 3)
     out["speedup_vs_fgmres_none"] = round(out["fgmres_none"]["seconds"] / out["seconds"], 3)
+    out["held_out"] = held_out_fields(cmx, L, mass, csw, net, kw)
     return out, net
+
+
+def held_out_fields(cmx, L, mass, csw, net, kw, seeds=(20, 21, 22, 23)):
+    """The trained network on four more fields of the recipe that training never saw (Haar + 6 stout
+    steps, seeds 20-23, the benchmark lattice and kappa): plain mixed CG against learned FGMRES-DR on a Z2
+    source, both to the FP64 true residual 1e-10."""
+    import numpy as np
+    Vh = int(np.prod(L)) // 2
+    rows = []
+    for seed in seeds:
+        U = cmx.gauge_apply_bc(L, stout_field(cmx, L, seed), (1, 1, 1, -1))
+        op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF,
+                          sloppy_recon=cmx.RECONS_12, autotune=True)
+        del U
+        cmx.net_attach(op, net)
+        chi = cmx.z2_source((Vh, 4, 3), seed)
+        for _ in range(2):
+            _, rp = op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=cmx.HALF,
+                             true_residual=True)
+        for _ in range(2):
+            _, rl = op.solve(chi, precond=cmx.PRECOND_LEARNED, **kw)
+        rows.append({"seed": seed, "plain_iterations": rp["n_count"], "plain_seconds": round(rp["solve_seconds"], 4),
+                     "learned_iterations": rl["n_count"], "learned_seconds": round(rl["solve_seconds"], 4),
+                     "learned_true_rel_resid": rl["rel_resid"],
+                     "speedup": round(rp["solve_seconds"] / rl["solve_seconds"], 3)})
+        print(f"# held-out field {seed}: plain {rp['n_count']} it {rp['solve_seconds']:.4f} s, learned "
+              f"{rl['n_count']} it {rl['solve_seconds']:.4f} s", file=sys.stderr, flush=True)
+    return rows
 
 
 def main():

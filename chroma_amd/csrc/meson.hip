@@ -84,16 +84,29 @@ std::vector<Term> meson_terms() {
   return t;
 }
 
-// one lane per (site, Gamma): c[n][site]
-__global__ void __launch_bounds__(256) k_meson_sites(long V, const double2* __restrict__ s1,
-                                                     const double2* __restrict__ s2, const Term* __restrict__ terms,
-                                                     double2* __restrict__ c) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 16 * V) return;
-  const long site = t >> 4;
-  const int n = (int)(t & 15);
-  const double2* p1 = s1 + site * 144;
-  const double2* p2 = s2 + site * 144;
+// MESON_SITES sites per block, one lane per (site, Gamma): the block first copies its sites' two
+// propagators (2 x 2304 B per site, contiguous) into LDS with coalesced loads, then each lane forms its
+// trace from LDS, so HBM sees each propagator once instead of the 16 re-reads a lane per (site, Gamma)
+// would make through the caches.  c[n][site]
+constexpr int MESON_SITES = 8;
+
+__global__ void __launch_bounds__(16 * MESON_SITES) k_meson_sites(long V, const double2* __restrict__ s1,
+                                                                  const double2* __restrict__ s2,
+                                                                  const Term* __restrict__ terms,
+                                                                  double2* __restrict__ c) {
+  __shared__ double2 sh1[MESON_SITES * 144], sh2[MESON_SITES * 144];
+  const long site0 = (long)blockIdx.x * MESON_SITES;
+  const int nsite = V - site0 < MESON_SITES ? (int)(V - site0) : MESON_SITES;
+  for (int q = threadIdx.x; q < nsite * 144; q += blockDim.x) {
+    sh1[q] = s1[site0 * 144 + q];
+    sh2[q] = s2[site0 * 144 + q];
+  }
+  __syncthreads();
+  const int ls = (int)threadIdx.x >> 4;
+  const int n = (int)threadIdx.x & 15;
+  if (ls >= nsite) return;
+  const double2* p1 = sh1 + ls * 144;
+  const double2* p2 = sh2 + ls * 144;
   double cr = 0, ci = 0;
   for (int k = 0; k < 16; ++k) {
     const Term tm = terms[n * 16 + k];
@@ -110,7 +123,7 @@ __global__ void __launch_bounds__(256) k_meson_sites(long V, const double2* __re
     cr += tm.coef.x * sr - tm.coef.y * si;
     ci += tm.coef.x * si + tm.coef.y * sr;
   }
-  c[(size_t)n * V + site] = make_double2(cr, ci);
+  c[(size_t)n * V + site0 + ls] = make_double2(cr, ci);
 }
 
 struct ProjGeom {
@@ -175,7 +188,7 @@ void meson_2pt(const int dims[4], const double* prop1, const double* prop2, cons
     CMX_REQUIRE(g.org[m] >= 0 && g.org[m] < dims[m], CMX_ERR_ARG, "meson_2pt: origin outside the lattice");
   }
   g.Vh = g.V / 2;
-  CMX_REQUIRE(16 * g.V <= (long)INT32_MAX * 256, CMX_ERR_ARG, "meson_2pt: lattice too large for one grid");
+  CMX_REQUIRE(g.V <= (long)INT32_MAX * MESON_SITES, CMX_ERR_ARG, "meson_2pt: lattice too large for one grid");
   const std::vector<Term> terms = meson_terms();
   const size_t pbytes = sizeof(double) * 288 * (size_t)g.V;
   DevBuf d1(pbytes), d2(pbytes), dc(sizeof(double2) * 16 * (size_t)g.V), dt(sizeof(Term) * terms.size()),
@@ -187,8 +200,8 @@ void meson_2pt(const int dims[4], const double* prop1, const double* prop2, cons
     CMX_HIP(hipMemcpy(d2.p, prop2, pbytes, hipMemcpyHostToDevice));
   CMX_HIP(hipMemcpy(dt.p, terms.data(), dt.bytes, hipMemcpyHostToDevice));
   CMX_HIP(hipMemcpy(dm.p, moms, dm.bytes, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_meson_sites, dim3((unsigned)((16 * g.V + 255) / 256)), dim3(256), 0, 0, g.V, d1.as<double2>(),
-                     d2.as<double2>(), dt.as<Term>(), dc.as<double2>());
+  hipLaunchKernelGGL(k_meson_sites, dim3((unsigned)((g.V + MESON_SITES - 1) / MESON_SITES)), dim3(16 * MESON_SITES), 0, 0,
+                     g.V, d1.as<double2>(), d2.as<double2>(), dt.as<Term>(), dc.as<double2>());
   CMX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_meson_project, dim3(g.L[3], nmom), dim3(256), 0, 0, g, dc.as<double2>(), dm.as<int>(), nmom,
                      dout.as<double2>());

@@ -8,9 +8,11 @@
 // product is re-projected onto SU(3) (rows 0 and 1 orthonormalised, row 2 = conj(row0 x row1)), which
 // moves it only at rounding level and keeps many steps from drifting off the group.
 //
-// Layout: the host gauge layout, [mu][site][3][3] complex with sites checkerboard-major and x fastest
-// within a checkerboard.  One lane per (mu, site); the 19 matrices a lane reads come mostly from L2
-// (each link is read by 13 lanes), so the kernel is bound by FP64 issue, about 10 kflop per link.
+// Layout: the host gauge layout ([mu][site][3][3] complex, sites checkerboard-major, x fastest within a
+// checkerboard) is transposed on the device into [mu][element][site], so that each of a lane's loads and
+// stores is one 16 B element and a wave's 64 lanes touch 1 KiB of consecutive sites; the steps run in
+// that layout and the result is transposed back.  One lane per (mu, site); each link is read by 13 lanes
+// (its own, and as a staple leg of 12 others), mostly from L2.
 #include <cmath>
 #include <string>
 #include <utility>
@@ -85,12 +87,29 @@ __device__ __forceinline__ long site_of(const StoutGeom& g, const int (&x)[4]) {
   return cb * g.Vh + ((((long)x[3] * g.L[2] + x[2]) * g.L[1] + x[1]) * g.L[0] + x[0]) / 2;
 }
 
+// [mu][element][site] layout on the device
 __device__ __forceinline__ M3 load(const double2* __restrict__ u, const StoutGeom& g, int mu, long s) {
-  const double2* p = u + ((size_t)mu * g.V + s) * 9;
+  const double2* p = u + (size_t)mu * 9 * g.V + s;
   M3 m;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) m.e[k] = p[k];
+  for (int k = 0; k < 9; ++k) m.e[k] = p[(size_t)k * g.V];
   return m;
+}
+
+// host layout [mu][site][9] <-> device layout [mu][9][site], one lane per link
+__global__ void __launch_bounds__(256) k_links_to_soa(long V, const double2* __restrict__ in, double2* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * V) return;
+  const long mu = t / V, s = t - mu * V;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) out[(mu * 9 + k) * V + s] = in[t * 9 + k];
+}
+__global__ void __launch_bounds__(256) k_links_from_soa(long V, const double2* __restrict__ in, double2* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * V) return;
+  const long mu = t / V, s = t - mu * V;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) out[t * 9 + k] = in[(mu * 9 + k) * V + s];
 }
 
 // exp(X) for a 3x3 X with |X|_F = O(1): X 2^-s with |X 2^-s|_F <= 1/4, Horner form of the degree-14
@@ -210,9 +229,9 @@ __global__ void __launch_bounds__(256) k_stout_step(StoutGeom g, const double2* 
   for (int i = 0; i < 3; ++i) X.e[4 * i] = make_double2(X.e[4 * i].x - tr.x, X.e[4 * i].y - tr.y);
   M3 N = mul(expm(X), U);
   reunit(N);
-  double2* o = next + ((size_t)mu * g.V + s) * 9;
+  double2* o = next + (size_t)mu * 9 * g.V + s;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) o[k] = N.e[k];
+  for (int k = 0; k < 9; ++k) o[(size_t)k * g.V] = N.e[k];
 }
 
 }  // namespace
@@ -235,14 +254,18 @@ void stout_smear(const int dims[4], double* links, int nstep, double rho) {
   DevBuf a(bytes), b(bytes);
   CMX_HIP(hipMemcpy(a.p, links, bytes, hipMemcpyHostToDevice));
   const dim3 grid((unsigned)((4 * g.V + 255) / 256));
-  double2* cur = a.as<double2>();
-  double2* nxt = b.as<double2>();
+  hipLaunchKernelGGL(k_links_to_soa, grid, dim3(256), 0, 0, g.V, a.as<double2>(), b.as<double2>());
+  CMX_HIP(hipGetLastError());
+  double2* cur = b.as<double2>();
+  double2* nxt = a.as<double2>();
   for (int k = 0; k < nstep; ++k) {
     hipLaunchKernelGGL(k_stout_step, grid, dim3(256), 0, 0, g, cur, nxt, rho);
     CMX_HIP(hipGetLastError());
     std::swap(cur, nxt);
   }
-  CMX_HIP(hipMemcpy(links, cur, bytes, hipMemcpyDeviceToHost));
+  hipLaunchKernelGGL(k_links_from_soa, grid, dim3(256), 0, 0, g.V, cur, nxt);
+  CMX_HIP(hipGetLastError());
+  CMX_HIP(hipMemcpy(links, nxt, bytes, hipMemcpyDeviceToHost));
 }
 
 }  // namespace cmx

@@ -197,7 +197,14 @@ int cmx_dslash_deriv(cmx_op h, double* ds_u, const double* chi, const double* ps
 int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho) {
   return guarded([&] {
     require_init();
-    cmx::stout_smear(dims, links, nstep, rho);
+    cmx::stout_smear(dims, links, nstep, rho, 15);
+  });
+}
+
+int cmx_gauge_stout_smear_dirs(const int dims[4], double* links, int nstep, double rho, int smear_dirs) {
+  return guarded([&] {
+    require_init();
+    cmx::stout_smear(dims, links, nstep, rho, smear_dirs);
   });
 }
 

@@ -172,8 +172,9 @@ void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, b
 // WilsonDslashBase::deriv (lwldslash_base_w.h:132-227) for the sites of cb, FP64 (deriv.hip): chi on cb,
 // psi on 1 - cb, ds = 4 * Vh * 9 double2 (direction, site of cb, 3x3 row-major)
 void hop_deriv(const Geom& g, int cb, int isign, VRef chi, VRef psi, void* ds, hipStream_t s);
-// nstep isotropic stout steps on the device, in place on host-layout links (stout.hip)
-void stout_smear(const int dims[4], double* links, int nstep, double rho);
+// nstep stout steps on the device over the directions in the bit mask dirs, in place on host-layout
+// links (stout.hip)
+void stout_smear(const int dims[4], double* links, int nstep, double rho, int dirs = 15);
 // meson two-point functions of two host-layout propagators for the 16 gamma insertions (meson.hip)
 void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
                const int* moms, double* corr);

@@ -1,5 +1,7 @@
-// Isotropic stout smearing on the device, in FP64: Stouting::smear_links (stout_utils.cc:936-962) with
-// every direction smeared and rho(mu,nu) = rho, the staple sum C and Q of getQsandCs (:105-180):
+// Stout smearing on the device, in FP64: Stouting::smear_links (stout_utils.cc:936-962) with the
+// parameters of StoutLinkSmearing (stout_link_smearing.cc:62-120): a set of smeared directions and
+// rho(mu,nu) = rho for mu != nu both smeared, else 0.  Links of an unsmeared direction are copied, and
+// only smeared directions contribute staples; the staple sum C and Q of getQsandCs (:105-180):
 //   C_mu(x) = rho sum_{nu != mu} [ U_nu(x) U_mu(x+nu) U_nu^dag(x+mu) + U_nu^dag(x-nu) U_mu(x-nu) U_nu(x-nu+mu) ],
 //   Omega = C U_mu^dag(x),  Q = (i/2) [ (Omega^dag - Omega) - tr(Omega^dag - Omega)/3 ],  U'_mu(x) = exp(iQ) U_mu(x).
 // The reference evaluates exp(iQ) by Cayley-Hamilton (getFsAndBs, :383-874); here it is a degree-14
@@ -174,11 +176,18 @@ __device__ void reunit(M3& U) {
 }
 
 __global__ void __launch_bounds__(256) k_stout_step(StoutGeom g, const double2* __restrict__ cur,
-                                                    double2* __restrict__ next, double rho) {
+                                                    double2* __restrict__ next, double rho, int dirs) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * g.V) return;
   const int mu = (int)(t / g.V);
   const long s = t - (long)mu * g.V;
+  if (!(dirs >> mu & 1)) {  // next[mu] = current[mu]
+    const double2* p = cur + (size_t)mu * 9 * g.V + s;
+    double2* o = next + (size_t)mu * 9 * g.V + s;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[(size_t)k * g.V] = p[(size_t)k * g.V];
+    return;
+  }
   int x[4];
   {
     const int cb = s >= g.Vh ? 1 : 0;
@@ -196,7 +205,7 @@ __global__ void __launch_bounds__(256) k_stout_step(StoutGeom g, const double2* 
 #pragma unroll
   for (int k = 0; k < 9; ++k) C.e[k] = make_double2(0, 0);
   for (int nu = 0; nu < 4; ++nu) {
-    if (nu == mu) continue;
+    if (nu == mu || !(dirs >> nu & 1)) continue;
     int y[4] = {x[0], x[1], x[2], x[3]};
     y[mu] = x[mu] + 1 == g.L[mu] ? 0 : x[mu] + 1;
     const long xpm = site_of(g, y);
@@ -236,8 +245,9 @@ __global__ void __launch_bounds__(256) k_stout_step(StoutGeom g, const double2* 
 
 }  // namespace
 
-void stout_smear(const int dims[4], double* links, int nstep, double rho) {
+void stout_smear(const int dims[4], double* links, int nstep, double rho, int dirs) {
   CMX_REQUIRE(dims && links, CMX_ERR_ARG, "gauge_stout_smear: null array");
+  CMX_REQUIRE(dirs >= 0 && dirs <= 15, CMX_ERR_ARG, "gauge_stout_smear: smeared directions are a 4-bit mask");
   CMX_REQUIRE(nstep >= 0, CMX_ERR_ARG, "gauge_stout_smear: nstep must be >= 0");
   CMX_REQUIRE(std::isfinite(rho), CMX_ERR_ARG, "gauge_stout_smear: rho must be finite");
   StoutGeom g;
@@ -259,7 +269,7 @@ void stout_smear(const int dims[4], double* links, int nstep, double rho) {
   double2* cur = b.as<double2>();
   double2* nxt = a.as<double2>();
   for (int k = 0; k < nstep; ++k) {
-    hipLaunchKernelGGL(k_stout_step, grid, dim3(256), 0, 0, g, cur, nxt, rho);
+    hipLaunchKernelGGL(k_stout_step, grid, dim3(256), 0, 0, g, cur, nxt, rho, dirs);
     CMX_HIP(hipGetLastError());
     std::swap(cur, nxt);
   }

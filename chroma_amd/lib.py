@@ -134,6 +134,7 @@ def lib():
         "cmx_gauge_apply_bc": (ctypes.c_int, [_ip, _dp, _ip, _dp, _ip]),
         "cmx_plaquette": (ctypes.c_double, [_ip, _dp]),
         "cmx_gauge_stout_smear": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double]),
+        "cmx_gauge_stout_smear_dirs": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
         "cmx_meson_2pt": (ctypes.c_int, [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]),
         "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
     }
@@ -190,12 +191,15 @@ def gauge_apply_bc(L, U, boundary=(1, 1, 1, 1), twist_by_pi=None, twist_dir=(0, 
     return U
 
 
-def gauge_stout_smear(L, U, nstep, rho):
-    """nstep isotropic stout steps on the device (Stouting::smear_links, stout_utils.cc:936-962); returns a
-    new complex128[4, V, 3, 3]."""
+def gauge_stout_smear(L, U, nstep, rho, smear_dirs=15):
+    """nstep stout steps on the device (Stouting::smear_links, stout_utils.cc:936-962) over the directions
+    whose bit is set in smear_dirs (15: all, 7: spatial); returns a new complex128[4, V, 3, 3]."""
     init()
     out = np.array(U, np.complex128, order="C", copy=True)
-    check(lib().cmx_gauge_stout_smear(_dims(L), _d(out), int(nstep), float(rho)))
+    if smear_dirs == 15:
+        check(lib().cmx_gauge_stout_smear(_dims(L), _d(out), int(nstep), float(rho)))
+    else:
+        check(lib().cmx_gauge_stout_smear_dirs(_dims(L), _d(out), int(nstep), float(rho), int(smear_dirs)))
     return out
 
 

@@ -328,6 +328,11 @@ int cmx_gauge_apply_bc(const int dims[4], double* links, const int boundary[4], 
  * getQsandCs (:105-180), exp(iQ) to rounding, each new link re-projected onto SU(3) at rounding level.
  * links: 4*V*18 doubles (host layout).  Needs cmx_init; FP64 throughout. */
 int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho);
+/* The same over a subset of directions, StoutLinkSmearing's smear_dirs / no_smear_dir
+ * (stout_link_smearing.cc:62-120): bit mu of smear_dirs set = direction mu smeared.  Links of the other
+ * directions are left as they are and contribute no staples (rho(mu,nu) = 0 when either is unsmeared);
+ * 15 is cmx_gauge_stout_smear, 7 the usual spatial smearing with no_smear_dir = 3. */
+int cmx_gauge_stout_smear_dirs(const int dims[4], double* links, int nstep, double rho, int smear_dirs);
 /* Meson two-point functions on the device: mesons() (mesons_w.cc:97-180) for the 16 insertions
  * Gamma(n) = gamma_1^n0 gamma_2^n1 gamma_3^n2 gamma_4^n3 with the sink-momentum projection of
  * SftMom::sft (sftmom.cc:405-460), decay direction 3:

@@ -45,7 +45,7 @@ def lib():
         L.orc_dslash_deriv.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_dslash_f32.argtypes = [_ip, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.orc_mesfield.argtypes = [_ip, _dp, _dp]
-        L.orc_stout.argtypes = [_ip, _dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.orc_stout.argtypes = [_ip, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_mesons.argtypes = [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]
         L.orc_make_clover.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_clover_invert.argtypes = [_ip, _dp, ctypes.c_int, _dp, _lp]
@@ -114,11 +114,12 @@ def dslash(L, U, psi, isign, cb, out=None):
     return chi
 
 
-def stout(L, U, rho, nstep=1):
-    """nstep isotropic stout steps (Stouting::smear_links, stout_utils.cc:936-962): complex128[4, V, 3, 3]."""
+def stout(L, U, rho, nstep=1, dirs=15):
+    """nstep stout steps (Stouting::smear_links, stout_utils.cc:936-962) over the directions in the bit
+    mask dirs: complex128[4, V, 3, 3]."""
     U = np.ascontiguousarray(U, np.complex128)
     out = np.empty_like(U)
-    _check(lib().orc_stout(_dims(L), _d(U), _d(out), float(rho), int(nstep)))
+    _check(lib().orc_stout(_dims(L), _d(U), _d(out), float(rho), int(nstep), int(dirs)))
     return out
 
 

@@ -258,15 +258,21 @@ static void exp_taylor(cd* E, const cd* X) {
   throw std::runtime_error("stout: exp series did not converge");
 }
 
-void stout_step(const Geom& g, const cd* U, cd* out, double rho) {
+// dirs: bit mu set = direction mu smeared (smear_in_this_dirP); an unsmeared direction is copied and
+// gives no staples (smear_links :945-958, getQsandCs :120).
+void stout_step(const Geom& g, const cd* U, cd* out, double rho, int dirs) {
   const long V = g.V;
 #pragma omp parallel for schedule(static)
   for (long t = 0; t < 4 * V; ++t) {
     const int mu = (int)(t / V);
     const long s = t % V;
+    if (!(dirs >> mu & 1)) {
+      for (int k = 0; k < 9; ++k) out[t * 9 + k] = U[t * 9 + k];
+      continue;
+    }
     cd C[9] = {}, a[9], b[9];
     for (int nu = 0; nu < 4; ++nu) {
-      if (nu == mu) continue;
+      if (nu == mu || !(dirs >> nu & 1)) continue;
       // forward staple
       mm(a, link(U, V, nu, s), link(U, V, mu, g.fwd(s, nu)));
       mma(b, a, link(U, V, nu, g.fwd(s, mu)));
@@ -1193,13 +1199,13 @@ int orc_mesons(const int L[4], const double* S1, const double* S2, const int ori
 }
 
 // nstep stout steps; U and out are [4][V][3][3] (may alias)
-int orc_stout(const int L[4], const double* U, double* out, double rho, int nstep) {
+int orc_stout(const int L[4], const double* U, double* out, double rho, int nstep, int dirs) {
   return guard([&] {
     Geom g(L);
     const size_t n = (size_t)4 * g.V * 9;
     std::vector<cd> a(reinterpret_cast<const cd*>(U), reinterpret_cast<const cd*>(U) + n), b(n);
     for (int k = 0; k < nstep; ++k) {
-      stout_step(g, a.data(), b.data(), rho);
+      stout_step(g, a.data(), b.data(), rho, dirs);
       a.swap(b);
     }
     std::memcpy(out, a.data(), sizeof(cd) * n);

@@ -476,7 +476,7 @@ def _one_link_field(L):
     return U
 
 
-def _one_link_closed_form(L, rho):
+def _one_link_closed_form(L, rho, dirs=15):
     """One stout step of _one_link_field by hand.  Every link stays diagonal, so exp(iQ) is a phase per
     colour.  For U_mu0(x0) all six staples are 1: C = 6 rho, Omega = 6 rho D^dag, and
     U' = exp(i (eps h - 6 rho (sin(eps h) - mean)));  a link with one staple equal to D (or D^dag) and
@@ -493,10 +493,11 @@ def _one_link_closed_form(L, rho):
             x[m] = (x[m] + d) % L[m]
         U[mu, index_of(L, x)] = np.diag(np.exp(1j * sign * ph))
 
-    U[mu0, index_of(L, x0)] = np.diag(np.exp(1j * (STOUT_EPS * STOUT_H - 6 * ph)))
-    for nu in range(4):
-        if nu == mu0:
-            continue
+    # with only the directions in `dirs` smeared, U_mu0(x0) has 2 staples per other smeared direction and
+    # links of unsmeared directions stay 1 (StoutLinkSmearing smear_dirs, stout_link_smearing.cc:62-120)
+    others = [nu for nu in range(4) if nu != mu0 and dirs >> nu & 1]
+    U[mu0, index_of(L, x0)] = np.diag(np.exp(1j * (STOUT_EPS * STOUT_H - 2 * len(others) * ph)))
+    for nu in others:
         at(nu, [], +1)
         at(nu, [(mu0, 1), (nu, -1)], +1)
         at(mu0, [(nu, 1)], +1)
@@ -506,11 +507,12 @@ def _one_link_closed_form(L, rho):
     return U
 
 
-@pytest.mark.parametrize("rho", [0.1, 0.22])
-def test_stout_one_link_closed_form(rho):
+@pytest.mark.parametrize("rho,dirs", [(0.1, 15), (0.22, 15), (0.1, 7), (0.15, 0b1010)])
+def test_stout_one_link_closed_form(rho, dirs):
+    """All directions, spatial only (no_smear_dir = 3), and mu0 with t only."""
     L = (4, 4, 4, 4)
-    out = oracle.stout(L, _one_link_field(L), rho)
-    assert np.max(np.abs(out - _one_link_closed_form(L, rho))) < 1e-14
+    out = oracle.stout(L, _one_link_field(L), rho, 1, dirs)
+    assert np.max(np.abs(out - _one_link_closed_form(L, rho, dirs))) < 1e-14
 
 
 def test_stout_gauge_covariance_unitarity_and_rho_zero():

@@ -21,6 +21,7 @@ namespace {
 void usage() {
   std::printf(
       "usage: mi355x_prop [--nrow Lx Ly Lz Lt] [--cfg unit|weak|haar] [--seed N] [--eps E] [--nstout N --rho R]\n"
+      "                   [--smear N R DIRS]   (then N stout steps on the device, rho R, direction bit mask DIRS)\n"
       "                   [--boundary bx by bz bt] [--twist px py pz] [--fermact WILSON|CLOVER] [--kappa K]\n"
       "                   [--csw C] [--solver CG|BICGSTAB|RELIABLE_CG|RELIABLE_BICGSTAB] [--rsd R] [--maxit N]\n"
       "                   [--sloppy DOUBLE|SINGLE|HALF] [--delta D] [--true-residual] [--src-site S] [--device D]\n"
@@ -43,6 +44,8 @@ int main(int argc, char** argv) {
   bool batched = false;
   std::vector<double> shifts;  // --multishift: the RHMC solves on the odd part of the first point source
   int chrono = 0;              // --chrono N: a sequence of M^dag M solves with the MRE predictor
+  int smear_n = 0, smear_dirs = 15;  // --smear N R DIRS: StoutLinkSmearing on the device
+  double smear_rho = 0.0;
   for (int a = 1; a < argc; ++a) {
     const std::string k = argv[a];
     auto need = [&](int n) {
@@ -103,6 +106,11 @@ int main(int argc, char** argv) {
     } else if (k == "--delta") {
       need(1);
       sp.Delta = std::atof(argv[++a]);
+    } else if (k == "--smear") {
+      need(3);
+      smear_n = std::atoi(argv[++a]);
+      smear_rho = std::atof(argv[++a]);
+      smear_dirs = std::atoi(argv[++a]);
     } else if (k == "--batch") {
       batched = true;
     } else if (k == "--chrono") {
@@ -135,6 +143,8 @@ int main(int argc, char** argv) {
     check(cmx_init(device), "cmx_init");
     std::vector<double> links((size_t)4 * V * 18);
     check(cmx_gauge_generate(nrow, kind, seed, eps, nstout, rho, links.data()), "cmx_gauge_generate");
+    if (smear_n > 0)
+      check(cmx_gauge_stout_smear_dirs(nrow, links.data(), smear_n, smear_rho, smear_dirs), "cmx_gauge_stout_smear_dirs");
     const int tdir[3] = {0, 1, 2};
     check(cmx_gauge_apply_bc(nrow, links.data(), boundary, twisted ? twist : nullptr, twisted ? tdir : nullptr),
           "cmx_gauge_apply_bc");

@@ -226,7 +226,7 @@ def train_net(cmx, mass, csw, a):
     return net, train
 
 
-def north_star_sources(cmx, L, kappa, csw, a, net=None):
+def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None):
     """The 12 spin-colour point sources of a propagator on a near-critical clover operator (Haar + 6 stout
     steps, seed 1), all to the FP64 true residual 1e-10: plain mixed-precision CG one after another and
     batched, against the learned FGMRES-DR one after another."""
@@ -264,7 +264,7 @@ def north_star_sources(cmx, L, kappa, csw, a, net=None):
                                          "worst_true_rel_resid": max(r["rel_resid"] for r in learned)},
            "speedup_vs_plain": round(ps / ls, 3), "speedup_vs_plain_batched": round(bsecs / ls, 3),
            "wall_seconds": round(time.time() - t0, 1)}
-    print(f"# north star {L}: {out}", file=sys.stderr, flush=True)
+    print(f"# 12 sources {L}: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -363,11 +363,10 @@ def main():
     ap.add_argument("--sources-lattice", type=str, default="48,48,48,96")
     ap.add_argument("--batch-group", type=int, default=12,
                     help="sources per batched solve of the 12 (6 fits 64^3x128 in HBM; 12 does not)")
-    ap.add_argument("--no-north-star", action="store_true",
+    ap.add_argument("--no-sources-comparison", action="store_true",
                     help="skip the 12 point sources at --sources-lattice, kappa 0.1355: plain mixed CG "
                          "(sequential, batched) vs learned FGMRES-DR with the network trained for the learned solve "
                          "(about 30 s)")
-    ap.add_argument("--north-star", action="store_true", help="(default; kept for old command lines)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -456,9 +455,9 @@ def main():
                    "propagator_sources": batched_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")),
                                                       a.batch_group)}
 
-    north = None
-    if learned is not None and not a.no_north_star:
-        north = north_star_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")), 0.1355, csw, a, net)
+    sources_cmp = None
+    if learned is not None and not a.no_sources_comparison:
+        sources_cmp = learned_vs_plain_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")), 0.1355, csw, a, net)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -493,7 +492,7 @@ def main():
         "learned": learned,
         "small_volume": small,
         "batched": batched,
-        "north_star_12_sources": north,
+        "sources_12_learned_vs_plain": sources_cmp,
         "solves": solves,
         "kernels": extra,
         "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}

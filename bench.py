@@ -51,35 +51,45 @@ def measured_traffic(key, lattice):
     return None, None
 
 
-def cpu_baseline(L, U, mass, csw, gpu_cg_iterations=None, seconds_target=8.0):
-    """The oracle (FP64, OpenMP over the host cores) on bounded samples of the same workload: the hopping
-    term (the headline figure), the EO clover operator and one CG iteration on it, and the 8^3x16
-    unpreconditioned Wilson CGNE solved to 1e-10 in full."""
+def _sample(op, what, target):
+    t1 = op.time(what, 1)  # one call to size the sample
+    n = max(1, int(target / max(t1, 1e-6)))
+    return n, op.time(what, n)
+
+
+def cpu_hop_baseline(L, U, seconds_target=8.0):
+    """The oracle's FP64 hopping term (OpenMP over all host cores) on a bounded sample of the headline
+    workload: the lattice and field of the GPU figure, FP64/18 algorithmic bytes (1536 per site)."""
     import numpy as np
     import oracle
     ncores = len(os.sched_getaffinity(0))
     oracle.set_threads(ncores)
     tag = "x".join(map(str, L))
-
-    def sample(op, what, target):
-        t1 = op.time(what, 1)  # one call to size the sample
-        n = max(1, int(target / max(t1, 1e-6)))
-        return n, op.time(what, n)
-
     op = oracle.Op(oracle.EO_WILSON, L, U, 0.0)  # the hop does not depend on the mass
-    n, t = sample(op, 0, seconds_target)
+    n, t = _sample(op, 0, seconds_target)
     del op
     Vh = int(np.prod(L)) // 2
     gbs = hop_bytes(0, 18) * Vh / t / 1e9
-    out = {"value": round(gbs, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
-           "sample": f"{n} FP64 hopping-term applications at {tag} (oracle, {t * 1e3:.1f} ms each)",
-           "gflops": round(1320.0 * Vh / t / 1e9, 3)}
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
+            "sample": f"{n} FP64 hopping-term applications at {tag} (oracle, {t * 1e3:.1f} ms each)",
+            "gflops": round(1320.0 * Vh / t / 1e9, 3)}
+
+
+def cpu_solve_baseline(L, U, mass, csw, gpu_cg_iterations=None, seconds_target=8.0):
+    """The oracle (FP64, OpenMP over the host cores) on the solve lattice: the clover build, the EO clover
+    operator and one CG iteration on it, and the 8^3x16 unpreconditioned Wilson CGNE solved to 1e-10 in
+    full."""
+    import numpy as np
+    import oracle
+    ncores = len(os.sched_getaffinity(0))
+    oracle.set_threads(ncores)
+    out = {"lattice": list(L), "cores": ncores}
     t0 = time.time()
     op = oracle.Op(oracle.EO_CLOVER, L, U, mass, csw, csw)
     out["clover_build_seconds"] = round(time.time() - t0, 2)
-    n, t = sample(op, 1, seconds_target / 4)
+    n, t = _sample(op, 1, seconds_target / 4)
     out["eo_clover_op_ms"] = round(t * 1e3, 2)
-    n, t = sample(op, 2, seconds_target / 4)
+    n, t = _sample(op, 2, seconds_target / 4)
     out["cg_iteration_ms"] = round(t * 1e3, 2)
     if gpu_cg_iterations:
         out["plain_cg_solve_seconds_extrapolated"] = round(t * gpu_cg_iterations, 2)
@@ -345,12 +355,53 @@ def held_out_fields(cmx, L, mass, csw, net, kw, seeds=(20, 21, 22, 23)):
     return rows
 
 
+def hop_kernels(cmx, op, Vh, steps, multi=True):
+    """FP32/FP64 hopping terms, the fused EO clover operator at every precision, the streaming copy and
+    (multi) the 12-source FP16 hop on one operator, each timed over `steps` launches with HIP events."""
+    extra = {}
+    r64 = op.link_recon(cmx.DOUBLE)  # 12 when the links are SU(3) to rounding
+    rows = [
+        ("hop_fp32_recon12", 0, cmx.SINGLE, hop_bytes(1, 12) * Vh),
+        (f"hop_fp64_recon{r64}", 0, cmx.DOUBLE, hop_bytes(0, r64) * Vh),
+        ("eo_clover_op_fp16", 1, cmx.HALF, eo_clover_bytes(2, 12) * Vh),
+        ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
+        ("eo_clover_op_fp64", 1, cmx.DOUBLE, eo_clover_bytes(0, r64) * Vh),
+        ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
+    ]
+    if multi:
+        # 12 sources: each spinor read and written once per source, each link once
+        rows.append(("hop_fp16_recon12_12src", 4, cmx.HALF, (12 * (48 + 4 + 48 + 4) + 8 * 12 * 2) * Vh))
+    for name, what, prec, nbytes in rows:
+        op.time(what, prec, 5)
+        t = op.time(what, prec, steps)
+        extra[name] = {"us": round(t * 1e3, 2), "GB/s": round(nbytes / (t * 1e-3) / 1e9, 1)}
+        print(f"# {name}: {t * 1e3:.1f} us  {extra[name]['GB/s']} GB/s", file=sys.stderr, flush=True)
+    return extra
+
+
+def fp16_hop(cmx, op, Vh, steps, warmup):
+    """The FP16/12 hopping term: `warmup` untimed launches, then exactly `steps` timed ones (HIP events on
+    the operator's stream); returns (ms per launch, algorithmic GB/s, record)."""
+    op.time(0, cmx.HALF, max(1, warmup))
+    ms = op.time(0, cmx.HALF, steps)
+    gbs = hop_bytes(cmx.HALF, 12) * Vh / (ms * 1e-3) / 1e9
+    # the no-reuse figure counts all 8 neighbour spinors and their norms, 8 links and the output per site
+    # (the QUDA-style accounting; not the graded one)
+    no_reuse = (8 * (48 + 4) + 8 * 12 * 2 + 48 + 4) * Vh / (ms * 1e-3) / 1e9
+    rec = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1), "GB/s_no_reuse": round(no_reuse, 1),
+           "gflops": round(1320.0 * Vh / (ms * 1e-3) / 1e9, 1)}
+    return ms, gbs, rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--lattice", type=str, default="32,32,32,64")
+    ap.add_argument("--lattice", type=str, default="48,48,48,96",
+                    help="lattice of the headline FP16 hopping term (the 12-source propagator lattice)")
+    ap.add_argument("--solve-lattice", type=str, default="32,32,32,64",
+                    help="lattice of the EO clover solves (kappa 0.1350 mixed CG, the learned preconditioner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -367,6 +418,8 @@ def main():
                     help="skip the 12 point sources at --sources-lattice, kappa 0.1355: plain mixed CG "
                          "(sequential, batched) vs learned FGMRES-DR with the network trained for the learned solve "
                          "(about 30 s)")
+    ap.add_argument("--no-solve-lattice-kernels", action="store_true",
+                    help="skip the kernel table of the solve lattice")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -376,17 +429,15 @@ def main():
     import numpy as np
     from chroma_amd import lib as cmx
 
-    L = tuple(int(v) for v in a.lattice.split(","))
-    V = int(np.prod(L))
-    Vh = V // 2
     kappa, csw = 0.1350, 1.0
+    # ---- headline: the FP16 hopping term on the 48^3x96 lattice -------------------------------------
+    L = tuple(int(v) for v in a.lattice.split(","))
+    Vh = int(np.prod(L)) // 2
     t0 = time.time()
     U = stout_field(cmx, L, 1)
     plaq = cmx.plaquette(L, U)
     U = cmx.gauge_apply_bc(L, U, (1, 1, 1, -1))
-    t_gen = time.time() - t0
-    print(f"# gauge {L}: plaquette {plaq:.6f} ({t_gen:.1f} s)", file=sys.stderr, flush=True)
-
+    print(f"# gauge {L}: plaquette {plaq:.6f} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     gauge = cmx.Gauge(L, U)
     t0 = time.time()
     op = cmx.Operator(gauge, cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), csw, sloppy_prec=cmx.HALF,
@@ -395,51 +446,58 @@ def main():
         op.set_hop_config(cmx.HALF, *[int(v) for v in a.hop_shape.split(",")])
     print(f"# operator built and launch shapes tuned ({time.time() - t0:.1f} s): "
           f"{ {p: (op.hop_config(p, 1), op.hop_config(p, 2)) for p in (0, 1, 2)} }", file=sys.stderr, flush=True)
-    # warm-up, then exactly `steps` timed applications (HIP events on the operator's stream)
-    op.time(0, cmx.HALF, max(1, a.warmup))
-    ms = op.time(0, cmx.HALF, a.steps)
-    b16 = hop_bytes(cmx.HALF, 12) * Vh
-    gbs = b16 / (ms * 1e-3) / 1e9
-    print(f"# hop FP16/12: {ms * 1e3:.1f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    ms, gbs, hop16 = fp16_hop(cmx, op, Vh, a.steps, a.warmup)
+    print(f"# hop FP16/12 {L}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    extra = hop_kernels(cmx, op, Vh, a.steps, multi=False)
+    extra["hop_fp16_recon12"] = hop16
+    shapes = {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}
+              for name, p in (("fp64", 0), ("fp32", 1), ("fp16", 2))}
+    del op, gauge
+    cpu_hop = None
+    if not a.no_cpu_baseline:
+        cpu_hop = cpu_hop_baseline(L, U)
+        print(f"# cpu baseline hop: {cpu_hop}", file=sys.stderr, flush=True)
+    del U
 
-    extra = {}
-    r64 = op.link_recon(cmx.DOUBLE)  # 12 when the links are SU(3) to rounding
-    for name, what, prec, nbytes in [
-        ("hop_fp32_recon12", 0, cmx.SINGLE, hop_bytes(1, 12) * Vh),
-        (f"hop_fp64_recon{r64}", 0, cmx.DOUBLE, hop_bytes(0, r64) * Vh),
-        ("eo_clover_op_fp16", 1, cmx.HALF, eo_clover_bytes(2, 12) * Vh),
-        ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
-        ("eo_clover_op_fp64", 1, cmx.DOUBLE, eo_clover_bytes(0, r64) * Vh),
-        ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
-        # 12 sources: each spinor read and written once per source, each link once
-        ("hop_fp16_recon12_12src", 4, cmx.HALF, (12 * (48 + 4 + 48 + 4) + 8 * 12 * 2) * Vh),
-    ]:
-        op.time(what, prec, 5)
-        t = op.time(what, prec, a.steps)
-        extra[name] = {"us": round(t * 1e3, 2), "GB/s": round(nbytes / (t * 1e-3) / 1e9, 1)}
-        print(f"# {name}: {t * 1e3:.1f} us  {extra[name]['GB/s']} GB/s", file=sys.stderr, flush=True)
-    # the no-reuse figure counts all 8 neighbour spinors and their norms, 8 links and the output per site
-    # (the QUDA-style accounting; not the graded one)
-    no_reuse = (8 * (48 + 4) + 8 * 12 * 2 + 48 + 4) * Vh / (ms * 1e-3) / 1e9
-    extra["hop_fp16_recon12"] = {"us": round(ms * 1e3, 2), "GB/s": round(gbs, 1),
-                                 "GB/s_no_reuse": round(no_reuse, 1)}
-    m12 = extra["hop_fp16_recon12_12src"]
-    m12["us_per_source"] = round(m12["us"] / 12, 2)
-    m12["speedup_vs_12_single"] = round(12 * ms * 1e3 / m12["us"], 3)
-    extra["hop_gflops_fp16"] = round(1320.0 * Vh / (ms * 1e-3) / 1e9, 1)
+    # ---- the solve lattice: kernels, mixed CG, the learned preconditioner ----------------------------
+    LS = tuple(int(v) for v in a.solve_lattice.split(","))
+    VhS = int(np.prod(LS)) // 2
+    US = cmx.gauge_apply_bc(LS, stout_field(cmx, LS, 1), (1, 1, 1, -1))
+    plaq_s = cmx.plaquette(LS, US)
+    gauge_s = cmx.Gauge(LS, US)
+    op_s = cmx.Operator(gauge_s, cmx.EO_CLOVER, cmx.kappa_to_mass(kappa), csw, sloppy_prec=cmx.HALF,
+                        sloppy_recon=cmx.RECONS_12, autotune=True)
+    if a.hop_shape:
+        op_s.set_hop_config(cmx.HALF, *[int(v) for v in a.hop_shape.split(",")])
+    kernels_s = None
+    if not a.no_solve_lattice_kernels:
+        ms_s, gbs_s, hop16_s = fp16_hop(cmx, op_s, VhS, a.steps, a.warmup)
+        print(f"# hop FP16/12 {LS}: {ms_s * 1e3:.1f} us, {gbs_s:.0f} GB/s", file=sys.stderr, flush=True)
+        kernels_s = hop_kernels(cmx, op_s, VhS, a.steps, multi=True)
+        kernels_s["hop_fp16_recon12"] = hop16_s
+        m12 = kernels_s["hop_fp16_recon12_12src"]
+        m12["us_per_source"] = round(m12["us"] / 12, 2)
+        m12["speedup_vs_12_single"] = round(12 * ms_s * 1e3 / m12["us"], 3)
+        kernels_s["lattice"] = list(LS)
+        kernels_s["launch_shapes"] = {name: {"operator": op_s.hop_config(p, 1), "hop": op_s.hop_config(p, 2)}
+                                      for name, p in (("fp64", 0), ("fp32", 1), ("fp16", 2))}
 
     solves = {}
     if not a.no_solve:
-        chi = cmx.z2_source((Vh, 4, 3), 2)
+        chi = cmx.z2_source((VhS, 4, 3), 2)
         for name, sp in [("plain_mixed_cg_fp16", cmx.HALF), ("plain_mixed_cg_fp32", cmx.SINGLE)]:
             for _ in range(2):  # the first solve allocates the solver's vectors; the second is reported
-                psi, res = op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1,
-                                    sloppy_prec=sp, true_residual=True)
+                psi, res = op_s.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1,
+                                      sloppy_prec=sp, true_residual=True)
             solves[name] = {"seconds": round(res["solve_seconds"], 4), "iterations": res["n_count"],
                             "true_rel_resid": res["rel_resid"], "reliable_updates": res["n_reliable"],
                             "restarts": res["n_restarts"], "gflops": round(res["gflops"], 1)}
             print(f"# solve {name}: {res['n_count']} it, {res['solve_seconds']:.3f} s, rel resid "
                   f"{res['rel_resid']:.2e}", file=sys.stderr, flush=True)
+        solves["lattice"] = list(LS)
+        solves["kappa"] = kappa
+        solves["plaquette"] = round(plaq_s, 6)
+    del op_s
 
     small = None
     if not a.no_solve:
@@ -447,7 +505,7 @@ def main():
 
     learned = None
     if not a.no_solve and not a.no_learned:
-        learned, net = learned_solve(cmx, gauge, L, 0.1355, csw, a)
+        learned, net = learned_solve(cmx, gauge_s, LS, 0.1355, csw, a)
 
     batched = None
     if not a.no_solve and not a.no_batch:
@@ -461,8 +519,9 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(L, U, cmx.kappa_to_mass(kappa), csw,
-                           (solves or {}).get("plain_mixed_cg_fp16", {}).get("iterations"))
+        cpu = dict(cpu_hop)
+        cpu["solve_lattice"] = cpu_solve_baseline(LS, US, cmx.kappa_to_mass(kappa), csw,
+                                                  (solves or {}).get("plain_mixed_cg_fp16", {}).get("iterations"))
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     traffic, traffic_src = measured_traffic("hop_fp16", L)
@@ -479,7 +538,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f16-storage/f32",
         "data": "synthetic (Haar + 6 stout steps, seeded)",
-        "config": {"workload": "EO Wilson-clover hopping term D_eo, 32^3x64, stout-smeared Haar field",
+        "config": {"workload": "EO Wilson-clover hopping term D_eo, " + "x".join(map(str, L)) +
+                               ", stout-smeared Haar field (the 12-source propagator lattice)",
                    "lattice": list(L), "kappa": kappa, "csw": csw, "plaquette": round(plaq, 6),
                    "storage": "FP16 spinors + per-site FP32 norm, 12-real FP16 links",
                    "bytes_per_site": hop_bytes(cmx.HALF, 12), "parallelism": "1 GPU"},
@@ -495,8 +555,8 @@ def main():
         "sources_12_learned_vs_plain": sources_cmp,
         "solves": solves,
         "kernels": extra,
-        "launch_shapes": {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}
-                          for name, p in (("fp64", 0), ("fp32", 1), ("fp16", 2))},
+        "launch_shapes": shapes,
+        "kernels_solve_lattice": kernels_s,
     }
     print(json.dumps(out))
 

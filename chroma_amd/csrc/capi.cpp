@@ -82,6 +82,19 @@ int cmx_gauge_create(const int dims[4], const double* links, cmx_gauge* out) {
   return guarded([&] {
     require_init();
     CMX_REQUIRE(out, CMX_ERR_ARG, "gauge_create: null output");
+    CMX_REQUIRE(dims && links, CMX_ERR_ARG, "gauge_create: null argument");
+    const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+    const double* dirs[4] = {links, links + V * 18, links + 2 * V * 18, links + 3 * V * 18};
+    auto h = std::make_unique<cmx_gauge_s>();
+    h->gd = cmx::make_gauge(dims, dirs);
+    *out = h.release();
+  });
+}
+
+int cmx_gauge_create_dirs(const int dims[4], const double* const links[4], cmx_gauge* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(out, CMX_ERR_ARG, "gauge_create_dirs: null output");
     auto h = std::make_unique<cmx_gauge_s>();
     h->gd = cmx::make_gauge(dims, links);
     *out = h.release();
@@ -318,20 +331,33 @@ int cmx_clover_pack_quda(cmx_op h, int which, int cb, double* out) {
     require_init();
     cmx::Op& op = O(h);
     CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_pack_quda: not a clover operator");
-    CMX_REQUIRE(out && (which == 0 || (which == 1 && cb == 0)) && (cb == 0 || cb == 1), CMX_ERR_ARG,
+    CMX_REQUIRE(out && (which == 0 || which == 1) && (cb == 0 || cb == 1), CMX_ERR_ARG,
                 "clover_pack_quda: bad argument");
     const int Vh = (int)op.Vh;
-    std::unique_ptr<double[]> tri(new double[(size_t)72 * 2 * Vh]);
+    std::unique_ptr<double[]> tri(new double[(size_t)72 * Vh]);
     cmx::DevBuf d((size_t)72 * sizeof(double) * Vh);
-    const cmx::DevBuf& src = which == 1 ? op.clovinv_even[cmx::P64] : (cb == 0 ? op.clov64_even : op.clov_odd[cmx::P64]);
-    cmx::clover_unpack(Vh, src.as<double2>(), d.as<double>(), op.stream);
+    if (which == 1 && cb == 1) {
+      // A_oo^-1 is not kept by the operator (the even-odd solve needs only A_ee^-1): LDL^dag of A_oo
+      // here, as the QUDA set-up inverts both checkerboards (choles(0), choles(1))
+      cmx::DevBuf inv((size_t)36 * 16 * Vh), ld(sizeof(double) * Vh), nn(sizeof(double) * Vh);
+      cmx::clover_invert(op.g, op.clov_odd[cmx::P64].as<double2>(), inv.as<double2>(), ld.as<double>(),
+                         nn.as<double>(), op.stream);
+      cmx::clover_unpack(Vh, inv.as<double2>(), d.as<double>(), op.stream);
+      op.sync();
+    } else {
+      const cmx::DevBuf& src = which == 1 ? op.clovinv_even[cmx::P64] : (cb == 0 ? op.clov64_even : op.clov_odd[cmx::P64]);
+      cmx::clover_unpack(Vh, src.as<double2>(), d.as<double>(), op.stream);
+    }
     CMX_HIP(hipMemcpyAsync(tri.get(), d.p, (size_t)72 * sizeof(double) * Vh, hipMemcpyDeviceToHost, op.stream));
     op.sync();
     // QUDAPackedClovSite: diag1[6], offDiag1[15][2], diag2[6], offDiag2[15][2]; the lower triangle
-    // column by column (packForQUDA, clover_term_qdp_w.h:2185-2222)
+    // column by column (packForQUDA, clover_term_qdp_w.h:2185-2222), written at the full-lattice site
+    // index of each site of checkerboard cb (quda_array[rb[cb].siteTable()[ssite]]); the other
+    // checkerboard's entries of out are left as they are
+    double* outcb = out + (size_t)72 * cb * Vh;
     for (int i = 0; i < Vh; ++i) {
       const double* t = tri.get() + (size_t)72 * i;
-      double* q = out + (size_t)72 * i;
+      double* q = outcb + (size_t)72 * i;
       for (int b = 0; b < 2; ++b) {
         double* qb = q + 36 * b;
         for (int k = 0; k < 6; ++k) qb[k] = t[6 * b + k];

@@ -190,6 +190,8 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   for (int i = 0; i < nd_max; ++i) spareZ[i] = &op.work(250 + i, pb, n);
   for (Vec* z : Z) op.blas(B_ZERO, {z});
   Vec &r = op.work(300, P64, n), &tmp = op.work(301, P64, n);
+  // the FP64 residual converted to the basis precision for <V_i, r> at a deflated restart
+  Vec* rs = nd_max > 0 ? (pb == P64 ? &r : &op.work(306, pb, n)) : nullptr;
   Vec *xs = nullptr, *ys = nullptr;  // operator-precision staging when pb != ps
   if (pb != ps) {
     xs = &op.work(302, ps, n);
@@ -203,12 +205,17 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   const int reorth = p.reorth >= 2 ? 2 : 1;
   const int nb = blas_blocks(n);
 
-  auto t0 = std::chrono::steady_clock::now();
-  hipEvent_t ev0, ev1;
-  CMX_HIP(hipEventCreate(&ev0));
-  CMX_HIP(hipEventCreate(&ev1));
-  CMX_HIP(hipEventRecord(ev0, op.stream));
-  (void)t0;
+  // the events are released on every exit, including an exception from inside the loop
+  struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EvPair() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  CMX_HIP(hipEventCreate(&ev.a));
+  CMX_HIP(hipEventCreate(&ev.b));
+  CMX_HIP(hipEventRecord(ev.a, op.stream));
 
   // |chi|, target (:463-465); r = chi - A psi, |r| (:468-475)
   op.blas(B_NORM2, {&chi});
@@ -272,8 +279,10 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
   double r_prev = r_norm;
   while (!finished) {
     ++cycles;
-    int nd = nd_max;
-    if (plain_next || nd_max == 0) {
+    // a cycle that ended early (cycle_reduction, a converged estimate, an exact breakdown) may have
+    // fewer than NDefl Arnoldi vectors: deflate at most that many harmonic Ritz vectors
+    int nd = std::min(nd_max, prev_dim);
+    if (plain_next || nd == 0) {
       // c = g = [|r|, 0, ...]; V_0 = r / |r|  (:491-513)
       std::fill(c.begin(), c.end(), cd(0));
       std::fill(g.begin(), g.end(), cd(0));
@@ -346,10 +355,9 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       std::fill(g.begin(), g.end(), cd(0));
       {
         // <V_i, r> with r in FP64: convert r once to the basis precision
-        Vec rs(pb, n);
-        op.blas(B_COPY, {&rs, &r});
+        if (rs != &r) op.blas(B_COPY, {rs, &r});
         for (int i = 0; i <= nd; ++i) vs[i] = V[i];
-        kd.mdot(vs, nd + 1, rs, K_H);
+        kd.mdot(vs, nd + 1, *rs, K_H);
         kd.get(K_H, 2 * (nd + 1));
         for (int i = 0; i <= nd; ++i) c[i] = g[i] = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
       }
@@ -439,12 +447,10 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       std::printf("FGMRESDR: cycle %d: %d iterations, |r| = %g target %g\n", cycles, dim - nd, r_norm, target);
   }
   (void)prev_nd;
-  CMX_HIP(hipEventRecord(ev1, op.stream));
-  CMX_HIP(hipEventSynchronize(ev1));
+  CMX_HIP(hipEventRecord(ev.b, op.stream));
+  CMX_HIP(hipEventSynchronize(ev.b));
   float ms = 0;
-  CMX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
+  CMX_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
   res.n_count = iters;
   res.resid = r_norm;
   res.rel_resid = norm_rhs > 0 ? r_norm / norm_rhs : r_norm;

@@ -102,8 +102,9 @@ long full_index(const int L[4], const int x[4], long Vh) {
 
 }  // namespace
 
-std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
-  CMX_REQUIRE(dims && links, CMX_ERR_ARG, "gauge_create: null argument");
+std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* const links[4]) {
+  CMX_REQUIRE(dims && links && links[0] && links[1] && links[2] && links[3], CMX_ERR_ARG,
+              "gauge_create: null argument");
   auto gd = std::make_shared<GaugeData>();
   long V = 1;
   for (int m = 0; m < 4; ++m) {
@@ -116,9 +117,12 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
   gd->Vh = V / 2;
   gd->g.Lxh = dims[0] / 2;
   gd->g.Vh = (int)gd->Vh;
+  // one copy per direction: the caller's arrays need not be contiguous (QUDA_QDP_GAUGE_ORDER)
   const size_t nd = (size_t)4 * V * 18;
   DevBuf tmp(nd * sizeof(double));
-  CMX_HIP(hipMemcpy(tmp.p, links, nd * sizeof(double), hipMemcpyHostToDevice));
+  for (int mu = 0; mu < 4; ++mu)
+    CMX_HIP(hipMemcpy(tmp.as<double>() + (size_t)mu * V * 18, links[mu], (size_t)V * 18 * sizeof(double),
+                      hipMemcpyHostToDevice));
   for (int p = 0; p < 2; ++p) gd->g64[p].alloc((size_t)4 * 9 * 16 * gd->Vh);
   gauge_pack64(gd->g, tmp.as<double>(), gd->g64[0].as<double2>(), gd->g64[1].as<double2>(), 0);
   CMX_HIP(hipGetLastError());
@@ -128,8 +132,8 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
   for (int mu = 0; mu < 4; ++mu) {
     int x0[4] = {0, 0, 0, 0}, x1[4] = {0, 0, 0, 0};
     x1[mu] = dims[mu] - 1;
-    const cd d0 = det3(links + ((size_t)mu * V + full_index(dims, x0, gd->Vh)) * 18);
-    const cd d1 = det3(links + ((size_t)mu * V + full_index(dims, x1, gd->Vh)) * 18);
+    const cd d0 = det3(links[mu] + (size_t)full_index(dims, x0, gd->Vh) * 18);
+    const cd d1 = det3(links[mu] + (size_t)full_index(dims, x1, gd->Vh) * 18);
     gd->ph[mu][0][0] = d0.real(), gd->ph[mu][0][1] = d0.imag();
     gd->ph[mu][1][0] = d1.real(), gd->ph[mu][1][1] = d1.imag();
   }
@@ -149,7 +153,7 @@ std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links) {
     x[0] = 2 * xh + ((cb + x[1] + x[2] + x[3]) & 1);
     const int last = x[mu] == dims[mu] - 1;
     const cd e(gd->ph[mu][last][0], gd->ph[mu][last][1]);
-    return link_dev(links + (size_t)t * 18, e);
+    return link_dev(links[mu] + (size_t)s * 18, e);
   });
   gd->su3_dev = dev;
   return gd;

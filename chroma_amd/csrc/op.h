@@ -152,7 +152,7 @@ struct Op {
   void autotune();
 };
 
-std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* links);
+std::shared_ptr<GaugeData> make_gauge(const int dims[4], const double* const links[4]);
 Op* make_op(std::shared_ptr<GaugeData> gd, const cmx_op_params& p);
 // A second solver context on the operator o: shares o's links, clover fields, staging buffer and
 // stream (o must outlive it) and owns its solver state and workspace.  o's fields for precision

@@ -91,6 +91,7 @@ def lib():
         "cmx_last_error": (ctypes.c_char_p, []),
         "cmx_version": (ctypes.c_int, []),
         "cmx_gauge_create": (ctypes.c_int, [_ip, _dp, ctypes.POINTER(_vp)]),
+        "cmx_gauge_create_dirs": (ctypes.c_int, [_ip, ctypes.POINTER(_dp), ctypes.POINTER(_vp)]),
         "cmx_gauge_destroy": (ctypes.c_int, [_vp]),
         "cmx_op_create": (ctypes.c_int, [_vp, ctypes.POINTER(OpParams), ctypes.POINTER(_vp)]),
         "cmx_op_destroy": (ctypes.c_int, [_vp]),
@@ -238,12 +239,21 @@ def kappa_to_mass(kappa):
 
 # ---- device objects -------------------------------------------------------------------------------
 class Gauge:
-    def __init__(self, L, U):
+    """Device gauge field from host links: one array U[4, V, 3, 3], or (per_direction=True) a sequence
+    of four arrays U[mu][V, 3, 3] handed over as separate pointers (QUDA_QDP_GAUGE_ORDER)."""
+
+    def __init__(self, L, U, per_direction=False):
         init()
         self.L = tuple(int(v) for v in L)
-        self._U = np.ascontiguousarray(U, np.complex128)
         h = _vp()
-        check(lib().cmx_gauge_create(_dims(L), _d(self._U), ctypes.byref(h)))
+        if per_direction:
+            dirs = [np.ascontiguousarray(u, np.complex128) for u in U]
+            assert len(dirs) == 4
+            ptrs = (_dp * 4)(*[_d(u) for u in dirs])
+            check(lib().cmx_gauge_create_dirs(_dims(L), ptrs, ctypes.byref(h)))
+        else:
+            U = np.ascontiguousarray(U, np.complex128)
+            check(lib().cmx_gauge_create(_dims(L), _d(U), ctypes.byref(h)))
         self.h = h
 
     def __del__(self):
@@ -322,8 +332,11 @@ class Operator:
         check(lib().cmx_clover_logdet(self.h, ctypes.byref(ld), ctypes.byref(nn)))
         return ld.value, nn.value
 
-    def clover_pack_quda(self, which, cb):
-        out = np.zeros((self.V // 2, 72), np.float64)
+    def clover_pack_quda(self, which, cb, out=None):
+        """QUDAPackedClovSite array over the full lattice (V x 72 doubles, host site order); only the
+        sites of checkerboard cb are written (packForQUDA, clover_term_qdp_w.h:2171-2228)."""
+        out = np.zeros((self.V, 72), np.float64) if out is None else out
+        assert out.shape == (self.V, 72) and out.dtype == np.float64 and out.flags.c_contiguous
         check(lib().cmx_clover_pack_quda(self.h, int(which), int(cb), _d(out)))
         return out
 

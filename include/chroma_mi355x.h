@@ -77,6 +77,10 @@ int cmx_version(void);
  * applied (state->getLinks(), simple_fermbc.h:87-104 / twisted_fermbc_w.h:96-119).  Copied to the
  * device at creation; the host array may be freed afterwards. */
 int cmx_gauge_create(const int dims[4], const double* links, cmx_gauge* out);
+/* The same from four per-direction arrays, each V*18 doubles [site][3][3][re,im]: the gauge argument
+ * of loadGaugeQuda in QUDA_QDP_GAUGE_ORDER, void* gauge[Nd] with gauge[mu] = &links[mu].elem(0)
+ * (syssolver_linop_clover_quda_w.h:510-523), so the caller needs no contiguous copy of the field. */
+int cmx_gauge_create_dirs(const int dims[4], const double* const links[4], cmx_gauge* out);
 int cmx_gauge_destroy(cmx_gauge g);
 
 /* ---- operator: EvenOddPrecCloverLinOp::create (eoprec_clover_linop_w.cc:19-39) ------------------
@@ -130,8 +134,11 @@ int cmx_clover_download(cmx_op op, int which, double* tri);
 /* Sum over the even sites of log|d_i| of the LDL^dag pivots (cholesDet(0), clover_term_qdp_w.h:580-606;
  * logDetEvenEvenLinOp, eoprec_clover_linop_w.cc:310-312) and the count of sites with negative pivots. */
 int cmx_clover_logdet(cmx_op op, double* logdet, long* n_neg_sites);
-/* QUDAPackedClovSite wire format (clover_term_qdp_w.h:26-32, packForQUDA :2171-2228) of checkerboard cb:
- * which 0 = A, 1 = A^-1 (cb 0 only).  out: Vh*72 doubles. */
+/* QUDAPackedClovSite wire format (clover_term_qdp_w.h:26-32) of checkerboard cb, as packForQUDA
+ * (:2171-2228) writes it: out holds V*72 doubles indexed by the full-lattice site (the host order
+ * above); the sites of checkerboard cb are written, the others left as they are, so two calls (cb 0,
+ * then cb 1) fill the array the way syssolver_linop_clover_quda_w.h:539-548 does.  which 0 = A,
+ * 1 = A^-1 (for cb 1 the LDL^dag inverse of A_oo is formed for the call, choles(1)). */
 int cmx_clover_pack_quda(cmx_op op, int which, int cb, double* out);
 
 /* ---- solver: replaces invertQuda (syssolver_linop_clover_quda_w.cc:98) --------------------------

@@ -163,6 +163,30 @@ def clover_apply(L, tri, psi, cb, out=None):
     return chi
 
 
+def pack_quda(tri, cb, out=None):
+    """QDPCloverTermT::packForQUDA (clover_term_qdp_w.h:2171-2228, qudaPackSiteLoop :2176-2224) from a
+    PrimitiveClovTriang array tri float64[V, 72] ({diag[2][6], offd[2][15][2]}, full-lattice host order):
+    QUDAPackedClovSite {diag1[6], offDiag1[15][2], diag2[6], offDiag2[15][2]} (:26-32) written at the
+    full-lattice index of every site of checkerboard cb; the other checkerboard's rows of out are left
+    as they are.  Within a chiral block the off-diagonal entries go column by column down the strict
+    lower triangle, target index t, from the triangle's source index row*(row-1)/2 + col (:2191-2199)."""
+    V = tri.shape[0]
+    Vh = V // 2
+    out = np.zeros((V, 72), np.float64) if out is None else out
+    sites = slice(cb * Vh, (cb + 1) * Vh)
+    for b in range(2):
+        q0 = 36 * b
+        out[sites, q0:q0 + 6] = tri[sites, 6 * b:6 * b + 6]
+        t = 0
+        for col in range(5):
+            for row in range(col + 1, 6):
+                src = row * (row - 1) // 2 + col
+                out[sites, q0 + 6 + 2 * t] = tri[sites, 12 + 30 * b + 2 * src]
+                out[sites, q0 + 6 + 2 * t + 1] = tri[sites, 12 + 30 * b + 2 * src + 1]
+                t += 1
+    return out
+
+
 def site_coords(L, idx):
     x = (ctypes.c_int * 4)()
     lib().orc_site_coords(_dims(L), int(idx), x)

@@ -65,10 +65,13 @@ class Givens:
         v[j + 1] = -self.s * a + self.c * b
 
 
-def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None, record=None):
+def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None, record=None, cycle_reduction=0.0):
     """LinOpSysSolverFGMRESDR::operator() (syssolver_linop_fgmres_dr.cc:459-774).
 
     A: callable on arrays shaped like chi.  precond(z0, v) -> z (None: identity).
+    cycle_reduction > 0 ends a cycle once its residual estimate has fallen by that factor below the
+    residual the cycle started from (the library's option; 0 is the reference's rule).  A cycle that
+    ends with fewer Arnoldi vectors than n_defl deflates only as many harmonic Ritz vectors as it has.
     Returns (psi, n_count, |chi - A psi|, cycles).  `record` (a dict) receives the Hessenberg matrix,
     the bases and residual norms of each cycle, for tests.
     """
@@ -94,8 +97,8 @@ def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None,
     plain_next, r_prev = True, r_norm
     while not finished:
         cycles += 1
-        nd = nd_max
-        if plain_next or nd_max == 0:
+        nd = min(nd_max, prev_dim)
+        if plain_next or nd == 0:
             c[:] = 0
             g[:] = 0
             c[0] = g[0] = r_norm
@@ -136,6 +139,7 @@ def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None,
             g[:nd + 1] = Qk.conj().T @ g[:nd + 1]
             if record is not None:
                 record.setdefault("harmonic_ritz", []).append(evals[order[:nd]])
+        cycle_stop = max(target, cycle_reduction * r_norm if cycle_reduction > 0 else 0.0)
         dim = nd
         for j in range(nd, nd + nk):
             Z[j] = V[j].copy() if precond is None else precond(Z[j], V[j])
@@ -159,7 +163,7 @@ def fgmres_dr(A, chi, psi, rsd, n_krylov, n_defl=0, max_iter=1000, precond=None,
             rots[j].apply(g)
             R[:j + 2, j] = col
             dim = j + 1
-            if abs(g[j + 1].real) <= target:  # (:172)
+            if abs(g[j + 1].real) <= cycle_stop:  # (:172)
                 break
         if dim > 0:
             for row in range(dim - 1, -1, -1):

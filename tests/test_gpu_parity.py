@@ -115,6 +115,39 @@ def test_clover_build_and_inverse(L):
     assert abs(ld - ld_ref) < 1e-10 * max(1.0, abs(ld_ref)) and nn == nn_ref
 
 
+def test_clover_pack_quda_and_per_direction_links():
+    """QUDAPackedClovSite wire format (packForQUDA, clover_term_qdp_w.h:2171-2228) of A and A^-1 on both
+    checkerboards against the oracle's restatement on the oracle's clover term, at full-lattice site
+    indices with the other checkerboard untouched; bit for bit against the same restatement applied
+    to the library's own FP64 clover download.  The operator is built from four per-direction link
+    arrays (QUDA_QDP_GAUGE_ORDER) and equals the one built from the contiguous array bit for bit."""
+    L = (4, 4, 4, 8)
+    say("clover pack quda")
+    V = int(np.prod(L))
+    Vh = V // 2
+    U = gauge_field(L, 33)
+    mass, csw = cmx.kappa_to_mass(0.125), 1.0
+    op = cmx.Operator(cmx.Gauge(L, [U[mu].copy() for mu in range(4)], per_direction=True), cmx.EO_CLOVER, mass, csw)
+    op_c = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw)
+    psi = random_fermion(Vh, 34)
+    assert np.array_equal(op(psi).view(np.uint64), op_c(psi).view(np.uint64))
+    ref = oracle.make_clover(L, U, csw, csw, mass)
+    out = np.full((V, 72), np.nan)
+    op.clover_pack_quda(0, 0, out)
+    assert np.isnan(out[Vh:]).all() and not np.isnan(out[:Vh]).any()
+    op.clover_pack_quda(0, 1, out)
+    want = oracle.pack_quda(ref, 1, oracle.pack_quda(ref, 0))
+    assert np.max(np.abs(out - want)) < 1e-13 * np.max(np.abs(want))
+    own = op.clover(0)
+    assert np.array_equal(out, oracle.pack_quda(own, 1, oracle.pack_quda(own, 0)))
+    inv_ref = ref.copy()
+    oracle.clover_invert(L, inv_ref, 0)
+    oracle.clover_invert(L, inv_ref, 1)
+    outi = op.clover_pack_quda(1, 1, op.clover_pack_quda(1, 0))
+    wanti = oracle.pack_quda(inv_ref, 1, oracle.pack_quda(inv_ref, 0))
+    assert np.max(np.abs(outi - wanti)) < 1e-12 * np.max(np.abs(wanti))
+
+
 @pytest.mark.parametrize("kind", [cmx.EO_WILSON, cmx.EO_CLOVER])
 @pytest.mark.parametrize("prec", [cmx.DOUBLE, cmx.SINGLE, cmx.HALF])
 def test_eo_operator(kind, prec):
@@ -336,6 +369,37 @@ def test_fgmres_dr_vs_oracle(n_defl, precond):
     assert abs(res["n_count"] - n_ref) <= 1, (res["n_count"], n_ref)
     assert res["rel_resid"] <= 1e-10
     assert rel(psi, psi_ref) < 1e-8
+
+
+@pytest.mark.parametrize("basis", [cmx.DOUBLE, cmx.HALF])
+def test_fgmres_dr_deflation_after_short_cycles(basis):
+    """NDefl = 8 with cycles that end after a residual reduction of 2 (fewer Arnoldi vectors than NDefl):
+    the deflated restart keeps at most as many harmonic Ritz vectors as the cycle built.  FP64 basis:
+    against the oracle's restatement; FP16 operator and basis: reaches the FP64 true residual 1e-10 and
+    agrees with the oracle's solution."""
+    from oracle import krylov
+    L = (8, 8, 8, 16)
+    say("fgmres-dr short cycles", basis)
+    V = int(np.prod(L))
+    U = gauge_field(L, 96)
+    mass = cmx.kappa_to_mass(0.125)
+    ref_op = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF)
+    chi = random_fermion(V, 17)[V // 2:]
+    A = _odd_op(ref_op, V)
+    rec = {}
+    psi_ref, n_ref, r_ref, cyc_ref = krylov.fgmres_dr(A, chi, np.zeros_like(chi), 1e-10, n_krylov=12, n_defl=8,
+                                                      max_iter=3000, record=rec, cycle_reduction=0.5)
+    assert min(c["dim"] - c["nd"] for c in rec["cycles"]) < 8
+    sp = cmx.DOUBLE if basis == cmx.DOUBLE else cmx.HALF
+    psi, res = op.solve(chi, solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=sp, basis_prec=basis,
+                        n_krylov=12, n_defl=8, reorth=2, cycle_reduction=0.5)
+    say("  n_count", res["n_count"], "oracle", n_ref, "cycles", res["n_reliable"], "oracle", cyc_ref,
+        "rel resid", res["rel_resid"])
+    assert res["rel_resid"] <= 1e-10
+    assert rel(psi, psi_ref) < 1e-8
+    if basis == cmx.DOUBLE:
+        assert abs(res["n_count"] - n_ref) <= 2, (res["n_count"], n_ref)
 
 
 @pytest.mark.parametrize("sloppy", [cmx.SINGLE, cmx.HALF])

@@ -224,6 +224,65 @@ def test_clover_dense_construction():
     assert np.max(np.abs(tri_to_dense(tri) - A)) < 1e-13
 
 
+def quda_packed_from_dense(A):
+    """QUDAPackedClovSite from dense Hermitian 12x12 site matrices (index spin*3+colour): per chiral
+    block b (spins 2b, 2b+1) the real diagonal, then the strict lower triangle column by column (col 0:
+    rows 1..5, col 1: rows 2..5, ...), (re, im) pairs."""
+    V = A.shape[0]
+    out = np.zeros((V, 72))
+    for b in range(2):
+        blk = A[:, 6 * b:6 * b + 6, 6 * b:6 * b + 6]
+        out[:, 36 * b:36 * b + 6] = np.real(np.diagonal(blk, axis1=1, axis2=2))
+        low = [blk[:, row, col] for col in range(6) for row in range(col + 1, 6)]
+        low = np.stack(low, axis=1)
+        out[:, 36 * b + 6:36 * b + 36:2] = low.real
+        out[:, 36 * b + 7:36 * b + 36:2] = low.imag
+    return out
+
+
+def test_pack_quda_wire_format():
+    """packForQUDA (clover_term_qdp_w.h:2171-2228) against the dense clover term of the gammas and the
+    field strength: diag1/offDiag1/diag2/offDiag2 in the column-major lower-triangle order of
+    QUDAPackedClovSite, at the full-lattice index of each site of the checkerboard, the other
+    checkerboard untouched; the same for the inverse on both checkerboards (choles(0), choles(1))."""
+    L = (2, 4, 2, 4)
+    U = numpy_gauge(L, 15)
+    mass, cR, cT = 0.07, 1.3, 0.9
+    tri = oracle.make_clover(L, U, cR, cT, mass)
+    Fn = field_strength_numpy(L, U)
+    V = int(np.prod(L))
+    Vh = V // 2
+    A = np.zeros((V, 12, 12), complex)
+    A[:] = (4 + mass) * np.eye(12)
+    for (mu, nu), f in Fn.items():
+        c = 0.5 * (cT if 3 in (mu, nu) else cR)
+        A -= c * np.einsum("ab,vij->vaibj", GAMMA[mu] @ GAMMA[nu], f).reshape(V, 12, 12)
+    want = quda_packed_from_dense(A)
+    out = np.full((V, 72), np.nan)
+    oracle.pack_quda(tri, 0, out)
+    assert np.isnan(out[Vh:]).all()
+    assert np.max(np.abs(out[:Vh] - want[:Vh])) < 1e-13
+    oracle.pack_quda(tri, 1, out)
+    assert np.max(np.abs(out - want)) < 1e-13
+    inv = tri.copy()
+    oracle.clover_invert(L, inv, 0)
+    oracle.clover_invert(L, inv, 1)
+    pinv = oracle.pack_quda(inv, 1, oracle.pack_quda(inv, 0))
+    assert np.max(np.abs(pinv - quda_packed_from_dense(np.linalg.inv(A)))) < 1e-12
+    # planted mistakes: the lower triangle row by row (the PrimitiveClovTriang order itself), and the
+    # upper triangle column by column (the conjugates), are other wire formats and fail
+    bad = want.copy()
+    for b in range(2):
+        low = np.stack([A[:, 6 * b + row, 6 * b + col] for row in range(6) for col in range(row)], axis=1)
+        bad[:, 36 * b + 6:36 * b + 36:2] = low.real
+        bad[:, 36 * b + 7:36 * b + 36:2] = low.imag
+    assert np.max(np.abs(out - bad)) > 1e-3
+    conj = want.copy()
+    for b in range(2):
+        conj[:, 36 * b + 7:36 * b + 36:2] *= -1
+    assert np.max(np.abs(out - conj)) > 1e-3
+
+
 def test_clover_unit_gauge_and_inverse():
     L = (4, 4, 4, 4)
     V = int(np.prod(L))

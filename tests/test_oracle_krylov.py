@@ -83,6 +83,25 @@ def test_deflation_reduces_iterations():
     assert n4 < n0
 
 
+def test_deflation_after_short_cycles():
+    """Cycles that end with fewer Arnoldi vectors than n_defl (cycle_reduction stops each cycle after a
+    factor 2): the deflated restart keeps only as many harmonic Ritz vectors as the cycle has, and the
+    solve still reaches the dense solution."""
+    V, M, A = wilson_system(seed=53, mass=0.05)
+    chi = random_fermion(V, 54)
+    rec = {}
+    psi, n, r, cycles = krylov.fgmres_dr(A, chi, np.zeros_like(chi), 1e-10, n_krylov=12, n_defl=8,
+                                         max_iter=5000, record=rec, cycle_reduction=0.5)
+    dims = [c["dim"] - c["nd"] for c in rec["cycles"]]
+    assert min(dims) < 8  # some cycles are shorter than n_defl
+    assert max(c["nd"] for c in rec["cycles"]) > 0  # and deflation does take place
+    for a, b in zip(rec["cycles"], rec["cycles"][1:]):
+        assert b["nd"] <= a["dim"]
+    exact = np.linalg.solve(M, chi.reshape(-1)).reshape(chi.shape)
+    assert rel(psi, exact) < 1e-8
+    assert r <= 1e-10 * np.linalg.norm(chi)
+
+
 def test_mr_step_is_optimal():
     # one MR step minimises |r - a A r| over complex a
     V, M, A = wilson_system(seed=51)

@@ -78,38 +78,44 @@ inline std::vector<cd> lu_solve(Mat A, std::vector<cd> b) {
 // Thin QR of A (m x n, m >= n) by Householder reflections: A = Q R, Q m x n with orthonormal
 // columns, R n x n upper triangular.  If `full`, Q is m x m.
 inline void qr(const Mat& A, Mat& Q, Mat& R, bool full = false) {
+  // Householder QR with LAPACK's reflector convention (zgeqrf / zlarfg / zungqr, which the reference
+  // calls): H_k = I - tau_k v_k v_k^H with v_k[k] = 1, R's diagonal real, beta = -sign(Re alpha) |x|.
+  // The phases of Q's columns, the completion columns of the full form included, are then the
+  // reference's, and so is the real part of the incremental residual g that FGMRES-DR tests (:172).
   const int m = A.rows, n = A.cols;
   if (m < n) throw std::runtime_error("dense::qr: rows < cols");
   Mat W = A;
   std::vector<std::vector<cd>> vs;
+  std::vector<cd> taus;
   for (int k = 0; k < n; ++k) {
-    double nrm = 0;
-    for (int r = k; r < m; ++r) nrm += std::norm(W(r, k));
-    nrm = std::sqrt(nrm);
     std::vector<cd> v(m, cd(0));
-    if (nrm == 0) {
-      vs.push_back(v);
-      continue;
+    v[k] = 1.0;
+    const cd alpha = W(k, k);
+    double xn2 = 0;
+    for (int r = k + 1; r < m; ++r) xn2 += std::norm(W(r, k));
+    cd tau = 0;
+    if (!(xn2 == 0 && alpha.imag() == 0)) {  // zlarfg
+      const double beta = -std::copysign(std::sqrt(std::norm(alpha) + xn2), alpha.real());
+      tau = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
+      const cd scal = 1.0 / (alpha - beta);
+      for (int r = k + 1; r < m; ++r) v[r] = W(r, k) * scal;
+      W(k, k) = beta;
+      for (int r = k + 1; r < m; ++r) W(r, k) = 0;
     }
-    const cd x0 = W(k, k);
-    const cd ph = std::abs(x0) > 0 ? x0 / std::abs(x0) : cd(1);
-    v[k] = x0 + ph * nrm;
-    for (int r = k + 1; r < m; ++r) v[r] = W(r, k);
-    double vn = 0;
-    for (int r = k; r < m; ++r) vn += std::norm(v[r]);
-    vn = std::sqrt(vn);
-    for (int r = k; r < m; ++r) v[r] /= vn;
-    // W <- (I - 2 v v^H) W
-    for (int c = k; c < n; ++c) {
+    // W(k:, k+1:) <- H_k^H W(k:, k+1:)   (zgeqr2 applies conj(tau))
+    for (int c = k + 1; c < n; ++c) {
       cd s = 0;
       for (int r = k; r < m; ++r) s += std::conj(v[r]) * W(r, c);
-      for (int r = k; r < m; ++r) W(r, c) -= 2.0 * v[r] * s;
+      s *= std::conj(tau);
+      for (int r = k; r < m; ++r) W(r, c) -= v[r] * s;
     }
     vs.push_back(v);
+    taus.push_back(tau);
   }
   R = Mat(n, n);
   for (int r = 0; r < n; ++r)
     for (int c = r; c < n; ++c) R(r, c) = W(r, c);
+  // Q = H_0 H_1 ... H_{n-1} applied to the first qc columns of the identity (zungqr)
   const int qc = full ? m : n;
   Q = Mat(m, qc);
   for (int c = 0; c < qc; ++c) Q(c, c) = 1.0;
@@ -118,7 +124,8 @@ inline void qr(const Mat& A, Mat& Q, Mat& R, bool full = false) {
     for (int c = 0; c < qc; ++c) {
       cd s = 0;
       for (int r = k; r < m; ++r) s += std::conj(v[r]) * Q(r, c);
-      for (int r = k; r < m; ++r) Q(r, c) -= 2.0 * v[r] * s;
+      s *= taus[k];
+      for (int r = k; r < m; ++r) Q(r, c) -= v[r] * s;
     }
   }
 }

@@ -67,6 +67,7 @@ int cmx_init(int device) {
     CMX_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, CMX_ERR_UNSUPPORTED,
                 std::string("cmx_init: built for gfx950, device is ") + prop.gcnArchName);
     g_device = device;
+    cmx::meson_scratch_setup();  // the allocation-free meson contraction's work space on this device
     g_init = true;
   });
 }
@@ -226,6 +227,14 @@ int cmx_meson_2pt(const int dims[4], const double* prop1, const double* prop2, c
   return guarded([&] {
     require_init();
     cmx::meson_2pt(dims, prop1, prop2, origin, nmom, moms, corr);
+  });
+}
+
+int cmx_meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4],
+                          int nmom, const int* moms, double* corr, void* stream) {
+  return guarded([&] {
+    require_init();
+    cmx::meson_2pt_noalloc(dims, prop1, prop2, origin, nmom, moms, corr, static_cast<hipStream_t>(stream));
   });
 }
 

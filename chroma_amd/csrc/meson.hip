@@ -9,7 +9,9 @@
 // non-zero entry, a power of i, per row, so each of the 16 spin blocks (a, b) of the trace reads one 3x3
 // colour block of S_2 and one of S_1 with a fixed coefficient; the host builds that table per Gamma.
 // Propagators: [site][sink spin*3 + colour][source spin*3 + colour] complex, sites in the host order.
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -87,16 +89,17 @@ std::vector<Term> meson_terms() {
 // MESON_SITES sites per block, one lane per (site, Gamma): the block first copies its sites' two
 // propagators (2 x 2304 B per site, contiguous) into LDS with coalesced loads, then each lane forms its
 // trace from LDS, so HBM sees each propagator once instead of the 16 re-reads a lane per (site, Gamma)
-// would make through the caches.  c[n][site]
+// would make through the caches.  The block range is the nsite sites from s1/s2 (already offset to the
+// range's first site); c[n * cstride + coff + i] for the range's site i.
 constexpr int MESON_SITES = 8;
 
-__global__ void __launch_bounds__(16 * MESON_SITES) k_meson_sites(long V, const double2* __restrict__ s1,
+__global__ void __launch_bounds__(16 * MESON_SITES) k_meson_sites(long nsites, const double2* __restrict__ s1,
                                                                   const double2* __restrict__ s2,
                                                                   const Term* __restrict__ terms,
-                                                                  double2* __restrict__ c) {
+                                                                  double2* __restrict__ c, long cstride, long coff) {
   __shared__ double2 sh1[MESON_SITES * 144], sh2[MESON_SITES * 144];
   const long site0 = (long)blockIdx.x * MESON_SITES;
-  const int nsite = V - site0 < MESON_SITES ? (int)(V - site0) : MESON_SITES;
+  const int nsite = nsites - site0 < MESON_SITES ? (int)(nsites - site0) : MESON_SITES;
   for (int q = threadIdx.x; q < nsite * 144; q += blockDim.x) {
     sh1[q] = s1[site0 * 144 + q];
     sh2[q] = s2[site0 * 144 + q];
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(16 * MESON_SITES) k_meson_sites(long V, const 
     cr += tm.coef.x * sr - tm.coef.y * si;
     ci += tm.coef.x * si + tm.coef.y * sr;
   }
-  c[(size_t)n * V + site0 + ls] = make_double2(cr, ci);
+  c[(size_t)n * cstride + coff + site0 + ls] = make_double2(cr, ci);
 }
 
 struct ProjGeom {
@@ -132,13 +135,20 @@ struct ProjGeom {
   int org[4];
 };
 
+// Where k_meson_project finds the correlator of a site (cb, lex) of time slice t0 + blockIdx.x:
+// c[n * cstride + cb * cb_off + lex / 2 - lex_off].  Whole lattice: cstride V, cb_off Vh, lex_off 0.
+struct CorrIndex {
+  int t0;
+  long cstride, cb_off, lex_off;
+};
+
 // one block per (time slice, momentum): out[n][m][t_eff] = sum over the slice of phase(x) c[n][x], summed in
 // a fixed order (strided per lane, then a tree over the block), so the result is deterministic
-__global__ void __launch_bounds__(256) k_meson_project(ProjGeom g, const double2* __restrict__ c,
+__global__ void __launch_bounds__(256) k_meson_project(ProjGeom g, CorrIndex ci, const double2* __restrict__ c,
                                                        const int* __restrict__ moms, int nmom,
                                                        double2* __restrict__ out) {
   __shared__ double2 red[256];
-  const int t = blockIdx.x, m = blockIdx.y;
+  const int t = ci.t0 + (int)blockIdx.x, m = blockIdx.y;
   const long Vs = (long)g.L[0] * g.L[1] * g.L[2];
   const double k0 = 6.283185307179586 * moms[3 * m] / g.L[0], k1 = 6.283185307179586 * moms[3 * m + 1] / g.L[1],
                k2 = 6.283185307179586 * moms[3 * m + 2] / g.L[2];
@@ -148,12 +158,12 @@ __global__ void __launch_bounds__(256) k_meson_project(ProjGeom g, const double2
   for (long v = threadIdx.x; v < Vs; v += blockDim.x) {
     const int x0 = (int)(v % g.L[0]), x1 = (int)(v / g.L[0] % g.L[1]), x2 = (int)(v / ((long)g.L[0] * g.L[1]));
     const long lex = v + Vs * t;
-    const long site = ((x0 + x1 + x2 + t) & 1) * g.Vh + lex / 2;
+    const long idx = ((x0 + x1 + x2 + t) & 1) * ci.cb_off + lex / 2 - ci.lex_off;
     double sn, cs;
     sincos(k0 * (x0 - g.org[0]) + k1 * (x1 - g.org[1]) + k2 * (x2 - g.org[2]), &sn, &cs);
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
-      const double2 z = c[(size_t)n * g.V + site];
+      const double2 z = c[(size_t)n * ci.cstride + idx];
       acc[n].x += cs * z.x - sn * z.y;
       acc[n].y += cs * z.y + sn * z.x;
     }
@@ -172,12 +182,7 @@ __global__ void __launch_bounds__(256) k_meson_project(ProjGeom g, const double2
   }
 }
 
-}  // namespace
-
-void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
-               const int* moms, double* corr) {
-  CMX_REQUIRE(dims && prop1 && prop2 && corr, CMX_ERR_ARG, "meson_2pt: null array");
-  CMX_REQUIRE(nmom >= 1 && nmom <= 4096 && moms, CMX_ERR_ARG, "meson_2pt: 1..4096 momenta");
+ProjGeom proj_geom(const int dims[4], const int origin[4]) {
   ProjGeom g;
   g.V = 1;
   for (int m = 0; m < 4; ++m) {
@@ -189,6 +194,36 @@ void meson_2pt(const int dims[4], const double* prop1, const double* prop2, cons
   }
   g.Vh = g.V / 2;
   CMX_REQUIRE(g.V <= (long)INT32_MAX * MESON_SITES, CMX_ERR_ARG, "meson_2pt: lattice too large for one grid");
+  return g;
+}
+
+void launch_sites(long nsites, const double2* s1, const double2* s2, const Term* terms, double2* c, long cstride,
+                  long coff, hipStream_t st) {
+  hipLaunchKernelGGL(k_meson_sites, dim3((unsigned)((nsites + MESON_SITES - 1) / MESON_SITES)), dim3(16 * MESON_SITES), 0,
+                     st, nsites, s1, s2, terms, c, cstride, coff);
+  CMX_HIP(hipGetLastError());
+}
+
+// The allocation-free variant's scratch: the correlators of up to MESON_SCRATCH_SITES sites (a pass of whole
+// time slices; 64 MiB) followed by the Gamma-term table (written once per device at set-up, never
+// overwritten).  Host code never uses this name as an address: it uses the per-device pointer that
+// hipGetSymbolAddress returns.
+constexpr long MESON_SCRATCH_SITES = 262144;  // 64^3: the largest spatial volume one pass holds
+constexpr size_t MESON_SCRATCH_CORR = sizeof(double2) * 16 * MESON_SCRATCH_SITES;
+constexpr size_t MESON_SCRATCH_BYTES = MESON_SCRATCH_CORR + sizeof(Term) * 256;
+static __device__ __align__(256) char scratch[MESON_SCRATCH_BYTES];
+
+constexpr int MESON_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MESON_MAX_DEVICES] = {};
+
+}  // namespace
+
+void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+               const int* moms, double* corr) {
+  CMX_REQUIRE(dims && prop1 && prop2 && corr, CMX_ERR_ARG, "meson_2pt: null array");
+  CMX_REQUIRE(nmom >= 1 && nmom <= 4096 && moms, CMX_ERR_ARG, "meson_2pt: 1..4096 momenta");
+  const ProjGeom g = proj_geom(dims, origin);
   const std::vector<Term> terms = meson_terms();
   const size_t pbytes = sizeof(double) * 288 * (size_t)g.V;
   DevBuf d1(pbytes), d2(pbytes), dc(sizeof(double2) * 16 * (size_t)g.V), dt(sizeof(Term) * terms.size()),
@@ -200,13 +235,60 @@ void meson_2pt(const int dims[4], const double* prop1, const double* prop2, cons
     CMX_HIP(hipMemcpy(d2.p, prop2, pbytes, hipMemcpyHostToDevice));
   CMX_HIP(hipMemcpy(dt.p, terms.data(), dt.bytes, hipMemcpyHostToDevice));
   CMX_HIP(hipMemcpy(dm.p, moms, dm.bytes, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_meson_sites, dim3((unsigned)((g.V + MESON_SITES - 1) / MESON_SITES)), dim3(16 * MESON_SITES), 0, 0,
-                     g.V, d1.as<double2>(), d2.as<double2>(), dt.as<Term>(), dc.as<double2>());
-  CMX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_meson_project, dim3(g.L[3], nmom), dim3(256), 0, 0, g, dc.as<double2>(), dm.as<int>(), nmom,
+  launch_sites(g.V, d1.as<double2>(), d2.as<double2>(), dt.as<Term>(), dc.as<double2>(), g.V, 0, 0);
+  const CorrIndex ci{0, g.V, g.Vh, 0};
+  hipLaunchKernelGGL(k_meson_project, dim3(g.L[3], nmom), dim3(256), 0, 0, g, ci, dc.as<double2>(), dm.as<int>(), nmom,
                      dout.as<double2>());
   CMX_HIP(hipGetLastError());
   CMX_HIP(hipMemcpy(corr, dout.p, dout.bytes, hipMemcpyDeviceToHost));
+}
+
+void meson_scratch_setup() {
+  int dev = 0;
+  CMX_HIP(hipGetDevice(&dev));
+  CMX_REQUIRE(dev >= 0 && dev < MESON_MAX_DEVICES, CMX_ERR_UNSUPPORTED, "meson_2pt_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* ptr = nullptr;
+  CMX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  CMX_REQUIRE(ptr != nullptr, CMX_ERR_HIP, "meson_2pt_noalloc: no device address for the scratch array");
+  const std::vector<Term> terms = meson_terms();
+  CMX_HIP(hipMemcpy(static_cast<char*>(ptr) + MESON_SCRATCH_CORR, terms.data(), sizeof(Term) * terms.size(),
+                    hipMemcpyHostToDevice));
+  g_scratch_ptr[dev] = ptr;
+}
+
+void meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+                       const int* moms, double* corr, hipStream_t stream) {
+  CMX_REQUIRE(dims && prop1 && prop2 && corr && moms, CMX_ERR_ARG, "meson_2pt_noalloc: null array");
+  CMX_REQUIRE(nmom >= 1 && nmom <= 4096, CMX_ERR_ARG, "meson_2pt_noalloc: 1..4096 momenta");
+  const ProjGeom g = proj_geom(dims, origin);
+  const long Vs = (long)g.L[0] * g.L[1] * g.L[2];
+  CMX_REQUIRE(Vs <= MESON_SCRATCH_SITES, CMX_ERR_ARG,
+              "meson_2pt_noalloc: spatial volume above 262144 sites (64^3), the scratch array's limit");
+  meson_scratch_setup();
+  int dev = 0;
+  CMX_HIP(hipGetDevice(&dev));
+  char* base = static_cast<char*>(g_scratch_ptr[dev]);
+  double2* c = reinterpret_cast<double2*>(base);
+  const Term* terms = reinterpret_cast<const Term*>(base + MESON_SCRATCH_CORR);
+  const double2* s1 = reinterpret_cast<const double2*>(prop1);
+  const double2* s2 = reinterpret_cast<const double2*>(prop2);
+  // passes of T whole time slices; the slice range [t0, t0 + T) is two contiguous site ranges, one per
+  // checkerboard, stored one after the other in the scratch
+  const int T = (int)std::min<long>(g.L[3], MESON_SCRATCH_SITES / Vs);
+  for (int t0 = 0; t0 < g.L[3]; t0 += T) {
+    const int nt = std::min(T, g.L[3] - t0);
+    const long half = Vs * nt / 2, lex0 = Vs * t0 / 2;
+    for (int cb = 0; cb < 2; ++cb) {
+      const size_t first = (size_t)(cb * g.Vh + lex0) * 144;
+      launch_sites(half, s1 + first, s2 + first, terms, c, Vs * nt, cb * half, stream);
+    }
+    const CorrIndex ci{t0, Vs * nt, half, lex0};
+    hipLaunchKernelGGL(k_meson_project, dim3(nt, nmom), dim3(256), 0, stream, g, ci, c, moms, nmom,
+                       reinterpret_cast<double2*>(corr));
+    CMX_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace cmx

@@ -178,6 +178,11 @@ void stout_smear(const int dims[4], double* links, int nstep, double rho, int di
 // meson two-point functions of two host-layout propagators for the 16 gamma insertions (meson.hip)
 void meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
                const int* moms, double* corr);
+// the same with device arrays and no allocation: the work space is a fixed device array of meson.hip,
+// set up once per device (meson_scratch_setup, with that device current)
+void meson_scratch_setup();
+void meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
+                       const int* moms, double* corr, hipStream_t stream);
 // MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
 void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
                       double* psi, const double* chi, cmx_solver_result* res);

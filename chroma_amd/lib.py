@@ -137,6 +137,7 @@ def lib():
         "cmx_gauge_stout_smear": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double]),
         "cmx_gauge_stout_smear_dirs": (ctypes.c_int, [_ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
         "cmx_meson_2pt": (ctypes.c_int, [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]),
+        "cmx_meson_2pt_noalloc": (ctypes.c_int, [_ip, _vp, _vp, _ip, ctypes.c_int, _vp, _vp, _vp]),
         "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
     }
     for name, (res, args) in sig.items():

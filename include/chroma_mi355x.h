@@ -350,6 +350,16 @@ int cmx_gauge_stout_smear_dirs(const int dims[4], double* links, int nstep, doub
  * complex (mesons() writes the real part).  Needs cmx_init; FP64 throughout. */
 int cmx_meson_2pt(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
                   const int* moms, double* corr);
+/* The same contraction without allocating: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * prop1, prop2, moms (3*nmom ints) and corr are DEVICE pointers with the layouts of cmx_meson_2pt; the
+ * kernels are queued on `stream` (a hipStream_t, NULL for the null stream) and the call returns without
+ * waiting, so corr is valid once the stream has drained.  The work space is a fixed 64 MiB device array
+ * of the library (per GPU, held until the process ends): the spatial volume Lx*Ly*Lz must not exceed
+ * 262144 sites (64^3); larger lattices are rejected with CMX_ERR_ARG.  Calls that overlap on one GPU share
+ * that array and race; serialise them.  The results equal cmx_meson_2pt's bit for bit. */
+int cmx_meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4],
+                          int nmom, const int* moms, double* corr, void* stream);
 /* Average plaquette Re tr(U_p)/3 over all sites and planes (MesPlq normalisation). */
 double cmx_plaquette(const int dims[4], const double* links);
 /* Z2 noise source: each real and imaginary part +-1/sqrt(2) from (seed, site, component). */

@@ -526,7 +526,7 @@ int cmx_op_get_hop_config(cmx_op h, int prec, int cfg[6]) {
 
 int cmx_op_set_multi_group(cmx_op h, int group) {
   return guarded([&] {
-    CMX_REQUIRE(group >= 0 && group <= 4, CMX_ERR_ARG, "set_multi_group: group must be 0..4");
+    CMX_REQUIRE(group >= 0 && group <= 5, CMX_ERR_ARG, "set_multi_group: group must be 0..5");
     O(h).multi_group = group;
   });
 }
@@ -595,30 +595,49 @@ int cmx_net_init_neumann(cmx_net net, double diag) {
   });
 }
 
-int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0) {
+int cmx_net_get_block(cmx_net net, int blk, double* w1, double* w2, double* w0) {
   return guarded([&] {
     cmx::LearnedNet& n = N(net);
-    auto put = [](const std::vector<cmx::LearnedNet::cf>& w, double* o) {
+    CMX_REQUIRE(blk >= 0 && blk < (int)n.blk.size(), CMX_ERR_ARG, "net_get_block: no such block");
+    auto put = [](const std::vector<cmx::PtcBlock::cf>& w, double* o) {
       if (!o) return;
       for (size_t k = 0; k < w.size(); ++k) o[2 * k] = w[k].real(), o[2 * k + 1] = w[k].imag();
     };
-    put(n.W1, w1);
-    put(n.W2, w2);
-    put(n.W0, w0);
+    put(n.blk[blk].W1, w1);
+    put(n.blk[blk].W2, w2);
+    put(n.blk[blk].W0, w0);
   });
 }
 
-int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0) {
+int cmx_net_set_block(cmx_net net, int blk, const double* w1, const double* w2, const double* w0) {
   return guarded([&] {
     cmx::LearnedNet& n = N(net);
-    auto get = [](std::vector<cmx::LearnedNet::cf>& w, const double* i) {
+    CMX_REQUIRE(blk >= 0 && blk < (int)n.blk.size(), CMX_ERR_ARG, "net_set_block: no such block");
+    auto get = [](std::vector<cmx::PtcBlock::cf>& w, const double* i) {
       if (!i) return;
-      for (size_t k = 0; k < w.size(); ++k) w[k] = cmx::LearnedNet::cf((float)i[2 * k], (float)i[2 * k + 1]);
+      for (size_t k = 0; k < w.size(); ++k) w[k] = cmx::PtcBlock::cf((float)i[2 * k], (float)i[2 * k + 1]);
     };
-    get(n.W1, w1);
-    get(n.W2, w2);
-    get(n.W0, w0);
+    get(n.blk[blk].W1, w1);
+    get(n.blk[blk].W2, w2);
+    get(n.blk[blk].W0, w0);
     n.version++;
+  });
+}
+
+int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0) { return cmx_net_get_block(net, 0, w1, w2, w0); }
+
+int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0) {
+  return cmx_net_set_block(net, 0, w1, w2, w0);
+}
+
+int cmx_net_set_depth(cmx_net net, int nblocks) {
+  return guarded([&] { cmx::net_set_depth(N(net), nblocks); });
+}
+
+int cmx_net_depth(cmx_net net, int* nblocks) {
+  return guarded([&] {
+    CMX_REQUIRE(nblocks, CMX_ERR_ARG, "net_depth: null pointer");
+    *nblocks = (int)N(net).blk.size();
   });
 }
 

@@ -17,5 +17,8 @@ struct MultiHopLaunch {
 
 // out[k] = b D(isign) in[k], k < nrhs; FP16 storage, 12-real links, real boundary phases
 void launch_hop_multi(const MultiHopLaunch& M, hipStream_t s);
+// the same with the link products on the matrix cores (hop_mfma.hip); M.group is not used
+constexpr int MULTI_MFMA = 5;
+void launch_hop_mfma(const MultiHopLaunch& M, hipStream_t s);
 
 }  // namespace cmx

@@ -408,6 +408,8 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
   M.h.cb = cb;
   M.h.epi = EPI_NONE;
   M.h.b = b;
+  // group 1..4: the vector-unit kernel with that many sources per lane; MULTI_MFMA: the link products
+  // on the matrix cores (hop_mfma.hip), all sources per launch
   auto run = [&]() {
     for (int r0 = 0; r0 < n; r0 += MULTI_MAXR) {
       M.nrhs = std::min(MULTI_MAXR, n - r0);
@@ -415,7 +417,10 @@ bool Op::dslash_multi(int prec, const VRef* out, const VRef* in, int n, int isig
         M.in[k] = in[r0 + k];
         M.out[k] = out[r0 + k];
       }
-      launch_hop_multi(M, stream);
+      if (M.group == MULTI_MFMA)
+        launch_hop_mfma(M, stream);
+      else
+        launch_hop_multi(M, stream);
     }
   };
   if (multi_group == 0) {  // first use: time groups of 1 to 4 sources (registers vs link reuse)

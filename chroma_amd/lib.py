@@ -117,6 +117,10 @@ def lib():
         "cmx_net_destroy": (ctypes.c_int, [_vp]),
         "cmx_net_init_neumann": (ctypes.c_int, [_vp, ctypes.c_double]),
         "cmx_net_get_weights": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+        "cmx_net_get_block": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+        "cmx_net_set_block": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+        "cmx_net_set_depth": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "cmx_net_depth": (ctypes.c_int, [_vp, _ip]),
         "cmx_net_set_weights": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
         "cmx_op_set_net": (ctypes.c_int, [_vp, _vp]),
         "cmx_net_apply": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
@@ -355,7 +359,7 @@ class Operator:
         return dict(block=cfg[0], tiled=cfg[1], zb=cfg[2], prefetch=cfg[3], fast=cfg[4])
 
     def set_multi_group(self, group):
-        """Sources per lane of dslash_multi: 0 timed at first use, or 1..4."""
+        """Kernel of dslash_multi: 1..4 sources per lane (vector units), 5 matrix cores, 0 timed at first use."""
         check(lib().cmx_op_set_multi_group(self.h, int(group)))
 
     def link_recon(self, prec):
@@ -466,19 +470,34 @@ class Net:
     def init_neumann(self, diag):
         check(lib().cmx_net_init_neumann(self.h, float(diag)))
 
-    def get_weights(self):
+    def set_depth(self, nblocks):
+        """1..4 blocks applied one after another; blocks added start as the identity."""
+        check(lib().cmx_net_set_depth(self.h, int(nblocks)))
+
+    def depth(self):
+        n = ctypes.c_int()
+        check(lib().cmx_net_depth(self.h, ctypes.byref(n)))
+        return n.value
+
+    def get_block(self, blk):
         w1 = np.zeros((64, 32), np.complex128)
         w2 = np.zeros((32, 64), np.complex128)
         w0 = np.zeros((4, 4), np.complex128)
-        check(lib().cmx_net_get_weights(self.h, _d(w1), _d(w2), _d(w0)))
+        check(lib().cmx_net_get_block(self.h, int(blk), _d(w1), _d(w2), _d(w0)))
         return w1, w2, w0
 
-    def set_weights(self, w1, w2, w0):
+    def set_block(self, blk, w1, w2, w0):
         w1 = np.ascontiguousarray(w1, np.complex128)
         w2 = np.ascontiguousarray(w2, np.complex128)
         w0 = np.ascontiguousarray(w0, np.complex128)
         assert w1.shape == (64, 32) and w2.shape == (32, 64) and w0.shape == (4, 4)
-        check(lib().cmx_net_set_weights(self.h, _d(w1), _d(w2), _d(w0)))
+        check(lib().cmx_net_set_block(self.h, int(blk), _d(w1), _d(w2), _d(w0)))
+
+    def get_weights(self):
+        return self.get_block(0)
+
+    def set_weights(self, w1, w2, w0):
+        self.set_block(0, w1, w2, w0)
 
 
 def net_attach(op, net):
@@ -498,13 +517,17 @@ def net_apply(op, v, prec=SINGLE):
 
 
 def net_grad(op, v, e, prec=SINGLE):
-    """(|P v - e|^2, d/d conj(W2 W1) complex[32, 32], d/d conj(W0) complex[4, 4])"""
+    """(|P v - e|^2, d/d conj(W2 W1) complex[32, 32], d/d conj(W0) complex[4, 4]) of a one-block network;
+    for a deeper network the gradients are complex[nblocks, 32, 32] and complex[nblocks, 4, 4]."""
     v = np.ascontiguousarray(v, np.complex128)
     e = np.ascontiguousarray(e, np.complex128)
+    nb = op.net.depth() if getattr(op, "net", None) is not None else 1
     loss = ctypes.c_double()
-    gwc = np.zeros((32, 32), np.complex128)
-    gw0 = np.zeros((4, 4), np.complex128)
+    gwc = np.zeros((nb, 32, 32), np.complex128)
+    gw0 = np.zeros((nb, 4, 4), np.complex128)
     check(lib().cmx_net_grad(op.h, int(prec), _d(v), _d(e), ctypes.byref(loss), _d(gwc), _d(gw0)))
+    if nb == 1:
+        return loss.value, gwc[0], gw0[0]
     return loss.value, gwc, gw0
 
 

@@ -241,20 +241,28 @@ int cmx_solve_batch(const cmx_op* ops, int n, const cmx_solver_params* p, const 
                     cmx_batch_result* bres);
 
 /* ---- learned preconditioner (CMX_PRECOND_LEARNED): no counterpart in the reference ------------------
- * A gauge-equivariant parallel-transport network on the odd checkerboard,
- *   P(v)(x) = W0 v(x) + sum_q U_q(x) g_q(x+q),  g_q(y) = sum_c W2[q][c] sum_p W1[c][p] U_p(y) v(y+p),
- * with one-hop paths p, q in {+-mu}, 16 hidden channels and 4x4 complex spin matrices W0, W1, W2 (colour
- * identity).  It is the right preconditioner of CMX_SOLVER_FGMRESDR (the slot of PrecondParams,
- * syssolver_linop_fgmres_dr.h:175-185).  A network is independent of the gauge field: train it on some
- * operators, attach it to others. */
+ * A gauge-equivariant parallel-transport network on the odd checkerboard, a chain of 1..4 blocks
+ * P = B_{n-1} o ... o B_0, each
+ *   B(v)(x) = W0 v(x) + sum_q U_q(x) Pi_q g_q(x+q),  g_q(y) = sum_c W2[q][c] sum_p W1[c][p] U_p(y) Pi_p v(y+p),
+ * with one-hop paths p, q in {+-mu}, their spin projectors Pi = (1 -+ gamma_mu)/2, 16 hidden channels and
+ * 4x4 complex spin matrices W0, W1, W2 (colour identity).  Every block carries its own transports, so the
+ * receptive field grows by two hops per block.  It is the right preconditioner of CMX_SOLVER_FGMRESDR (the
+ * slot of PrecondParams, syssolver_linop_fgmres_dr.h:175-185).  A network is independent of the gauge
+ * field: train it on some operators, attach it to others. */
 typedef struct cmx_net_s* cmx_net;
-int cmx_net_create(cmx_net* out);
+int cmx_net_create(cmx_net* out);  /* one block */
 int cmx_net_destroy(cmx_net net);
-/* W0 = 1/diag, W2 W1 = (1/(4 diag^3)) (1 -+ gamma_q)(1 -+ gamma_p): the two-term Neumann series of the
- * even-odd operator with the clover term replaced by its diagonal diag = Nd + m */
+/* 1..4 blocks; blocks added start as the identity (W0 = 1, W2 = 0, W1 channel c < 8 carrying path c) */
+int cmx_net_set_depth(cmx_net net, int nblocks);
+int cmx_net_depth(cmx_net net, int* nblocks);
+/* block 0: W0 = 1/diag, W2 W1 = (1/(4 diag^3)) (1 -+ gamma_q)(1 -+ gamma_p), the two-term Neumann series
+ * of the even-odd operator with the clover term replaced by its diagonal diag = Nd + m; the other blocks
+ * the identity */
 int cmx_net_init_neumann(cmx_net net, double diag);
-/* weights as interleaved (re, im) doubles: W1[(c*4+s)*32 + p*4+t] (64 x 32), W2[(q*4+s)*64 + c*4+t]
- * (32 x 64), W0[s*4+t] (4 x 4); path index p = 2 mu + (0 forward, 1 backward) */
+/* block weights as interleaved (re, im) doubles: W1[(c*4+s)*32 + p*4+t] (64 x 32), W2[(q*4+s)*64 + c*4+t]
+ * (32 x 64), W0[s*4+t] (4 x 4); path index p = 2 mu + (0 forward, 1 backward).  get/set_weights: block 0 */
+int cmx_net_get_block(cmx_net net, int blk, double* w1, double* w2, double* w0);
+int cmx_net_set_block(cmx_net net, int blk, const double* w1, const double* w2, const double* w0);
 int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0);
 int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0);
 int cmx_op_set_net(cmx_op op, cmx_net net);
@@ -264,7 +272,8 @@ int cmx_op_set_net(cmx_op op, cmx_net net);
 int cmx_op_set_net_variant(cmx_op op, int prec, int variant);
 /* out = P(in) on the odd checkerboard at storage precision prec (host arrays, Vh*24 doubles) */
 int cmx_net_apply(cmx_op op, int prec, double* out, const double* in);
-/* loss = |P(v) - e|^2; g_wc = d loss / d conj(W2 W1) (32 x 32 complex), g_w0 = d loss / d conj(W0) */
+/* loss = |P(v) - e|^2; per block b (nblocks of them, in order): g_wc[b] = d loss / d conj(W2 W1) of
+ * block b (32 x 32 complex), g_w0[b] = d loss / d conj(W0) of block b (4 x 4 complex) */
 int cmx_net_grad(cmx_op op, int prec, const double* v, const double* e, double* loss, double* g_wc, double* g_w0);
 typedef struct {
   int n_sources;      /* Gaussian sources solved by plain FP64 CGNE                                 */
@@ -302,9 +311,10 @@ int cmx_op_autotune(cmx_op op);
  * link is SU(3) times its boundary phase to 1e-14, else as 18; FP32 follows cmx_op_params.sloppy_recon;
  * FP16 is always 12.  Like QUDA's cuda_reconstruct (syssolver_quda_clover_params.cc:71-88). */
 int cmx_op_link_recon(cmx_op op, int prec, int* recon);
-/* Sources per lane of cmx_dslash_multi (the links are loaded once per lane for all of them): 0 timed
- * at first use (default), or 1..4.  A tuning knob like cmx_op_set_hop_config; no counterpart in the
- * reference. */
+/* Kernel of cmx_dslash_multi: 1..4 sources per lane on the vector units (the links are loaded once per
+ * lane for all of them), 5 the link products of all sources on the matrix cores (one site per 4 lanes,
+ * v_mfma_f32_4x4x4_16b_f16), 0 timed at first use (default).  A tuning knob like
+ * cmx_op_set_hop_config; no counterpart in the reference. */
 int cmx_op_set_multi_group(cmx_op op, int group);
 
 /* ---- measurement -------------------------------------------------------------------------------

@@ -474,6 +474,33 @@ def test_dslash_multi_source(L, nrhs):
                 assert np.array_equal(got[0].view(np.uint64), single.view(np.uint64))
 
 
+@pytest.mark.parametrize("L", [(8, 8, 8, 16), (16, 16, 8, 16), (6, 4, 8, 10), (12, 12, 12, 24)])
+@pytest.mark.parametrize("nrhs", [1, 5, 12])
+def test_dslash_multi_source_mfma(L, nrhs):
+    """The multi-source hop with the link products on the matrix cores (cmx_op_set_multi_group 5): every
+    source against the oracle at the FP16 tolerance and against the single-source kernel (the same FP16
+    operands; products summed in FP32 in another order), both site orders, both link cache policies."""
+    say("dslash multi mfma", L, nrhs)
+    V = int(np.prod(L))
+    Vh = V // 2
+    U = gauge_field(L, 22)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_WILSON, 0.1, sloppy_prec=cmx.HALF, sloppy_recon=12)
+    psis = np.stack([random_fermion(V, 200 + k) for k in range(nrhs)])
+    op.set_multi_group(5)
+    tiled_ok = L[3] % 8 == 0
+    for fast, tiled in [(1, 0), (3, 0)] + ([(1, 1), (3, 1)] if tiled_ok else []):
+        op.set_hop_config(cmx.HALF, 256, tiled, 1, 0, fast, target=1)
+        for cb in (0, 1):
+            for isign in (+1, -1):
+                src = np.ascontiguousarray(psis[:, (1 - cb) * Vh:(2 - cb) * Vh])
+                got = op.dslash_multi(src, isign, cb, cmx.HALF)
+                for k in range(nrhs):
+                    ref = oracle.dslash(L, U, psis[k], isign, cb)[cb * Vh:(cb + 1) * Vh]
+                    assert rel(got[k], ref) < TOL[cmx.HALF], (fast, tiled, cb, isign, k)
+                    single = op.dslash(src[k], isign, cb, cmx.HALF)
+                    assert rel(got[k], single) < 1e-3, (fast, tiled, cb, isign, k)
+
+
 @pytest.mark.parametrize("solver,sloppy", [(cmx.CG, cmx.DOUBLE), (cmx.BICGSTAB, cmx.DOUBLE),
                                            (cmx.RELIABLE_CG, cmx.HALF), (cmx.RELIABLE_CG, cmx.SINGLE)])
 def test_graph_replay_matches_eager(solver, sloppy):

@@ -36,22 +36,37 @@ void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
 // inference: variant 0 operands in registers, 1 transports staged in LDS
 void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s);
-void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s);
+// adj: the block's adjoint W0^dag delta + G^dag(L.G) (L.W0 holding W0^dag, L.G the mixed adjoint)
+void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_grad(const PtcLaunch& L, double* out1024, hipStream_t s);
 int launch_w0_grad(const PtcLaunch& L, hipStream_t s);
 
-// Network weights (host, FP32 complex) with Adam state.  Component indices: path/channel-major,
-// spin-minor: W1[(c*4 + s) * 32 + (p*4 + t)], W2[(q*4 + s) * 64 + (c*4 + t)], W0[s*4 + t].
-struct LearnedNet {
+constexpr int PTC_MAX_BLOCKS = 4;
+
+// One block of the network (host, FP32 complex) with its Adam state.  Component indices: path/channel-
+// major, spin-minor: W1[(c*4 + s) * 32 + (p*4 + t)], W2[(q*4 + s) * 64 + (c*4 + t)], W0[s*4 + t].
+//   B(v)(x) = W0 v(x) + sum_q U_q(x) Pi_q g_q(x+q),  g_q(y) = sum_p Wc[q][p] U_p(y) (Pi_p v)(y+p),  Wc = W2 W1
+struct PtcBlock {
   using cf = std::complex<float>;
   std::vector<cf> W1, W2, W0;
   std::vector<float> m1, v1, m2, v2, m0, v0;  // Adam moments over (re, im)
-  long step = 0;
-  long version = 0;  // bumped on every weight change; operators re-upload when it differs
-  LearnedNet();
+  PtcBlock();
   // Wc = W2 W1 (32 x 32 complex), row-major over (q*4+s, p*4+t); the network applies it through the
   // half-spinor projectors of the paths (Pi_q Wc Pi_p)
   std::vector<cf> contracted() const;
+  // B = 1: W0 the identity, W2 = 0, and W1 channel c < 8 carrying path c with its projector (so that the
+  // gradient reaches W2 from the start)
+  void set_identity();
+  void reset_moments();
+};
+
+// The network: blocks applied one after another, P = B_{n-1} o ... o B_1 o B_0.  Every block carries
+// its own transports, so the receptive field grows by two hops per block.
+struct LearnedNet {
+  std::vector<PtcBlock> blk;
+  long step = 0;
+  long version = 0;  // bumped on every weight change; operators re-upload when it differs
+  LearnedNet() : blk(1) {}
 };
 
 }  // namespace cmx

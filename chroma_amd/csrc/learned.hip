@@ -24,7 +24,8 @@
 //   k_ptc_gather_mix             both of the above in one kernel, transports packed straight into the
 //                                MFMA operands (inference path)
 //   k_ptc_gather_mix_lds         the same with the transports staged in LDS tiles
-//   k_ptc_scatter  (odd sites)   P(v) = W0 v + sum_q U_q R_q gh_q
+//   k_ptc_scatter  (odd sites)   P(v) = W0 v + sum_q U_q R_q gh_q (with ADJ: a block's adjoint,
+//                                W0^dag delta + G^dag(dth), from the mixed adjoint dth = Wh^dag dgh)
 //   k_ptc_grad     (MFMA)        Sh = sum_{y,c} dgh(y,c) th(y,c)^dag, the gradient w.r.t. conj(Wh)
 //   k_w0_grad                    sum_x delta(x) v(x)^dag, the gradient w.r.t. conj(W0)
 #include "hop_kernel.h"
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
 }
 
 // P(v)(x) = W0 v(x) + sum_q U_q(x) R_q gh_q(x+q) at odd sites x
-template <int P, int RECON, int MU>
+template <int P, int RECON, int MU, bool ADJ>
 __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4], int xi, float (&acc)[24]) {
   using R = typename SpinorIO<P>::R;
   constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
@@ -385,7 +386,10 @@ __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4]
     const bool fwd = d == 0;
     bool last;
     const int yn = ptc_neighbour(g, x, MU, fwd, last);
-    const int q = 2 * MU + d;
+    // ADJ: the adjoint of the gather, G^dag(t)(x) = sum_p Phat_p^dag U_p(x-p)^dag t_p(x-p): the path of the
+    // opposite direction (t_{-q} at x + q) reconstructed with R_{-q} = 2 Phat_{-q}^dag (the factor 1/2 is in
+    // the weights); the transports are the scatter's
+    const int q = 2 * MU + (ADJ ? (d ^ 1) : d);
     float gh[12];  // gh_q(y) as [s'][colour][re,im]
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -406,11 +410,11 @@ __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4]
       LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
       transport2<true>(u, gh, o);
     }
-    reconstruct_add<MU>(o, fwd ? -1.f : 1.f, acc);
+    reconstruct_add<MU>(o, (fwd ? -1.f : 1.f) * (ADJ ? -1.f : 1.f), acc);
   }
 }
 
-template <int P, int RECON>
+template <int P, int RECON, bool ADJ>
 __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
   const int xi = blockIdx.x * blockDim.x + threadIdx.x;
   const Geom& g = L.g;
@@ -438,10 +442,10 @@ __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
         acc[2 * (3 * s + c) + 1] = im;
       }
   }
-  scatter_mu<P, RECON, 0>(L, x, xi, acc);
-  scatter_mu<P, RECON, 1>(L, x, xi, acc);
-  scatter_mu<P, RECON, 2>(L, x, xi, acc);
-  scatter_mu<P, RECON, 3>(L, x, xi, acc);
+  scatter_mu<P, RECON, 0, ADJ>(L, x, xi, acc);
+  scatter_mu<P, RECON, 1, ADJ>(L, x, xi, acc);
+  scatter_mu<P, RECON, 2, ADJ>(L, x, xi, acc);
+  scatter_mu<P, RECON, 3, ADJ>(L, x, xi, acc);
   typename SpinorIO<P>::R out[24];
 #pragma unroll
   for (int k = 0; k < 24; ++k) out[k] = (typename SpinorIO<P>::R)acc[k];
@@ -562,10 +566,13 @@ void launch_ptc_mix(const PtcLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(k_ptc_mix, dim3((waves + 3) / 4), dim3(256), 0, s, L);
 }
 
-void launch_ptc_scatter(const PtcLaunch& L, hipStream_t s) {
+void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s) {
   const dim3 grid((L.g.Vh + 255) / 256), block(256);
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
-    hipLaunchKernelGGL((k_ptc_scatter<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
+    if (adj)
+      hipLaunchKernelGGL((k_ptc_scatter<decltype(P)::value, decltype(RC)::value, true>), grid, block, 0, s, L);
+    else
+      hipLaunchKernelGGL((k_ptc_scatter<decltype(P)::value, decltype(RC)::value, false>), grid, block, 0, s, L);
   });
 }
 

@@ -162,6 +162,8 @@ Op* clone_op(Op& o, int prec);
 // Solvers (solvers.cpp)
 // Learned preconditioner (learned.cpp)
 void net_init_neumann(LearnedNet& net, double diag);
+// 1..4 blocks; blocks added start as the identity
+void net_set_depth(LearnedNet& net, int nblocks);
 void net_apply_host(Op& op, int prec, double* out, const double* in);
 void net_set_variant(Op& op, int prec, int variant);
 void net_grad_host(Op& op, int prec, const double* v, const double* e, double* loss, double* gwc, double* gw0);

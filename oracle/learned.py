@@ -108,14 +108,24 @@ def ptc_apply(L, U, W1, W2, W0, v_odd, return_parts=False):
     return out
 
 
-def ptc_grad(L, U, W1, W2, W0, v_odd, e_odd):
-    """(|P v - e|^2, dL/d conj(Wc) [32, 32], dL/d conj(W0) [4, 4]) with Wc = W2 W1."""
+def ptc_block_grad(L, U, W1, W2, W0, v_odd, delta):
+    """(dL/d conj(Wc) [32, 32], dL/d conj(W0) [4, 4]) of one block B with input v and output gradient
+    delta = dL/d conj(B(v))."""
     V = int(np.prod(L))
     Vh = V // 2
-    out, tk = ptc_apply(L, U, W1, W2, W0, v_odd, return_parts=True)
-    delta = out - e_odd
-    loss = float(np.vdot(delta, delta).real)
-    # adjoint of the scatter: dg_q(y) = U_q(x)^dag delta(x) for y = x + q
+    full = np.zeros((V, 4, 3), complex)
+    full[Vh:] = v_odd
+    tk = _transports(L, U, full, np.arange(Vh)).transpose(1, 0, 2, 3).reshape(Vh, 32, 3)
+    dgk = _scatter_adjoint(L, U, delta)
+    S = np.einsum("kqc,kpc->qp", dgk, tk.conj())
+    S0 = np.einsum("ksc,ktc->st", delta, v_odd.conj())
+    return S, S0
+
+
+def _scatter_adjoint(L, U, delta):
+    """dg_q(y) = Pi_q U_q(x)^dag delta(x) for y = x + q: the adjoint of the scatter, as [Vh, 32, 3]."""
+    V = int(np.prod(L))
+    Vh = V // 2
     fwd, bwd = neighbour_tables(L)
     odd = np.arange(Vh, V)
     dg = np.zeros((8, V, 4, 3), complex)
@@ -126,7 +136,67 @@ def ptc_grad(L, U, W1, W2, W0, v_odd, e_odd):
         np.add.at(dg[2 * mu + 1], yb, np.einsum("kab,ksb->ksa", U[mu, yb], delta))
     for q in range(8):  # the adjoint of Pi_q (Hermitian)
         dg[q] = np.einsum("st,ktc->ksc", projector(q), dg[q])
-    dgk = dg[:, :Vh].transpose(1, 0, 2, 3).reshape(Vh, 32, 3)
-    S = np.einsum("kqc,kpc->qp", dgk, tk.conj())
-    S0 = np.einsum("ksc,ktc->st", delta, v_odd.conj())
+    return dg[:, :Vh].transpose(1, 0, 2, 3).reshape(Vh, 32, 3)
+
+
+def ptc_grad(L, U, W1, W2, W0, v_odd, e_odd):
+    """(|P v - e|^2, dL/d conj(Wc) [32, 32], dL/d conj(W0) [4, 4]) with Wc = W2 W1."""
+    out = ptc_apply(L, U, W1, W2, W0, v_odd)
+    delta = out - e_odd
+    loss = float(np.vdot(delta, delta).real)
+    S, S0 = ptc_block_grad(L, U, W1, W2, W0, v_odd, delta)
     return loss, S, S0
+
+
+def ptc_adjoint(L, U, W1, W2, W0, d_odd):
+    """B^dag d = W0^dag d + G^dag(Wc^dag S^dag d) for one block: S^dag the adjoint of the scatter, Wc^dag
+    the adjoint mixing, G^dag(w)(x) = sum_p Pi_p U_p(x-p)^dag w_p(x-p) the adjoint of the transports."""
+    V = int(np.prod(L))
+    Vh = V // 2
+    dgk = _scatter_adjoint(L, U, d_odd)
+    Wc = W2 @ W1
+    dtk = np.einsum("qp,kqc->kpc", Wc.conj(), dgk).reshape(Vh, 8, 4, 3)
+    fwd, bwd = neighbour_tables(L)
+    even = np.arange(Vh)
+    full = np.zeros((V, 4, 3), complex)
+    for mu in range(4):
+        w = np.einsum("st,ktc->ksc", projector(2 * mu), dtk[:, 2 * mu])
+        np.add.at(full, fwd[mu][even], np.einsum("kba,ksb->ksa", U[mu, even].conj(), w))
+        yb = bwd[mu][even]
+        w = np.einsum("st,ktc->ksc", projector(2 * mu + 1), dtk[:, 2 * mu + 1])
+        np.add.at(full, yb, np.einsum("kab,ksb->ksa", U[mu, yb], w))
+    return np.einsum("ts,kta->ksa", W0.conj(), d_odd) + full[Vh:]
+
+
+def ptc_apply_deep(L, U, blocks, v_odd):
+    """P(v) = B_{n-1}(... B_0(v)) for blocks = [(W1, W2, W0), ...]."""
+    out = v_odd
+    for W1, W2, W0 in blocks:
+        out = ptc_apply(L, U, W1, W2, W0, out)
+    return out
+
+
+def ptc_grad_deep(L, U, blocks, v_odd, e_odd):
+    """(|P v - e|^2, [(dL/d conj(Wc_k), dL/d conj(W0_k)) per block k]): back-propagation through the
+    chain, the output gradient of block k carried to its input by B_k^dag."""
+    us = [v_odd]
+    for W1, W2, W0 in blocks[:-1]:
+        us.append(ptc_apply(L, U, W1, W2, W0, us[-1]))
+    W1, W2, W0 = blocks[-1]
+    delta = ptc_apply(L, U, W1, W2, W0, us[-1]) - e_odd
+    loss = float(np.vdot(delta, delta).real)
+    grads = [None] * len(blocks)
+    for k in range(len(blocks) - 1, -1, -1):
+        W1, W2, W0 = blocks[k]
+        grads[k] = ptc_block_grad(L, U, W1, W2, W0, us[k], delta)
+        if k > 0:
+            delta = ptc_adjoint(L, U, W1, W2, W0, delta)
+    return loss, grads
+
+
+def identity_block():
+    """The identity block (W0 = 1, W2 = 0, W1 channel c < 8 carrying path c with 1 -+ gamma_mu)."""
+    W1 = np.zeros((64, 32), complex)
+    for p in range(8):
+        W1[p * 4:p * 4 + 4, p * 4:p * 4 + 4] = 2 * projector(p)
+    return W1, np.zeros((32, 64), complex), np.eye(4, dtype=complex)

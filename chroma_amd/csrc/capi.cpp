@@ -208,6 +208,95 @@ int cmx_dslash_deriv(cmx_op h, double* ds_u, const double* chi, const double* ps
   });
 }
 
+namespace {
+// ds (device, [4][V][9] double2) -> host ds_u [4][V][18] doubles
+void download_force(cmx::Op& op, const cmx::DevBuf& ds, double* ds_u) {
+  CMX_HIP(hipMemcpyAsync(ds_u, ds.p, sizeof(double) * 4 * 18 * op.V, hipMemcpyDeviceToHost, op.stream));
+  op.sync();
+}
+}  // namespace
+
+int cmx_clover_deriv(cmx_op h, double* ds_u, const double* chi, const double* psi, int isign, int cb) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_deriv: not a clover operator");
+    CMX_REQUIRE(ds_u && chi && psi, CMX_ERR_ARG, "clover_deriv: null array");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "clover_deriv: isign must be +1 or -1");
+    CMX_REQUIRE(cb >= -1 && cb <= 1, CMX_ERR_ARG, "clover_deriv: cb must be 0, 1 or -1");
+    const long Vh = op.Vh;
+    cmx::Vec a(cmx::P64, (int)Vh), b(cmx::P64, (int)Vh);
+    cmx::DevBuf ds((size_t)4 * 9 * 16 * op.V);
+    CMX_HIP(hipMemsetAsync(ds.p, 0, ds.bytes, op.stream));
+    for (int c = 0; c < 2; ++c) {
+      if (cb >= 0 && c != cb) continue;
+      op.pack(chi + (size_t)c * 24 * Vh, (int)Vh, a);
+      op.pack(psi + (size_t)c * 24 * Vh, (int)Vh, b);
+      cmx::clover_force(op, false, c, a.view(), b.view(), 1.0, ds.p);
+    }
+    download_force(op, ds, ds_u);
+  });
+}
+
+int cmx_clover_deriv_trln(cmx_op h, double* ds_u) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_ARG, "clover_deriv_trln: not a clover operator");
+    CMX_REQUIRE(ds_u, CMX_ERR_ARG, "clover_deriv_trln: null array");
+    cmx::DevBuf ds((size_t)4 * 9 * 16 * op.V);
+    CMX_HIP(hipMemsetAsync(ds.p, 0, ds.bytes, op.stream));
+    cmx::clover_force(op, true, 0, cmx::VRef{}, cmx::VRef{}, 1.0, ds.p);
+    download_force(op, ds, ds_u);
+  });
+}
+
+int cmx_op_deriv(cmx_op h, double* ds_u, const double* chi, const double* psi, int isign) {
+  return guarded([&] {
+    require_init();
+    cmx::Op& op = O(h);
+    CMX_REQUIRE(op.clover(), CMX_ERR_UNSUPPORTED, "op_deriv: the even-odd clover operator only");
+    CMX_REQUIRE(ds_u && chi && psi, CMX_ERR_ARG, "op_deriv: null array");
+    CMX_REQUIRE(isign == 1 || isign == -1, CMX_ERR_ARG, "op_deriv: isign must be +1 or -1");
+    const int Vh = (int)op.Vh;
+    cmx::Vec co(cmx::P64, Vh), po(cmx::P64, Vh), t2(cmx::P64, Vh), t3(cmx::P64, Vh);
+    cmx::DevBuf ds((size_t)4 * 9 * 16 * op.V), dcb((size_t)4 * 9 * 16 * Vh);
+    CMX_HIP(hipMemsetAsync(ds.p, 0, ds.bytes, op.stream));
+    op.pack(chi, Vh, co);
+    op.pack(psi, Vh, po);
+    // t = A_ee^-1 D_eo(sign) v, the first kernel of the even-odd operator at FP64
+    auto ainv_d = [&](cmx::Vec& t, const cmx::Vec& v, int sign) {
+      cmx::HopLaunch hh = op.hop_base(cmx::P64);
+      hh.isign = sign;
+      hh.cb = 0;
+      const cmx::VRef vi = v.view(), vo = t.view();
+      hh.in = vi.v, hh.in_nrm = nullptr, hh.in_stride = vi.stride;
+      hh.out = vo.v, hh.out_nrm = nullptr, hh.out_stride = vo.stride;
+      hh.epi = cmx::EPI_CLOV;
+      hh.clov = op.clovinv_even[cmx::P64].p;
+      hh.b = 1.0;
+      hh.partials = nullptr;
+      cmx::launch_hop(hh, op.stream);
+      CMX_HIP(hipGetLastError());
+    };
+    // EvenOddPrecLogDetLinearOperator::deriv (eoprec_logdet_linop.h:114-157) with the reference's
+    // tmp2 = A_ee^-1 (-1/2 D_eo psi) = -t2/2 and tmp3 = A_ee^-1 (-1/2 D_eo(-isign) chi) = -t3/2:
+    //   ds  = clov.deriv(chi, psi, cb 1)
+    //   ds -= -1/2 D.deriv(chi, tmp2, cb 1)     = -1/4 D.deriv(chi, t2, cb 1)
+    //   ds += clov.deriv(tmp3, tmp2, cb 0)      = +1/4 clov.deriv(t3, t2, cb 0)
+    //   ds -= -1/2 D.deriv(tmp3, psi, cb 0)     = -1/4 D.deriv(t3, psi, cb 0)
+    cmx::clover_force(op, false, 1, co.view(), po.view(), 1.0, ds.p);
+    ainv_d(t2, po, isign);
+    cmx::hop_deriv(op.g, 1, isign, co.view(), t2.view(), dcb.p, op.stream);
+    cmx::force_axpy(op, 1, -0.25, dcb.p, ds.p);
+    ainv_d(t3, co, -isign);
+    cmx::clover_force(op, false, 0, t3.view(), t2.view(), 0.25, ds.p);
+    cmx::hop_deriv(op.g, 0, isign, t3.view(), po.view(), dcb.p, op.stream);
+    cmx::force_axpy(op, 0, -0.25, dcb.p, ds.p);
+    download_force(op, ds, ds_u);
+  });
+}
+
 int cmx_gauge_stout_smear(const int dims[4], double* links, int nstep, double rho) {
   return guarded([&] {
     require_init();

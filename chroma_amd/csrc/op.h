@@ -174,6 +174,12 @@ void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, b
 // WilsonDslashBase::deriv (lwldslash_base_w.h:132-227) for the sites of cb, FP64 (deriv.hip): chi on cb,
 // psi on 1 - cb, ds = 4 * Vh * 9 double2 (direction, site of cb, 3x3 row-major)
 void hop_deriv(const Geom& g, int cb, int isign, VRef chi, VRef psi, void* ds, hipStream_t s);
+// Clover force (force.hip), FP64, accumulated into ds_full = [4][V][9] double2 (host site order) times
+// scale: trln = false: CloverTermBase::deriv on cb from chi, psi (views of cb); trln = true: derivTrLn
+// of the inverted even-even term (cb = 0; chi, psi unused).  force_axpy adds f times a hopping-term
+// derivative of one checkerboard (hop_deriv's layout) into ds_full.
+void clover_force(Op& op, bool trln, int cb, VRef chi, VRef psi, double scale, void* ds_full);
+void force_axpy(Op& op, int cb, double f, const void* ds_cb, void* ds_full);
 // nstep stout steps on the device over the directions in the bit mask dirs, in place on host-layout
 // links (stout.hip)
 void stout_smear(const int dims[4], double* links, int nstep, double rho, int dirs = 15);

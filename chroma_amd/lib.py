@@ -101,6 +101,9 @@ def lib():
         "cmx_op_eo_piece": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
         "cmx_dslash_multi": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "cmx_dslash_deriv": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+        "cmx_clover_deriv": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+        "cmx_clover_deriv_trln": (ctypes.c_int, [_vp, _dp]),
+        "cmx_op_deriv": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int]),
         "cmx_clover_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
         "cmx_clover_logdet": (ctypes.c_int, [_vp, _dp, _lp]),
         "cmx_clover_pack_quda": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
@@ -317,6 +320,26 @@ class Operator:
         ds = np.zeros((4, self.V, 3, 3), np.complex128)
         check(lib().cmx_dslash_deriv(self.h, _d(ds), _d(np.ascontiguousarray(chi_full, np.complex128)),
                                      _d(np.ascontiguousarray(psi_full, np.complex128)), int(isign), int(cb)))
+        return ds
+
+    def clover_deriv(self, chi_full, psi_full, isign=PLUS, cb=-1):
+        """CloverTermBase::deriv: ds[4, V, 3, 3] of chi^dag A' psi over cb (-1: both checkerboards)."""
+        ds = np.zeros((4, self.V, 3, 3), np.complex128)
+        check(lib().cmx_clover_deriv(self.h, _d(ds), _d(np.ascontiguousarray(chi_full, np.complex128)),
+                                     _d(np.ascontiguousarray(psi_full, np.complex128)), int(isign), int(cb)))
+        return ds
+
+    def clover_deriv_trln(self):
+        """CloverTermBase::derivTrLn of the inverted even-even clover term (the even-even log det force)."""
+        ds = np.zeros((4, self.V, 3, 3), np.complex128)
+        check(lib().cmx_clover_deriv_trln(self.h, _d(ds)))
+        return ds
+
+    def deriv(self, chi_odd, psi_odd, isign=PLUS):
+        """EvenOddPrecLogDetLinearOperator::deriv: ds[4, V, 3, 3] of chi^dag M'(isign) psi (odd arrays)."""
+        ds = np.zeros((4, self.V, 3, 3), np.complex128)
+        check(lib().cmx_op_deriv(self.h, _d(ds), _d(np.ascontiguousarray(chi_odd, np.complex128)),
+                                 _d(np.ascontiguousarray(psi_odd, np.complex128)), int(isign)))
         return ds
 
     def piece(self, which, psi_full, isign=PLUS):

@@ -127,6 +127,21 @@ int cmx_op_eo_piece(cmx_op op, int which, double* chi, const double* psi, int is
  * fermions (V*24 doubles); ds_u: 4*V*18 doubles in the link layout (no link factor: the force is
  * taproj(U ds_u) as the reference forms it). */
 int cmx_dslash_deriv(cmx_op op, double* ds_u, const double* chi, const double* psi, int isign, int cb);
+/* Clover force terms, FP64 on the device, ds_u as for cmx_dslash_deriv ([4][V][3][3] complex, host order):
+ *   cmx_clover_deriv      CloverTermBase::deriv (clover_term_base_w.h:691-745): ds of chi^dag A' psi over the
+ *                         sites of cb (-1: both), chi and psi full-lattice host arrays (V*24 doubles); the
+ *                         term is Hermitian, so isign only selects the reference's overload;
+ *   cmx_clover_deriv_trln CloverTermBase::derivTrLn (:825-870) of the inverted even-even term, i.e.
+ *                         EvenOddPrecCloverLinOp::derivLogDetEvenEvenLinOp (eoprec_clover_linop_w.cc:225-235);
+ *   cmx_op_deriv          EvenOddPrecLogDetLinearOperator::deriv (eoprec_logdet_linop.h:114-157) of the
+ *                         even-odd clover operator: ds of chi^dag M'(isign) psi, chi and psi odd-checkerboard
+ *                         arrays (Vh*24 doubles).
+ * ds is the derivative with respect to the occurrences of U (not U^dag), as WilsonDslashBase::deriv's:
+ * for U_mu(y) -> exp(eps X) U_mu(y), d<chi, M psi>/d eps = tr(X U ds(chi, psi, isign)) +
+ * conj tr(X U ds(psi, chi, -isign)).  Clover operators only. */
+int cmx_clover_deriv(cmx_op op, double* ds_u, const double* chi, const double* psi, int isign, int cb);
+int cmx_clover_deriv_trln(cmx_op op, double* ds_u);
+int cmx_op_deriv(cmx_op op, double* ds_u, const double* chi, const double* psi, int isign);
 
 /* Clover term download (FP64): which 0 = A on all sites, 1 = A_ee^-1 on the even sites.
  * tri: V*72 doubles in PrimitiveClovTriang layout; sites not covered are left untouched. */

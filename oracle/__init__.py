@@ -43,6 +43,10 @@ def lib():
         L.orc_op_eo_piece.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.orc_dslash.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_dslash_deriv.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_clover_deriv.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_clover_deriv_trln.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_eo_clover_deriv.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                          ctypes.c_int, _dp]
         L.orc_dslash_f32.argtypes = [_ip, _fp, _fp, _fp, ctypes.c_int, ctypes.c_int]
         L.orc_mesfield.argtypes = [_ip, _dp, _dp]
         L.orc_stout.argtypes = [_ip, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
@@ -100,6 +104,34 @@ def dslash_deriv(L, chi, psi, isign, cb=-1):
     ds = np.zeros((4, int(np.prod(L)), 3, 3), np.complex128)
     _check(lib().orc_dslash_deriv(_dims(L), _d(ds), _d(np.ascontiguousarray(chi, np.complex128)),
                                   _d(np.ascontiguousarray(psi, np.complex128)), int(isign), int(cb)))
+    return ds
+
+
+def clover_deriv(L, U, chi, psi, csw_r=1.0, csw_t=None, cb=-1):
+    """CloverTermBase::deriv (clover_term_base_w.h:691-745): ds[4, V, 3, 3] of chi^dag A' psi on cb (-1: both)."""
+    ds = np.zeros((4, int(np.prod(L)), 3, 3), complex)
+    _check(lib().orc_clover_deriv(_dims(L), _d(np.ascontiguousarray(U, np.complex128)), float(csw_r),
+                                  float(csw_r if csw_t is None else csw_t), _d(np.ascontiguousarray(chi, np.complex128)),
+                                  _d(np.ascontiguousarray(psi, np.complex128)), int(cb), _d(ds)))
+    return ds
+
+
+def clover_deriv_trln(L, U, mass, csw_r=1.0, csw_t=None, cb=0):
+    """CloverTermBase::derivTrLn (clover_term_base_w.h:825-870) of the clover term inverted on cb."""
+    ds = np.zeros((4, int(np.prod(L)), 3, 3), complex)
+    _check(lib().orc_clover_deriv_trln(_dims(L), _d(np.ascontiguousarray(U, np.complex128)), float(csw_r),
+                                       float(csw_r if csw_t is None else csw_t), float(mass), int(cb), _d(ds)))
+    return ds
+
+
+def eo_clover_deriv(L, U, mass, chi, psi, isign=+1, csw_r=1.0, csw_t=None):
+    """EvenOddPrecLogDetLinearOperator::deriv (eoprec_logdet_linop.h:114-157) of the EO clover operator:
+    ds of chi^dag M' psi, chi and psi full-lattice (odd half used)."""
+    ds = np.zeros((4, int(np.prod(L)), 3, 3), complex)
+    _check(lib().orc_eo_clover_deriv(_dims(L), _d(np.ascontiguousarray(U, np.complex128)), float(mass), float(csw_r),
+                                     float(csw_r if csw_t is None else csw_t),
+                                     _d(np.ascontiguousarray(chi, np.complex128)),
+                                     _d(np.ascontiguousarray(psi, np.complex128)), int(isign), _d(ds)))
     return ds
 
 

@@ -519,6 +519,207 @@ template <typename R> void clover_apply(const Geom& g, const Clov<R>* tri, cplx<
 }
 
 // ------------------------------------------------------------------------------------------------
+// Clover force terms (CloverTermBase, clover_term_base_w.h).
+//   deriv (:691-745): for every plane mu < nu,
+//     Lambda(x) = -(1/8) c_munu traceSpin(outerProduct(Gamma(2^mu + 2^nu) psi, chi))(x)  on cb, 0 elsewhere,
+//     c_munu = getCloverCoeff(mu, nu) (clover_term_qdp_w.h:1525-1547: the 1/2-scaled clovCoeffR, or
+//     clovCoeffT when nu is the time direction), then deriv_loops(mu, nu, cb, Lambda) -> ds_mu, ds_nu and
+//     ds[mu] += ds_mu, ds[nu] -= ds_nu; the clover term is Hermitian, so isign does not enter.
+//   derivTrLn (:825-870): the same with Lambda = -(1/8) c_munu triacntr(2^mu + 2^nu) of the (inverted)
+//     clover term on cb, triacntr(mat)(x) = traceSpin(Gamma(mat) A(x)) (clover_term_qdp_w.h:1213-1500:
+//     B_ij = sum_{s,t} Gamma_st A_(t,i),(s,j) for the six mu-nu matrices).
+// deriv_loops (:320-683): the 16 insertions of Lambda into the four clover leaves of the plane, with
+// Lambda on cb only; the corners and staples on all sites, then the terms on cb and on 1-cb, and the
+// two accumulators shifted back by nu and mu.
+template <typename R> static inline void aam(cplx<R>* c, const cplx<R>* a, const cplx<R>* b) {  // a^dag b^dag
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cplx<R> t = 0;
+      for (int k = 0; k < 3; ++k) t += std::conj(a[k * 3 + i]) * std::conj(b[j * 3 + k]);
+      c[i * 3 + j] = t;
+    }
+}
+template <typename R> static inline void adj3(cplx<R>* c, const cplx<R>* a) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) c[i * 3 + j] = std::conj(a[j * 3 + i]);
+}
+template <typename R> static inline void mm3(cplx<R>* d, const cplx<R>* a, const cplx<R>* b, const cplx<R>* c) {  // a b c
+  cplx<R> t[9];
+  mm(t, a, b);
+  mm(d, t, c);
+}
+
+template <typename R>
+static void deriv_loops(const Geom& g, const cplx<R>* U, int mu, int nu, int cb, const cplx<R>* Lam, cplx<R>* ds_mu,
+                        cplx<R>* ds_nu) {
+  const long V = g.V;
+  std::vector<cplx<R>> ulc(V * 9), urc(V * 9), lrc(V * 9), llc(V * 9), sfor(V * 9), sright(V * 9), sleft(V * 9),
+      sback(V * 9), tmu(V * 9), tnu(V * 9);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < V; ++s) {
+    const cplx<R>* unu_fmu = link(U, V, nu, g.fwd(s, mu));  // u_nu_for_mu
+    const cplx<R>* umu_fnu = link(U, V, mu, g.fwd(s, nu));  // u_mu_for_nu
+    const cplx<R>* umu = link(U, V, mu, s);
+    const cplx<R>* unu = link(U, V, nu, s);
+    aam(&ulc[s * 9], umu_fnu, unu);                 // up_left_corner  = adj(u_mu_for_nu) adj(u_nu)
+    mma(&urc[s * 9], unu_fmu, umu_fnu);             // up_right_corner = u_nu_for_mu adj(u_mu_for_nu)
+    aam(&lrc[s * 9], unu_fmu, umu);                 // low_right_corner = adj(u_nu_for_mu) adj(u_mu)
+    amm(&llc[s * 9], umu, unu);                     // low_left_corner = adj(u_mu) u_nu
+    mm(&sfor[s * 9], unu_fmu, &ulc[s * 9]);         // staple_for
+    mm(&sright[s * 9], &ulc[s * 9], umu);           // staple_right
+    mm(&sleft[s * 9], umu_fnu, &lrc[s * 9]);        // staple_left
+    amm(&sback[s * 9], unu_fmu, &llc[s * 9]);       // staple_back = adj(u_nu_for_mu) low_left_corner
+  }
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < V; ++s) {
+    const long sm = g.fwd(s, mu), sn = g.fwd(s, nu), smn = g.fwd(sm, nu);
+    const cplx<R>* Lx = Lam + s * 9;
+    const cplx<R>* Lxm = Lam + sm * 9;
+    const cplx<R>* Lxn = Lam + sn * 9;
+    const cplx<R>* Lxmn = Lam + smn * 9;
+    const cplx<R>* unu_fmu = link(U, V, nu, sm);
+    const cplx<R>* umu_fnu = link(U, V, mu, sn);
+    const cplx<R>* umu = link(U, V, mu, s);
+    const cplx<R>* unu = link(U, V, nu, s);
+    cplx<R> a[9], b[9], c[9], d[9];
+    if (g.parity(s) == cb) {
+      mm3(a, unu_fmu, Lxmn, &ulc[s * 9]);           // 1) mu
+      mm(b, &sfor[s * 9], Lx);                      // 5) mu
+      for (int k = 0; k < 9; ++k) ds_mu[s * 9 + k] = a[k] + b[k];
+      mm3(a, &ulc[s * 9], Lx, umu);                 // 2) nu
+      mm(b, Lxmn, &sright[s * 9]);                  // 6) nu
+      for (int k = 0; k < 9; ++k) tnu[s * 9 + k] = a[k] + b[k];
+      mm3(a, &lrc[s * 9], Lx, unu);                 // 4) mu
+      mm(b, Lxmn, &sback[s * 9]);                   // 8) mu
+      for (int k = 0; k < 9; ++k) tmu[s * 9 + k] = a[k] + b[k];
+      mm3(a, umu_fnu, Lxmn, &lrc[s * 9]);           // 4) nu
+      mm(b, &sleft[s * 9], Lx);                     // 7) nu
+      for (int k = 0; k < 9; ++k) ds_nu[s * 9 + k] = a[k] + b[k];
+    } else {
+      adj3(c, umu_fnu);
+      adj3(d, &llc[s * 9]);
+      mm3(a, c, Lxn, d);                            // 1) nu
+      mm(b, &sright[s * 9], Lxm);                   // 5) nu
+      for (int k = 0; k < 9; ++k) tnu[s * 9 + k] = a[k] + b[k];
+      adj3(c, unu);
+      mm3(a, &urc[s * 9], Lxn, c);                  // 2) mu
+      mm(b, Lxm, &sfor[s * 9]);                     // 6) mu
+      for (int k = 0; k < 9; ++k) ds_mu[s * 9 + k] = a[k] + b[k];
+      adj3(c, unu_fmu);
+      mm3(a, c, Lxm, &llc[s * 9]);                  // 3) mu
+      mm(b, &sback[s * 9], Lxn);                    // 7) mu
+      for (int k = 0; k < 9; ++k) tmu[s * 9 + k] = a[k] + b[k];
+      adj3(c, &urc[s * 9]);
+      adj3(d, umu);
+      mm3(a, c, Lxm, d);                            // 3) nu
+      mm(b, Lxn, &sleft[s * 9]);                    // 8) nu
+      for (int k = 0; k < 9; ++k) ds_nu[s * 9 + k] = a[k] + b[k];
+    }
+  }
+  // ds_mu -= shift(tmp_mu, BACKWARD, nu); ds_nu -= shift(tmp_nu, BACKWARD, mu)
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < V; ++s) {
+    const long bn = g.bwd(s, nu), bm = g.bwd(s, mu);
+    for (int k = 0; k < 9; ++k) {
+      ds_mu[s * 9 + k] -= tmu[bn * 9 + k];
+      ds_nu[s * 9 + k] -= tnu[bm * 9 + k];
+    }
+  }
+}
+
+static inline double clover_coeff(int mu, int nu, double csw_r, double csw_t) {  // getCloverCoeff, 1/2 scaled
+  return 0.5 * ((mu == 3 || nu == 3) ? csw_t : csw_r);
+}
+
+static void gamma_munu(cd G[4][4], int mu, int nu) {
+  for (int s = 0; s < 4; ++s)
+    for (int t = 0; t < 4; ++t) {
+      cd a = 0;
+      for (int k = 0; k < 4; ++k) a += kGamma[mu][s][k] * kGamma[nu][k][t];
+      G[s][t] = a;
+    }
+}
+
+// ds += the planes' insertions of Lambda_munu (given per plane by lam(mu, nu, Lambda[V*9]))
+template <typename R, typename F>
+static void clover_force(const Geom& g, const cplx<R>* U, int cb, cplx<R>* ds, F&& lam) {
+  const long V = g.V;
+  for (long k = 0; k < 4 * V * 9; ++k) ds[k] = 0;
+  std::vector<cplx<R>> L(V * 9), dmu(V * 9), dnu(V * 9);
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = mu + 1; nu < 4; ++nu) {
+      std::fill(L.begin(), L.end(), cplx<R>(0));
+      lam(mu, nu, L.data());
+      deriv_loops<R>(g, U, mu, nu, cb, L.data(), dmu.data(), dnu.data());
+      for (long k = 0; k < V * 9; ++k) {
+        ds[(long)mu * V * 9 + k] += dmu[k];
+        ds[(long)nu * V * 9 + k] -= dnu[k];
+      }
+    }
+}
+
+template <typename R>
+void clover_deriv(const Geom& g, const cplx<R>* U, double csw_r, double csw_t, const cplx<R>* chi, const cplx<R>* psi,
+                  int cb, cplx<R>* ds) {
+  clover_force<R>(g, U, cb, ds, [&](int mu, int nu, cplx<R>* L) {
+    cd G[4][4];
+    gamma_munu(G, mu, nu);
+    const R f = R(-clover_coeff(mu, nu, csw_r, csw_t) / 8.0);
+    for (long s = cb * g.Vh; s < (cb + 1) * g.Vh; ++s) {
+      cplx<R> gp[4][3];  // Gamma psi
+      for (int sp = 0; sp < 4; ++sp)
+        for (int c = 0; c < 3; ++c) {
+          cplx<R> t = 0;
+          for (int k = 0; k < 4; ++k) t += cplx<R>(G[sp][k]) * psi[s * 12 + k * 3 + c];
+          gp[sp][c] = t;
+        }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          cplx<R> t = 0;
+          for (int sp = 0; sp < 4; ++sp) t += gp[sp][i] * std::conj(chi[s * 12 + sp * 3 + j]);
+          L[s * 9 + i * 3 + j] = f * t;
+        }
+    }
+  });
+}
+
+// the 12x12 (spin x colour) matrix of a packed clover site (two Hermitian 6x6 chiral blocks)
+template <typename R> static void clover_dense(const Clov<R>& t, cplx<R> A[12][12]) {
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < 12; ++j) A[i][j] = 0;
+  for (int b = 0; b < 2; ++b) {
+    for (int i = 0; i < 6; ++i) A[6 * b + i][6 * b + i] = t.diag[b][i];
+    for (int i = 1; i < 6; ++i)
+      for (int j = 0; j < i; ++j) {
+        const cplx<R> v = t.offd[b][i * (i - 1) / 2 + j];
+        A[6 * b + i][6 * b + j] = v;
+        A[6 * b + j][6 * b + i] = std::conj(v);
+      }
+  }
+}
+
+template <typename R>
+void clover_deriv_trln(const Geom& g, const cplx<R>* U, double csw_r, double csw_t, const Clov<R>* tri_inv, int cb,
+                       cplx<R>* ds) {
+  clover_force<R>(g, U, cb, ds, [&](int mu, int nu, cplx<R>* L) {
+    cd G[4][4];
+    gamma_munu(G, mu, nu);
+    const R f = R(-clover_coeff(mu, nu, csw_r, csw_t) / 8.0);
+    for (long s = cb * g.Vh; s < (cb + 1) * g.Vh; ++s) {
+      cplx<R> A[12][12];
+      clover_dense(tri_inv[s], A);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          cplx<R> t = 0;  // triacntr: B_ij = sum_{s,t} Gamma_st A_(t,i),(s,j)
+          for (int sp = 0; sp < 4; ++sp)
+            for (int tp = 0; tp < 4; ++tp) t += cplx<R>(G[sp][tp]) * A[tp * 3 + i][sp * 3 + j];
+          L[s * 9 + i * 3 + j] = f * t;
+        }
+    }
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
 // BLAS over a checkerboard range.  QDP++ norm2 / innerProduct return Double / DComplex.
 struct Range {
   long lo, hi;  // site range
@@ -627,6 +828,33 @@ template <typename R> struct EOClover : EOLinOp<R> {
     axpby<R>(chi, R(-0.25), t1.data(), R(1), this->subset());
   }
 };
+
+// EvenOddPrecLogDetLinearOperator::deriv (eoprec_logdet_linop.h:114-157) of the EO clover operator with
+// its pieces (eoprec_clover_linop_w.cc:192-260): derivOddOdd / derivEvenEven = clov.deriv(cb 1 / 0),
+// derivOddEven / derivEvenOdd = -1/2 D.deriv(cb 1 / 0).  chi, psi full-lattice (odd part used); ds [4][V][3][3].
+//   ds  = chi^dag A'_oo psi
+//   ds -= chi^dag D'_oe A_ee^-1 D_eo psi                         (tmp2 = A_ee^-1 (-1/2 D_eo psi))
+//   ds += chi^dag D_oe A_ee^-1 A'_ee A_ee^-1 D_eo psi           (tmp3 = A_ee^-1 (-1/2 D_eo(msign) chi))
+//   ds -= chi^dag D_oe A_ee^-1 D'_eo psi
+template <typename R>
+void eo_clover_deriv(const EOClover<R>& M, double csw_r, double csw_t, const cplx<R>* chi, const cplx<R>* psi,
+                     int isign, cplx<R>* ds) {
+  const Geom& g = *M.g;
+  const long V = g.V;
+  const int msign = -isign;
+  std::vector<cplx<R>> t1(V * 12), t2(V * 12), t3(V * 12), d1((size_t)4 * V * 9);
+  clover_deriv<R>(g, M.U, csw_r, csw_t, chi, psi, 1, ds);                 // derivOddOddLinOp
+  M.evenOddLinOp(t1.data(), psi, isign);
+  M.evenEvenInvLinOp(t2.data(), t1.data(), isign);
+  dslash_deriv<R>(g, d1.data(), chi, t2.data(), isign, 1);                // derivOddEvenLinOp / (-1/2)
+  for (size_t k = 0; k < d1.size(); ++k) ds[k] -= R(-0.5) * d1[k];
+  M.evenOddLinOp(t1.data(), chi, msign);
+  M.evenEvenInvLinOp(t3.data(), t1.data(), msign);
+  clover_deriv<R>(g, M.U, csw_r, csw_t, t3.data(), t2.data(), 0, d1.data());  // derivEvenEvenLinOp
+  for (size_t k = 0; k < d1.size(); ++k) ds[k] += d1[k];
+  dslash_deriv<R>(g, d1.data(), t3.data(), psi, isign, 0);                // derivEvenOddLinOp / (-1/2)
+  for (size_t k = 0; k < d1.size(); ++k) ds[k] -= R(-0.5) * d1[k];
+}
 
 // EvenOddPrecWilsonLinOp, eoprec_wilson_linop_w.cc:48-166 and eoprec_wilson_linop_w.h:75-110
 template <typename R> struct EOWilson : EOLinOp<R> {
@@ -1319,6 +1547,51 @@ int orc_solve_mdagm(void* h, int solver, double* psi, const double* chi, double 
 
 // WilsonDslashBase::deriv (lwldslash_base_w.h:132-227): cb = 0 or 1, or -1 for both checkerboards
 // (the sum, :110-124).  chi, psi full-lattice fermions; ds_u [4][V][3][3].
+// CloverTermBase::deriv on cb (cb = -1: both), derivTrLn of the clover term inverted on cb (the even-even
+// block's log det derivative of EvenOddPrecCloverLinOp::derivLogDetEvenEvenLinOp for cb = 0), and the EO
+// operator's derivative.  U carries the boundary conditions; ds is [4][V][3][3].
+int orc_clover_deriv(const int dims[4], const double* U, double csw_r, double csw_t, const double* chi,
+                     const double* psi, int cb, double* ds_u) {
+  return guard([&] {
+    Geom g(dims);
+    auto* ds = reinterpret_cast<cd*>(ds_u);
+    auto* u = reinterpret_cast<const cd*>(U);
+    auto* c = reinterpret_cast<const cd*>(chi);
+    auto* p = reinterpret_cast<const cd*>(psi);
+    if (cb >= 0) {
+      clover_deriv<double>(g, u, csw_r, csw_t, c, p, cb, ds);
+    } else {
+      std::vector<cd> t((size_t)4 * g.V * 9);
+      clover_deriv<double>(g, u, csw_r, csw_t, c, p, 0, ds);
+      clover_deriv<double>(g, u, csw_r, csw_t, c, p, 1, t.data());
+      for (size_t k = 0; k < t.size(); ++k) ds[k] += t[k];
+    }
+  });
+}
+
+int orc_clover_deriv_trln(const int dims[4], const double* U, double csw_r, double csw_t, double mass, int cb,
+                          double* ds_u) {
+  return guard([&] {
+    Geom g(dims);
+    auto* u = reinterpret_cast<const cd*>(U);
+    std::vector<Clov<double>> tri(g.V);
+    make_clover<double>(g, u, csw_r, csw_t, mass, tri.data());
+    long nneg = 0;
+    clover_invert<double>(g, tri.data(), cb, &nneg);
+    clover_deriv_trln<double>(g, u, csw_r, csw_t, tri.data(), cb, reinterpret_cast<cd*>(ds_u));
+  });
+}
+
+int orc_eo_clover_deriv(const int dims[4], const double* U, double mass, double csw_r, double csw_t, const double* chi,
+                        const double* psi, int isign, double* ds_u) {
+  return guard([&] {
+    Geom g(dims);
+    EOClover<double> M(g, reinterpret_cast<const cd*>(U), mass, csw_r, csw_t);
+    eo_clover_deriv<double>(M, csw_r, csw_t, reinterpret_cast<const cd*>(chi), reinterpret_cast<const cd*>(psi), isign,
+                            reinterpret_cast<cd*>(ds_u));
+  });
+}
+
 int orc_dslash_deriv(const int dims[4], double* ds_u, const double* chi, const double* psi, int isign, int cb) {
   return guard([&] {
     Geom g(dims);

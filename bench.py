@@ -221,9 +221,11 @@ def stout_field(cmx, L, seed):
     return cmx.gauge_stout_smear(L, cmx.gauge_generate(L, kind=2, seed=seed), 6, 0.1)
 
 
-def train_net(cmx, mass, csw, a):
-    """The learned preconditioner's network, trained on separate fields of the benchmark recipe."""
+def train_net(cmx, mass, csw, a, depth=1, pairs_per_source=4, lr=1e-3):
+    """The learned preconditioner's network (`depth` blocks, the first starting as the two-term Neumann
+    series, the others as the identity), trained on separate fields of the benchmark recipe."""
     net = cmx.Net()
+    net.set_depth(depth)
     net.init_neumann(4.0 + mass)
     TL = tuple(int(v) for v in a.train_lattice.split(","))
     train = []
@@ -231,21 +233,119 @@ def train_net(cmx, mass, csw, a):
         U = cmx.gauge_apply_bc(TL, stout_field(cmx, TL, 10 + k), (1, 1, 1, -1))
         top = cmx.Operator(cmx.Gauge(TL, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF)
         cmx.net_attach(top, net)
-        train.append(cmx.net_train(top, n_sources=8, snapshot_every=1, pairs_per_source=4, normalize=False,
-                                   epochs=4, lr=1e-3, seed=100 + k, prec=cmx.HALF))
+        train.append(cmx.net_train(top, n_sources=8, snapshot_every=1, pairs_per_source=pairs_per_source,
+                                   normalize=False, epochs=4, lr=lr, seed=100 + k, prec=cmx.HALF, max_pairs=100000))
     return net, train
 
 
-def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None):
-    """The 12 spin-colour point sources of a propagator on a near-critical clover operator (Haar + 6 stout
+FGMRES_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, basis_prec=2, n_krylov=16, cycle_reduction=3e-3)
+
+
+def sigma_min(cmx, op, Vh, iters=6):
+    """Smallest singular value of the even-odd operator M by inverse iteration on (M^dag M)^-1 =
+    M^-1 g5 M^-1 g5 (M^dag = g5 M g5), each inverse an FGMRES-DR solve with an FP32 operator and basis to
+    1e-6; the Rayleigh quotient 1 / <x, (M^dag M)^-1 x> of the last unit vector x estimates sigma_min^2
+    from above.  None when a solve does not converge."""
+    import numpy as np
+    g5 = np.array([1.0, 1.0, -1.0, -1.0])[None, :, None]
+    x = cmx.z2_source((Vh, 4, 3), 77)
+    x /= np.linalg.norm(x)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-6, maxit=5000, sloppy_prec=cmx.SINGLE, basis_prec=cmx.SINGLE, n_krylov=16)
+    lam = None
+    try:
+        for _ in range(iters):
+            w, _ = op.solve(g5 * x, **kw)
+            y, _ = op.solve(g5 * w, **kw)
+            lam = 1.0 / float(np.vdot(x, y).real)
+            x = y / np.linalg.norm(y)
+    except cmx.CmxError as e:
+        print(f"# sigma_min: {e}", file=sys.stderr, flush=True)
+        return None
+    return float(np.sqrt(lam))
+
+
+def critical_scan(cmx, gauge, L, kappas=(0.1355, 0.150, 0.155, 0.158, 0.160)):
+    """Where the synthetic ensemble (Haar + 6 stout steps) is critical: at each kappa, the smallest
+    singular value of the even-odd operator, the plain mixed CG iterations on a Z2 source (to kappa
+    0.158; it needs 18 000 at 0.160 and does not converge at 0.162) and the FGMRES-DR(16) iterations; kappa_c
+    is where sigma_min, linear in 1/kappa near the critical point, extrapolates to zero (last two points)."""
+    import numpy as np
+    Vh = int(np.prod(L)) // 2
+    chi = cmx.z2_source((Vh, 4, 3), 2)
+    rows = []
+    for k in kappas:
+        op = cmx.Operator(gauge, cmx.EO_CLOVER, cmx.kappa_to_mass(k), 1.0, sloppy_prec=cmx.HALF, autotune=True)
+        sm = sigma_min(cmx, op, Vh)
+        row = {"kappa": k, "sigma_min": None if sm is None else round(sm, 6)}
+        _, r = op.solve(chi, solver=cmx.FGMRESDR, **FGMRES_KW)
+        row["fgmres_iterations"] = r["n_count"]
+        if k <= 0.158:
+            _, r = op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=cmx.HALF,
+                            true_residual=True)
+            row["plain_cg_iterations"] = r["n_count"]
+        rows.append(row)
+        print(f"# critical scan {row}", file=sys.stderr, flush=True)
+        del op
+    pts = [(1.0 / r["kappa"], r["sigma_min"]) for r in rows if r["sigma_min"] is not None][-2:]
+    kc = None
+    if len(pts) == 2:
+        (k1, s1), (k2, s2) = pts
+        kc = round(1.0 / (k2 - s2 * (k2 - k1) / (s2 - s1)), 5)
+    return {"lattice": list(L), "rows": rows, "kappa_c": kc,
+            "method": "sigma_min(M) by inverse iteration, linear in 1/kappa through the two points nearest to it"}
+
+
+def near_critical_solve(cmx, gauge, L, kappa, net):
+    """One Z2 source at a near-critical kappa, all to the FP64 true residual 1e-10: plain mixed CG (FP16
+    inner, FP64 reliable updates), FGMRES-DR(16) without a preconditioner, with the untrained network
+    (its Neumann start) and with the trained network."""
+    import numpy as np
+    mass = cmx.kappa_to_mass(kappa)
+    op = cmx.Operator(gauge, cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF, autotune=True)
+    Vh = int(np.prod(L)) // 2
+    chi = cmx.z2_source((Vh, 4, 3), 2)
+    out = {"lattice": list(L), "kappa": kappa}
+    try:  # allocates the solver's vectors; 20 iterations do not converge here
+        op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=20, delta=0.1, sloppy_prec=cmx.HALF)
+    except cmx.CmxError:
+        pass
+    _, r = op.solve(chi, solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=cmx.HALF,
+                    true_residual=True)
+    out["plain_mixed_cg"] = {"seconds": round(r["solve_seconds"], 4), "iterations": r["n_count"],
+                             "true_rel_resid": r["rel_resid"]}
+    start = cmx.Net()
+    start.set_depth(net.depth())
+    start.init_neumann(4.0 + mass)
+    for name, pc, nn in (("fgmres_none", cmx.PRECOND_NONE, None), ("fgmres_neumann_start", cmx.PRECOND_LEARNED, start),
+                         ("fgmres_trained", cmx.PRECOND_LEARNED, net)):
+        if nn is not None:
+            cmx.net_attach(op, nn)
+        op.solve(chi, solver=cmx.FGMRESDR, precond=pc, **FGMRES_KW)
+        _, r = op.solve(chi, solver=cmx.FGMRESDR, precond=pc, **FGMRES_KW)
+        out[name] = {"seconds": round(r["solve_seconds"], 4), "iterations": r["n_count"],
+                     "true_rel_resid": r["rel_resid"]}
+    p = out["plain_mixed_cg"]["seconds"]
+    out["speedup_trained_vs_plain"] = round(p / out["fgmres_trained"]["seconds"], 3)
+    out["speedup_trained_vs_fgmres_none"] = round(out["fgmres_none"]["seconds"] / out["fgmres_trained"]["seconds"], 3)
+    out["speedup_trained_vs_neumann_start"] = round(out["fgmres_neumann_start"]["seconds"] /
+                                                    out["fgmres_trained"]["seconds"], 3)
+    print(f"# near-critical {L} kappa {kappa}: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, variants=False):
+    """The 12 spin-colour point sources of a propagator on the clover operator at kappa (Haar + 6 stout
     steps, seed 1), all to the FP64 true residual 1e-10: plain mixed-precision CG one after another and
-    batched, against the learned FGMRES-DR one after another."""
+    batched, against the learned FGMRES-DR one after another (with `variants`: also FGMRES-DR without a
+    preconditioner and with the network's untrained Neumann start)."""
     import numpy as np
     mass = cmx.kappa_to_mass(kappa)
     t0 = time.time()
-    U = cmx.gauge_apply_bc(L, stout_field(cmx, L, 1), (1, 1, 1, -1))
-    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF, autotune=True)
-    del U
+    if gauge is None:
+        U = cmx.gauge_apply_bc(L, stout_field(cmx, L, 1), (1, 1, 1, -1))
+        gauge = cmx.Gauge(L, U)
+        del U
+    op = cmx.Operator(gauge, cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF, autotune=True)
     Vh = int(np.prod(L)) // 2
     chis = []
     for s in range(4):
@@ -259,22 +359,45 @@ def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None):
     rb, bsecs, _ = batch_in_groups(cmx, op, chis, a.batch_group, **kwp)
     if net is None:
         net, _ = train_net(cmx, mass, csw, a)
-    cmx.net_attach(op, net)
-    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, basis_prec=cmx.HALF, n_krylov=16,
-              cycle_reduction=3e-3, precond=cmx.PRECOND_LEARNED)
-    op.solve(chis[0], **kw)
-    learned = [op.solve(c, **kw)[1] for c in chis]
-    ps, ls = sum(r["solve_seconds"] for r in plain), sum(r["solve_seconds"] for r in learned)
+    kw = dict(FGMRES_KW, solver=cmx.FGMRESDR)
+    runs = [("learned_fgmres_sequential", cmx.PRECOND_LEARNED, net)]
+    if variants:
+        start = cmx.Net()
+        start.set_depth(net.depth())
+        start.init_neumann(4.0 + mass)
+        runs = [("fgmres_none_sequential", cmx.PRECOND_NONE, None),
+                ("neumann_start_fgmres_sequential", cmx.PRECOND_LEARNED, start)] + runs
+    ps = sum(r["solve_seconds"] for r in plain)
     out = {"lattice": list(L), "kappa": kappa, "sources": 12,
            "plain_mixed_cg_sequential": {"seconds": round(ps, 4), "iterations": [r["n_count"] for r in plain],
                                          "worst_true_rel_resid": max(r["rel_resid"] for r in plain)},
            "plain_mixed_cg_batched": {"seconds": round(bsecs, 4), "group": a.batch_group,
-                                      "worst_true_rel_resid": max(r["rel_resid"] for r in rb)},
-           "learned_fgmres_sequential": {"seconds": round(ls, 4), "iterations": [r["n_count"] for r in learned],
-                                         "worst_true_rel_resid": max(r["rel_resid"] for r in learned)},
-           "speedup_vs_plain": round(ps / ls, 3), "speedup_vs_plain_batched": round(bsecs / ls, 3),
-           "wall_seconds": round(time.time() - t0, 1)}
-    print(f"# 12 sources {L}: {out}", file=sys.stderr, flush=True)
+                                      "worst_true_rel_resid": max(r["rel_resid"] for r in rb)}}
+    for name, pc, nn in runs:
+        if nn is not None:
+            cmx.net_attach(op, nn)
+        op.solve(chis[0], precond=pc, **kw)
+        res = [op.solve(c, precond=pc, **kw)[1] for c in chis]
+        out[name] = {"seconds": round(sum(r["solve_seconds"] for r in res), 4),
+                     "iterations": [r["n_count"] for r in res],
+                     "worst_true_rel_resid": max(r["rel_resid"] for r in res)}
+    # the same learned solves batched: one Arnoldi step of every source per host read-back
+    cmx.net_attach(op, net)
+    psb, rbl, summ = cmx.solve_batch([op] * 12, chis, precond=cmx.PRECOND_LEARNED, **kw)
+    out["learned_fgmres_batched"] = {"seconds": round(summ["solve_seconds"], 4),
+                                     "iterations": [r["n_count"] for r in rbl],
+                                     "worst_true_rel_resid": max(r["rel_resid"] for r in rbl),
+                                     "arnoldi_ticks": summ["n_iter"]}
+    del psb
+    ls = out["learned_fgmres_sequential"]["seconds"]
+    lb = out["learned_fgmres_batched"]["seconds"]
+    out["network_blocks"] = net.depth()
+    out["speedup_vs_plain"] = round(ps / ls, 3)
+    out["speedup_vs_plain_batched"] = round(bsecs / ls, 3)
+    out["speedup_batched_learned_vs_plain_batched"] = round(bsecs / lb, 3)
+    out["speedup_batched_learned_vs_sequential_learned"] = round(ls / lb, 3)
+    out["wall_seconds"] = round(time.time() - t0, 1)
+    print(f"# 12 sources {L} kappa {kappa}: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -420,6 +543,10 @@ def main():
                          "(about 30 s)")
     ap.add_argument("--no-solve-lattice-kernels", action="store_true",
                     help="skip the kernel table of the solve lattice")
+    ap.add_argument("--near-kappa", type=float, default=0.158,
+                    help="near-critical kappa of the stout ensemble (kappa_c is about 0.161 at 32^3x64)")
+    ap.add_argument("--no-near-critical", action="store_true",
+                    help="skip the critical scan and the near-critical solves (about 60 s)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -452,7 +579,7 @@ def main():
     extra["hop_fp16_recon12"] = hop16
     shapes = {name: {"operator": op.hop_config(p, 1), "hop": op.hop_config(p, 2)}
               for name, p in (("fp64", 0), ("fp32", 1), ("fp16", 2))}
-    del op, gauge
+    del op
     cpu_hop = None
     if not a.no_cpu_baseline:
         cpu_hop = cpu_hop_baseline(L, U)
@@ -507,6 +634,22 @@ def main():
     if not a.no_solve and not a.no_learned:
         learned, net = learned_solve(cmx, gauge_s, LS, 0.1355, csw, a)
 
+    # near the critical kappa: where the ensemble is critical, one source on both lattices, the 12
+    # point sources on the solve lattice; the network has two blocks and is trained at that kappa
+    critical = near = sources_nc = None
+    if not a.no_solve and not a.no_near_critical:
+        critical = critical_scan(cmx, gauge_s, LS)
+        t0 = time.time()
+        net_nc, train_nc = train_net(cmx, cmx.kappa_to_mass(a.near_kappa), csw, a, depth=2, pairs_per_source=16,
+                                     lr=5e-4)
+        near = {"network": {"blocks": 2, "train_seconds": round(time.time() - t0, 2),
+                            "train": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in t.items()}
+                                      for t in train_nc]},
+                "solve_lattice": near_critical_solve(cmx, gauge_s, LS, a.near_kappa, net_nc),
+                "headline_lattice": near_critical_solve(cmx, gauge, L, a.near_kappa, net_nc)}
+        sources_nc = learned_vs_plain_sources(cmx, LS, a.near_kappa, csw, a, net_nc, gauge=gauge_s, variants=True)
+    del gauge
+
     batched = None
     if not a.no_solve and not a.no_batch:
         batched = {"uneven_volumes": batched_uneven_solves(cmx),
@@ -553,6 +696,9 @@ def main():
         "small_volume": small,
         "batched": batched,
         "sources_12_learned_vs_plain": sources_cmp,
+        "critical": critical,
+        "near_critical": near,
+        "sources_12_near_critical": sources_nc,
         "solves": solves,
         "kernels": extra,
         "launch_shapes": shapes,

@@ -12,7 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchroma_mi355x.so")
 
 SOURCES = ["hop_p64.hip", "hop_p32.hip", "hop_p16.hip", "hop_multi.hip", "hop_mfma.hip", "blas.hip", "setup.hip", "krylov.hip", "learned.hip",
-           "op.cpp", "batch.cpp", "solvers.cpp", "multishift.cpp", "deriv.hip", "force.hip", "stout.hip", "meson.hip", "fgmres.cpp", "learned.cpp", "capi.cpp", "gauge_gen.cpp"]
+           "op.cpp", "batch.cpp", "solvers.cpp", "multishift.cpp", "deriv.hip", "force.hip", "stout.hip", "meson.hip", "mg.hip", "mg.cpp", "fgmres.cpp", "learned.cpp", "capi.cpp", "gauge_gen.cpp", "qio.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

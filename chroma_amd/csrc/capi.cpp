@@ -1,11 +1,13 @@
 // extern "C" entry points declared in include/chroma_mi355x.h.  No exception crosses this boundary:
 // every call returns CMX_OK or a CMX_ERR_* code and records the message for cmx_last_error().
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "mg.h"
 #include "op.h"
 #include "setup.h"
 
@@ -511,7 +513,8 @@ int cmx_solve_batch(const cmx_op* hs, int n, const cmx_solver_params* p, const c
     // a repeated operator gets a solver context of its own (sharing its fields and stream)
     std::vector<std::unique_ptr<cmx::Op>> clones;
     std::vector<cmx::Op*> ops(n);
-    const int ps = p->solver == CMX_SOLVER_RELIABLE_CG ? p->sloppy_prec : cmx::P64;
+    const bool sloppy = p->solver == CMX_SOLVER_RELIABLE_CG || p->solver == CMX_SOLVER_FGMRESDR;
+    const int ps = sloppy && p->sloppy_prec >= 0 && p->sloppy_prec <= 2 ? p->sloppy_prec : cmx::P64;
     for (int i = 0; i < n; ++i) {
       CMX_REQUIRE(hs[i] && psi[i] && chi[i], CMX_ERR_ARG, "solve_batch: null operator or field");
       cmx::Op& o = O(hs[i]);
@@ -519,6 +522,8 @@ int cmx_solve_batch(const cmx_op* hs, int n, const cmx_solver_params* p, const c
       for (int j = 0; j < i; ++j) seen = seen || hs[j] == hs[i];
       if (seen) {
         clones.emplace_back(cmx::clone_op(o, ps));
+        // the learned preconditioner: the clone uses the operator's network and device buffers
+        if (p->solver == CMX_SOLVER_FGMRESDR && p->precond == CMX_PRECOND_LEARNED) cmx::learned_share(o, *clones.back());
         ops[i] = clones.back().get();
       } else {
         ops[i] = &o;
@@ -767,6 +772,96 @@ int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res) {
     require_init();
     CMX_REQUIRE(p && res, CMX_ERR_ARG, "net_train: bad argument");
     cmx::net_train(O(op), *p, *res);
+  });
+}
+
+int cmx_field_write_scidac(const char* path, const int dims[4], int kind, int prec, const double* field,
+                           const char* file_xml, const char* record_xml) {
+  return guarded([&] {
+    CMX_REQUIRE(dims, CMX_ERR_ARG, "field_write_scidac: null dims");
+    cmx::scidac_write(path, dims, kind, prec, field, file_xml, record_xml);
+  });
+}
+
+int cmx_field_read_scidac(const char* path, const int dims[4], int kind, double* field, char* file_xml,
+                          int file_xml_cap, char* record_xml, int record_xml_cap) {
+  return guarded([&] {
+    CMX_REQUIRE(dims, CMX_ERR_ARG, "field_read_scidac: null dims");
+    std::string fx, rx;
+    cmx::scidac_read(path, dims, kind, field, file_xml ? &fx : nullptr, record_xml ? &rx : nullptr);
+    auto give = [](const std::string& v, char* out, int cap) {
+      if (!out || cap <= 0) return;
+      const size_t n = std::min(v.size(), (size_t)cap - 1);
+      std::memcpy(out, v.data(), n);
+      out[n] = 0;
+    };
+    give(fx, file_xml, file_xml_cap);
+    give(rx, record_xml, record_xml_cap);
+  });
+}
+
+int cmx_mg_params_default(cmx_mg_params* p) {
+  return guarded([&] {
+    CMX_REQUIRE(p, CMX_ERR_ARG, "mg_params_default: null argument");
+    std::memset(p, 0, sizeof(*p));
+    for (int mu = 0; mu < 4; ++mu) p->blocking[mu] = 4;
+    p->nvec = 16;
+    p->nu_pre = 2;
+    p->nu_post = 2;
+    p->omega = 1.0;
+    p->coarse_tol = 1e-4;
+    p->coarse_maxit = 12;
+    p->coarse_nkrylov = 12;
+    p->setup_maxit = 500;
+    p->seed = 1;
+  });
+}
+
+int cmx_mg_setup(cmx_op h, const cmx_mg_params* p, cmx_mg_result* res) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(p && res, CMX_ERR_ARG, "mg_setup: null argument");
+    cmx::mg_setup(O(h), *p, *res);
+  });
+}
+
+int cmx_mg_get_nullvecs(cmx_op h, double* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(out, CMX_ERR_ARG, "mg_get_nullvecs: null argument");
+    cmx::mg_get_nullvecs(O(h), out);
+  });
+}
+
+int cmx_mg_get_coarse_op(cmx_op h, double* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(out, CMX_ERR_ARG, "mg_get_coarse_op: null argument");
+    cmx::mg_get_coarse_op(O(h), out);
+  });
+}
+
+int cmx_mg_restrict(cmx_op h, const double* full_in, double* coarse_out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(full_in && coarse_out, CMX_ERR_ARG, "mg_restrict: null argument");
+    cmx::mg_restrict_host(O(h), full_in, coarse_out);
+  });
+}
+
+int cmx_mg_prolong(cmx_op h, const double* coarse_in, double* full_out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(coarse_in && full_out, CMX_ERR_ARG, "mg_prolong: null argument");
+    cmx::mg_prolong_host(O(h), coarse_in, full_out);
+  });
+}
+
+int cmx_mg_coarse_apply(cmx_op h, const double* in, double* out) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(in && out, CMX_ERR_ARG, "mg_coarse_apply: null argument");
+    cmx::mg_coarse_apply_host(O(h), in, out);
   });
 }
 

@@ -25,6 +25,8 @@
 #include <memory>
 
 #include "dense.h"
+#include "fgmres.h"
+#include "mg.h"
 #include "op.h"
 
 namespace cmx {
@@ -120,9 +122,13 @@ struct KryDev {
     op.sync();
   }
   void get(int slot, int count) {
+    get_async(slot, count);
+    op.sync();
+  }
+  // the copy only: the values are in op.hS once the stream has reached it
+  void get_async(int slot, int count) {
     CMX_HIP(hipMemcpyAsync(op.hS + slot, static_cast<double*>(op.state_S.p) + slot, sizeof(double) * count,
                            hipMemcpyDeviceToHost, op.stream));
-    op.sync();
   }
 };
 
@@ -133,128 +139,115 @@ constexpr int K_WN = K_H2 + 2 * KRY_MAXV;  // |w|^2, 1/|w|
 constexpr int K_C = K_WN + 2;              // host-supplied coefficients (2 * KRY_MAXV)
 static_assert(K_C + 2 * KRY_MAXV <= S_MR, "Krylov slots overlap the MR slots");
 
-// MR preconditioner (LinOpSysSolverMR -> InvMR, invmr.cc:55-160) in precision ps: z is the initial
-// guess on entry (the flexible solver passes Z_j) and the result on exit.
-struct MRPrecond {
-  Op& op;
-  int ps, n, maxit;
-  double rsd, omega;
-  Vec &r, &mr;
-  MRPrecond(Op& o, int ps_, int n_, int maxit_, double rsd_, double omega_, Vec& r_, Vec& mr_)
-      : op(o), ps(ps_), n(n_), maxit(maxit_), rsd(rsd_), omega(omega_), r(r_), mr(mr_) {}
-  void apply(Vec& z, const Vec& v) {
-    const int nb = blas_blocks(n);
-    int* F = op.state_F.as<int>();
-    op.hS[S_MR + 9] = omega;
-    op.hS[S_MR + 10] = rsd;
-    CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + S_MR + 9, op.hS + S_MR + 9, 2 * sizeof(double),
-                           hipMemcpyHostToDevice, op.stream));
-    op.blas(B_NORM2, {&v});
-    op.finalize(nb, 1, PG_STORE, S_MR + 7);  // |chi|^2
-    op.apply(ps, mr.view(), z.view(), +1, nullptr);
-    op.blas(B_XMY_NORM, {&r, &v, &mr});      // r = chi - M psi
-    op.finalize(nb, 1, PG_MR_INIT);
-    for (int k = 0; k < maxit; ++k) {
-      op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
-      op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
-      op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
-      op.blas(B_CAXPY, {&z, &r}, S_MR + 3, 0, F_MRDONE);              // psi += a r
-      op.blas(B_AXPY_CNORM, {&r, &mr}, S_MR + 5, 0, F_MRDONE);        // r -= a M r
-      op.finalize(nb, 1, PG_MR_R);
-    }
-  }
-};
 
 }  // namespace
 
-void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
-  const int n = op.subset_sites();
-  const int ps = p.sloppy_prec;
-  CMX_REQUIRE(ps == P64 || ps == P32 || ps == P16, CMX_ERR_ARG, "FGMRES-DR: bad sloppy precision");
-  const int nk = p.n_krylov, nd_max = p.n_defl;
-  CMX_REQUIRE(nk >= 1 && nd_max >= 0 && nd_max < nk && nk + nd_max + 1 <= KRY_MAXV, CMX_ERR_ARG,
-              "FGMRES-DR: need 1 <= NKrylov, 0 <= NDefl < NKrylov, NKrylov + NDefl < 48");
-  CMX_REQUIRE(p.precond >= CMX_PRECOND_NONE && p.precond <= CMX_PRECOND_LEARNED, CMX_ERR_ARG,
-              "FGMRES-DR: unknown preconditioner");
-  // basis precision pb (V, Z) may be wider than the precision ps of the operator and preconditioner
-  const int pb = p.basis_prec < 0 ? ps : p.basis_prec;
-  CMX_REQUIRE(pb == P64 || pb == P32 || pb == P16, CMX_ERR_ARG, "FGMRES-DR: bad basis precision");
-  const int total = nk + nd_max;
+struct FgmresDR::Impl {
+  Op& op;
+  const cmx_solver_params p;
+  Vec& psi;
+  const Vec& chi;
+  int n, ps, pb, nk, nd_max, total, reorth, nb;
   // the bases, the spare vectors of the deflated restart and the scratch vectors live in the
-  // operator's workspace (slots 100-305), so repeated solves allocate nothing; V and Z map basis
+  // operator's workspace (slots 100-306), so repeated solves allocate nothing; V and Z map basis
   // indices to those vectors and are permuted, not copied, when the deflated basis is formed
-  std::vector<Vec*> V(total + 1), Z(total), spareV(nd_max + 1), spareZ(nd_max);
-  for (int i = 0; i <= total; ++i) V[i] = &op.work(100 + i, pb, n);
-  for (int i = 0; i < total; ++i) Z[i] = &op.work(150 + i, pb, n);
-  for (int i = 0; i <= nd_max; ++i) spareV[i] = &op.work(200 + i, pb, n);
-  for (int i = 0; i < nd_max; ++i) spareZ[i] = &op.work(250 + i, pb, n);
-  for (Vec* z : Z) op.blas(B_ZERO, {z});
-  Vec &r = op.work(300, P64, n), &tmp = op.work(301, P64, n);
-  // the FP64 residual converted to the basis precision for <V_i, r> at a deflated restart
-  Vec* rs = nd_max > 0 ? (pb == P64 ? &r : &op.work(306, pb, n)) : nullptr;
-  Vec *xs = nullptr, *ys = nullptr;  // operator-precision staging when pb != ps
-  if (pb != ps) {
-    xs = &op.work(302, ps, n);
-    ys = &op.work(303, ps, n);
-  }
-  KryDev kd(op, n);
+  std::vector<Vec*> V, Z, spareV, spareZ, vs;
+  Vec *r = nullptr, *tmp = nullptr;
+  Vec* rs = nullptr;                  // the FP64 residual in the basis precision (deflated restarts)
+  Vec *xs = nullptr, *ys = nullptr;   // operator-precision staging when pb != ps
+  KryDev kd;
   std::unique_ptr<MRPrecond> mr;
-  if (p.precond == CMX_PRECOND_MR)
-    mr.reset(new MRPrecond(op, ps, n, std::max(0, p.precond_iters), p.precond_rsd,
-                           p.precond_omega != 0 ? p.precond_omega : 1.0, op.work(304, ps, n), op.work(305, ps, n)));
-  const int reorth = p.reorth >= 2 ? 2 : 1;
-  const int nb = blas_blocks(n);
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double norm_rhs = 0, target = 0, r_norm = 0, r_prev = 0;
+  Mat H, R, Qk;  // Qk: Q of the deflated block (nd+1 x nd+1)
+  std::vector<cd> g, c, eta;
+  std::vector<Givens> rot;
+  int iters = 0, cycles = 0, prev_dim = 0;
+  bool done = false;
+  bool plain_next = true;  // the first cycle, and any cycle after a deflated one that stalled
+  // the cycle in progress
+  int nd = 0, j = 0, dim = 0;
+  double cycle_stop = 0;
 
-  // the events are released on every exit, including an exception from inside the loop
-  struct EvPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    ~EvPair() {
-      if (a) (void)hipEventDestroy(a);
-      if (b) (void)hipEventDestroy(b);
+  Impl(Op& o, const cmx_solver_params& p_, Vec& psi_, const Vec& chi_)
+      : op(o), p(p_), psi(psi_), chi(chi_), n(o.subset_sites()), ps(p_.sloppy_prec), kd(o, o.subset_sites()) {
+    CMX_REQUIRE(ps == P64 || ps == P32 || ps == P16, CMX_ERR_ARG, "FGMRES-DR: bad sloppy precision");
+    nk = p.n_krylov, nd_max = p.n_defl;
+    CMX_REQUIRE(nk >= 1 && nd_max >= 0 && nd_max < nk && nk + nd_max + 1 <= KRY_MAXV, CMX_ERR_ARG,
+                "FGMRES-DR: need 1 <= NKrylov, 0 <= NDefl < NKrylov, NKrylov + NDefl < 48");
+    CMX_REQUIRE(p.precond >= CMX_PRECOND_NONE && p.precond <= CMX_PRECOND_MG, CMX_ERR_ARG,
+                "FGMRES-DR: unknown preconditioner");
+    // basis precision pb (V, Z) may be wider than the precision ps of the operator and preconditioner
+    pb = p.basis_prec < 0 ? ps : p.basis_prec;
+    CMX_REQUIRE(pb == P64 || pb == P32 || pb == P16, CMX_ERR_ARG, "FGMRES-DR: bad basis precision");
+    total = nk + nd_max;
+    prev_dim = nk;
+    V.resize(total + 1), Z.resize(total), spareV.resize(nd_max + 1), spareZ.resize(nd_max), vs.resize(total + 1);
+    for (int i = 0; i <= total; ++i) V[i] = &op.work(100 + i, pb, n);
+    for (int i = 0; i < total; ++i) Z[i] = &op.work(150 + i, pb, n);
+    for (int i = 0; i <= nd_max; ++i) spareV[i] = &op.work(200 + i, pb, n);
+    for (int i = 0; i < nd_max; ++i) spareZ[i] = &op.work(250 + i, pb, n);
+    for (Vec* z : Z) op.blas(B_ZERO, {z});
+    r = &op.work(300, P64, n);
+    tmp = &op.work(301, P64, n);
+    rs = nd_max > 0 ? (pb == P64 ? r : &op.work(306, pb, n)) : nullptr;
+    if (pb != ps) {
+      xs = &op.work(302, ps, n);
+      ys = &op.work(303, ps, n);
     }
-  } ev;
-  CMX_HIP(hipEventCreate(&ev.a));
-  CMX_HIP(hipEventCreate(&ev.b));
-  CMX_HIP(hipEventRecord(ev.a, op.stream));
+    if (p.precond == CMX_PRECOND_MR)
+      mr.reset(new MRPrecond(op, ps, n, std::max(0, p.precond_iters), p.precond_rsd,
+                             p.precond_omega != 0 ? p.precond_omega : 1.0, op.work(304, ps, n), op.work(305, ps, n)));
+    reorth = p.reorth >= 2 ? 2 : 1;
+    nb = blas_blocks(n);
+    CMX_HIP(hipEventCreate(&ev_a));
+    CMX_HIP(hipEventCreate(&ev_b));
+    CMX_HIP(hipEventRecord(ev_a, op.stream));
 
-  // |chi|, target (:463-465); r = chi - A psi, |r| (:468-475)
-  op.blas(B_NORM2, {&chi});
-  op.finalize(nb, 1, PG_STORE, K_WN);
-  kd.get(K_WN, 1);
-  const double norm_rhs = std::sqrt(op.hS[K_WN]);
-  const double target = norm_rhs * p.rsd_target;
-  auto true_residual = [&](bool psi_zero = false) {
+    // |chi|, target (:463-465); r = chi - A psi, |r| (:468-475)
+    op.blas(B_NORM2, {&chi});
+    op.finalize(nb, 1, PG_STORE, K_WN);
+    kd.get(K_WN, 1);
+    norm_rhs = std::sqrt(op.hS[K_WN]);
+    target = norm_rhs * p.rsd_target;
+    // a zero initial guess (the usual case) needs no operator application for the first residual
+    op.blas(B_NORM2, {&psi});
+    op.finalize(nb, 1, PG_STORE, K_WN);
+    kd.get(K_WN, 1);
+    r_norm = true_residual(op.hS[K_WN] == 0.0);
+    r_prev = r_norm;
+    H = Mat(total + 1, total);
+    R = Mat(total + 1, total);
+    g.assign(total + 1, cd(0));
+    c.assign(total + 1, cd(0));
+    eta.assign(total, cd(0));
+    rot.assign(total, Givens());
+    done = r_norm <= target;
+  }
+  ~Impl() {
+    if (ev_a) (void)hipEventDestroy(ev_a);
+    if (ev_b) (void)hipEventDestroy(ev_b);
+  }
+
+  double true_residual(bool psi_zero = false) {
     if (psi_zero)  // A 0 = 0 exactly: the same r without the FP64 operator
-      op.blas(B_ZERO, {&tmp});
+      op.blas(B_ZERO, {tmp});
     else
-      op.apply(P64, tmp.view(), psi.view(), +1, nullptr);
-    op.blas(B_XMY_NORM, {&r, &chi, &tmp});
+      op.apply(P64, tmp->view(), psi.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {r, &chi, tmp});
     op.finalize(nb, 1, PG_STORE, K_WN);
     kd.get(K_WN, 1);
     return std::sqrt(op.hS[K_WN]);
-  };
-  // a zero initial guess (the usual case) needs no operator application for the first residual
-  op.blas(B_NORM2, {&psi});
-  op.finalize(nb, 1, PG_STORE, K_WN);
-  kd.get(K_WN, 1);
-  double r_norm = true_residual(op.hS[K_WN] == 0.0);
-
-  Mat H(total + 1, total), R(total + 1, total);
-  std::vector<cd> g(total + 1), c(total + 1), eta(total);
-  std::vector<Givens> rot(total);
-  Mat Qk;  // Q of the deflated block (nd+1 x nd+1)
-  int iters = 0, cycles = 0, prev_dim = nk, prev_nd = 0;
-  bool finished = r_norm <= target;
-  std::vector<Vec*> vs(total + 1);
-
-  auto precondition_s = [&](Vec& z, const Vec& v) {  // both in precision ps
+  }
+  void precondition_s(Vec& z, const Vec& v) {  // both in precision ps
     switch (p.precond) {
       case CMX_PRECOND_NONE: op.blas(B_COPY, {&z, &v}); break;
       case CMX_PRECOND_MR: mr->apply(z, v); break;
+      case CMX_PRECOND_MG: mg_precondition(op, ps, v, z); break;
       default: learned_precond_apply(op, ps, v, z); break;
     }
-  };
-  auto precondition = [&](Vec& z, const Vec& v) {  // z, v in the basis precision
+  }
+  void precondition(Vec& z, const Vec& v) {  // z, v in the basis precision
     if (pb == ps || p.precond == CMX_PRECOND_NONE) {
       if (pb == ps) precondition_s(z, v);
       else op.blas(B_COPY, {&z, &v});
@@ -264,8 +257,8 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     if (p.precond == CMX_PRECOND_MR) op.blas(B_COPY, {ys, &z});  // MR starts from z
     precondition_s(*ys, *xs);
     op.blas(B_COPY, {&z, ys});
-  };
-  auto apply_a = [&](Vec& w, const Vec& z) {  // w = A z at the operator precision
+  }
+  void apply_a(Vec& w, const Vec& z) {  // w = A z at the operator precision
     if (pb == ps) {
       op.apply(ps, w.view(), z.view(), +1, nullptr);
       return;
@@ -273,15 +266,13 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     op.blas(B_COPY, {xs, &z});
     op.apply(ps, ys->view(), xs->view(), +1, nullptr);
     op.blas(B_COPY, {&w, ys});
-  };
+  }
 
-  bool plain_next = true;  // the first cycle, and any cycle after a deflated one that stalled
-  double r_prev = r_norm;
-  while (!finished) {
+  void begin_cycle() {
     ++cycles;
     // a cycle that ended early (cycle_reduction, a converged estimate, an exact breakdown) may have
     // fewer than NDefl Arnoldi vectors: deflate at most that many harmonic Ritz vectors
-    int nd = std::min(nd_max, prev_dim);
+    nd = std::min(nd_max, prev_dim);
     if (plain_next || nd == 0) {
       // c = g = [|r|, 0, ...]; V_0 = r / |r|  (:491-513)
       std::fill(c.begin(), c.end(), cd(0));
@@ -290,138 +281,151 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
       H = Mat(total + 1, total);
       R = Mat(total + 1, total);
       kd.put1(K_WN + 1, 1.0 / r_norm);
-      op.blas(B_SCALE, {V[0], &r}, K_WN + 1);
+      op.blas(B_SCALE, {V[0], r}, K_WN + 1);
       nd = 0;
     } else {
-      // deflated restart (:514-688): harmonic Ritz vectors of the previous cycle
-      const int m = prev_dim;
-      std::vector<cd> hm(m), f(m);
-      Mat Hm(m, m);
-      for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) Hm(i, j) = H(i, j);
-      for (int j = 0; j < m; ++j) hm[j] = std::conj(H(m, j));
-      f = dense::lu_solve(dense::adjoint(Hm), hm);  // H_m^H f = h_m
-      Mat Ht(m, m);
-      for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) Ht(i, j) = H(i, j) + f[i] * std::conj(hm[j]);
-      std::vector<cd> evals;
-      Mat evecs;
-      dense::eig(Ht, evals, evecs);
-      std::vector<int> order(m);
-      for (int i = 0; i < m; ++i) order[i] = i;
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int a, int b) { return std::norm(evals[a]) < std::norm(evals[b]); });
-      // G_{k+1} = [ [G_k; 0] | c - H eta ]  ((m+1) x (nd+1)), then Q_{k+1} from its QR
-      Mat G(m + 1, nd + 1);
-      for (int col = 0; col < nd; ++col)
-        for (int row = 0; row < m; ++row) G(row, col) = evecs(row, order[col]);
-      for (int row = 0; row <= m; ++row) {
-        cd s = c[row];
-        for (int col = 0; col < m; ++col) s -= H(row, col) * eta[col];
-        G(row, nd) = s;
-      }
-      Mat Q, Rq;
-      dense::qr(G, Q, Rq);
-      // H_new = Q_{k+1}^H H Q_k   ((nd+1) x nd)
-      Mat Hfull(m + 1, m), Qk_(m, nd);
-      for (int i = 0; i <= m; ++i)
-        for (int j = 0; j < m; ++j) Hfull(i, j) = H(i, j);
-      for (int i = 0; i < m; ++i)
-        for (int j = 0; j < nd; ++j) Qk_(i, j) = Q(i, j);
-      Mat Hn = dense::matmul(dense::adjoint(Q), dense::matmul(Hfull, Qk_));
-      // new V_i = sum_j V_j Q(j, i) (i <= nd); new Z_i = sum_j Z_j Q(j, i) (i < nd), formed in the
-      // spare vectors (basis precision), which then take the places of V_i, Z_i
-      for (int j = 0; j <= m; ++j) vs[j] = V[j];
-      for (int i = 0; i <= nd; ++i) {
-        std::vector<cd> col(m + 1);
-        for (int j = 0; j <= m; ++j) col[j] = Q(j, i);
-        kd.put(K_C, col);
-        kd.maxpy(*spareV[i], vs, m + 1, K_C, 1.0, true, -1);
-      }
-      for (int j = 0; j < m; ++j) vs[j] = Z[j];
-      for (int i = 0; i < nd; ++i) {
-        std::vector<cd> col(m);
-        for (int j = 0; j < m; ++j) col[j] = Q(j, i);
-        kd.put(K_C, col);
-        kd.maxpy(*spareZ[i], vs, m, K_C, 1.0, true, -1);
-      }
-      for (int i = 0; i <= nd; ++i) std::swap(V[i], spareV[i]);
-      for (int i = 0; i < nd; ++i) std::swap(Z[i], spareZ[i]);
-      for (int i = nd; i < total; ++i) op.blas(B_ZERO, {Z[i]});
-      // reset H, R, c, g; c = V_{k+1}^H r (:640-650)
-      H = Mat(total + 1, total);
-      R = Mat(total + 1, total);
-      std::fill(c.begin(), c.end(), cd(0));
-      std::fill(g.begin(), g.end(), cd(0));
-      {
-        // <V_i, r> with r in FP64: convert r once to the basis precision
-        if (rs != &r) op.blas(B_COPY, {rs, &r});
-        for (int i = 0; i <= nd; ++i) vs[i] = V[i];
-        kd.mdot(vs, nd + 1, *rs, K_H);
-        kd.get(K_H, 2 * (nd + 1));
-        for (int i = 0; i <= nd; ++i) c[i] = g[i] = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
-      }
-      for (int i = 0; i <= nd; ++i)
-        for (int j = 0; j < nd; ++j) H(i, j) = Hn(i, j);
-      // QR of H_k ((nd+1) x nd): R block and g = Q_k^H c (:652-670)
-      Mat Hk(nd + 1, nd);
-      for (int i = 0; i <= nd; ++i)
-        for (int j = 0; j < nd; ++j) Hk(i, j) = Hn(i, j);
-      Mat Rk;
-      dense::qr(Hk, Qk, Rk, true);
-      for (int i = 0; i < nd; ++i)
-        for (int j = i; j < nd; ++j) R(i, j) = Rk(i, j);
-      std::vector<cd> gq(nd + 1, cd(0));
-      for (int i = 0; i <= nd; ++i)
-        for (int j = 0; j <= nd; ++j) gq[i] += std::conj(Qk(j, i)) * g[j];
-      for (int i = 0; i <= nd; ++i) g[i] = gq[i];
+      deflated_restart();
     }
-
     // flexible Arnoldi (:78-183); with cycle_reduction > 0 the cycle also ends once its estimate has
     // fallen by that factor below the residual it started from
-    const double cycle_stop = std::max(target, p.cycle_reduction > 0 ? p.cycle_reduction * r_norm : 0.0);
-    int dim = nd;
-    for (int j = nd; j < nd + nk; ++j) {
-      precondition(*Z[j], *V[j]);                                     // z_j = M v_j
-      apply_a(*V[j + 1], *Z[j]);                                     // w = A z_j
-      for (int i = 0; i <= j; ++i) vs[i] = V[i];
-      kd.mdot(vs, j + 1, *V[j + 1], K_H);                             // h_i = <v_i, w>
-      kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
-      if (reorth == 2) {
-        kd.mdot(vs, j + 1, *V[j + 1], K_H2);
-        kd.maxpy(*V[j + 1], vs, j + 1, K_H2, -1.0, false, K_WN);
-      }
-      kd.get(K_H, K_WN + 2 - K_H);  // the new column (both passes), |w|^2 and 1/|w| in one read-back
-      for (int i = 0; i <= j; ++i) {
-        H(i, j) = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
-        if (reorth == 2) H(i, j) += cd(op.hS[K_H2 + 2 * i], op.hS[K_H2 + 2 * i + 1]);
-      }
-      const double wnorm = std::sqrt(op.hS[K_WN]);
-      H(j + 1, j) = wnorm;
-      if (std::fabs(wnorm) < 1e-14) {  // exact convergence (:117-127)
-        dim = j;
-        break;
-      }
-      op.blas(B_SCALE, {V[j + 1], V[j + 1]}, K_WN + 1);  // v_{j+1} = w / |w|
-      std::vector<cd> col(j + 2);
-      for (int i = 0; i <= j + 1; ++i) col[i] = H(i, j);
-      if (nd > 0) {  // R_col <- Q_k^H R_col on the deflated rows (:133-152)
-        std::vector<cd> t(nd + 1, cd(0));
-        for (int i = 0; i <= nd; ++i)
-          for (int k = 0; k <= nd; ++k) t[i] += std::conj(Qk(k, i)) * col[k];
-        for (int i = 0; i <= nd; ++i) col[i] = t[i];
-      }
-      for (int i = nd; i < j; ++i) rot[i].apply(col);
-      rot[j] = Givens(j, col[j], col[j + 1]);
-      col[j] = rot[j].r;
-      col[j + 1] = 0;
-      rot[j].apply(g);
-      for (int i = 0; i <= j + 1; ++i) R(i, j) = col[i];
-      dim = j + 1;
-      const double accum = std::fabs(g[j + 1].real());  // (:172)
-      if (p.verbose) std::printf("FGMRESDR: iter %d |r| ~ %g target %g\n", iters + j - nd + 1, accum, target);
-      if (accum <= cycle_stop) break;
+    cycle_stop = std::max(target, p.cycle_reduction > 0 ? p.cycle_reduction * r_norm : 0.0);
+    dim = nd;
+    j = nd;
+  }
+
+  // deflated restart (:514-688): harmonic Ritz vectors of the previous cycle
+  void deflated_restart() {
+    const int m = prev_dim;
+    std::vector<cd> hm(m), f(m);
+    Mat Hm(m, m);
+    for (int i = 0; i < m; ++i)
+      for (int k = 0; k < m; ++k) Hm(i, k) = H(i, k);
+    for (int k = 0; k < m; ++k) hm[k] = std::conj(H(m, k));
+    f = dense::lu_solve(dense::adjoint(Hm), hm);  // H_m^H f = h_m
+    Mat Ht(m, m);
+    for (int i = 0; i < m; ++i)
+      for (int k = 0; k < m; ++k) Ht(i, k) = H(i, k) + f[i] * std::conj(hm[k]);
+    std::vector<cd> evals;
+    Mat evecs;
+    dense::eig(Ht, evals, evecs);
+    std::vector<int> order(m);
+    for (int i = 0; i < m; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return std::norm(evals[a]) < std::norm(evals[b]); });
+    // G_{k+1} = [ [G_k; 0] | c - H eta ]  ((m+1) x (nd+1)), then Q_{k+1} from its QR
+    Mat G(m + 1, nd + 1);
+    for (int col = 0; col < nd; ++col)
+      for (int row = 0; row < m; ++row) G(row, col) = evecs(row, order[col]);
+    for (int row = 0; row <= m; ++row) {
+      cd s = c[row];
+      for (int col = 0; col < m; ++col) s -= H(row, col) * eta[col];
+      G(row, nd) = s;
     }
+    Mat Q, Rq;
+    dense::qr(G, Q, Rq);
+    // H_new = Q_{k+1}^H H Q_k   ((nd+1) x nd)
+    Mat Hfull(m + 1, m), Qk_(m, nd);
+    for (int i = 0; i <= m; ++i)
+      for (int k = 0; k < m; ++k) Hfull(i, k) = H(i, k);
+    for (int i = 0; i < m; ++i)
+      for (int k = 0; k < nd; ++k) Qk_(i, k) = Q(i, k);
+    Mat Hn = dense::matmul(dense::adjoint(Q), dense::matmul(Hfull, Qk_));
+    // new V_i = sum_k V_k Q(k, i) (i <= nd); new Z_i = sum_k Z_k Q(k, i) (i < nd), formed in the
+    // spare vectors (basis precision), which then take the places of V_i, Z_i
+    for (int k = 0; k <= m; ++k) vs[k] = V[k];
+    for (int i = 0; i <= nd; ++i) {
+      std::vector<cd> col(m + 1);
+      for (int k = 0; k <= m; ++k) col[k] = Q(k, i);
+      kd.put(K_C, col);
+      kd.maxpy(*spareV[i], vs, m + 1, K_C, 1.0, true, -1);
+    }
+    for (int k = 0; k < m; ++k) vs[k] = Z[k];
+    for (int i = 0; i < nd; ++i) {
+      std::vector<cd> col(m);
+      for (int k = 0; k < m; ++k) col[k] = Q(k, i);
+      kd.put(K_C, col);
+      kd.maxpy(*spareZ[i], vs, m, K_C, 1.0, true, -1);
+    }
+    for (int i = 0; i <= nd; ++i) std::swap(V[i], spareV[i]);
+    for (int i = 0; i < nd; ++i) std::swap(Z[i], spareZ[i]);
+    for (int i = nd; i < total; ++i) op.blas(B_ZERO, {Z[i]});
+    // reset H, R, c, g; c = V_{k+1}^H r (:640-650)
+    H = Mat(total + 1, total);
+    R = Mat(total + 1, total);
+    std::fill(c.begin(), c.end(), cd(0));
+    std::fill(g.begin(), g.end(), cd(0));
+    {
+      // <V_i, r> with r in FP64: convert r once to the basis precision
+      if (rs != r) op.blas(B_COPY, {rs, r});
+      for (int i = 0; i <= nd; ++i) vs[i] = V[i];
+      kd.mdot(vs, nd + 1, *rs, K_H);
+      kd.get(K_H, 2 * (nd + 1));
+      for (int i = 0; i <= nd; ++i) c[i] = g[i] = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
+    }
+    for (int i = 0; i <= nd; ++i)
+      for (int k = 0; k < nd; ++k) H(i, k) = Hn(i, k);
+    // QR of H_k ((nd+1) x nd): R block and g = Q_k^H c (:652-670)
+    Mat Hk(nd + 1, nd);
+    for (int i = 0; i <= nd; ++i)
+      for (int k = 0; k < nd; ++k) Hk(i, k) = Hn(i, k);
+    Mat Rk;
+    dense::qr(Hk, Qk, Rk, true);
+    for (int i = 0; i < nd; ++i)
+      for (int k = i; k < nd; ++k) R(i, k) = Rk(i, k);
+    std::vector<cd> gq(nd + 1, cd(0));
+    for (int i = 0; i <= nd; ++i)
+      for (int k = 0; k <= nd; ++k) gq[i] += std::conj(Qk(k, i)) * g[k];
+    for (int i = 0; i <= nd; ++i) g[i] = gq[i];
+  }
+
+  void issue_step() {
+    precondition(*Z[j], *V[j]);  // z_j = M v_j
+    apply_a(*V[j + 1], *Z[j]);   // w = A z_j
+    for (int i = 0; i <= j; ++i) vs[i] = V[i];
+    kd.mdot(vs, j + 1, *V[j + 1], K_H);  // h_i = <v_i, w>
+    kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
+    if (reorth == 2) {
+      kd.mdot(vs, j + 1, *V[j + 1], K_H2);
+      kd.maxpy(*V[j + 1], vs, j + 1, K_H2, -1.0, false, K_WN);
+    }
+    kd.get_async(K_H, K_WN + 2 - K_H);  // the new column (both passes), |w|^2 and 1/|w| in one read-back
+  }
+
+  bool finish_step() {
+    for (int i = 0; i <= j; ++i) {
+      H(i, j) = cd(op.hS[K_H + 2 * i], op.hS[K_H + 2 * i + 1]);
+      if (reorth == 2) H(i, j) += cd(op.hS[K_H2 + 2 * i], op.hS[K_H2 + 2 * i + 1]);
+    }
+    const double wnorm = std::sqrt(op.hS[K_WN]);
+    H(j + 1, j) = wnorm;
+    if (std::fabs(wnorm) < 1e-14) {  // exact convergence (:117-127)
+      dim = j;
+      return false;
+    }
+    op.blas(B_SCALE, {V[j + 1], V[j + 1]}, K_WN + 1);  // v_{j+1} = w / |w|
+    std::vector<cd> col(j + 2);
+    for (int i = 0; i <= j + 1; ++i) col[i] = H(i, j);
+    if (nd > 0) {  // R_col <- Q_k^H R_col on the deflated rows (:133-152)
+      std::vector<cd> t(nd + 1, cd(0));
+      for (int i = 0; i <= nd; ++i)
+        for (int k = 0; k <= nd; ++k) t[i] += std::conj(Qk(k, i)) * col[k];
+      for (int i = 0; i <= nd; ++i) col[i] = t[i];
+    }
+    for (int i = nd; i < j; ++i) rot[i].apply(col);
+    rot[j] = Givens(j, col[j], col[j + 1]);
+    col[j] = rot[j].r;
+    col[j + 1] = 0;
+    rot[j].apply(g);
+    for (int i = 0; i <= j + 1; ++i) R(i, j) = col[i];
+    dim = j + 1;
+    const double accum = std::fabs(g[j + 1].real());  // (:172)
+    if (p.verbose) std::printf("FGMRESDR: iter %d |r| ~ %g target %g\n", iters + j - nd + 1, accum, target);
+    if (accum <= cycle_stop) return false;
+    ++j;
+    return j < nd + nk;
+  }
+
+  void end_cycle() {
     // eta = R^-1 g by back substitution (:240-262); psi += sum eta_j Z_j (:716-724)
     if (dim > 0) {
       for (int row = dim - 1; row >= 0; --row) {
@@ -429,7 +433,7 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
         for (int col = row + 1; col < dim; ++col) s -= R(row, col) * eta[col];
         eta[row] = s / R(row, row);
       }
-      for (int j = 0; j < dim; ++j) vs[j] = Z[j];
+      for (int k = 0; k < dim; ++k) vs[k] = Z[k];
       kd.put(K_C, std::vector<cd>(eta.begin(), eta.begin() + dim));
       kd.maxpy(psi, vs, dim, K_C, 1.0, false, -1);
     }
@@ -440,25 +444,64 @@ void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi
     // from the true residual (a plain FGMRES cycle).
     plain_next = nd > 0 && r_norm > 0.5 * r_prev;
     r_prev = r_norm;
-    finished = r_norm <= target || iters >= p.max_iter || (dim == nd && !plain_next);
+    done = r_norm <= target || iters >= p.max_iter || (dim == nd && !plain_next);
     prev_dim = dim;
-    prev_nd = nd;
     if (p.verbose)
       std::printf("FGMRESDR: cycle %d: %d iterations, |r| = %g target %g\n", cycles, dim - nd, r_norm, target);
   }
-  (void)prev_nd;
-  CMX_HIP(hipEventRecord(ev.b, op.stream));
-  CMX_HIP(hipEventSynchronize(ev.b));
-  float ms = 0;
-  CMX_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
-  res.n_count = iters;
-  res.resid = r_norm;
-  res.rel_resid = norm_rhs > 0 ? r_norm / norm_rhs : r_norm;
-  res.n_reliable = cycles;
-  res.solve_seconds = ms * 1e-3;
-  res.gflops = 0;
-  if (r_norm > target)
-    throw Error(CMX_ERR_NOCONV, "FGMRES-DR: no convergence in " + std::to_string(p.max_iter) + " iterations");
+
+  void result(cmx_solver_result& res) {
+    CMX_HIP(hipEventRecord(ev_b, op.stream));
+    CMX_HIP(hipEventSynchronize(ev_b));
+    float ms = 0;
+    CMX_HIP(hipEventElapsedTime(&ms, ev_a, ev_b));
+    res.n_count = iters;
+    res.resid = r_norm;
+    res.rel_resid = norm_rhs > 0 ? r_norm / norm_rhs : r_norm;
+    res.n_reliable = cycles;
+    res.solve_seconds = ms * 1e-3;
+    res.gflops = 0;
+    if (r_norm > target)
+      throw Error(CMX_ERR_NOCONV, "FGMRES-DR: no convergence in " + std::to_string(p.max_iter) + " iterations");
+  }
+};
+
+FgmresDR::FgmresDR(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi) : m_(new Impl(op, p, psi, chi)) {}
+void FgmresDR::reserve(Op& op, const cmx_solver_params& p) {
+  const int n = op.subset_sites(), ps = p.sloppy_prec, pb = p.basis_prec < 0 ? ps : p.basis_prec;
+  CMX_REQUIRE(ps >= P64 && ps <= P16 && pb >= P64 && pb <= P16 && p.n_krylov >= 1 && p.n_defl >= 0 &&
+                  p.n_krylov + p.n_defl + 1 <= KRY_MAXV, CMX_ERR_ARG, "FGMRES-DR: bad parameters");
+  const int nd = p.n_defl, total = p.n_krylov + nd;
+  for (int i = 0; i <= total; ++i) op.work(100 + i, pb, n);
+  for (int i = 0; i < total; ++i) op.work(150 + i, pb, n);
+  for (int i = 0; i <= nd; ++i) op.work(200 + i, pb, n);
+  for (int i = 0; i < nd; ++i) op.work(250 + i, pb, n);
+  op.work(300, P64, n);
+  op.work(301, P64, n);
+  if (nd > 0 && pb != P64) op.work(306, pb, n);
+  if (pb != ps) op.work(302, ps, n), op.work(303, ps, n);
+  if (p.precond == CMX_PRECOND_MR) op.work(304, ps, n), op.work(305, ps, n);
+  KryDev::partials_of(op);
+}
+FgmresDR::~FgmresDR() = default;
+bool FgmresDR::finished() const { return m_->done; }
+void FgmresDR::begin_cycle() { m_->begin_cycle(); }
+void FgmresDR::issue_step() { m_->issue_step(); }
+bool FgmresDR::finish_step() { return m_->finish_step(); }
+void FgmresDR::end_cycle() { m_->end_cycle(); }
+void FgmresDR::result(cmx_solver_result& res) { m_->result(res); }
+
+void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res) {
+  FgmresDR run(op, p, psi, chi);
+  while (!run.finished()) {
+    run.begin_cycle();
+    do {
+      run.issue_step();
+      op.sync();
+    } while (run.finish_step());
+    run.end_cycle();
+  }
+  run.result(res);
 }
 
 }  // namespace cmx

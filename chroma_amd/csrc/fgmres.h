@@ -1,0 +1,72 @@
+// FGMRES-DR as a stepping object, so that several solves can advance together on one stream with one
+// host read-back per Arnoldi step for all of them (the batched solver), while each performs exactly the
+// kernels and host arithmetic of a single solve.
+#pragma once
+#include <memory>
+
+#include "op.h"
+
+namespace cmx {
+
+// MR preconditioner (LinOpSysSolverMR -> InvMR, invmr.cc:55-160) in precision ps: z is the initial
+// guess on entry (the flexible solver passes Z_j) and the result on exit.
+struct MRPrecond {
+  Op& op;
+  int ps, n, maxit;
+  double rsd, omega;
+  Vec &r, &mr;
+  MRPrecond(Op& o, int ps_, int n_, int maxit_, double rsd_, double omega_, Vec& r_, Vec& mr_)
+      : op(o), ps(ps_), n(n_), maxit(maxit_), rsd(rsd_), omega(omega_), r(r_), mr(mr_) {}
+  void apply(Vec& z, const Vec& v) {
+    const int nb = blas_blocks(n);
+    int* F = op.state_F.as<int>();
+    op.hS[S_MR + 9] = omega;
+    op.hS[S_MR + 10] = rsd;
+    CMX_HIP(hipMemcpyAsync(static_cast<double*>(op.state_S.p) + S_MR + 9, op.hS + S_MR + 9, 2 * sizeof(double),
+                           hipMemcpyHostToDevice, op.stream));
+    op.blas(B_NORM2, {&v});
+    op.finalize(nb, 1, PG_STORE, S_MR + 7);  // |chi|^2
+    op.apply(ps, mr.view(), z.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {&r, &v, &mr});      // r = chi - M psi
+    op.finalize(nb, 1, PG_MR_INIT);
+    for (int k = 0; k < maxit; ++k) {
+      op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
+      op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
+      op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
+      op.blas(B_CAXPY, {&z, &r}, S_MR + 3, 0, F_MRDONE);              // psi += a r
+      op.blas(B_AXPY_CNORM, {&r, &mr}, S_MR + 5, 0, F_MRDONE);        // r -= a M r
+      op.finalize(nb, 1, PG_MR_R);
+    }
+  }
+};
+
+class FgmresDR {
+ public:
+  // Sets up the workspace of `op` and computes the initial residual (synchronises op's stream)
+  FgmresDR(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi);
+  ~FgmresDR();
+  FgmresDR(const FgmresDR&) = delete;
+  FgmresDR& operator=(const FgmresDR&) = delete;
+  // allocates the workspace a solve with these parameters uses on op (the constructor does so too)
+  static void reserve(Op& op, const cmx_solver_params& p);
+  bool finished() const;
+  // the restart: plain (V_0 = r / |r|) or deflated (harmonic Ritz vectors of the previous cycle)
+  void begin_cycle();
+  // issues one Arnoldi step on op's stream, ending with an asynchronous read-back of the new column
+  void issue_step();
+  // after op's stream has reached the read-back: the Givens update; false when the cycle has ended
+  bool finish_step();
+  // psi += Z eta, the FP64 residual, the stopping decision
+  void end_cycle();
+  // fills res; throws CMX_ERR_NOCONV when the target was not reached
+  void result(cmx_solver_result& res);
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> m_;
+};
+
+// one solve (LinOpSysSolverFGMRESDR::operator(), syssolver_linop_fgmres_dr.cc:448-774)
+void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res);
+
+}  // namespace cmx

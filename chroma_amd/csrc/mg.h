@@ -1,0 +1,77 @@
+// Two-level multigrid preconditioner (the role of QUDA's multigrid behind MULTIGRIDSolverParams,
+// quda_multigrid_params.h:12-113) for the even-odd operator: kernels (mg.hip) and host state (mg.cpp).
+//
+// The coarse space aggregates the null vectors of the unpreconditioned operator M over blocks of the
+// full lattice, split by chirality (gamma_5 is diagonal in the DeGrand-Rossi basis: spins 0,1 and 2,3):
+// coarse degrees of freedom per block D = 2 nvec.  The coarse operator is the Galerkin product
+// A_c = P^dag M P with a nearest-neighbour (9-point) block stencil.  The V-cycle preconditions the
+// even-odd Schur operator A = A_oo - D_oe A_ee^-1 D_eo / 4 through the identity A^-1 = (M^-1)_oo: MR
+// smoothing on A, restriction of the odd residual, a GCR solve of the coarse system, prolongation of
+// the odd part, MR smoothing.
+#pragma once
+#include "op.h"
+
+namespace cmx {
+
+constexpr int MG_MAXVEC = 16;           // null vectors
+constexpr int MG_MAXD = 2 * MG_MAXVEC;  // coarse degrees of freedom per block
+constexpr int MG_NST = 9;               // coarse stencil: the block, then +x, -x, +y, -y, +z, -z, +t, -t
+constexpr int MG_MAXKRY = 32;           // coarse GCR directions
+
+struct MGGeom {
+  Geom g;          // fine lattice
+  int B[4], Lc[4]; // block extents, coarse extents
+  int Nb, nvec, D, bvol;
+};
+
+struct MGVecs {
+  const float4* P[MG_MAXVEC];  // null vectors: full lattice (even then odd), FP32, 6 chunks of 2 Vh
+};
+
+// block orthonormalisation of the null vectors per (block, chirality), FP64 sums
+void launch_mg_orth(const MGGeom& m, float4* const* P, hipStream_t s);
+// out[b * D + c * nvec + k] = sum over the block's sites of P_k^c(x)^dag v(x): full = true: v a full-lattice
+// vector (view of the whole vector, stride 2 Vh); false: v an odd-checkerboard vector (its own stride)
+void launch_mg_restrict(const MGGeom& m, const MGVecs& P, int prec, VRef v, bool full, float2* out, hipStream_t s);
+// out (+)= P e (precision prec): full = true: out a full-lattice vector; false: odd checkerboard
+void launch_mg_prolong(const MGGeom& m, const MGVecs& P, int prec, const float2* e, VRef out, bool full, bool add,
+                       hipStream_t s);
+// probe vector: full-lattice FP64, P_k^c on the blocks of colour col, zero elsewhere
+void launch_mg_probe(const MGGeom& m, const MGVecs& P, const int* colour, int col, int k, int c, VRef out,
+                     hipStream_t s);
+// Ac[b][d][r][dof] = y[b][r] for the first stencil entry d whose block has colour col
+void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int col, int dof, const float2* y,
+                       float2* Ac, hipStream_t s);
+// y = A_c x
+void launch_mg_coarse_apply(const MGGeom& m, const int* nbr, const float2* Ac, const float2* x, float2* y,
+                            hipStream_t s);
+// coarse vectors (n complex): partials[j][block] = sum over the block's elements of conj(v_j) w, j < m
+int mg_cdot_blocks(int n);
+void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, hipStream_t s);
+// GCR update: q_j = (q_j - sum_i c_i q_i) inv; p_j likewise; x += alpha p_j; r -= alpha q_j
+struct MGUpdate {
+  float2* q[MG_MAXKRY];
+  float2* p[MG_MAXKRY];
+  float2 c[MG_MAXKRY];
+  int j, n;
+  float inv;
+  float2 alpha;
+  float2 *x, *r;
+};
+void launch_mg_update(const MGUpdate& u, hipStream_t s);
+
+struct MGState;
+// Builds the null space, the block-orthonormal prolongator and the coarse operator on op
+void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res);
+// z = K v: one V-cycle on the odd-checkerboard system at precision prec (v, z in that precision)
+void mg_precondition(Op& op, int prec, const Vec& v, Vec& z);
+// test and inspection access: null vectors (host order, FP64 copies), the coarse operator, and the
+// restriction / prolongation / coarse operator on host arrays
+void mg_get_nullvecs(Op& op, double* out);
+void mg_get_coarse_op(Op& op, double* out);
+void mg_restrict_host(Op& op, const double* full_in, double* coarse_out);
+void mg_prolong_host(Op& op, const double* coarse_in, double* full_out);
+void mg_coarse_apply_host(Op& op, const double* in, double* out);
+void mg_info(Op& op, int* nb, int* d);
+
+}  // namespace cmx

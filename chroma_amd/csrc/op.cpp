@@ -265,6 +265,7 @@ Op* clone_op(Op& o, int prec) {
   CMX_HIP(hipHostMalloc((void**)&c->hS, sizeof(double) * S_NSLOTS, 0));
   CMX_HIP(hipHostMalloc((void**)&c->hF, sizeof(int) * F_NFLAGS, 0));
   c->staging.alias(o.staging);
+  c->mg = o.mg;  // the multigrid state is read-only in a solve, but for its coarse work space (one stream)
   return c.release();
 }
 

@@ -1,6 +1,7 @@
 // Device-resident operator state behind the cmx_gauge / cmx_op handles.
 #pragma once
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "blas.h"
@@ -118,6 +119,8 @@ struct Op {
   std::shared_ptr<LearnedNet> net;
   long net_version = -1;
   std::shared_ptr<void> ptc;
+  // multigrid preconditioner state (mg.cpp)
+  std::shared_ptr<void> mg;
 
   ~Op();
   bool eo() const { return par.kind != CMX_OP_UNPREC_WILSON; }
@@ -168,6 +171,9 @@ void net_apply_host(Op& op, int prec, double* out, const double* in);
 void net_set_variant(Op& op, int prec, int variant);
 void net_grad_host(Op& op, int prec, const double* v, const double* e, double* loss, double* gwc, double* gw0);
 void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
+// `to` (a solver context on the same operator, clone_op) uses from's network and its device buffers; the
+// two must issue their work on one stream
+void learned_share(Op& from, Op& to);
 
 // mdagm: solve M^dag M psi = chi instead of M psi = chi
 void solve(Op& op, const cmx_solver_params& p, double* psi, const double* chi, bool mdagm, cmx_solver_result& res);
@@ -191,6 +197,11 @@ void meson_2pt(const int dims[4], const double* prop1, const double* prop2, cons
 void meson_scratch_setup();
 void meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4], int nmom,
                        const int* moms, double* corr, hipStream_t stream);
+// SciDAC single-file fields (qio.cpp, host only): kind 0 propagator, 1 fermion; prec P64 / P32 in the file
+void scidac_write(const char* path, const int dims[4], int kind, int prec, const double* host, const char* file_xml,
+                  const char* record_xml);
+void scidac_read(const char* path, const int dims[4], int kind, double* host, std::string* file_xml,
+                 std::string* record_xml);
 // MInvCG2 (minvcg2.cc:72-355): (M^dag M + shifts[s]) psi[s] = chi, s < ns, FP64 (multishift.cpp)
 void solve_multishift(Op& op, const cmx_solver_params& p, int ns, const double* shifts, const double* rsd,
                       double* psi, const double* chi, cmx_solver_result* res);

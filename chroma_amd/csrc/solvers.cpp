@@ -11,19 +11,21 @@
 // The MdagM system solvers (psi = (M^dag M)^-1 chi) are the same iterations without the M^dag chi
 // source preparation (CG, reliable CG), or two BiCGStab solves, M^dag Y = chi then M psi = Y
 // (syssolver_mdagm_bicgstab.h:64-100, syssolver_mdagm_rel_bicgstab_clover.h:104-176).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <utility>
 #include <vector>
 
 #include "batch.h"
+#include "fgmres.h"
 #include "op.h"
 
 namespace cmx {
 
-void solve_fgmresdr(Op& op, const cmx_solver_params& p, Vec& psi, const Vec& chi, cmx_solver_result& res);
 
 namespace {
 
@@ -528,6 +530,151 @@ std::vector<HopConfig> batch_candidates(const std::vector<Op*>& ops, int ps) {
 
 }  // namespace
 
+namespace {
+
+// Device bytes of one FGMRES-DR solve's workspace (the solution, source and residual vectors in FP64,
+// the bases and spares in the basis precision, the staging and MR vectors in the operator precision)
+size_t fgmres_workspace_bytes(const Op& op, const cmx_solver_params& p) {
+  const long n = op.subset_sites();
+  const int ps = p.sloppy_prec, pb = p.basis_prec < 0 ? ps : p.basis_prec;
+  const int nd = std::max(0, p.n_defl), total = p.n_krylov + nd;
+  size_t b = 6 * spinor_bytes(P64, n);
+  b += (size_t)(2 * total + 1 + 2 * nd + 1 + (nd > 0 && pb != P64 ? 1 : 0)) * spinor_bytes(pb, n);
+  if (pb != ps) b += 2 * spinor_bytes(ps, n);
+  if (p.precond == CMX_PRECOND_MR) b += 2 * spinor_bytes(ps, n);
+  return b;
+}
+
+// FGMRES-DR over n problems: every solve is a FgmresDR stepping object on its own solver context; at each
+// tick every solve in its Arnoldi loop issues one step, the streams are synchronised once, and each solve
+// takes its new column (the Givens update, on the host).  Cycle ends and restarts run per solve.  Each
+// solve performs exactly the kernels and host arithmetic of cmx_solve, so its result is the same bit for
+// bit.  As many solves run at once as their workspaces fit in free device memory; the others wait.
+void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p, double* const* psi_h,
+                        const double* const* chi_h, cmx_solver_result* res, cmx_batch_result& bres) {
+  const int np = (int)ops.size();
+  std::vector<hipStream_t> streams;
+  for (Op* o : ops)
+    if (std::find(streams.begin(), streams.end(), o->stream) == streams.end()) streams.push_back(o->stream);
+  auto sync_all = [&]() {
+    for (hipStream_t s : streams) CMX_HIP(hipStreamSynchronize(s));
+  };
+  // concurrency from free memory and the per-solve workspace
+  size_t free_b = 0, total_b = 0;
+  CMX_HIP(hipMemGetInfo(&free_b, &total_b));
+  size_t need = 0;
+  for (Op* o : ops) need = std::max(need, fgmres_workspace_bytes(*o, p));
+  const int conc = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, (size_t)(0.9 * (double)free_b) / std::max<size_t>(need, 1)));
+
+  std::vector<cmx_solver_params> q(np, p);
+  for (int i = 0; i < np; ++i) {
+    std::memset(&res[i], 0, sizeof(res[i]));
+    if (p.true_residual) q[i].rsd_target = 0.5 * p.rsd_target;
+  }
+  std::vector<std::unique_ptr<FgmresDR>> run(np);
+  std::vector<int> stepping(np, 0);  // 1: inside an Arnoldi cycle
+  std::vector<int> err(np, CMX_OK);
+  std::vector<std::string> msg(np);
+  std::vector<int> pending, active;
+  std::vector<char> packed(np, 0);
+  for (int i = 0; i < np; ++i) pending.push_back(i);
+  // the source and guess on the device and the workspace allocated
+  auto prepare = [&](int i) {
+    Op& op = *ops[i];
+    const int n = op.subset_sites();
+    op.pack(chi_h[i], n, op.work(1, P64, n));
+    op.pack(psi_h[i], n, op.work(0, P64, n));
+    op.work(2, P64, n);
+    op.work(3, P64, n);
+    FgmresDR::reserve(op, q[i]);
+    packed[i] = 1;
+  };
+  auto start = [&](int i) {
+    Op& op = *ops[i];
+    const int n = op.subset_sites();
+    if (!packed[i]) prepare(i);
+    run[i].reset(new FgmresDR(op, q[i], op.work(0, P64, n), op.work(1, P64, n)));
+    stepping[i] = 0;
+  };
+  // the first wave is prepared before the timed region
+  for (int k = 0; k < std::min(conc, np); ++k) prepare(k);
+  sync_all();
+  // a solve that has stopped: its result and true residual, a true-residual restart, or its solution
+  // to the host (and a solver context of the batch's own frees its workspace)
+  auto stop = [&](int i) -> bool {
+    Op& op = *ops[i];
+    const int n = op.subset_sites();
+    cmx_solver_result part;
+    std::memset(&part, 0, sizeof(part));
+    try {
+      run[i]->result(part);
+    } catch (const Error& e) {
+      err[i] = e.code;
+      msg[i] = e.what();
+    }
+    run[i].reset();
+    res[i].n_count += part.n_count;
+    res[i].n_reliable += part.n_reliable;
+    true_residual(op, op.work(0, P64, n), op.work(1, P64, n), false, res[i]);
+    if (err[i] == CMX_OK && p.true_residual && res[i].rel_resid > p.rsd_target && res[i].n_restarts < 8 &&
+        res[i].n_count < p.max_iter) {
+      q[i].rsd_target = q[i].rsd_target * std::max(0.02, 0.5 * p.rsd_target / res[i].rel_resid);
+      q[i].max_iter = std::max(1, p.max_iter - res[i].n_count);
+      res[i].n_restarts++;
+      start(i);
+      return false;
+    }
+    op.unpack(psi_h[i], n, op.work(0, P64, n));
+    if (op.shared_stream) {  // a clone made for this batch
+      op.ws.clear();
+      ++op.ws_gen;
+    }
+    return true;
+  };
+
+  Timer tm(ops[0]->stream);
+  tm.start();
+  long ticks = 0;
+  while (!pending.empty() || !active.empty()) {
+    while ((int)active.size() < conc && !pending.empty()) {
+      const int i = pending.front();
+      pending.erase(pending.begin());
+      start(i);
+      active.push_back(i);
+    }
+    // cycle starts (and solves that stopped before any step)
+    std::vector<int> still;
+    for (int i : active) {
+      bool live = true;
+      while (live && !stepping[i]) {
+        if (run[i]->finished()) {
+          live = !stop(i);
+        } else {
+          run[i]->begin_cycle();
+          stepping[i] = 1;
+        }
+      }
+      if (live) still.push_back(i);
+    }
+    active = still;
+    if (active.empty()) continue;
+    for (int i : active) run[i]->issue_step();
+    sync_all();
+    ++ticks;
+    for (int i : active)
+      if (!run[i]->finish_step()) {
+        run[i]->end_cycle();
+        stepping[i] = 0;
+      }
+  }
+  bres.solve_seconds = tm.stop();
+  bres.n_iter = (int)ticks;
+  for (int i = 0; i < np; ++i)
+    if (err[i] != CMX_OK) throw Error(err[i], "solve_batch: problem " + std::to_string(i) + ": " + msg[i]);
+}
+
+}  // namespace
+
 void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const cmx_batch_params& bp,
                  double* const* psi_h, const double* const* chi_h, bool mdagm, cmx_solver_result* res,
                  cmx_batch_result& bres) {
@@ -535,11 +682,22 @@ void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const 
   const int np = (int)ops.size();
   CMX_REQUIRE(np >= 1, CMX_ERR_ARG, "solve_batch: no problems");
   CMX_REQUIRE(p.max_iter > 0 && p.rsd_target > 0, CMX_ERR_ARG, "solve: MaxIter and RsdTarget must be positive");
-  CMX_REQUIRE(p.solver == CMX_SOLVER_CG || p.solver == CMX_SOLVER_RELIABLE_CG, CMX_ERR_UNSUPPORTED,
-              "solve_batch: CMX_SOLVER_CG or CMX_SOLVER_RELIABLE_CG");
+  CMX_REQUIRE(p.solver == CMX_SOLVER_CG || p.solver == CMX_SOLVER_RELIABLE_CG || p.solver == CMX_SOLVER_FGMRESDR,
+              CMX_ERR_UNSUPPORTED, "solve_batch: CMX_SOLVER_CG, CMX_SOLVER_RELIABLE_CG or CMX_SOLVER_FGMRESDR");
   for (int i = 0; i < np; ++i)
     for (int j = 0; j < i; ++j)
       CMX_REQUIRE(ops[i] != ops[j], CMX_ERR_ARG, "solve_batch: each problem needs its own solver context");
+  if (p.solver == CMX_SOLVER_FGMRESDR) {
+    CMX_REQUIRE(!mdagm, CMX_ERR_UNSUPPORTED, "FGMRES-DR is a LinOp system solver (no MdagM form)");
+    std::memset(&bres, 0, sizeof(bres));
+    solve_batch_fgmres(ops, p, psi_h, chi_h, res, bres);
+    bres.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i = 0; i < np; ++i) {
+      res[i].solve_seconds = bres.solve_seconds;
+      res[i].total_seconds = bres.total_seconds;
+    }
+    return;
+  }
   std::memset(&bres, 0, sizeof(bres));
   const bool rel = p.solver == CMX_SOLVER_RELIABLE_CG;
   const int ps = rel ? p.sloppy_prec : P64;

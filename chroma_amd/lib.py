@@ -17,7 +17,7 @@ DOUBLE, SINGLE, HALF = 0, 1, 2
 RECONS_18, RECONS_12 = 18, 12
 UNPREC_WILSON, EO_WILSON, EO_CLOVER = 0, 1, 2
 CG, BICGSTAB, RELIABLE_CG, RELIABLE_BICGSTAB, FGMRESDR = 0, 1, 2, 3, 4
-PRECOND_NONE, PRECOND_MR, PRECOND_LEARNED = 0, 1, 2
+PRECOND_NONE, PRECOND_MR, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2, 3
 PLUS, MINUS = 1, -1
 
 
@@ -40,6 +40,20 @@ class SolverParams(ctypes.Structure):
                 ("n_defl", ctypes.c_int), ("precond", ctypes.c_int), ("precond_iters", ctypes.c_int),
                 ("precond_rsd", ctypes.c_double), ("precond_omega", ctypes.c_double), ("reorth", ctypes.c_int),
                 ("basis_prec", ctypes.c_int), ("cycle_reduction", ctypes.c_double), ("graph", ctypes.c_int)]
+
+
+class MGParams(ctypes.Structure):
+    """cmx_mg_params: the two-level form of MULTIGRIDSolverParams (quda_multigrid_params.h:12-113)."""
+    _fields_ = [("blocking", ctypes.c_int * 4), ("nvec", ctypes.c_int), ("nu_pre", ctypes.c_int),
+                ("nu_post", ctypes.c_int), ("omega", ctypes.c_double), ("coarse_tol", ctypes.c_double),
+                ("coarse_maxit", ctypes.c_int), ("coarse_nkrylov", ctypes.c_int), ("setup_maxit", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong), ("verbose", ctypes.c_int)]
+
+
+class MGResult(ctypes.Structure):
+    _fields_ = [("n_blocks", ctypes.c_int), ("coarse_dof", ctypes.c_int), ("n_probes", ctypes.c_int),
+                ("setup_seconds", ctypes.c_double), ("nullspace_seconds", ctypes.c_double),
+                ("coarse_op_seconds", ctypes.c_double), ("nullvec_rel_residual_max", ctypes.c_double)]
 
 
 class TrainParams(ctypes.Structure):
@@ -130,6 +144,13 @@ def lib():
         "cmx_op_set_net_variant": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "cmx_net_grad": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
         "cmx_net_train": (ctypes.c_int, [_vp, ctypes.POINTER(TrainParams), ctypes.POINTER(TrainResult)]),
+        "cmx_mg_params_default": (ctypes.c_int, [ctypes.POINTER(MGParams)]),
+        "cmx_mg_setup": (ctypes.c_int, [_vp, ctypes.POINTER(MGParams), ctypes.POINTER(MGResult)]),
+        "cmx_mg_get_nullvecs": (ctypes.c_int, [_vp, _dp]),
+        "cmx_mg_get_coarse_op": (ctypes.c_int, [_vp, _dp]),
+        "cmx_mg_restrict": (ctypes.c_int, [_vp, _dp, _dp]),
+        "cmx_mg_prolong": (ctypes.c_int, [_vp, _dp, _dp]),
+        "cmx_mg_coarse_apply": (ctypes.c_int, [_vp, _dp, _dp]),
         "cmx_op_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
         "cmx_layout_roundtrip": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "cmx_op_get_hop_config": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
@@ -146,6 +167,10 @@ def lib():
         "cmx_meson_2pt": (ctypes.c_int, [_ip, _dp, _dp, _ip, ctypes.c_int, _ip, _dp]),
         "cmx_meson_2pt_noalloc": (ctypes.c_int, [_ip, _vp, _vp, _ip, ctypes.c_int, _vp, _vp, _vp]),
         "cmx_fill_z2": (ctypes.c_int, [ctypes.c_long, ctypes.c_ulonglong, _dp]),
+        "cmx_field_write_scidac": (ctypes.c_int, [ctypes.c_char_p, _ip, ctypes.c_int, ctypes.c_int, _dp,
+                                                  ctypes.c_char_p, ctypes.c_char_p]),
+        "cmx_field_read_scidac": (ctypes.c_int, [ctypes.c_char_p, _ip, ctypes.c_int, _dp, ctypes.c_char_p,
+                                                 ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -228,6 +253,31 @@ def meson_2pt(L, prop1, prop2=None, origin=(0, 0, 0, 0), moms=((0, 0, 0),)):
     check(lib().cmx_meson_2pt(_dims(L), _d(p1), _d(p2), org, int(mom.shape[0]),
                               mom.ctypes.data_as(_ip), _d(out)))
     return out
+
+
+SCIDAC_PROPAGATOR, SCIDAC_FERMION = 0, 1
+
+
+def field_write_scidac(path, L, field, kind=SCIDAC_PROPAGATOR, prec=DOUBLE, file_xml="", record_xml=""):
+    """SciDAC single-file output of a propagator (complex[V, 12, 12]) or fermion (complex[V, 4, 3]) in the
+    host site order (cmx_field_write_scidac; QIO_WRITE_NAMED_OBJECT's file).  Host only."""
+    V = int(np.prod(L))
+    f = np.ascontiguousarray(field, np.complex128)
+    if f.size != V * (144 if kind == SCIDAC_PROPAGATOR else 12):
+        raise ValueError("field_write_scidac: field size does not match the lattice and kind")
+    check(lib().cmx_field_write_scidac(os.fsencode(path), _dims(L), int(kind), int(prec), _d(f),
+                                       file_xml.encode(), record_xml.encode()))
+
+
+def field_read_scidac(path, L, kind=SCIDAC_PROPAGATOR, xml_cap=1 << 20):
+    """Reads a SciDAC single-file propagator or fermion written in double or single precision; checks
+    dims, datatype and checksum.  Returns (field, file_xml, record_xml)."""
+    V = int(np.prod(L))
+    out = np.zeros((V, 12, 12) if kind == SCIDAC_PROPAGATOR else (V, 4, 3), np.complex128)
+    fx = ctypes.create_string_buffer(xml_cap)
+    rx = ctypes.create_string_buffer(xml_cap)
+    check(lib().cmx_field_read_scidac(os.fsencode(path), _dims(L), int(kind), _d(out), fx, xml_cap, rx, xml_cap))
+    return out, fx.value.decode(), rx.value.decode()
 
 
 def plaquette(L, U):
@@ -368,6 +418,50 @@ class Operator:
         check(lib().cmx_clover_pack_quda(self.h, int(which), int(cb), _d(out)))
         return out
 
+    # ---- multigrid (cmx_mg_*) ----
+    def mg_setup(self, params=None, **kw):
+        """Builds the two-level multigrid on this operator (cmx_mg_setup); params: MGParams (defaults of
+        mg_params()), keyword overrides applied on top.  Returns the set-up summary."""
+        p = mg_params(**kw) if params is None else params
+        for k, v in kw.items():
+            if params is not None:
+                _mg_set(p, k, v)
+        r = MGResult()
+        check(lib().cmx_mg_setup(self.h, ctypes.byref(p), ctypes.byref(r)))
+        self._mg = (r.n_blocks, r.coarse_dof, p.nvec)
+        return {f: getattr(r, f) for f, _ in MGResult._fields_}
+
+    def mg_nullvecs(self):
+        nb, d, nvec = self._mg
+        out = np.zeros((nvec, self.V, 4, 3), np.complex128)
+        check(lib().cmx_mg_get_nullvecs(self.h, _d(out)))
+        return out
+
+    def mg_coarse_op(self):
+        nb, d, _ = self._mg
+        out = np.zeros((nb, 9, d, d), np.complex128)
+        check(lib().cmx_mg_get_coarse_op(self.h, _d(out)))
+        return out
+
+    def mg_restrict(self, full):
+        nb, d, _ = self._mg
+        f = np.ascontiguousarray(full, np.complex128)
+        out = np.zeros((nb, d), np.complex128)
+        check(lib().cmx_mg_restrict(self.h, _d(f), _d(out)))
+        return out
+
+    def mg_prolong(self, coarse):
+        c = np.ascontiguousarray(coarse, np.complex128)
+        out = np.zeros((self.V, 4, 3), np.complex128)
+        check(lib().cmx_mg_prolong(self.h, _d(c), _d(out)))
+        return out
+
+    def mg_coarse_apply(self, coarse):
+        c = np.ascontiguousarray(coarse, np.complex128)
+        out = np.zeros_like(c)
+        check(lib().cmx_mg_coarse_apply(self.h, _d(c), _d(out)))
+        return out
+
     def layout_roundtrip(self, psi_cb, prec):
         psi_cb = np.ascontiguousarray(psi_cb, np.complex128)
         out = np.zeros_like(psi_cb)
@@ -446,17 +540,22 @@ class Operator:
 
 
 def solve_batch(ops, chis, psis=None, solver=RELIABLE_CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=HALF,
-                check_every=0, mdagm=False, true_residual=False, tune=True, graph=False):
-    """n independent solves advanced together, every kernel launched once for all of them
-    (cmx_solve_batch).  ops: Operators (an Operator may repeat: the sources of one propagator);
-    chis: subset arrays, one per problem.  Returns (psis, per-problem results, batch summary)."""
+                check_every=0, mdagm=False, true_residual=False, tune=True, graph=False, n_krylov=16, n_defl=0,
+                precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1,
+                basis_prec=-1, cycle_reduction=0.0):
+    """n independent solves advanced together (cmx_solve_batch): CG and reliable CG launch every kernel
+    once for all problems; FGMRES-DR advances all solves one Arnoldi step per host read-back.
+    ops: Operators (an Operator may repeat: the sources of one propagator); chis: subset arrays, one per
+    problem.  Returns (psis, per-problem results, batch summary)."""
     n = len(ops)
     assert len(chis) == n
     chis = [np.ascontiguousarray(c, np.complex128) for c in chis]
     psis = [np.zeros_like(c) for c in chis] if psis is None else \
         [np.ascontiguousarray(x, np.complex128).copy() for x in psis]
     p = SolverParams(int(solver), int(maxit), float(rsd), float(delta), int(sloppy_prec), int(check_every),
-                     0, int(bool(true_residual)), 16, 0, PRECOND_NONE, 4, 1e-12, 1.0, 1, -1, 0.0, 0)
+                     0, int(bool(true_residual)), int(n_krylov), int(n_defl), int(precond), int(precond_iters),
+                     float(precond_rsd), float(precond_omega), int(reorth), int(basis_prec), float(cycle_reduction),
+                     0)
     bp = BatchParams(int(bool(tune)), int(bool(graph)))
     hs = (_vp * n)(*[o.h for o in ops])
     pp = (_dp * n)(*[_d(x) for x in psis])
@@ -552,6 +651,84 @@ def net_grad(op, v, e, prec=SINGLE):
     if nb == 1:
         return loss.value, gwc[0], gw0[0]
     return loss.value, gwc, gw0
+
+
+def _mg_set(p, k, v):
+    if k == "blocking":
+        for i in range(4):
+            p.blocking[i] = int(v[i])
+    else:
+        setattr(p, k, type(getattr(p, k))(v))
+
+
+def mg_params(**kw):
+    """cmx_mg_params with the defaults of cmx_mg_params_default (the reader's), keyword overrides applied."""
+    p = MGParams()
+    check(lib().cmx_mg_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        _mg_set(p, k, v)
+    return p
+
+
+def mg_params_from_xml(xml):
+    """MULTIGRIDSolverParams read from a <MULTIGRIDParams> element, with the element names and defaults of
+    quda_multigrid_params.cc:44-235, restricted to two levels (one Blocking entry; per-level arrays give
+    their first-level value, or their coarse-level value for the coarse solver's CoarseResidual and
+    MaxCoarseIterations; a single value is broadcast as the reader does).  Elements without a counterpart
+    here (Precision, Reconstruct, SchwarzType, SetupOnGPU, ...) are accepted and ignored."""
+    import xml.etree.ElementTree as ET
+    root = ET.fromstring(xml) if isinstance(xml, str) else xml
+    if root.tag != "MULTIGRIDParams":
+        found = root.find(".//MULTIGRIDParams")
+        if found is None:
+            raise ValueError("mg_params_from_xml: no MULTIGRIDParams element")
+        root = found
+
+    def arr(name, conv):
+        e = root.find(name)
+        return None if e is None or e.text is None else [conv(t) for t in e.text.split()]
+
+    p = mg_params()
+    blk = root.find("Blocking")
+    if blk is None:
+        raise ValueError("mg_params_from_xml: Blocking is required")
+    elems = [[int(t) for t in e.text.split()] for e in blk.findall("elem")]
+    if len(elems) != 1:
+        raise ValueError(f"mg_params_from_xml: {len(elems)} blockings; this multigrid has two levels (one blocking)")
+    _mg_set(p, "blocking", elems[0])
+    levels = 2
+
+    def level_value(name, conv, level):
+        v = arr(name, conv)
+        if v is None:
+            return None
+        if len(v) == 1:
+            return v[0]
+        if name in ("NullVectors", "Pre-SmootherApplications", "Post-SmootherApplications",
+                    "MaxIterSubspaceCreate") and len(v) == levels - 1:
+            return v[level]
+        if len(v) != levels:
+            raise ValueError(f"mg_params_from_xml: {name} has {len(v)} entries for {levels} levels")
+        return v[level]
+
+    for name, field, conv, level in (("NullVectors", "nvec", int, 0),
+                                     ("Pre-SmootherApplications", "nu_pre", int, 0),
+                                     ("Post-SmootherApplications", "nu_post", int, 0),
+                                     ("RelaxationOmegaMG", "omega", float, 0),
+                                     ("CoarseResidual", "coarse_tol", float, 1),
+                                     ("MaxCoarseIterations", "coarse_maxit", int, 1),
+                                     ("MaxIterSubspaceCreate", "setup_maxit", int, 0)):
+        v = level_value(name, conv, level)
+        if v is not None:
+            _mg_set(p, field, v)
+    for name, field in (("PrecondGCRNKrylov", "coarse_nkrylov"),):
+        e = root.find(name)
+        if e is not None:
+            _mg_set(p, field, int(e.text))
+    e = root.find("Verbosity")
+    if e is not None:
+        p.verbose = 1 if e.text.strip().lower() == "true" else 0
+    return p
 
 
 def net_train(op, n_sources=4, snapshot_every=4, solve_rsd=1e-9, epochs=4, lr=1e-3, seed=1, max_pairs=64,

@@ -59,6 +59,7 @@ extern "C" {
 #define CMX_PRECOND_NONE 0        /* identity                                                       */
 #define CMX_PRECOND_MR 1          /* MR_INVERTER: InvMR (invmr.cc:55-160) in the sloppy precision     */
 #define CMX_PRECOND_LEARNED 2     /* gauge-equivariant network loaded with cmx_op_set_learned_precond */
+#define CMX_PRECOND_MG 3          /* two-level multigrid V-cycle set up with cmx_mg_setup              */
 
 #define CMX_PLUS 1
 #define CMX_MINUS (-1)
@@ -234,8 +235,11 @@ int cmx_solve_multishift(cmx_op op, const cmx_solver_params* p, int nshift, cons
  * chi_i) on ops[i]; operators may differ in lattice, gauge field and mass, and may repeat (the 12
  * sources of quarkProp4_a, quarkprop4_w.cc:70-117, on one operator).  Solver CMX_SOLVER_CG or
  * CMX_SOLVER_RELIABLE_CG; each problem's iterations, stopping rule and result are those of cmx_solve
- * with the same launch shapes (true_residual restarts are not done).  psi[i] is the initial guess on
- * entry; res[i].solve_seconds is the device time of the whole batch. */
+ * with the same launch shapes.  Solver CMX_SOLVER_FGMRESDR (any preconditioner; a repeated operator's
+ * solves share its network): the solves advance one Arnoldi step per tick with one host read-back for
+ * all of them, each with the kernels and host arithmetic of cmx_solve, so each result equals cmx_solve's
+ * bit for bit; as many run at once as their workspaces fit in free device memory.  psi[i] is the initial
+ * guess on entry; res[i].solve_seconds is the device time of the whole batch. */
 typedef struct {
   int tune;   /* 1: time candidate launch shapes for the batch (plane-grid or site-linear hopping
                  kernel, 64/128/256 threads) and keep the fastest; 0: the operators' own shapes, which
@@ -312,6 +316,47 @@ typedef struct {
 } cmx_train_result;
 int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
 
+/* ---- multigrid (CMX_PRECOND_MG) ----------------------------------------------------------------------
+ * The two-level form of MULTIGRIDSolverParams (quda_multigrid_params.h:12-113, read by
+ * quda_multigrid_params.cc:44-235), behind the same role as QUDA_MULTIGRID_CLOVER_INVERTER's solver: the
+ * right preconditioner of CMX_SOLVER_FGMRESDR (the outer flexible solver, OuterGCRNKrylov ~ NKrylov).
+ * Null vectors: nvec random starts relaxed by setup_maxit MR steps on A x = 0 (A the even-odd operator,
+ * FP32), extended to the full lattice by the even-odd reconstruction, block-orthonormalised per block and
+ * chirality.  Coarse operator: the Galerkin product P^dag M P of the unpreconditioned operator M (FP64
+ * probes, FP32 storage), a 9-point block stencil, 2 nvec degrees of freedom per block.  V-cycle on the
+ * odd checkerboard, A^-1 = (M^-1)_oo: nu_pre MR steps, restriction of the odd residual, GCR(coarse_nkrylov)
+ * on the coarse system to coarse_tol or coarse_maxit steps, prolongation, nu_post MR steps; smoother and
+ * V-cycle at the solver's sloppy precision.  Defaults (cmx_mg_params_default) follow the reader's. */
+typedef struct {
+  int blocking[4];       /* Blocking/elem (first level): even extents dividing the lattice (4 4 4 4)    */
+  int nvec;              /* NullVectors (first level), 1..16 (16)                                       */
+  int nu_pre, nu_post;   /* Pre-SmootherApplications, Post-SmootherApplications (2, 2)                  */
+  double omega;          /* RelaxationOmegaMG: the MR smoother's over-relaxation (1.0)                 */
+  double coarse_tol;     /* CoarseResidual (coarse level): relative residual of the coarse GCR (1e-4)   */
+  int coarse_maxit;      /* MaxCoarseIterations (coarse level) (12)                                     */
+  int coarse_nkrylov;    /* PrecondGCRNKrylov: GCR directions before a restart, <= 32 (12)             */
+  int setup_maxit;       /* MaxIterSubspaceCreate: MR relaxation steps per null vector (500)           */
+  unsigned long long seed;
+  int verbose;           /* Verbosity                                                                    */
+} cmx_mg_params;
+typedef struct {
+  int n_blocks, coarse_dof;        /* coarse sites, degrees of freedom per coarse site (2 nvec)       */
+  int n_probes;                    /* operator applications that built the coarse operator            */
+  double setup_seconds, nullspace_seconds, coarse_op_seconds;
+  double nullvec_rel_residual_max; /* max over the null vectors of |A x_o| / |x_o| after relaxation    */
+} cmx_mg_result;
+int cmx_mg_params_default(cmx_mg_params* p);
+int cmx_mg_setup(cmx_op op, const cmx_mg_params* p, cmx_mg_result* res);
+/* Inspection and test access: the null vectors after block orthonormalisation (nvec * V * 24 doubles,
+ * host order), the coarse operator (n_blocks * 9 * D * D complex: block, stencil entry -- the block,
+ * then +x, -x, +y, -y, +z, -z, +t, -t --, row, column), restriction R = P^dag of a full-lattice field
+ * (V * 24 doubles -> n_blocks * D complex), prolongation (the reverse) and the coarse operator. */
+int cmx_mg_get_nullvecs(cmx_op op, double* out);
+int cmx_mg_get_coarse_op(cmx_op op, double* out);
+int cmx_mg_restrict(cmx_op op, const double* full_in, double* coarse_out);
+int cmx_mg_prolong(cmx_op op, const double* coarse_in, double* full_out);
+int cmx_mg_coarse_apply(cmx_op op, const double* in, double* out);
+
 /* ---- launch shape of the hopping kernels ---------------------------------------------------------
  * cfg = {threads per work-group (64, 128, 256), tiled work-group order (0/1), z planes per tile,
  * double-buffered direction loads (0/1), kernel (0 site-linear, 1 plane-grid -- 12-real links, real
@@ -385,6 +430,21 @@ int cmx_meson_2pt(const int dims[4], const double* prop1, const double* prop2, c
  * that array and race; serialise them.  The results equal cmx_meson_2pt's bit for bit. */
 int cmx_meson_2pt_noalloc(const int dims[4], const double* prop1, const double* prop2, const int origin[4],
                           int nmom, const int* moms, double* corr, void* stream);
+/* SciDAC single-file output (the file QIO_WRITE_NAMED_OBJECT writes through QDPFileWriter with
+ * QDPIO_SINGLEFILE, inline_qio_write_obj.cc:150-180, qio_write_obj_funcmap.cc:30-100) and input (the
+ * QIO_READ_NAMED_OBJECT path).  Host-only: no device work, no cmx_init needed.  kind 0: a propagator
+ * (field: V*144 complex, sites in the host order, then sink spin*3+colour, then source spin*3+colour),
+ * datatype QDP_D3_DiracPropagator / QDP_F3_DiracPropagator; kind 1: a fermion (V*12 complex,
+ * QDP_D3_DiracFermion / QDP_F3_DiracFermion).  prec CMX_DOUBLE or CMX_SINGLE (the file's
+ * precision; the host array is always double).  The LIME records: scidac-private-file-xml,
+ * scidac-file-xml (file_xml), scidac-private-record-xml, scidac-record-xml (record_xml),
+ * scidac-binary-data (lexicographic sites, big-endian), scidac-checksum (the SciDAC CRC-32 sums).
+ * The reader checks dims, datatype and checksum, and returns the user XML records truncated to the
+ * capacities given (NULL: not returned). */
+int cmx_field_write_scidac(const char* path, const int dims[4], int kind, int prec, const double* field,
+                           const char* file_xml, const char* record_xml);
+int cmx_field_read_scidac(const char* path, const int dims[4], int kind, double* field, char* file_xml,
+                          int file_xml_cap, char* record_xml, int record_xml_cap);
 /* Average plaquette Re tr(U_p)/3 over all sites and planes (MesPlq normalisation). */
 double cmx_plaquette(const int dims[4], const double* links);
 /* Z2 noise source: each real and imaginary part +-1/sqrt(2) from (seed, site, component). */

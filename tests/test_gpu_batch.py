@@ -1,5 +1,6 @@
 """GPU: batched solves (cmx_solve_batch) -- several problems advancing through one sequence of batched
-kernels -- against the same problems solved one at a time.
+kernels (CG, reliable CG), or one Arnoldi step per host read-back (FGMRES-DR) -- against the same
+problems solved one at a time.
 
 With the operators' own launch shapes (tune=0) a batched problem runs the kernels of its single solve
 with the same work-group decomposition and reduction order, so its solution must equal the single
@@ -108,6 +109,61 @@ def test_batch_sources_on_one_operator():
         assert res[k]["rel_resid"] <= 1e-10
         assert res[k]["n_count"] == rr["n_count"]
         assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
+
+
+FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0), (cmx.PRECOND_LEARNED, cmx.SINGLE, 4),
+                (cmx.PRECOND_MR, cmx.SINGLE, 4), (cmx.PRECOND_NONE, cmx.DOUBLE, 4)]
+
+
+@pytest.mark.parametrize("precond,sloppy,ndefl", FGMRES_CASES)
+def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl):
+    """The 12 point sources on one operator through the batched FGMRES-DR (a repeated operator's solves
+    share its network): each solution and iteration count equals the single solve's bit for bit, and
+    reaches the FP64 true residual 1e-10."""
+    L = (8, 8, 8, 16)
+    say("batch fgmres", precond, sloppy, ndefl)
+    kappa = 0.13
+    op, _ = make_problem(L, 1030, kappa, cmx.HALF if sloppy == cmx.HALF else cmx.SINGLE)
+    if precond == cmx.PRECOND_LEARNED:
+        net = cmx.Net()
+        net.init_neumann(4.0 + cmx.kappa_to_mass(kappa))
+        net.set_depth(2)
+        cmx.net_attach(op, net)
+    Vh = int(np.prod(L)) // 2
+    chis = []
+    for s in range(4):
+        for c in range(3):
+            chi = np.zeros((Vh, 4, 3), complex)
+            chi[0, s, c] = 1.0
+            chis.append(chi)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=sloppy, basis_prec=sloppy, n_krylov=16,
+              n_defl=ndefl, precond=precond, precond_iters=4, true_residual=True,
+              cycle_reduction=3e-3 if sloppy == cmx.HALF else 0.0)
+    psis, res, summ = cmx.solve_batch([op] * 12, chis, **kw)
+    say("  summary", summ)
+    for k in range(12):
+        ref, rr = op.solve(chis[k], **kw)
+        say("  source", k, "batch", res[k]["n_count"], res[k]["rel_resid"], "single", rr["n_count"])
+        assert res[k]["rel_resid"] <= 1e-10
+        assert res[k]["n_count"] == rr["n_count"]
+        assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
+
+
+def test_batch_fgmres_uneven_matches_single():
+    """FGMRES-DR (MR preconditioner, deflation) over problems on different lattices, gauge fields and
+    masses: each equals its single solve bit for bit."""
+    say("batch fgmres uneven")
+    probs = [make_problem(L, s, k, cmx.SINGLE) for L, s, k in UNEVEN]
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=cmx.SINGLE, n_krylov=12, n_defl=3,
+              precond=cmx.PRECOND_MR, precond_iters=3)
+    single = [op.solve(chi, **kw) for op, chi in probs]
+    psis, res, summ = cmx.solve_batch([p[0] for p in probs], [p[1] for p in probs], **kw)
+    say("  summary", summ)
+    for k in range(len(probs)):
+        say("  problem", k, "single", single[k][1]["n_count"], "batch", res[k]["n_count"], res[k]["rel_resid"])
+        assert res[k]["n_count"] == single[k][1]["n_count"]
+        assert np.array_equal(psis[k].view(np.uint64), single[k][0].view(np.uint64))
+        assert res[k]["rel_resid"] <= 1e-10
 
 
 def test_batch_rejects_bad_input():

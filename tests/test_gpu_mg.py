@@ -1,0 +1,137 @@
+"""GPU: the two-level multigrid preconditioner (cmx_mg_*).
+
+* The null vectors are orthonormal on every (block, chirality) aggregate.
+* Restriction and prolongation equal R = P^dag and P computed in numpy from the downloaded null vectors.
+* The coarse operator built by probing equals the Galerkin product: A_c x = R M P x for random coarse x,
+  with M the oracle's FP64 unpreconditioned clover operator (unprecLinOp, eoprec_linop.h:147-165) and
+  R, P from numpy -- an independent composition that a wrong colouring, stencil entry or scatter fails.
+  Blockings with coarse extents 2 (stencil entries that coincide) and 4 are both covered.
+* FGMRES-DR with the V-cycle reaches the FP64 true residual 1e-10 in fewer iterations than without it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from chroma_amd import lib as cmx
+from tests.util import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def host_blocks(L, B):
+    """block index (x fastest over the coarse lattice) of every site in the host order"""
+    V = int(np.prod(L))
+    lex = np.arange(V)
+    x = lex % L[0]
+    y = (lex // L[0]) % L[1]
+    z = (lex // (L[0] * L[1])) % L[2]
+    t = lex // (L[0] * L[1] * L[2])
+    par = (x + y + z + t) % 2
+    host = par * (V // 2) + lex // 2
+    Lc = [L[m] // B[m] for m in range(4)]
+    blk = x // B[0] + Lc[0] * (y // B[1] + Lc[1] * (z // B[2] + Lc[2] * (t // B[3])))
+    out = np.empty(V, np.int64)
+    out[host] = blk
+    return out, int(np.prod(Lc))
+
+
+def make(L, seed, kappa, stout=False):
+    if stout:
+        U = cmx.gauge_generate(L, kind=2, seed=seed, nstout=6, rho=0.1)
+    else:
+        U = cmx.gauge_generate(L, kind=1, seed=seed, eps=0.3)
+    U = cmx.gauge_apply_bc(L, U, (1, 1, 1, -1))
+    mass = cmx.kappa_to_mass(kappa)
+    op = cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.SINGLE)
+    return U, mass, op
+
+
+def numpy_restrict(P, blk, nb, v):
+    nvec = P.shape[0]
+    out = np.zeros((nb, 2 * nvec), complex)
+    for c in range(2):
+        sl = slice(2 * c, 2 * c + 2)
+        for k in range(nvec):
+            prod = np.sum(np.conj(P[k][:, sl, :]) * v[:, sl, :], axis=(1, 2))
+            out[:, c * nvec + k] = np.bincount(blk, weights=prod.real, minlength=nb) + \
+                1j * np.bincount(blk, weights=prod.imag, minlength=nb)
+    return out
+
+
+def numpy_prolong(P, blk, e):
+    nvec = P.shape[0]
+    out = np.zeros(P.shape[1:], complex)
+    for c in range(2):
+        sl = slice(2 * c, 2 * c + 2)
+        for k in range(nvec):
+            out[:, sl, :] += P[k][:, sl, :] * e[blk, c * nvec + k][:, None, None]
+    return out
+
+
+@pytest.mark.parametrize("B", [(2, 2, 2, 2), (4, 4, 4, 4)])
+def test_mg_prolongator_and_coarse_operator(B):
+    L = (8, 8, 8, 16)
+    say("mg setup", L, B)
+    U, mass, op = make(L, 3001, 0.128)
+    nvec = 4
+    info = op.mg_setup(blocking=B, nvec=nvec, setup_maxit=60, seed=5)
+    say("  setup", info)
+    blk, nb = host_blocks(L, B)
+    assert info["n_blocks"] == nb and info["coarse_dof"] == 2 * nvec
+    P = op.mg_nullvecs()
+    # orthonormal per (block, chirality)
+    worst = 0.0
+    for c in range(2):
+        sl = slice(2 * c, 2 * c + 2)
+        for b in range(nb):
+            sel = blk == b
+            M = P[:, sel][:, :, sl, :].reshape(nvec, -1)
+            G = np.conj(M) @ M.T
+            worst = max(worst, float(np.max(np.abs(G - np.eye(nvec)))))
+    say("  orthonormality error", worst)
+    assert worst < 1e-5
+    rng = np.random.default_rng(7)
+    V = int(np.prod(L))
+    v = rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))
+    r_dev = op.mg_restrict(v)
+    r_np = numpy_restrict(P, blk, nb, v)
+    say("  restrict", rel(r_dev, r_np))
+    assert rel(r_dev, r_np) < 1e-5
+    e = rng.standard_normal((nb, 2 * nvec)) + 1j * rng.standard_normal((nb, 2 * nvec))
+    p_dev = op.mg_prolong(e)
+    p_np = numpy_prolong(P, blk, e)
+    say("  prolong", rel(p_dev, p_np))
+    assert rel(p_dev, p_np) < 1e-5
+    # Galerkin: A_c e = R M P e with the oracle's FP64 operator
+    ref = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    Me = ref.piece("unprec", p_np)
+    want = numpy_restrict(P, blk, nb, Me)
+    got = op.mg_coarse_apply(e)
+    say("  coarse operator vs R M P", rel(got, want))
+    assert rel(got, want) < 1e-4
+
+
+def test_fgmres_mg_preconditioner():
+    L = (16, 16, 16, 32)
+    say("fgmres + mg", L)
+    U, mass, op = make(L, 3002, 0.1575, stout=True)
+    Vh = int(np.prod(L)) // 2
+    chi = cmx.z2_source((Vh, 4, 3), 9)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=4000, sloppy_prec=cmx.SINGLE, n_krylov=16)
+    psi0, r_none = op.solve(chi, precond=cmx.PRECOND_NONE, **kw)
+    info = op.mg_setup(blocking=(4, 4, 4, 4), nvec=8, nu_pre=0, nu_post=4, setup_maxit=300, coarse_tol=0.1,
+                       coarse_maxit=12, seed=3)
+    say("  setup", info)
+    psi, r_mg = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
+    say("  iterations: none", r_none["n_count"], r_none["solve_seconds"], "mg", r_mg["n_count"],
+        r_mg["solve_seconds"], "resid", r_mg["rel_resid"])
+    assert r_mg["rel_resid"] <= 1e-10
+    assert rel(psi, psi0) < 1e-8
+    assert r_mg["n_count"] < r_none["n_count"] / 2

@@ -10,7 +10,6 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
-#include <random>
 
 #include "fgmres.h"
 #include "mg.h"
@@ -27,12 +26,8 @@ struct MGState {
   MGVecs pv{};
   DevBuf Ac, nbr, colour;
   // coarse GCR work space
-  DevBuf xc, rc, bc, part;
+  DevBuf xc, rc, bc, part, S, F;
   std::vector<DevBuf> pk, qk;
-  double2* hpart = nullptr;  // pinned
-  ~MGState() {
-    if (hpart) (void)hipHostFree(hpart);
-  }
 };
 
 namespace {
@@ -109,74 +104,53 @@ double norm2_host(Op& op, const Vec& v) {
   return op.hS[S_TR];
 }
 
-// sums the cdot partials of m vectors on the host
-void cdot_host(MGState& st, Op& op, const float2* const* v, int m, const float2* w, int n, std::vector<std::complex<double>>& out) {
-  const int nb = mg_cdot_blocks(n);
-  launch_mg_cdot(v, m, w, n, st.part.as<double2>(), op.stream);
-  CMX_HIP(hipMemcpyAsync(st.hpart, st.part.p, sizeof(double2) * nb * m, hipMemcpyDeviceToHost, op.stream));
-  op.sync();
-  out.assign(m, 0.0);
-  for (int j = 0; j < m; ++j) {
-    double re = 0, im = 0;
-    for (int b = 0; b < nb; ++b) re += st.hpart[j * nb + b].x, im += st.hpart[j * nb + b].y;
-    out[j] = std::complex<double>(re, im);
-  }
-}
-
-// x = A_c^-1 b by GCR(nk) with restarts, to |r| <= tol |b| or maxit steps (x, b: coarse vectors)
+// x = A_c^-1 b by GCR(nk) with restarts, to |r| <= tol |b| or maxit steps (x, b: coarse vectors).  Every
+// scalar stays on the device: the kernels after convergence return at once, and the host never waits.
 void coarse_gcr(MGState& st, Op& op, const float2* b, float2* x) {
   const MGGeom& m = st.m;
   const int n = m.Nb * m.D;
   const int nk = st.par.coarse_nkrylov;
+  const int nblk = mg_cdot_blocks(n);
+  const double tol2 = st.par.coarse_tol * st.par.coarse_tol;
   float2* r = st.rc.as<float2>();
+  double* S = st.S.as<double>();
+  int* F = st.F.as<int>();
+  const int* skip = F + MGF_SKIP;
+  double2* part = st.part.as<double2>();
   CMX_HIP(hipMemsetAsync(x, 0, sizeof(float2) * n, op.stream));
   CMX_HIP(hipMemcpyAsync(r, b, sizeof(float2) * n, hipMemcpyDeviceToDevice, op.stream));
-  std::vector<std::complex<double>> d;
   const float2* rv[1] = {r};
-  cdot_host(st, op, rv, 1, r, n, d);
-  const double b2 = d[0].real();
-  if (!(b2 > 0)) return;
-  double r2 = b2;
-  const double stop = st.par.coarse_tol * st.par.coarse_tol * b2;
+  launch_mg_cdot(rv, 1, r, n, part, nullptr, op.stream);
+  launch_mg_gcr_fin(part, nblk, 1, 0, MG_FIN_INIT, tol2, S, F, op.stream);
   int j = 0;
-  for (int it = 0; it < st.par.coarse_maxit && r2 > stop; ++it) {
+  for (int it = 0; it < st.par.coarse_maxit; ++it) {
     float2* pj = st.pk[j].as<float2>();
     float2* qj = st.qk[j].as<float2>();
     CMX_HIP(hipMemcpyAsync(pj, r, sizeof(float2) * n, hipMemcpyDeviceToDevice, op.stream));
-    launch_mg_coarse_apply(m, st.nbr.as<int>(), st.Ac.as<float2>(), pj, qj, op.stream);
+    launch_mg_coarse_apply(m, st.nbr.as<int>(), st.Ac.as<float2>(), pj, qj, skip, op.stream);
     // <q_i, q_j> (i < j), <q_j, q_j>, conj(<q_j, r>) in one pass
     const float2* vs[MG_MAXKRY + 2];
     for (int i = 0; i <= j; ++i) vs[i] = st.qk[i].as<float2>();
     vs[j + 1] = r;
-    cdot_host(st, op, vs, j + 2, qj, n, d);
+    launch_mg_cdot(vs, j + 2, qj, n, part, skip, op.stream);
+    launch_mg_gcr_fin(part, nblk, j + 2, j, MG_FIN_STEP, tol2, S, F, op.stream);
     MGUpdate u;
     std::memset(&u, 0, sizeof(u));
-    double qq = d[j].real();
-    for (int i = 0; i < j; ++i) {
-      qq -= std::norm(d[i]);
-      u.c[i] = make_float2((float)d[i].real(), (float)d[i].imag());
-    }
-    if (!(qq > 1e-30 * b2)) break;  // no new direction
-    const double nq = std::sqrt(qq);
-    const std::complex<double> alpha = std::conj(d[j + 1]) / nq;  // <q_j, r> / |q_j'|
-    for (int i = 0; i < MG_MAXKRY; ++i) {
-      u.q[i] = i <= j ? st.qk[i].as<float2>() : nullptr;
-      u.p[i] = i <= j ? st.pk[i].as<float2>() : nullptr;
-    }
+    for (int i = 0; i <= j; ++i) u.q[i] = st.qk[i].as<float2>(), u.p[i] = st.pk[i].as<float2>();
     u.j = j;
     u.n = n;
-    u.inv = (float)(1.0 / nq);
-    u.alpha = make_float2((float)alpha.real(), (float)alpha.imag());
+    u.S = S;
+    u.F = F;
     u.x = x;
     u.r = r;
     launch_mg_update(u, op.stream);
-    r2 -= std::norm(alpha);
-    if (++j == nk) {  // restart: the residual recomputed from its recursion's vector
+    if (++j == nk && it + 1 < st.par.coarse_maxit) {  // restart: |r|^2 recomputed from its vector
       j = 0;
-      cdot_host(st, op, rv, 1, r, n, d);
-      r2 = d[0].real();
+      launch_mg_cdot(rv, 1, r, n, part, skip, op.stream);
+      launch_mg_gcr_fin(part, nblk, 1, 0, MG_FIN_RESTART, tol2, S, F, op.stream);
     }
   }
+  CMX_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -213,12 +187,8 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
   // null space: relaxed random vectors on the odd checkerboard, extended to the full lattice
   Vec x(P32, Vh), zero(P32, Vh), r(P32, Vh), mr(P32, Vh), F(P64, 2 * Vh);
   op.blas(B_ZERO, {&zero});
-  std::vector<double> host((size_t)Vh * 24);
   for (int k = 0; k < m.nvec; ++k) {
-    std::mt19937_64 rng(p.seed + 7919ull * (unsigned long long)k);
-    std::normal_distribution<double> nd(0.0, 1.0);
-    for (double& h : host) h = nd(rng);
-    op.pack(host.data(), Vh, x);
+    launch_mg_noise(x.view(), Vh, p.seed * 0x100000001b3ull + 7919ull * (unsigned long long)(k + 1), op.stream);
     MRPrecond relax(op, P32, Vh, p.setup_maxit, 0.0, 1.0, r, mr);
     relax.apply(x, zero);
     op.apply(P32, mr.view(), x.view(), +1, nullptr);
@@ -240,18 +210,25 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
   op.sync();
   const auto t1 = std::chrono::steady_clock::now();
 
-  // block stencil and colouring: period p_mu divides the coarse extent and is >= 3 where the extent is,
-  // so the 9 stencil entries of a block fall on distinct colours (extents 1, 2: the entries coincide)
+  // block stencil and colouring.  Even (or unit) coarse extents: period 2 (1) per direction, and each
+  // probe is restricted per block and face (launch_mg_face_restrict): 2^4 colours.  Otherwise: period
+  // p_mu >= 3 dividing the extent, so the 9 stencil entries of a block fall on distinct colours, and the
+  // whole-block restriction is scattered to the entry of that colour.
+  bool face = true;
+  for (int mu = 0; mu < 4; ++mu) face = face && (m.Lc[mu] == 1 || m.Lc[mu] % 2 == 0);
   std::vector<int> nbr((size_t)m.Nb * MG_NST), col(m.Nb);
   st->ncol = 1;
   for (int mu = 0; mu < 4; ++mu) {
     int per = m.Lc[mu];
-    if (per > 2)
+    if (face) {
+      per = m.Lc[mu] == 1 ? 1 : 2;
+    } else if (per > 2) {
       for (int q = 3; q <= m.Lc[mu]; ++q)
         if (m.Lc[mu] % q == 0) {
           per = q;
           break;
         }
+    }
     st->period[mu] = per;
     st->ncol *= per;
   }
@@ -288,9 +265,14 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
       for (int k = 0; k < m.nvec; ++k) {
         launch_mg_probe(m, st->pv, st->colour.as<int>(), cl, k, c, F.view(0), op.stream);
         apply_full64(op, F, G);
-        launch_mg_restrict(m, st->pv, P64, G.view(0), true, y.as<float2>(), op.stream);
-        launch_mg_scatter(m, st->nbr.as<int>(), st->colour.as<int>(), cl, c * m.nvec + k, y.as<float2>(),
-                          st->Ac.as<float2>(), op.stream);
+        if (face) {
+          launch_mg_face_restrict(m, st->pv, st->colour.as<int>(), st->nbr.as<int>(), cl, c * m.nvec + k,
+                                  G.view(0), st->Ac.as<float2>(), op.stream);
+        } else {
+          launch_mg_restrict(m, st->pv, P64, G.view(0), true, y.as<float2>(), op.stream);
+          launch_mg_scatter(m, st->nbr.as<int>(), st->colour.as<int>(), cl, c * m.nvec + k, y.as<float2>(),
+                            st->Ac.as<float2>(), op.stream);
+        }
         ++res.n_probes;
       }
   CMX_HIP(hipGetLastError());
@@ -307,7 +289,8 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
   for (int j = 0; j < p.coarse_nkrylov; ++j) st->pk[j].alloc(cv), st->qk[j].alloc(cv);
   const int nbk = mg_cdot_blocks(m.Nb * m.D);
   st->part.alloc(sizeof(double2) * nbk * (MG_MAXKRY + 2));
-  CMX_HIP(hipHostMalloc((void**)&st->hpart, sizeof(double2) * nbk * (MG_MAXKRY + 2), 0));
+  st->S.alloc(sizeof(double) * MGS_N);
+  st->F.alloc(sizeof(int) * MGF_N);
 
   op.mg = std::shared_ptr<void>(st.release(), [](void* q) { delete static_cast<MGState*>(q); });
   res.n_blocks = m.Nb;
@@ -394,7 +377,7 @@ void mg_coarse_apply_host(Op& op, const double* in, double* out) {
   for (size_t i = 0; i < n; ++i) h[i] = make_float2((float)in[2 * i], (float)in[2 * i + 1]);
   CMX_HIP(hipMemcpy(st.xc.p, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
   launch_mg_coarse_apply(st.m, st.nbr.as<int>(), st.Ac.as<float2>(), st.xc.as<float2>(), st.rc.as<float2>(),
-                         op.stream);
+                         nullptr, op.stream);
   CMX_HIP(hipMemcpyAsync(h.data(), st.rc.p, sizeof(float2) * n, hipMemcpyDeviceToHost, op.stream));
   op.sync();
   for (size_t i = 0; i < n; ++i) out[2 * i] = h[i].x, out[2 * i + 1] = h[i].y;

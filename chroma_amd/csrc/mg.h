@@ -39,23 +39,40 @@ void launch_mg_prolong(const MGGeom& m, const MGVecs& P, int prec, const float2*
 // probe vector: full-lattice FP64, P_k^c on the blocks of colour col, zero elsewhere
 void launch_mg_probe(const MGGeom& m, const MGVecs& P, const int* colour, int col, int k, int c, VRef out,
                      hipStream_t s);
+// FP32 odd-checkerboard Z2 noise (+-1/sqrt 2 per real) from (seed, site)
+void launch_mg_noise(VRef out, int n, unsigned long long seed, hipStream_t s);
+// the coarse operator's column dof from v = M probe under a colouring of period 2 (1) per direction:
+// each block's entries whose neighbour has colour col, from the block or its two faces in that direction
+void launch_mg_face_restrict(const MGGeom& m, const MGVecs& P, const int* colour, const int* nbr, int col, int dof,
+                             VRef v, float2* Ac, hipStream_t s);
 // Ac[b][d][r][dof] = y[b][r] for the first stencil entry d whose block has colour col
 void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int col, int dof, const float2* y,
                        float2* Ac, hipStream_t s);
-// y = A_c x
+// y = A_c x; skip (nullable): a device flag, the launch does nothing while it is set
 void launch_mg_coarse_apply(const MGGeom& m, const int* nbr, const float2* Ac, const float2* x, float2* y,
-                            hipStream_t s);
+                            const int* skip, hipStream_t s);
 // coarse vectors (n complex): partials[j][block] = sum over the block's elements of conj(v_j) w, j < m
 int mg_cdot_blocks(int n);
-void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, hipStream_t s);
-// GCR update: q_j = (q_j - sum_i c_i q_i) inv; p_j likewise; x += alpha p_j; r -= alpha q_j
+void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, const int* skip,
+                    hipStream_t s);
+// The coarse GCR's scalars live on the device (S: doubles, F: flags), so a V-cycle needs no host
+// synchronisation.  mg_gcr_fin sums the partials of m dot products and, by stage: INIT (|r|^2 of the
+// right-hand side: target and flags), STEP (the Gram-Schmidt coefficients of direction j, its
+// normalisation, alpha, the residual norm recursion, the stopping flag), RESTART (|r|^2 recomputed).
+enum { MGS_B2 = 0, MGS_R2 = 1, MGS_STOP = 2, MGS_INV = 3, MGS_AR = 4, MGS_AI = 5, MGS_C = 8,
+       MGS_N = MGS_C + 2 * MG_MAXKRY };
+enum { MGF_SKIP = 0, MGF_UPD = 1, MGF_N = 4 };
+enum { MG_FIN_INIT = 0, MG_FIN_STEP = 1, MG_FIN_RESTART = 2 };
+void launch_mg_gcr_fin(const double2* partials, int nblk, int m, int j, int stage, double tol2, double* S, int* F,
+                       hipStream_t s);
+// GCR update with the device scalars: when F[UPD]: q_j = (q_j - sum_i c_i q_i) inv; p_j likewise;
+// x += alpha p_j; r -= alpha q_j
 struct MGUpdate {
   float2* q[MG_MAXKRY];
   float2* p[MG_MAXKRY];
-  float2 c[MG_MAXKRY];
   int j, n;
-  float inv;
-  float2 alpha;
+  const double* S;
+  const int* F;
   float2 *x, *r;
 };
 void launch_mg_update(const MGUpdate& u, hipStream_t s);

@@ -289,6 +289,117 @@ template <int PR, bool FULL> __global__ void __launch_bounds__(256) k_mg_prolong
   SpinorIO<PR>::store(A.out.v, A.out.nrm, A.out.stride, i, s);
 }
 
+// ---- random starts of the null vectors: Z2 noise from a counter-based hash (splitmix64) --------------
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_mg_noise(VRef out, int n, unsigned long long seed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long h = mix64(seed ^ mix64((unsigned long long)i));
+  float s[24];
+#pragma unroll
+  for (int q = 0; q < 24; ++q) s[q] = ((h >> q) & 1ull) ? 0.70710678f : -0.70710678f;
+  SpinorIO<P32>::store(out.v, nullptr, out.stride, i, s);
+}
+
+// ---- the coarse operator from face-restricted probes ---------------------------------------------
+// With a colouring of period 2 in every coarse direction (period 1 where the extent is 1), a probe on
+// the blocks of colour cl reaches a block b of another colour only through one direction mu, from
+// b + mu across b's +mu face and from b - mu across its -mu face: restricting M probe over the whole
+// block (colour cl: the block's own entry) or over each of the two faces separates the stencil entries.
+// One work-group per block; the entries go straight into Ac[b][d][.][dof].
+struct FaceArgs {
+  MGGeom m;
+  MGVecs P;
+  const int* colour;
+  const int* nbr;
+  int col, dof;
+  VRef v;  // M probe, full lattice FP64
+  float2* Ac;
+};
+
+__device__ __forceinline__ int face_site(const MGGeom& m, int b, int mu, int side, int f) {
+  int bc[4], x[4];
+  block_coords(m, b, bc);
+  int r = f;
+#pragma unroll
+  for (int nu = 0; nu < 4; ++nu) {
+    int l;
+    if (nu == mu) {
+      l = side ? m.B[nu] - 1 : 0;
+    } else {
+      l = r % m.B[nu];
+      r /= m.B[nu];
+    }
+    x[nu] = bc[nu] * m.B[nu] + l;
+  }
+  const int par = (x[0] + x[1] + x[2] + x[3]) & 1;
+  return par * m.g.Vh + lex_of(m.g, x) / 2;
+}
+
+__global__ void __launch_bounds__(256) k_mg_face_restrict(FaceArgs A) {
+  __shared__ float2 sh[MG_MAXD * 4], tot[MG_MAXD];
+  const MGGeom& m = A.m;
+  const int b = blockIdx.x;
+  const int stride = 2 * m.g.Vh;
+  int mu = -1;
+  if (A.colour[b] != A.col) {
+    for (int nu = 0; nu < 4; ++nu)
+      if (A.colour[A.nbr[b * MG_NST + 1 + 2 * nu]] == A.col) {
+        mu = nu;
+        break;
+      }
+    if (mu < 0) return;  // no stencil entry of this block has the probed colour
+  }
+  const int nregion = mu < 0 ? 1 : 2;
+  for (int reg = 0; reg < nregion; ++reg) {
+    const int nsite = mu < 0 ? m.bvol : m.bvol / m.B[mu];
+    float2 acc[MG_MAXD];
+#pragma unroll
+    for (int j = 0; j < MG_MAXD; ++j) acc[j] = make_float2(0.f, 0.f);
+    for (int l = threadIdx.x; l < nsite; l += blockDim.x) {
+      const int hs = mu < 0 ? full_site(m, b, l) : face_site(m, b, mu, reg == 0 ? 1 : 0, l);
+      double sd[24];
+      SpinorIO<P64>::load(A.v.v, nullptr, A.v.stride, hs, sd);
+      float v[24];
+#pragma unroll
+      for (int q = 0; q < 24; ++q) v[q] = (float)sd[q];
+#pragma unroll
+      for (int k = 0; k < MG_MAXVEC; ++k) {
+        if (k < m.nvec) {
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            float pk[12];
+            load_half(A.P.P[k], stride, hs, c, pk);
+            float re = 0.f, im = 0.f;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+              const float ar = pk[2 * q], ai = pk[2 * q + 1], br = v[12 * c + 2 * q], bi = v[12 * c + 2 * q + 1];
+              re += ar * br + ai * bi;
+              im += ar * bi - ai * br;
+            }
+            acc[c * MG_MAXVEC + k].x += re;
+            acc[c * MG_MAXVEC + k].y += im;
+          }
+        }
+      }
+    }
+    group_reduce<MG_MAXD>(acc, sh, tot);
+    // stencil entry: the block itself, or +mu (region 0, the +mu face) / -mu (region 1)
+    const int d = mu < 0 ? 0 : 1 + 2 * mu + reg;
+    if ((int)threadIdx.x < m.D) {
+      const int c = threadIdx.x / m.nvec, k = threadIdx.x % m.nvec;
+      A.Ac[(((size_t)b * MG_NST + d) * m.D + threadIdx.x) * m.D + A.dof] = tot[c * MG_MAXVEC + k];
+    }
+    __syncthreads();
+  }
+}
+
 // ---- probes and the coarse operator ---------------------------------------------------------------
 struct ProbeArgs {
   MGGeom m;
@@ -328,7 +439,8 @@ __global__ void __launch_bounds__(256) k_mg_scatter(MGGeom m, const int* nbr, co
 }
 
 __global__ void __launch_bounds__(256) k_mg_coarse_apply(MGGeom m, const int* nbr, const float2* Ac, const float2* x,
-                                                         float2* y) {
+                                                         float2* y, const int* skip) {
+  if (skip && *skip) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m.Nb * m.D) return;
   const int b = t / m.D, r = t % m.D;
@@ -351,10 +463,12 @@ struct CdotArgs {
   const float2* w;
   int n;
   double2* partials;
+  const int* skip;
 };
 
 __global__ void __launch_bounds__(256) k_mg_cdot(CdotArgs A) {
   __shared__ double2 sh[4], tot[1];
+  if (A.skip && *A.skip) return;
   const float2* v = A.v[blockIdx.y];
   double re = 0, im = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
@@ -367,25 +481,80 @@ __global__ void __launch_bounds__(256) k_mg_cdot(CdotArgs A) {
   if (threadIdx.x == 0) A.partials[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot[0];
 }
 
+// one work-group: the m dot products from their partials, then the stage's scalar work in thread 0
+__global__ void __launch_bounds__(256) k_mg_gcr_fin(const double2* part, int nblk, int m, int j, int stage,
+                                                    double tol2, double* S, int* F) {
+  __shared__ double2 sh[4], tot[1], d[MG_MAXKRY + 2];
+  if (stage != MG_FIN_INIT && F[MGF_SKIP]) {
+    if (threadIdx.x == 0) F[MGF_UPD] = 0;
+    return;
+  }
+  for (int q = 0; q < m; ++q) {
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) re += part[q * nblk + b].x, im += part[q * nblk + b].y;
+    const double2 acc[1] = {make_double2(re, im)};
+    group_reduce<1>(acc, sh, tot);
+    if (threadIdx.x == 0) d[q] = tot[0];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (stage == MG_FIN_INIT) {
+    const double b2 = d[0].x;
+    S[MGS_B2] = b2;
+    S[MGS_R2] = b2;
+    S[MGS_STOP] = tol2 * b2;
+    F[MGF_SKIP] = b2 > 0 ? 0 : 1;
+    F[MGF_UPD] = 0;
+  } else if (stage == MG_FIN_RESTART) {
+    S[MGS_R2] = d[0].x;
+    if (d[0].x <= S[MGS_STOP]) F[MGF_SKIP] = 1;
+    F[MGF_UPD] = 0;
+  } else {
+    double qq = d[j].x;
+    for (int i = 0; i < j; ++i) {
+      qq -= d[i].x * d[i].x + d[i].y * d[i].y;
+      S[MGS_C + 2 * i] = d[i].x;
+      S[MGS_C + 2 * i + 1] = d[i].y;
+    }
+    if (!(qq > 1e-30 * S[MGS_B2])) {  // no new direction
+      F[MGF_SKIP] = 1;
+      F[MGF_UPD] = 0;
+      return;
+    }
+    const double nq = sqrt(qq);
+    // alpha = <q_j', r> / |q_j'| = conj(d[j + 1]) / nq   (r is orthogonal to the earlier q_i)
+    const double ar = d[j + 1].x / nq, ai = -d[j + 1].y / nq;
+    S[MGS_INV] = 1.0 / nq;
+    S[MGS_AR] = ar;
+    S[MGS_AI] = ai;
+    S[MGS_R2] -= ar * ar + ai * ai;
+    F[MGF_UPD] = 1;
+    if (S[MGS_R2] <= S[MGS_STOP]) F[MGF_SKIP] = 1;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_mg_update(MGUpdate u) {
+  if (!u.F[MGF_UPD]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= u.n) return;
   float2 q = u.q[u.j][i], p = u.p[u.j][i];
   for (int k = 0; k < u.j; ++k) {
-    const float2 c = u.c[k], qk = u.q[k][i], pk = u.p[k][i];
-    q.x -= c.x * qk.x - c.y * qk.y;
-    q.y -= c.x * qk.y + c.y * qk.x;
-    p.x -= c.x * pk.x - c.y * pk.y;
-    p.y -= c.x * pk.y + c.y * pk.x;
+    const float cr = (float)u.S[MGS_C + 2 * k], ci = (float)u.S[MGS_C + 2 * k + 1];
+    const float2 qk = u.q[k][i], pk = u.p[k][i];
+    q.x -= cr * qk.x - ci * qk.y;
+    q.y -= cr * qk.y + ci * qk.x;
+    p.x -= cr * pk.x - ci * pk.y;
+    p.y -= cr * pk.y + ci * pk.x;
   }
-  q.x *= u.inv, q.y *= u.inv, p.x *= u.inv, p.y *= u.inv;
+  const float inv = (float)u.S[MGS_INV], ar = (float)u.S[MGS_AR], ai = (float)u.S[MGS_AI];
+  q.x *= inv, q.y *= inv, p.x *= inv, p.y *= inv;
   u.q[u.j][i] = q;
   u.p[u.j][i] = p;
   float2 x = u.x[i], r = u.r[i];
-  x.x += u.alpha.x * p.x - u.alpha.y * p.y;
-  x.y += u.alpha.x * p.y + u.alpha.y * p.x;
-  r.x -= u.alpha.x * q.x - u.alpha.y * q.y;
-  r.y -= u.alpha.x * q.y + u.alpha.y * q.x;
+  x.x += ar * p.x - ai * p.y;
+  x.y += ar * p.y + ai * p.x;
+  r.x -= ar * q.x - ai * q.y;
+  r.y -= ar * q.y + ai * q.x;
   u.x[i] = x;
   u.r[i] = r;
 }
@@ -437,25 +606,42 @@ void launch_mg_probe(const MGGeom& m, const MGVecs& P, const int* colour, int co
   hipLaunchKernelGGL(k_mg_probe, dim3(grid_of(2L * m.g.Vh)), dim3(256), 0, s, A);
 }
 
+void launch_mg_noise(VRef out, int n, unsigned long long seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_noise, dim3(grid_of(n)), dim3(256), 0, s, out, n, seed);
+}
+
+void launch_mg_face_restrict(const MGGeom& m, const MGVecs& P, const int* colour, const int* nbr, int col, int dof,
+                             VRef v, float2* Ac, hipStream_t s) {
+  FaceArgs A{m, P, colour, nbr, col, dof, v, Ac};
+  hipLaunchKernelGGL(k_mg_face_restrict, dim3(m.Nb), dim3(256), 0, s, A);
+}
+
 void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int col, int dof, const float2* y,
                        float2* Ac, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_scatter, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr, colour, col, dof, y, Ac);
 }
 
 void launch_mg_coarse_apply(const MGGeom& m, const int* nbr, const float2* Ac, const float2* x, float2* y,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_coarse_apply, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr, Ac, x, y);
+                            const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_coarse_apply, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr, Ac, x, y, skip);
 }
 
 int mg_cdot_blocks(int n) { return std::max(1, std::min(256, grid_of(n))); }
 
-void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, hipStream_t s) {
+void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, const int* skip,
+                    hipStream_t s) {
   CdotArgs A;
   for (int j = 0; j < MG_MAXKRY + 2; ++j) A.v[j] = j < m ? v[j] : nullptr;
   A.w = w;
   A.n = n;
   A.partials = partials;
+  A.skip = skip;
   hipLaunchKernelGGL(k_mg_cdot, dim3(mg_cdot_blocks(n), m), dim3(256), 0, s, A);
+}
+
+void launch_mg_gcr_fin(const double2* partials, int nblk, int m, int j, int stage, double tol2, double* S, int* F,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_gcr_fin, dim3(1), dim3(256), 0, s, partials, nblk, m, j, stage, tol2, S, F);
 }
 
 void launch_mg_update(const MGUpdate& u, hipStream_t s) {

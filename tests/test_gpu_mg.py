@@ -5,7 +5,8 @@
 * The coarse operator built by probing equals the Galerkin product: A_c x = R M P x for random coarse x,
   with M the oracle's FP64 unpreconditioned clover operator (unprecLinOp, eoprec_linop.h:147-165) and
   R, P from numpy -- an independent composition that a wrong colouring, stencil entry or scatter fails.
-  Blockings with coarse extents 2 (stencil entries that coincide) and 4 are both covered.
+  Covered: coarse extents 2 (the +mu and -mu entries on one block) and 4 (face-restricted probes, 2^4
+  colours), and odd extents 3 (whole-block probes with 3^3 x 4 colours).
 * FGMRES-DR with the V-cycle reaches the FP64 true residual 1e-10 in fewer iterations than without it.
 """
 import sys
@@ -75,9 +76,9 @@ def numpy_prolong(P, blk, e):
     return out
 
 
-@pytest.mark.parametrize("B", [(2, 2, 2, 2), (4, 4, 4, 4)])
-def test_mg_prolongator_and_coarse_operator(B):
-    L = (8, 8, 8, 16)
+@pytest.mark.parametrize("L,B", [((8, 8, 8, 16), (2, 2, 2, 2)), ((8, 8, 8, 16), (4, 4, 4, 4)),
+                                 ((6, 6, 6, 8), (2, 2, 2, 2))])
+def test_mg_prolongator_and_coarse_operator(L, B):
     say("mg setup", L, B)
     U, mass, op = make(L, 3001, 0.128)
     nvec = 4

@@ -239,6 +239,10 @@ def train_net(cmx, mass, csw, a, depth=1, pairs_per_source=4, lr=1e-3):
 
 
 FGMRES_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, basis_prec=2, n_krylov=16, cycle_reduction=3e-3)
+# two-level multigrid: 4^4 blocks, 8 null vectors (16 coarse degrees of freedom per block), MR post-smoothing
+# at FP16 storage, coarse GCR to 0.1 or 16 steps
+MG_KW = dict(nvec=8, nu_pre=0, nu_post=6, setup_maxit=300, coarse_tol=0.1, coarse_maxit=16, seed=3)
+FGMRES_MG_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, n_krylov=16)
 
 
 def sigma_min(cmx, op, Vh, iters=6):
@@ -298,7 +302,7 @@ def critical_scan(cmx, gauge, L, kappas=(0.1355, 0.150, 0.155, 0.158, 0.160)):
 def near_critical_solve(cmx, gauge, L, kappa, net):
     """One Z2 source at a near-critical kappa, all to the FP64 true residual 1e-10: plain mixed CG (FP16
     inner, FP64 reliable updates), FGMRES-DR(16) without a preconditioner, with the untrained network
-    (its Neumann start) and with the trained network."""
+    (its Neumann start), with the trained network, and with the two-level multigrid V-cycle."""
     import numpy as np
     mass = cmx.kappa_to_mass(kappa)
     op = cmx.Operator(gauge, cmx.EO_CLOVER, mass, 1.0, sloppy_prec=cmx.HALF, autotune=True)
@@ -324,6 +328,16 @@ def near_critical_solve(cmx, gauge, L, kappa, net):
         _, r = op.solve(chi, solver=cmx.FGMRESDR, precond=pc, **FGMRES_KW)
         out[name] = {"seconds": round(r["solve_seconds"], 4), "iterations": r["n_count"],
                      "true_rel_resid": r["rel_resid"]}
+    # two-level multigrid as the right preconditioner (set-up reported apart: it serves every later solve)
+    try:
+        info = op.mg_setup(blocking=(4, 4, 4, 4), **MG_KW)
+        op.solve(chi, solver=cmx.FGMRESDR, precond=cmx.PRECOND_MG, **FGMRES_MG_KW)
+        _, r = op.solve(chi, solver=cmx.FGMRESDR, precond=cmx.PRECOND_MG, **FGMRES_MG_KW)
+        out["fgmres_multigrid"] = {"seconds": round(r["solve_seconds"], 4), "iterations": r["n_count"],
+                                   "true_rel_resid": r["rel_resid"], "setup_seconds": round(info["setup_seconds"], 3),
+                                   "params": dict(MG_KW, blocking=[4, 4, 4, 4])}
+    except cmx.CmxError as e:
+        print(f"# multigrid: {e}", file=sys.stderr, flush=True)
     p = out["plain_mixed_cg"]["seconds"]
     out["speedup_trained_vs_plain"] = round(p / out["fgmres_trained"]["seconds"], 3)
     out["speedup_trained_vs_fgmres_none"] = round(out["fgmres_none"]["seconds"] / out["fgmres_trained"]["seconds"], 3)

@@ -378,8 +378,15 @@ struct FgmresDR::Impl {
     for (int i = 0; i <= nd; ++i) g[i] = gq[i];
   }
 
-  void issue_step() {
-    precondition(*Z[j], *V[j]);  // z_j = M v_j
+  bool learned_step_io(const Vec*& v, Vec*& z) const {
+    if (p.precond != CMX_PRECOND_LEARNED || pb != ps) return false;
+    v = V[j];
+    z = Z[j];
+    return true;
+  }
+
+  void issue_step(bool preconditioned) {
+    if (!preconditioned) precondition(*Z[j], *V[j]);  // z_j = M v_j
     apply_a(*V[j + 1], *Z[j]);   // w = A z_j
     for (int i = 0; i <= j; ++i) vs[i] = V[i];
     kd.mdot(vs, j + 1, *V[j + 1], K_H);  // h_i = <v_i, w>
@@ -486,7 +493,8 @@ void FgmresDR::reserve(Op& op, const cmx_solver_params& p) {
 FgmresDR::~FgmresDR() = default;
 bool FgmresDR::finished() const { return m_->done; }
 void FgmresDR::begin_cycle() { m_->begin_cycle(); }
-void FgmresDR::issue_step() { m_->issue_step(); }
+void FgmresDR::issue_step(bool preconditioned) { m_->issue_step(preconditioned); }
+bool FgmresDR::learned_step_io(const Vec*& v, Vec*& z) const { return m_->learned_step_io(v, z); }
 bool FgmresDR::finish_step() { return m_->finish_step(); }
 void FgmresDR::end_cycle() { m_->end_cycle(); }
 void FgmresDR::result(cmx_solver_result& res) { m_->result(res); }

@@ -52,8 +52,13 @@ class FgmresDR {
   bool finished() const;
   // the restart: plain (V_0 = r / |r|) or deflated (harmonic Ritz vectors of the previous cycle)
   void begin_cycle();
-  // issues one Arnoldi step on op's stream, ending with an asynchronous read-back of the new column
-  void issue_step();
+  // the next step's preconditioner input and output when that step applies the learned network directly
+  // to basis vectors (precond LEARNED, basis precision = operator precision): a batched caller applies
+  // it for several solves at once and then calls issue_step(true)
+  bool learned_step_io(const Vec*& v, Vec*& z) const;
+  // issues one Arnoldi step on op's stream, ending with an asynchronous read-back of the new column;
+  // preconditioned: z_j = M v_j has been formed already
+  void issue_step(bool preconditioned = false);
   // after op's stream has reached the read-back: the Givens update; false when the cycle has ended
   bool finish_step();
   // psi += Z eta, the FP64 residual, the stopping decision

@@ -31,11 +31,26 @@ struct PtcLaunch {
   double* w0part;        // W0 gradient partials (blocks x 32 doubles)
 };
 
+// Several sources through one block in one launch per kernel (inference only): per-source input,
+// output and mixed field G; strides and everything else from the PtcLaunch
+constexpr int PTC_MAXSRC = 16;
+struct PtcBatch {
+  int ns;
+  const void* in[PTC_MAXSRC];
+  const float* in_nrm[PTC_MAXSRC];
+  void* out[PTC_MAXSRC];
+  float* out_nrm[PTC_MAXSRC];
+  unsigned short* G[PTC_MAXSRC];
+};
+
 int ptc_waves();
 void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
 // inference: variant 0 operands in registers, 1 transports staged in LDS
 void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s);
+// variant 0 or 1 of launch_ptc_gather_mix, and the scatter, on B.ns sources
+void launch_ptc_gather_mix_batch(const PtcLaunch& L, const PtcBatch& B, int variant, hipStream_t s);
+void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, hipStream_t s);
 // adj: the block's adjoint W0^dag delta + G^dag(L.G) (L.W0 holding W0^dag, L.G the mixed adjoint)
 void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_grad(const PtcLaunch& L, double* out1024, hipStream_t s);

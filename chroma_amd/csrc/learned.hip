@@ -159,7 +159,8 @@ __device__ __forceinline__ void transport2(const R (&u)[18], const float (&h)[12
 // (coordinates x): th[d][s'][c].  ADJ: the adjoint gather of the loss gradient, dgh_{q}(y) for q = p^1,
 // i.e. the projection of the opposite path with a factor 2 (R^dag = 2 Phat), stored under slot p^1.
 template <int P, int RECON, int MU, bool ADJ>
-__device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4], int y, float (&th)[2][12]) {
+__device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4], int y, float (&th)[2][12],
+                                          const void* in, const float* in_nrm) {
   using R = typename SpinorIO<P>::R;
   constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   const Geom& g = L.g;
@@ -169,7 +170,7 @@ __device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4],
     bool last;
     const int yn = ptc_neighbour(g, x, MU, fwd, last);
     R v[24], u[18];
-    SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, yn, v);
+    SpinorIO<P>::load(in, in_nrm, L.in_stride, yn, v);
     const float sigma_p = fwd ? -1.f : 1.f;
     float h[12];
     half_project<MU>(v, ADJ ? -sigma_p : sigma_p, ADJ ? 1.f : 0.5f, h);
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(256) k_ptc_gather(PtcLaunch L) {
   auto one = [&](auto MUc) {
     constexpr int MU = decltype(MUc)::value;
     float th[2][12];
-    gather_mu<P, RECON, MU, ADJ>(L, x, yi, th);
+    gather_mu<P, RECON, MU, ADJ>(L, x, yi, th, L.in, L.in_nrm);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       bf4 re, im;
@@ -280,11 +281,12 @@ __global__ void __launch_bounds__(256) k_ptc_mix(PtcLaunch L) {
 // the directions mu = h, 2 + h (h = l / 32), both paths of each: in the B-operand layout these are
 // exactly the rows lane l holds, so each lane packs its four transports straight into its fragments.
 template <int P, int RECON>
-__global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
+__device__ __forceinline__ void gather_mix_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
+                                                unsigned short* G) {
   const Geom& g = L.g;
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
-  const int y = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 32 + (lane & 31);
+  const int y = (int)((wg * blockDim.x + threadIdx.x) >> 6) * 32 + (lane & 31);
   const bool live = y < g.Vh;
   bf4 b[3][4];
   if (live) {
@@ -296,14 +298,14 @@ __global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
       for (int c = 0; c < 3; ++c) pack_mu(th, c, b[c][ksr], b[c][2 + ksr]);
     };
     if (h == 0) {
-      gather_mu<P, RECON, 0, false>(L, x, y, th);
+      gather_mu<P, RECON, 0, false>(L, x, y, th, in, in_nrm);
       put(0);
-      gather_mu<P, RECON, 2, false>(L, x, y, th);
+      gather_mu<P, RECON, 2, false>(L, x, y, th, in, in_nrm);
       put(1);
     } else {
-      gather_mu<P, RECON, 1, false>(L, x, y, th);
+      gather_mu<P, RECON, 1, false>(L, x, y, th, in, in_nrm);
       put(0);
-      gather_mu<P, RECON, 3, false>(L, x, y, th);
+      gather_mu<P, RECON, 3, false>(L, x, y, th, in, in_nrm);
       put(1);
     }
   } else {
@@ -321,18 +323,23 @@ __global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
     f16v acc = {0};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], b[c][ks], acc, 0, 0, 0);
-    if (live) store_gh(L.G, g.Vh, y, c, h, acc);
+    if (live) store_gh(G, g.Vh, y, c, h, acc);
   }
+}
+
+template <int P, int RECON>
+__global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
+  gather_mix_body<P, RECON>(L, blockIdx.x, L.in, L.in_nrm, L.G);
 }
 
 // Fused gather + mix for inference: a work-group of 128 threads transports its 128 even sites into an
 // LDS copy of their four T tiles (24 KiB), then its two waves run the MFMA mix of those 12 (block,
 // colour) tiles and write G.  T never goes to HBM.
 template <int P, int RECON>
-__global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
-  __shared__ bf4 lds[4 * 3 * 4 * 64];
+__device__ __forceinline__ void gather_mix_lds_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
+                                                    unsigned short* G, bf4* lds) {
   const Geom& g = L.g;
-  const int yi = blockIdx.x * 128 + threadIdx.x;
+  const int yi = wg * 128 + threadIdx.x;
   const int lb = threadIdx.x >> 5, n = threadIdx.x & 31;  // local block, site within the block
   auto lds_at = [&](int c, int ks, int h) -> bf4& { return lds[((lb * 3 + c) * 4 + ks) * 64 + h * 32 + n]; };
   if (yi < g.Vh) {
@@ -341,7 +348,7 @@ __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
     auto one = [&](auto MUc) {
       constexpr int MU = decltype(MUc)::value;
       float th[2][12];
-      gather_mu<P, RECON, MU, false>(L, x, yi, th);
+      gather_mu<P, RECON, MU, false>(L, x, yi, th, in, in_nrm);
 #pragma unroll
       for (int c = 0; c < 3; ++c) pack_mu(th, c, lds_at(c, MU >> 1, MU & 1), lds_at(c, 2 + (MU >> 1), MU & 1));
     };
@@ -370,14 +377,21 @@ __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], lds[((tb * 3 + c) * 4 + ks) * 64 + lane], acc, 0, 0, 0);
-    const int y = blockIdx.x * 128 + tb * 32 + (lane & 31);
-    if (y < g.Vh) store_gh(L.G, g.Vh, y, c, lane >> 5, acc);
+    const int y = wg * 128 + tb * 32 + (lane & 31);
+    if (y < g.Vh) store_gh(G, g.Vh, y, c, lane >> 5, acc);
   }
+}
+
+template <int P, int RECON>
+__global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
+  __shared__ bf4 lds[4 * 3 * 4 * 64];
+  gather_mix_lds_body<P, RECON>(L, blockIdx.x, L.in, L.in_nrm, L.G, lds);
 }
 
 // P(v)(x) = W0 v(x) + sum_q U_q(x) R_q gh_q(x+q) at odd sites x
 template <int P, int RECON, int MU, bool ADJ>
-__device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4], int xi, float (&acc)[24]) {
+__device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4], int xi, float (&acc)[24],
+                                           const unsigned short* G) {
   using R = typename SpinorIO<P>::R;
   constexpr int LCH = link_chunks(P, RECON), LCB = link_chunk_bytes(P, RECON);
   const Geom& g = L.g;
@@ -393,7 +407,7 @@ __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4]
     float gh[12];  // gh_q(y) as [s'][colour][re,im]
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const ushort4 w = *reinterpret_cast<const ushort4*>(L.G + ((size_t)(q * 3 + c) * g.Vh + yn) * 4);
+      const ushort4 w = *reinterpret_cast<const ushort4*>(G + ((size_t)(q * 3 + c) * g.Vh + yn) * 4);
       gh[2 * c] = bf2f(w.x);
       gh[2 * c + 1] = bf2f(w.y);
       gh[2 * (3 + c)] = bf2f(w.z);
@@ -415,8 +429,9 @@ __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4]
 }
 
 template <int P, int RECON, bool ADJ>
-__global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
-  const int xi = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scatter_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
+                                             void* out, float* out_nrm, const unsigned short* G) {
+  const int xi = wg * blockDim.x + threadIdx.x;
   const Geom& g = L.g;
   if (xi >= g.Vh) return;
   using R = typename SpinorIO<P>::R;
@@ -425,7 +440,7 @@ __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
   float acc[24];
   {  // W0 v(x)
     R v[24];
-    SpinorIO<P>::load(L.in, L.in_nrm, L.in_stride, xi, v);
+    SpinorIO<P>::load(in, in_nrm, L.in_stride, xi, v);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -442,14 +457,52 @@ __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
         acc[2 * (3 * s + c) + 1] = im;
       }
   }
-  scatter_mu<P, RECON, 0, ADJ>(L, x, xi, acc);
-  scatter_mu<P, RECON, 1, ADJ>(L, x, xi, acc);
-  scatter_mu<P, RECON, 2, ADJ>(L, x, xi, acc);
-  scatter_mu<P, RECON, 3, ADJ>(L, x, xi, acc);
-  typename SpinorIO<P>::R out[24];
+  scatter_mu<P, RECON, 0, ADJ>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 1, ADJ>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 2, ADJ>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 3, ADJ>(L, x, xi, acc, G);
+  typename SpinorIO<P>::R o24[24];
 #pragma unroll
-  for (int k = 0; k < 24; ++k) out[k] = (typename SpinorIO<P>::R)acc[k];
-  SpinorIO<P>::store(L.out, L.out_nrm, L.out_stride, xi, out);
+  for (int k = 0; k < 24; ++k) o24[k] = (typename SpinorIO<P>::R)acc[k];
+  SpinorIO<P>::store(out, out_nrm, L.out_stride, xi, o24);
+}
+
+template <int P, int RECON, bool ADJ>
+__global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
+  scatter_body<P, RECON, ADJ>(L, blockIdx.x, L.in, L.in_nrm, L.out, L.out_nrm, L.G);
+}
+
+// ---- several sources at once (the batched flexible solver) -----------------------------------------
+// Work-group g runs on XCD g % 8 (round-robin dispatch); its k-th work-group there, k = g / 8, takes
+// source k % ns of block 8 (k / ns) + g % 8.  A block's work-groups for all sources follow each other on
+// one XCD, so its links (and transported neighbours' lines) are read from that XCD's L2 after the first.
+__device__ __forceinline__ bool batch_map(int g, int nblk, int ns, int& blk, int& src) {
+  const int k = g >> 3;
+  src = k % ns;
+  blk = 8 * (k / ns) + (g & 7);
+  return blk < nblk;
+}
+
+template <int P, int RECON>
+__global__ void __launch_bounds__(256) k_ptc_gather_mix_b(PtcLaunch L, PtcBatch B) {
+  int blk, src;
+  if (!batch_map(blockIdx.x, (L.g.Vh + 127) / 128, B.ns, blk, src)) return;
+  gather_mix_body<P, RECON>(L, blk, B.in[src], B.in_nrm[src], B.G[src]);
+}
+
+template <int P, int RECON>
+__global__ void __launch_bounds__(128) k_ptc_gather_mix_lds_b(PtcLaunch L, PtcBatch B) {
+  __shared__ bf4 lds[4 * 3 * 4 * 64];
+  int blk, src;
+  if (!batch_map(blockIdx.x, (L.g.Vh + 127) / 128, B.ns, blk, src)) return;
+  gather_mix_lds_body<P, RECON>(L, blk, B.in[src], B.in_nrm[src], B.G[src], lds);
+}
+
+template <int P, int RECON>
+__global__ void __launch_bounds__(256) k_ptc_scatter_b(PtcLaunch L, PtcBatch B) {
+  int blk, src;
+  if (!batch_map(blockIdx.x, (L.g.Vh + 255) / 256, B.ns, blk, src)) return;
+  scatter_body<P, RECON, false>(L, blk, B.in[src], B.in_nrm[src], B.out[src], B.out_nrm[src], B.G[src]);
 }
 
 // Sh_real (32x32) = sum over tiles of dG_tile (32 x 32 sites) T_tile^T (32 sites x 32), per wave into
@@ -557,6 +610,28 @@ void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s) {
   const dim3 grid((L.g.Vh + 127) / 128), block(256);  // 4 waves x 32 sites, operands in registers
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
     hipLaunchKernelGGL((k_ptc_gather_mix<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
+  });
+}
+
+static int batch_grid(int nblk, int ns) { return 8 * ns * ((nblk + 7) / 8); }
+
+void launch_ptc_gather_mix_batch(const PtcLaunch& L, const PtcBatch& B, int variant, hipStream_t s) {
+  const int nblk = (L.g.Vh + 127) / 128;
+  dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
+    if (variant == 1)
+      hipLaunchKernelGGL((k_ptc_gather_mix_lds_b<decltype(P)::value, decltype(RC)::value>),
+                         dim3(batch_grid(nblk, B.ns)), dim3(128), 0, s, L, B);
+    else
+      hipLaunchKernelGGL((k_ptc_gather_mix_b<decltype(P)::value, decltype(RC)::value>),
+                         dim3(batch_grid(nblk, B.ns)), dim3(256), 0, s, L, B);
+  });
+}
+
+void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, hipStream_t s) {
+  const int nblk = (L.g.Vh + 255) / 256;
+  dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
+    hipLaunchKernelGGL((k_ptc_scatter_b<decltype(P)::value, decltype(RC)::value>), dim3(batch_grid(nblk, B.ns)),
+                       dim3(256), 0, s, L, B);
   });
 }
 

@@ -663,7 +663,30 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     }
     active = still;
     if (active.empty()) continue;
-    for (int i : active) run[i]->issue_step();
+    // the learned network of all solves on one operator's network and links in one launch per kernel
+    std::vector<char> pre(np, 0);
+    {
+      std::vector<const Vec*> vin;
+      std::vector<Vec*> zout;
+      std::vector<int> who;
+      for (int i : active) {
+        const Vec* v = nullptr;
+        Vec* z = nullptr;
+        if (!run[i]->learned_step_io(v, z)) continue;
+        if (!who.empty()) {
+          const Op &a = *ops[who[0]], &b = *ops[i];
+          if (a.gd != b.gd || a.net != b.net || a.ptc != b.ptc || a.stream != b.stream) continue;
+        }
+        vin.push_back(v);
+        zout.push_back(z);
+        who.push_back(i);
+      }
+      if (who.size() >= 2) {
+        learned_precond_apply_batch(*ops[who[0]], p.sloppy_prec, vin, zout);
+        for (int i : who) pre[i] = 1;
+      }
+    }
+    for (int i : active) run[i]->issue_step(pre[i] != 0);
     sync_all();
     ++ticks;
     for (int i : active)

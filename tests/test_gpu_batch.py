@@ -111,17 +111,19 @@ def test_batch_sources_on_one_operator():
         assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
 
 
-FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0), (cmx.PRECOND_LEARNED, cmx.SINGLE, 4),
-                (cmx.PRECOND_MR, cmx.SINGLE, 4), (cmx.PRECOND_NONE, cmx.DOUBLE, 4)]
+FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0, 0), (cmx.PRECOND_LEARNED, cmx.HALF, 0, 1),
+                (cmx.PRECOND_LEARNED, cmx.SINGLE, 4, 1), (cmx.PRECOND_LEARNED, cmx.SINGLE, 0, 2),
+                (cmx.PRECOND_MR, cmx.SINGLE, 4, -1), (cmx.PRECOND_NONE, cmx.DOUBLE, 4, -1)]
 
 
-@pytest.mark.parametrize("precond,sloppy,ndefl", FGMRES_CASES)
-def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl):
+@pytest.mark.parametrize("precond,sloppy,ndefl,variant", FGMRES_CASES)
+def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl, variant):
     """The 12 point sources on one operator through the batched FGMRES-DR (a repeated operator's solves
-    share its network): each solution and iteration count equals the single solve's bit for bit, and
-    reaches the FP64 true residual 1e-10."""
+    share its network, applied to all sources in one launch per kernel with the inference path `variant`:
+    0 operands in registers, 1 transports in LDS, 2 unfused, one source at a time): each solution and
+    iteration count equals the single solve's bit for bit, and reaches the FP64 true residual 1e-10."""
     L = (8, 8, 8, 16)
-    say("batch fgmres", precond, sloppy, ndefl)
+    say("batch fgmres", precond, sloppy, ndefl, variant)
     kappa = 0.13
     op, _ = make_problem(L, 1030, kappa, cmx.HALF if sloppy == cmx.HALF else cmx.SINGLE)
     if precond == cmx.PRECOND_LEARNED:
@@ -129,6 +131,7 @@ def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl):
         net.init_neumann(4.0 + cmx.kappa_to_mass(kappa))
         net.set_depth(2)
         cmx.net_attach(op, net)
+        cmx.net_set_variant(op, sloppy, variant)
     Vh = int(np.prod(L)) // 2
     chis = []
     for s in range(4):

@@ -53,6 +53,8 @@ void BatchPlan::build(const std::vector<Recorder>& recs) {
     s.kind = r0.kind;
     s.nprob = np;
     s.rec0 = r0.bytes;
+    std::memcpy(s.key, r0.key, sizeof(s.key));
+    for (int p = 0; p < np; ++p) s.nb.push_back(recs[p].recs[j].nblocks);
     if (s.kind == REC_BLAS) {
       s.opc = r0.key[0];
       for (int k = 0; k < 4; ++k) s.prec[k] = r0.key[1 + k];
@@ -97,6 +99,38 @@ void BatchPlan::build(const std::vector<Recorder>& recs) {
       CMX_HIP(hipMemcpy(s.map.p, map.data(), sizeof(int4) * map.size(), hipMemcpyHostToDevice));
     }
     st_.push_back(std::move(s));
+  }
+}
+
+void BatchPlan::refresh(const std::vector<Recorder>& recs, hipStream_t strm) {
+  const int np = (int)recs.size();
+  bool same = np >= 1 && !st_.empty();
+  for (int p = 0; same && p < np; ++p) same = recs[p].recs.size() == st_.size();
+  for (size_t j = 0; same && j < st_.size(); ++j) {
+    const Stage& s = st_[j];
+    same = s.nprob == np;
+    for (int p = 0; same && p < np; ++p) {
+      const LaunchRec& r = recs[p].recs[j];
+      same = r.kind == s.kind && r.nblocks == s.nb[p] && std::memcmp(r.key, s.key, sizeof(s.key)) == 0 &&
+             r.bytes.size() == s.rec0.size();
+    }
+  }
+  if (!same) {
+    build(recs);
+    return;
+  }
+  for (size_t j = 0; j < st_.size(); ++j) {
+    Stage& s = st_[j];
+    const size_t rb = s.rec0.size(), bytes = rb * np;
+    if (!s.htab) {
+      void* h = nullptr;
+      CMX_HIP(hipHostMalloc(&h, bytes, 0));
+      s.htab = std::shared_ptr<void>(h, [](void* q) { (void)hipHostFree(q); });
+    }
+    unsigned char* h = static_cast<unsigned char*>(s.htab.get());
+    for (int p = 0; p < np; ++p) std::memcpy(h + rb * p, recs[p].recs[j].bytes.data(), rb);
+    s.rec0 = recs[0].recs[j].bytes;
+    CMX_HIP(hipMemcpyAsync(s.tab.p, h, bytes, hipMemcpyHostToDevice, strm));
   }
 }
 

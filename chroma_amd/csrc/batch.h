@@ -6,6 +6,7 @@
 // arithmetic of the single-problem iteration on each problem (same kernels, same work-group
 // decomposition, same reduction order).
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "op.h"
@@ -45,6 +46,10 @@ class BatchPlan {
  public:
   // recs[p] is problem p's recording; every recording must hold the same kernels in the same order
   void build(const std::vector<Recorder>& recs);
+  // the same kernel sequence with new launch records (other fields, same kernels and work-group counts):
+  // the records are copied to the device on stream s through pinned staging, the work-group maps kept;
+  // anything else rebuilds the plan.  The previous run of the plan must have completed.
+  void refresh(const std::vector<Recorder>& recs, hipStream_t s);
   void run(hipStream_t s) const;
   size_t stages() const { return st_.size(); }
   long blocks() const;  // work-groups over all stages
@@ -56,6 +61,9 @@ class BatchPlan {
     int nprob = 0, nblocks = 0;
     std::vector<unsigned char> rec0;  // the first problem's record: selects the kernel
     DevBuf tab, map;
+    int key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<int> nb;              // work-groups of each problem's own launch
+    std::shared_ptr<void> htab;       // pinned staging of the records (refresh)
   };
   std::vector<Stage> st_;
 };

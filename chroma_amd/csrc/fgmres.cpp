@@ -385,9 +385,16 @@ struct FgmresDR::Impl {
     return true;
   }
 
-  void issue_step(bool preconditioned) {
+  bool apply_step_io(const Vec*& z, Vec*& w) const {
+    if (pb != ps) return false;
+    z = Z[j];
+    w = V[j + 1];
+    return true;
+  }
+
+  void issue_step(bool preconditioned, bool applied) {
     if (!preconditioned) precondition(*Z[j], *V[j]);  // z_j = M v_j
-    apply_a(*V[j + 1], *Z[j]);   // w = A z_j
+    if (!applied) apply_a(*V[j + 1], *Z[j]);          // w = A z_j
     for (int i = 0; i <= j; ++i) vs[i] = V[i];
     kd.mdot(vs, j + 1, *V[j + 1], K_H);  // h_i = <v_i, w>
     kd.maxpy(*V[j + 1], vs, j + 1, K_H, -1.0, false, reorth == 1 ? K_WN : -1);
@@ -493,7 +500,9 @@ void FgmresDR::reserve(Op& op, const cmx_solver_params& p) {
 FgmresDR::~FgmresDR() = default;
 bool FgmresDR::finished() const { return m_->done; }
 void FgmresDR::begin_cycle() { m_->begin_cycle(); }
-void FgmresDR::issue_step(bool preconditioned) { m_->issue_step(preconditioned); }
+void FgmresDR::issue_step(bool preconditioned, bool applied) { m_->issue_step(preconditioned, applied); }
+bool FgmresDR::apply_step_io(const Vec*& z, Vec*& w) const { return m_->apply_step_io(z, w); }
+void FgmresDR::issue_precondition() { m_->precondition(*m_->Z[m_->j], *m_->V[m_->j]); }
 bool FgmresDR::learned_step_io(const Vec*& v, Vec*& z) const { return m_->learned_step_io(v, z); }
 bool FgmresDR::finish_step() { return m_->finish_step(); }
 void FgmresDR::end_cycle() { m_->end_cycle(); }

@@ -56,9 +56,14 @@ class FgmresDR {
   // to basis vectors (precond LEARNED, basis precision = operator precision): a batched caller applies
   // it for several solves at once and then calls issue_step(true)
   bool learned_step_io(const Vec*& v, Vec*& z) const;
+  // z_j = M v_j of the next step alone (then issue_step(true, ...))
+  void issue_precondition();
+  // the next step's operator input z_j and output w when the operator acts on basis vectors directly
+  // (basis precision = operator precision): a batched caller applies it for several solves at once
+  bool apply_step_io(const Vec*& z, Vec*& w) const;
   // issues one Arnoldi step on op's stream, ending with an asynchronous read-back of the new column;
-  // preconditioned: z_j = M v_j has been formed already
-  void issue_step(bool preconditioned = false);
+  // preconditioned: z_j = M v_j has been formed already; applied: also w = A z_j
+  void issue_step(bool preconditioned = false, bool applied = false);
   // after op's stream has reached the read-back: the Givens update; false when the cycle has ended
   bool finish_step();
   // psi += Z eta, the FP64 residual, the stopping decision

@@ -637,6 +637,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     return true;
   };
 
+  BatchPlan apply_plan;
   Timer tm(ops[0]->stream);
   tm.start();
   long ticks = 0;
@@ -686,7 +687,42 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         for (int i : who) pre[i] = 1;
       }
     }
-    for (int i : active) run[i]->issue_step(pre[i] != 0);
+    // the other solves' preconditioners, one solve at a time, before any operator reads z_j
+    for (int i : active)
+      if (!pre[i]) {
+        run[i]->issue_precondition();
+        pre[i] = 1;
+      }
+    // the operator of all solves that share one operator's fields and launch shapes in one batched grid
+    // per kernel (the CG batch's plan, its records refreshed every tick)
+    std::vector<char> app(np, 0);
+    {
+      std::vector<int> who;
+      std::vector<Recorder> recs;
+      for (int i : active) {
+        const Vec* z = nullptr;
+        Vec* w = nullptr;
+        if (!run[i]->apply_step_io(z, w)) continue;
+        if (!who.empty()) {
+          const Op &a = *ops[who[0]], &b = *ops[i];
+          if (a.gd != b.gd || a.stream != b.stream || a.Vh != b.Vh || a.par.kind != b.par.kind ||
+              std::memcmp(&a.hcfg[p.sloppy_prec][0], &b.hcfg[p.sloppy_prec][0], sizeof(HopConfig)) != 0)
+            continue;
+        }
+        recs.emplace_back();
+        {
+          RecordScope rs(recs.back());
+          ops[i]->apply(p.sloppy_prec, w->view(), z->view(), +1, nullptr);
+        }
+        who.push_back(i);
+      }
+      if (who.size() >= 2) {
+        apply_plan.refresh(recs, ops[who[0]]->stream);
+        apply_plan.run(ops[who[0]]->stream);
+        for (int i : who) app[i] = 1;
+      }
+    }
+    for (int i : active) run[i]->issue_step(pre[i] != 0, app[i] != 0);
     sync_all();
     ++ticks;
     for (int i : active)

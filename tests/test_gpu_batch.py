@@ -113,14 +113,16 @@ def test_batch_sources_on_one_operator():
 
 FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0, 0), (cmx.PRECOND_LEARNED, cmx.HALF, 0, 1),
                 (cmx.PRECOND_LEARNED, cmx.SINGLE, 4, 1), (cmx.PRECOND_LEARNED, cmx.SINGLE, 0, 2),
-                (cmx.PRECOND_MR, cmx.SINGLE, 4, -1), (cmx.PRECOND_NONE, cmx.DOUBLE, 4, -1)]
+                (cmx.PRECOND_MR, cmx.SINGLE, 4, -1), (cmx.PRECOND_NONE, cmx.DOUBLE, 4, -1),
+                (cmx.PRECOND_MG, cmx.SINGLE, 0, -1)]
 
 
 @pytest.mark.parametrize("precond,sloppy,ndefl,variant", FGMRES_CASES)
 def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl, variant):
     """The 12 point sources on one operator through the batched FGMRES-DR (a repeated operator's solves
     share its network, applied to all sources in one launch per kernel with the inference path `variant`:
-    0 operands in registers, 1 transports in LDS, 2 unfused, one source at a time): each solution and
+    0 operands in registers, 1 transports in LDS, 2 unfused, one source at a time; or its multigrid; the
+    operator applied to all sources in one batched grid): each solution and
     iteration count equals the single solve's bit for bit, and reaches the FP64 true residual 1e-10."""
     L = (8, 8, 8, 16)
     say("batch fgmres", precond, sloppy, ndefl, variant)
@@ -132,6 +134,8 @@ def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl, variant):
         net.set_depth(2)
         cmx.net_attach(op, net)
         cmx.net_set_variant(op, sloppy, variant)
+    if precond == cmx.PRECOND_MG:  # the clones share the operator's multigrid
+        op.mg_setup(blocking=(2, 2, 2, 2), nvec=4, nu_post=3, setup_maxit=30, coarse_tol=0.1, seed=1)
     Vh = int(np.prod(L)) // 2
     chis = []
     for s in range(4):

@@ -369,8 +369,13 @@ def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, varian
             chis.append(chi)
     kwp = dict(solver=cmx.RELIABLE_CG, rsd=1e-10, maxit=50000, delta=0.1, sloppy_prec=cmx.HALF, true_residual=True)
     op.solve(chis[0], **kwp)
-    plain = [op.solve(c, **kwp)[1] for c in chis]
+    plain = []
+    for c in chis:  # one line per solve: a long leg keeps reporting progress
+        plain.append(op.solve(c, **kwp)[1])
+        print(f"#   plain source {len(plain)}: {plain[-1]['n_count']} it {plain[-1]['solve_seconds']:.3f} s",
+              file=sys.stderr, flush=True)
     rb, bsecs, _ = batch_in_groups(cmx, op, chis, a.batch_group, **kwp)
+    print(f"#   plain batched: {bsecs:.3f} s", file=sys.stderr, flush=True)
     if net is None:
         net, _ = train_net(cmx, mass, csw, a)
     kw = dict(FGMRES_KW, solver=cmx.FGMRESDR)
@@ -557,6 +562,8 @@ def main():
                          "(about 30 s)")
     ap.add_argument("--no-solve-lattice-kernels", action="store_true",
                     help="skip the kernel table of the solve lattice")
+    ap.add_argument("--near-sources-lattice", type=str, default="",
+                    help="lattice of the 12 point sources near the critical point (default: the solve lattice)")
     ap.add_argument("--near-kappa", type=float, default=0.158,
                     help="near-critical kappa of the stout ensemble (kappa_c is about 0.161 at 32^3x64)")
     ap.add_argument("--no-near-critical", action="store_true",
@@ -661,7 +668,9 @@ def main():
                                       for t in train_nc]},
                 "solve_lattice": near_critical_solve(cmx, gauge_s, LS, a.near_kappa, net_nc),
                 "headline_lattice": near_critical_solve(cmx, gauge, L, a.near_kappa, net_nc)}
-        sources_nc = learned_vs_plain_sources(cmx, LS, a.near_kappa, csw, a, net_nc, gauge=gauge_s, variants=True)
+        NSL = tuple(int(v) for v in a.near_sources_lattice.split(",")) if a.near_sources_lattice else LS
+        sources_nc = learned_vs_plain_sources(cmx, NSL, a.near_kappa, csw, a, net_nc,
+                                              gauge=gauge_s if NSL == LS else None, variants=NSL == LS)
     del gauge
 
     batched = None

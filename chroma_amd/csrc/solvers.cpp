@@ -542,6 +542,9 @@ size_t fgmres_workspace_bytes(const Op& op, const cmx_solver_params& p) {
   b += (size_t)(2 * total + 1 + 2 * nd + 1 + (nd > 0 && pb != P64 ? 1 : 0)) * spinor_bytes(pb, n);
   if (pb != ps) b += 2 * spinor_bytes(ps, n);
   if (p.precond == CMX_PRECOND_MR) b += 2 * spinor_bytes(ps, n);
+  b += 2 * spinor_bytes(ps, n);  // the operator's even-checkerboard temporaries of a solver context
+  // the batched network: a mixed field (8 paths x 3 colours x 8 B per site) and two block outputs per source
+  if (p.precond == CMX_PRECOND_LEARNED) b += (size_t)192 * n + 2 * spinor_bytes(ps, n);
   return b;
 }
 
@@ -550,8 +553,9 @@ size_t fgmres_workspace_bytes(const Op& op, const cmx_solver_params& p) {
 // takes its new column (the Givens update, on the host).  Cycle ends and restarts run per solve.  Each
 // solve performs exactly the kernels and host arithmetic of cmx_solve, so its result is the same bit for
 // bit.  As many solves run at once as their workspaces fit in free device memory; the others wait.
-void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p, double* const* psi_h,
-                        const double* const* chi_h, cmx_solver_result* res, cmx_batch_result& bres) {
+void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p, int max_concurrent,
+                        double* const* psi_h, const double* const* chi_h, cmx_solver_result* res,
+                        cmx_batch_result& bres) {
   const int np = (int)ops.size();
   std::vector<hipStream_t> streams;
   for (Op* o : ops)
@@ -564,8 +568,17 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   CMX_HIP(hipMemGetInfo(&free_b, &total_b));
   size_t need = 0;
   for (Op* o : ops) need = std::max(need, fgmres_workspace_bytes(*o, p));
-  const int conc = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, (size_t)(0.9 * (double)free_b) / std::max<size_t>(need, 1)));
+  int conc = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, (size_t)(0.85 * (double)free_b) / std::max<size_t>(need, 1)));
+  if (max_concurrent > 0) conc = std::min(conc, max_concurrent);
 
+  // problem i runs on solver context ctx[i]: its own, or the context of a finished problem on the same
+  // operator (no new workspace)
+  std::vector<Op*> ctx(ops);
+  auto same_operator = [](const Op& a, const Op& b) {
+    return a.gd == b.gd && std::memcmp(&a.par, &b.par, sizeof(a.par)) == 0 && a.net == b.net && a.ptc == b.ptc &&
+           a.mg == b.mg && std::memcmp(a.hcfg, b.hcfg, sizeof(a.hcfg)) == 0 &&
+           std::memcmp(a.recon, b.recon, sizeof(a.recon)) == 0 && a.stream == b.stream;
+  };
   std::vector<cmx_solver_params> q(np, p);
   for (int i = 0; i < np; ++i) {
     std::memset(&res[i], 0, sizeof(res[i]));
@@ -580,7 +593,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   for (int i = 0; i < np; ++i) pending.push_back(i);
   // the source and guess on the device and the workspace allocated
   auto prepare = [&](int i) {
-    Op& op = *ops[i];
+    Op& op = *ctx[i];
     const int n = op.subset_sites();
     op.pack(chi_h[i], n, op.work(1, P64, n));
     op.pack(psi_h[i], n, op.work(0, P64, n));
@@ -589,10 +602,43 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     FgmresDR::reserve(op, q[i]);
     packed[i] = 1;
   };
+  // the timed region: device time of the solves, without the host transfers of the solves that start
+  // late or finish early (they pause it)
+  struct Seg {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Seg() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } seg;
+  CMX_HIP(hipEventCreate(&seg.a));
+  CMX_HIP(hipEventCreate(&seg.b));
+  double timed = 0;
+  bool timing = false;
+  hipStream_t s0 = ctx[0]->stream;
+  auto resume = [&]() {
+    if (timing) return;
+    CMX_HIP(hipEventRecord(seg.a, s0));
+    timing = true;
+  };
+  auto pause = [&]() {
+    if (!timing) return;
+    sync_all();
+    CMX_HIP(hipEventRecord(seg.b, s0));
+    CMX_HIP(hipEventSynchronize(seg.b));
+    float ms = 0;
+    CMX_HIP(hipEventElapsedTime(&ms, seg.a, seg.b));
+    timed += ms * 1e-3;
+    timing = false;
+  };
   auto start = [&](int i) {
-    Op& op = *ops[i];
+    Op& op = *ctx[i];
     const int n = op.subset_sites();
-    if (!packed[i]) prepare(i);
+    if (!packed[i]) {
+      pause();
+      prepare(i);
+      resume();
+    }
     run[i].reset(new FgmresDR(op, q[i], op.work(0, P64, n), op.work(1, P64, n)));
     stepping[i] = 0;
   };
@@ -602,7 +648,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   // a solve that has stopped: its result and true residual, a true-residual restart, or its solution
   // to the host (and a solver context of the batch's own frees its workspace)
   auto stop = [&](int i) -> bool {
-    Op& op = *ops[i];
+    Op& op = *ctx[i];
     const int n = op.subset_sites();
     cmx_solver_result part;
     std::memset(&part, 0, sizeof(part));
@@ -615,7 +661,9 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     run[i].reset();
     res[i].n_count += part.n_count;
     res[i].n_reliable += part.n_reliable;
+    pause();  // the check after the solve, as cmx_solve's, is not solve time
     true_residual(op, op.work(0, P64, n), op.work(1, P64, n), false, res[i]);
+    resume();
     if (err[i] == CMX_OK && p.true_residual && res[i].rel_resid > p.rsd_target && res[i].n_restarts < 8 &&
         res[i].n_count < p.max_iter) {
       q[i].rsd_target = q[i].rsd_target * std::max(0.02, 0.5 * p.rsd_target / res[i].rel_resid);
@@ -627,9 +675,19 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     // the solution goes to the host now only when a waiting solve needs this workspace; otherwise after
     // the timed region
     if (!pending.empty()) {
+      pause();
       op.unpack(psi_h[i], n, op.work(0, P64, n));
+      resume();
       unpacked[i] = 1;
-      if (op.shared_stream) {  // a clone made for this batch
+      int next = -1;
+      for (int j : pending)
+        if (!packed[j] && same_operator(op, *ops[j])) {
+          next = j;
+          break;
+        }
+      if (next >= 0) {
+        ctx[next] = &op;  // the waiting solve takes this context and its workspace
+      } else if (op.shared_stream) {  // a clone made for this batch
         op.ws.clear();
         ++op.ws_gen;
       }
@@ -638,8 +696,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
   };
 
   BatchPlan apply_plan;
-  Timer tm(ops[0]->stream);
-  tm.start();
+  resume();
   long ticks = 0;
   while (!pending.empty() || !active.empty()) {
     while ((int)active.size() < conc && !pending.empty()) {
@@ -675,7 +732,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         Vec* z = nullptr;
         if (!run[i]->learned_step_io(v, z)) continue;
         if (!who.empty()) {
-          const Op &a = *ops[who[0]], &b = *ops[i];
+          const Op &a = *ctx[who[0]], &b = *ctx[i];
           if (a.gd != b.gd || a.net != b.net || a.ptc != b.ptc || a.stream != b.stream) continue;
         }
         vin.push_back(v);
@@ -683,7 +740,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         who.push_back(i);
       }
       if (who.size() >= 2) {
-        learned_precond_apply_batch(*ops[who[0]], p.sloppy_prec, vin, zout);
+        learned_precond_apply_batch(*ctx[who[0]], p.sloppy_prec, vin, zout);
         for (int i : who) pre[i] = 1;
       }
     }
@@ -704,7 +761,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         Vec* w = nullptr;
         if (!run[i]->apply_step_io(z, w)) continue;
         if (!who.empty()) {
-          const Op &a = *ops[who[0]], &b = *ops[i];
+          const Op &a = *ctx[who[0]], &b = *ctx[i];
           if (a.gd != b.gd || a.stream != b.stream || a.Vh != b.Vh || a.par.kind != b.par.kind ||
               std::memcmp(&a.hcfg[p.sloppy_prec][0], &b.hcfg[p.sloppy_prec][0], sizeof(HopConfig)) != 0)
             continue;
@@ -712,13 +769,13 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         recs.emplace_back();
         {
           RecordScope rs(recs.back());
-          ops[i]->apply(p.sloppy_prec, w->view(), z->view(), +1, nullptr);
+          ctx[i]->apply(p.sloppy_prec, w->view(), z->view(), +1, nullptr);
         }
         who.push_back(i);
       }
       if (who.size() >= 2) {
-        apply_plan.refresh(recs, ops[who[0]]->stream);
-        apply_plan.run(ops[who[0]]->stream);
+        apply_plan.refresh(recs, ctx[who[0]]->stream);
+        apply_plan.run(ctx[who[0]]->stream);
         for (int i : who) app[i] = 1;
       }
     }
@@ -731,10 +788,11 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
         stepping[i] = 0;
       }
   }
-  bres.solve_seconds = tm.stop();
+  pause();
+  bres.solve_seconds = timed;
   bres.n_iter = (int)ticks;
   for (int i = 0; i < np; ++i)
-    if (!unpacked[i]) ops[i]->unpack(psi_h[i], ops[i]->subset_sites(), ops[i]->work(0, P64, ops[i]->subset_sites()));
+    if (!unpacked[i]) ctx[i]->unpack(psi_h[i], ctx[i]->subset_sites(), ctx[i]->work(0, P64, ctx[i]->subset_sites()));
   for (int i = 0; i < np; ++i)
     if (err[i] != CMX_OK) throw Error(err[i], "solve_batch: problem " + std::to_string(i) + ": " + msg[i]);
 }
@@ -756,7 +814,7 @@ void solve_batch(const std::vector<Op*>& ops, const cmx_solver_params& p, const 
   if (p.solver == CMX_SOLVER_FGMRESDR) {
     CMX_REQUIRE(!mdagm, CMX_ERR_UNSUPPORTED, "FGMRES-DR is a LinOp system solver (no MdagM form)");
     std::memset(&bres, 0, sizeof(bres));
-    solve_batch_fgmres(ops, p, psi_h, chi_h, res, bres);
+    solve_batch_fgmres(ops, p, bp.max_concurrent, psi_h, chi_h, res, bres);
     bres.total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int i = 0; i < np; ++i) {
       res[i].solve_seconds = bres.solve_seconds;

@@ -244,7 +244,7 @@ std::vector<SystemSolverResults_t> LinOpSysSolverMI355X::operator()(std::vector<
     cp[k] = chi[k].data() + off;
   }
   std::vector<cmx_solver_result> r(n);
-  cmx_batch_params bp{/*tune*/ 0, /*graph*/ 0};
+  cmx_batch_params bp{/*tune*/ 0, /*graph*/ 0, /*max_concurrent*/ 0};
   cmx_batch_result br{};
   const int st = cmx_solve_batch(ops.data(), n, &sp_, &bp, pp.data(), cp.data(), 0, r.data(), &br);
   if (st != CMX_OK && !(silent_ && st == CMX_ERR_NOCONV)) check(st, "cmx_solve_batch");

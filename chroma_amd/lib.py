@@ -75,7 +75,7 @@ class SolverResult(ctypes.Structure):
 
 
 class BatchParams(ctypes.Structure):
-    _fields_ = [("tune", ctypes.c_int), ("graph", ctypes.c_int)]
+    _fields_ = [("tune", ctypes.c_int), ("graph", ctypes.c_int), ("max_concurrent", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -542,7 +542,7 @@ class Operator:
 def solve_batch(ops, chis, psis=None, solver=RELIABLE_CG, rsd=1e-10, maxit=10000, delta=0.1, sloppy_prec=HALF,
                 check_every=0, mdagm=False, true_residual=False, tune=True, graph=False, n_krylov=16, n_defl=0,
                 precond=PRECOND_NONE, precond_iters=4, precond_rsd=1e-12, precond_omega=1.0, reorth=1,
-                basis_prec=-1, cycle_reduction=0.0):
+                basis_prec=-1, cycle_reduction=0.0, max_concurrent=0):
     """n independent solves advanced together (cmx_solve_batch): CG and reliable CG launch every kernel
     once for all problems; FGMRES-DR advances all solves one Arnoldi step per host read-back.
     ops: Operators (an Operator may repeat: the sources of one propagator); chis: subset arrays, one per
@@ -556,7 +556,7 @@ def solve_batch(ops, chis, psis=None, solver=RELIABLE_CG, rsd=1e-10, maxit=10000
                      0, int(bool(true_residual)), int(n_krylov), int(n_defl), int(precond), int(precond_iters),
                      float(precond_rsd), float(precond_omega), int(reorth), int(basis_prec), float(cycle_reduction),
                      0)
-    bp = BatchParams(int(bool(tune)), int(bool(graph)))
+    bp = BatchParams(int(bool(tune)), int(bool(graph)), int(max_concurrent))
     hs = (_vp * n)(*[o.h for o in ops])
     pp = (_dp * n)(*[_d(x) for x in psis])
     cp = (_dp * n)(*[_d(c) for c in chis])

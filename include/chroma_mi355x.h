@@ -245,6 +245,9 @@ typedef struct {
                  kernel, 64/128/256 threads) and keep the fastest; 0: the operators' own shapes, which
                  must select the same kernel on every problem */
   int graph;  /* 1: replay the batched iterations from a captured HIP graph */
+  int max_concurrent; /* FGMRES-DR: at most this many solves in flight (0: as many as fit in free device
+                         memory); a waiting solve takes over the context of a finished one on the same
+                         operator */
 } cmx_batch_params;
 typedef struct {
   double solve_seconds;  /* device time of the batched iterations (HIP events)                   */

@@ -156,6 +156,29 @@ def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl, variant):
         assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
 
 
+def test_batch_fgmres_limited_concurrency():
+    """12 sources with at most 3 solves in flight: each waiting solve takes over the solver context of a
+    finished one; every solution still equals its single solve bit for bit."""
+    L = (8, 8, 8, 16)
+    say("batch fgmres, 3 in flight")
+    kappa = 0.13
+    op, _ = make_problem(L, 1031, kappa, cmx.HALF)
+    net = cmx.Net()
+    net.init_neumann(4.0 + cmx.kappa_to_mass(kappa))
+    cmx.net_attach(op, net)
+    Vh = int(np.prod(L)) // 2
+    chis = [cmx.z2_source((Vh, 4, 3), 40 + k) for k in range(12)]
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=3000, sloppy_prec=cmx.HALF, basis_prec=cmx.HALF, n_krylov=16,
+              precond=cmx.PRECOND_LEARNED, cycle_reduction=3e-3)
+    psis, res, summ = cmx.solve_batch([op] * 12, chis, max_concurrent=3, **kw)
+    say("  summary", summ)
+    for k in range(12):
+        ref, rr = op.solve(chis[k], **kw)
+        assert res[k]["n_count"] == rr["n_count"]
+        assert np.array_equal(psis[k].view(np.uint64), ref.view(np.uint64))
+        assert res[k]["rel_resid"] <= 1e-10
+
+
 def test_batch_fgmres_uneven_matches_single():
     """FGMRES-DR (MR preconditioner, deflation) over problems on different lattices, gauge fields and
     masses: each equals its single solve bit for bit."""

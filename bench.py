@@ -469,6 +469,56 @@ def learned_solve(cmx, gauge, L, kappa, csw, a):
     return out, net
 
 
+def learned_config(cmx, a):
+    """The learned-preconditioner configuration as stated: 32^3x64, clover csw 1.0, kappa 0.1355; the network
+    (one block: two channel layers, 16 channels) trained on about 20 000 (residual, error) pairs from plain
+    CG solves on 8 seeded stout-smeared fields of that lattice (seeds 10-17), streamed in rounds of 128 pairs
+    (one field per round, the fields in turn, 2 epochs per round), then tested inside FGMRES-DR(16) on 4
+    held-out fields (seeds 20-23) against plain mixed CG, beside its untrained Neumann start."""
+    import numpy as np
+    L, kappa, csw = (32, 32, 32, 64), 0.1355, 1.0
+    mass = cmx.kappa_to_mass(kappa)
+    t0 = time.time()
+    net = cmx.Net()
+    net.set_depth(a.c4_depth)
+    net.init_neumann(4.0 + mass)
+    start = cmx.Net()
+    start.set_depth(a.c4_depth)
+    start.init_neumann(4.0 + mass)
+    ops = []
+    for k in range(8):
+        U = cmx.gauge_apply_bc(L, stout_field(cmx, L, 10 + k), (1, 1, 1, -1))
+        ops.append(cmx.Operator(cmx.Gauge(L, U), cmx.EO_CLOVER, mass, csw, sloppy_prec=cmx.HALF))
+        del U
+    pairs, rounds, loss = 0, 0, []
+    while pairs < a.c4_pairs:
+        op = ops[rounds % 8]
+        cmx.net_attach(op, net)
+        t = cmx.net_train(op, n_sources=8, snapshot_every=1, pairs_per_source=16, normalize=False, epochs=2,
+                          lr=a.c4_lr, seed=1000 + rounds, prec=cmx.HALF, max_pairs=128)
+        pairs += t["n_pairs"]
+        rounds += 1
+        loss.append((round(t["loss_first"], 5), round(t["loss_last"], 5)))
+        if rounds % 10 == 0:
+            print(f"#   c4 training: round {rounds}, {pairs} pairs, loss {t['loss_first']:.5f} -> {t['loss_last']:.5f}",
+                  file=sys.stderr, flush=True)
+    del ops
+    t_train = time.time() - t0
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=5000, sloppy_prec=cmx.HALF, basis_prec=cmx.HALF, n_krylov=16,
+              cycle_reduction=3e-3)
+    trained = held_out_fields(cmx, L, mass, csw, net, kw)
+    untrained = held_out_fields(cmx, L, mass, csw, start, kw)
+    out = {"lattice": list(L), "kappa": kappa, "network_blocks": a.c4_depth, "train_fields": 8,
+           "train_pairs": pairs, "train_rounds": rounds, "train_seconds": round(t_train, 1),
+           "loss_first_round": loss[0], "loss_last_round": loss[-1],
+           "held_out_trained": trained, "held_out_neumann_start": untrained,
+           "speedup_trained_vs_plain": round(float(np.mean([r["speedup"] for r in trained])), 3),
+           "speedup_trained_vs_neumann_start": round(float(np.mean(
+               [u["learned_seconds"] / t["learned_seconds"] for t, u in zip(trained, untrained)])), 3)}
+    print(f"# learned configuration: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def held_out_fields(cmx, L, mass, csw, net, kw, seeds=(20, 21, 22, 23)):
     """The trained network on four more fields of the recipe that training never saw (Haar + 6 stout
     steps, seeds 20-23, the benchmark lattice and kappa): plain mixed CG against learned FGMRES-DR on a Z2
@@ -562,6 +612,12 @@ def main():
                          "(about 30 s)")
     ap.add_argument("--no-solve-lattice-kernels", action="store_true",
                     help="skip the kernel table of the solve lattice")
+    ap.add_argument("--learned-config", action="store_true",
+                    help="also run the learned-preconditioner configuration as stated (32^3x64, 20 000 training "
+                         "pairs from 8 fields, 4 held-out fields; about 25 minutes, --c4-pairs to shorten)")
+    ap.add_argument("--c4-pairs", type=int, default=20000)
+    ap.add_argument("--c4-depth", type=int, default=1)
+    ap.add_argument("--c4-lr", type=float, default=1e-3)
     ap.add_argument("--near-sources-lattice", type=str, default="",
                     help="lattice of the 12 point sources near the critical point (default: the solve lattice)")
     ap.add_argument("--near-kappa", type=float, default=0.158,
@@ -683,6 +739,8 @@ def main():
     if learned is not None and not a.no_sources_comparison:
         sources_cmp = learned_vs_plain_sources(cmx, tuple(int(v) for v in a.sources_lattice.split(",")), 0.1355, csw, a, net)
 
+    learned_c4 = learned_config(cmx, a) if a.learned_config else None
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = dict(cpu_hop)
@@ -716,6 +774,7 @@ def main():
         "solve_time_plain_s": (learned or {}).get("plain_mixed_cg_fp16", {}).get("seconds"),
         "solve_time_learned_s": (learned or {}).get("seconds"),
         "learned": learned,
+        "learned_config": learned_c4,
         "small_volume": small,
         "batched": batched,
         "sources_12_learned_vs_plain": sources_cmp,

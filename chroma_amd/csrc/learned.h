@@ -46,13 +46,15 @@ struct PtcBatch {
 int ptc_waves();
 void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s);
 void launch_ptc_mix(const PtcLaunch& L, hipStream_t s);
-// inference: variant 0 operands in registers, 1 transports staged in LDS
+// inference: variant 0 operands in registers, 1 transports staged in LDS; 3, 4 the same with the links
+// streamed (non-temporal loads)
 void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s);
 // variant 0 or 1 of launch_ptc_gather_mix, and the scatter, on B.ns sources
 void launch_ptc_gather_mix_batch(const PtcLaunch& L, const PtcBatch& B, int variant, hipStream_t s);
-void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, hipStream_t s);
+void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, bool nt, hipStream_t s);
 // adj: the block's adjoint W0^dag delta + G^dag(L.G) (L.W0 holding W0^dag, L.G the mixed adjoint)
-void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s);
+// nt: the links streamed (inference variants 3, 4)
+void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s, bool nt = false);
 void launch_ptc_grad(const PtcLaunch& L, double* out1024, hipStream_t s);
 int launch_w0_grad(const PtcLaunch& L, hipStream_t s);
 

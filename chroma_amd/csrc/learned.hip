@@ -158,7 +158,7 @@ __device__ __forceinline__ void transport2(const R (&u)[18], const float (&h)[12
 // The transported half spinors of the two paths 2 mu (forward) and 2 mu + 1 (backward) at even site y
 // (coordinates x): th[d][s'][c].  ADJ: the adjoint gather of the loss gradient, dgh_{q}(y) for q = p^1,
 // i.e. the projection of the opposite path with a factor 2 (R^dag = 2 Phat), stored under slot p^1.
-template <int P, int RECON, int MU, bool ADJ>
+template <int P, int RECON, int MU, bool ADJ, int LCP = 0>
 __device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4], int y, float (&th)[2][12],
                                           const void* in, const float* in_nrm) {
   using R = typename SpinorIO<P>::R;
@@ -177,11 +177,11 @@ __device__ __forceinline__ void gather_mu(const PtcLaunch& L, const int (&x)[4],
     const int slot = ADJ ? (d ^ 1) : d;
     if (fwd) {  // U_mu(y) from the even-parity links
       const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)MU * LCH * LCB * g.Vh;
-      LinkIO<P, RECON>::load(gb, g.Vh, y, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      LinkIO<P, RECON>::template load<LCP>(gb, g.Vh, y, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
       transport2<false>(u, h, th[slot]);
     } else {    // U_mu(y-mu)^dag from the odd-parity links
       const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)MU * LCH * LCB * g.Vh;
-      LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      LinkIO<P, RECON>::template load<LCP>(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
       transport2<true>(u, h, th[slot]);
     }
   }
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) k_ptc_mix(PtcLaunch L) {
 // Fused gather + mix for inference, no LDS.  A wave owns 32 even sites; lane l takes site l % 32 and
 // the directions mu = h, 2 + h (h = l / 32), both paths of each: in the B-operand layout these are
 // exactly the rows lane l holds, so each lane packs its four transports straight into its fragments.
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __device__ __forceinline__ void gather_mix_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
                                                 unsigned short* G) {
   const Geom& g = L.g;
@@ -298,14 +298,14 @@ __device__ __forceinline__ void gather_mix_body(const PtcLaunch& L, int wg, cons
       for (int c = 0; c < 3; ++c) pack_mu(th, c, b[c][ksr], b[c][2 + ksr]);
     };
     if (h == 0) {
-      gather_mu<P, RECON, 0, false>(L, x, y, th, in, in_nrm);
+      gather_mu<P, RECON, 0, false, LCP>(L, x, y, th, in, in_nrm);
       put(0);
-      gather_mu<P, RECON, 2, false>(L, x, y, th, in, in_nrm);
+      gather_mu<P, RECON, 2, false, LCP>(L, x, y, th, in, in_nrm);
       put(1);
     } else {
-      gather_mu<P, RECON, 1, false>(L, x, y, th, in, in_nrm);
+      gather_mu<P, RECON, 1, false, LCP>(L, x, y, th, in, in_nrm);
       put(0);
-      gather_mu<P, RECON, 3, false>(L, x, y, th, in, in_nrm);
+      gather_mu<P, RECON, 3, false, LCP>(L, x, y, th, in, in_nrm);
       put(1);
     }
   } else {
@@ -327,15 +327,15 @@ __device__ __forceinline__ void gather_mix_body(const PtcLaunch& L, int wg, cons
   }
 }
 
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __global__ void __launch_bounds__(256) k_ptc_gather_mix(PtcLaunch L) {
-  gather_mix_body<P, RECON>(L, blockIdx.x, L.in, L.in_nrm, L.G);
+  gather_mix_body<P, RECON, LCP>(L, blockIdx.x, L.in, L.in_nrm, L.G);
 }
 
 // Fused gather + mix for inference: a work-group of 128 threads transports its 128 even sites into an
 // LDS copy of their four T tiles (24 KiB), then its two waves run the MFMA mix of those 12 (block,
 // colour) tiles and write G.  T never goes to HBM.
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __device__ __forceinline__ void gather_mix_lds_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
                                                     unsigned short* G, bf4* lds) {
   const Geom& g = L.g;
@@ -348,7 +348,7 @@ __device__ __forceinline__ void gather_mix_lds_body(const PtcLaunch& L, int wg, 
     auto one = [&](auto MUc) {
       constexpr int MU = decltype(MUc)::value;
       float th[2][12];
-      gather_mu<P, RECON, MU, false>(L, x, yi, th, in, in_nrm);
+      gather_mu<P, RECON, MU, false, LCP>(L, x, yi, th, in, in_nrm);
 #pragma unroll
       for (int c = 0; c < 3; ++c) pack_mu(th, c, lds_at(c, MU >> 1, MU & 1), lds_at(c, 2 + (MU >> 1), MU & 1));
     };
@@ -382,14 +382,14 @@ __device__ __forceinline__ void gather_mix_lds_body(const PtcLaunch& L, int wg, 
   }
 }
 
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds(PtcLaunch L) {
   __shared__ bf4 lds[4 * 3 * 4 * 64];
-  gather_mix_lds_body<P, RECON>(L, blockIdx.x, L.in, L.in_nrm, L.G, lds);
+  gather_mix_lds_body<P, RECON, LCP>(L, blockIdx.x, L.in, L.in_nrm, L.G, lds);
 }
 
 // P(v)(x) = W0 v(x) + sum_q U_q(x) R_q gh_q(x+q) at odd sites x
-template <int P, int RECON, int MU, bool ADJ>
+template <int P, int RECON, int MU, bool ADJ, int LCP = 0>
 __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4], int xi, float (&acc)[24],
                                            const unsigned short* G) {
   using R = typename SpinorIO<P>::R;
@@ -417,18 +417,18 @@ __device__ __forceinline__ void scatter_mu(const PtcLaunch& L, const int (&x)[4]
     float o[12];
     if (fwd) {  // U_mu(x), odd-parity links
       const char* gb = static_cast<const char*>(L.gauge[1]) + (size_t)MU * LCH * LCB * g.Vh;
-      LinkIO<P, RECON>::load(gb, g.Vh, xi, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      LinkIO<P, RECON>::template load<LCP>(gb, g.Vh, xi, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
       transport2<false>(u, gh, o);
     } else {    // U_mu(x-mu)^dag, even-parity links
       const char* gb = static_cast<const char*>(L.gauge[0]) + (size_t)MU * LCH * LCB * g.Vh;
-      LinkIO<P, RECON>::load(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
+      LinkIO<P, RECON>::template load<LCP>(gb, g.Vh, yn, u, R(L.ph[MU][last][0]), R(L.ph[MU][last][1]));
       transport2<true>(u, gh, o);
     }
     reconstruct_add<MU>(o, (fwd ? -1.f : 1.f) * (ADJ ? -1.f : 1.f), acc);
   }
 }
 
-template <int P, int RECON, bool ADJ>
+template <int P, int RECON, bool ADJ, int LCP>
 __device__ __forceinline__ void scatter_body(const PtcLaunch& L, int wg, const void* in, const float* in_nrm,
                                              void* out, float* out_nrm, const unsigned short* G) {
   const int xi = wg * blockDim.x + threadIdx.x;
@@ -457,19 +457,19 @@ __device__ __forceinline__ void scatter_body(const PtcLaunch& L, int wg, const v
         acc[2 * (3 * s + c) + 1] = im;
       }
   }
-  scatter_mu<P, RECON, 0, ADJ>(L, x, xi, acc, G);
-  scatter_mu<P, RECON, 1, ADJ>(L, x, xi, acc, G);
-  scatter_mu<P, RECON, 2, ADJ>(L, x, xi, acc, G);
-  scatter_mu<P, RECON, 3, ADJ>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 0, ADJ, LCP>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 1, ADJ, LCP>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 2, ADJ, LCP>(L, x, xi, acc, G);
+  scatter_mu<P, RECON, 3, ADJ, LCP>(L, x, xi, acc, G);
   typename SpinorIO<P>::R o24[24];
 #pragma unroll
   for (int k = 0; k < 24; ++k) o24[k] = (typename SpinorIO<P>::R)acc[k];
   SpinorIO<P>::store(out, out_nrm, L.out_stride, xi, o24);
 }
 
-template <int P, int RECON, bool ADJ>
+template <int P, int RECON, bool ADJ, int LCP>
 __global__ void __launch_bounds__(256) k_ptc_scatter(PtcLaunch L) {
-  scatter_body<P, RECON, ADJ>(L, blockIdx.x, L.in, L.in_nrm, L.out, L.out_nrm, L.G);
+  scatter_body<P, RECON, ADJ, LCP>(L, blockIdx.x, L.in, L.in_nrm, L.out, L.out_nrm, L.G);
 }
 
 // ---- several sources at once (the batched flexible solver) -----------------------------------------
@@ -483,26 +483,26 @@ __device__ __forceinline__ bool batch_map(int g, int nblk, int ns, int& blk, int
   return blk < nblk;
 }
 
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __global__ void __launch_bounds__(256) k_ptc_gather_mix_b(PtcLaunch L, PtcBatch B) {
   int blk, src;
   if (!batch_map(blockIdx.x, (L.g.Vh + 127) / 128, B.ns, blk, src)) return;
-  gather_mix_body<P, RECON>(L, blk, B.in[src], B.in_nrm[src], B.G[src]);
+  gather_mix_body<P, RECON, LCP>(L, blk, B.in[src], B.in_nrm[src], B.G[src]);
 }
 
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __global__ void __launch_bounds__(128) k_ptc_gather_mix_lds_b(PtcLaunch L, PtcBatch B) {
   __shared__ bf4 lds[4 * 3 * 4 * 64];
   int blk, src;
   if (!batch_map(blockIdx.x, (L.g.Vh + 127) / 128, B.ns, blk, src)) return;
-  gather_mix_lds_body<P, RECON>(L, blk, B.in[src], B.in_nrm[src], B.G[src], lds);
+  gather_mix_lds_body<P, RECON, LCP>(L, blk, B.in[src], B.in_nrm[src], B.G[src], lds);
 }
 
-template <int P, int RECON>
+template <int P, int RECON, int LCP>
 __global__ void __launch_bounds__(256) k_ptc_scatter_b(PtcLaunch L, PtcBatch B) {
   int blk, src;
   if (!batch_map(blockIdx.x, (L.g.Vh + 255) / 256, B.ns, blk, src)) return;
-  scatter_body<P, RECON, false>(L, blk, B.in[src], B.in_nrm[src], B.out[src], B.out_nrm[src], B.G[src]);
+  scatter_body<P, RECON, false, LCP>(L, blk, B.in[src], B.in_nrm[src], B.out[src], B.out_nrm[src], B.G[src]);
 }
 
 // Sh_real (32x32) = sum over tiles of dG_tile (32 x 32 sites) T_tile^T (32 sites x 32), per wave into
@@ -599,17 +599,21 @@ void launch_ptc_gather(const PtcLaunch& L, bool adj, hipStream_t s) {
   });
 }
 
+// variants 0 (operands in registers) and 1 (transports in LDS) load the links like the spinors; 3 and 4
+// are the same with the links streamed (non-temporal), so that they displace less of the neighbours' lines
 void launch_ptc_gather_mix(const PtcLaunch& L, int variant, hipStream_t s) {
-  if (variant == 1) {  // transports staged in LDS
-    const dim3 grid((L.g.Vh + 127) / 128), block(128);
-    dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
-      hipLaunchKernelGGL((k_ptc_gather_mix_lds<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
-    });
-    return;
-  }
-  const dim3 grid((L.g.Vh + 127) / 128), block(256);  // 4 waves x 32 sites, operands in registers
+  const bool nt = variant >= 3;
+  const int v = nt ? variant - 3 : variant;
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
-    hipLaunchKernelGGL((k_ptc_gather_mix<decltype(P)::value, decltype(RC)::value>), grid, block, 0, s, L);
+    constexpr int PP = decltype(P)::value, RR = decltype(RC)::value;
+    const dim3 grid((L.g.Vh + 127) / 128);
+    if (v == 1) {
+      if (nt) hipLaunchKernelGGL((k_ptc_gather_mix_lds<PP, RR, CMX_NT>), grid, dim3(128), 0, s, L);
+      else hipLaunchKernelGGL((k_ptc_gather_mix_lds<PP, RR, 0>), grid, dim3(128), 0, s, L);
+    } else {
+      if (nt) hipLaunchKernelGGL((k_ptc_gather_mix<PP, RR, CMX_NT>), grid, dim3(256), 0, s, L);
+      else hipLaunchKernelGGL((k_ptc_gather_mix<PP, RR, 0>), grid, dim3(256), 0, s, L);
+    }
   });
 }
 
@@ -617,21 +621,28 @@ static int batch_grid(int nblk, int ns) { return 8 * ns * ((nblk + 7) / 8); }
 
 void launch_ptc_gather_mix_batch(const PtcLaunch& L, const PtcBatch& B, int variant, hipStream_t s) {
   const int nblk = (L.g.Vh + 127) / 128;
+  const bool nt = variant >= 3;
+  const int v = nt ? variant - 3 : variant;
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
-    if (variant == 1)
-      hipLaunchKernelGGL((k_ptc_gather_mix_lds_b<decltype(P)::value, decltype(RC)::value>),
-                         dim3(batch_grid(nblk, B.ns)), dim3(128), 0, s, L, B);
-    else
-      hipLaunchKernelGGL((k_ptc_gather_mix_b<decltype(P)::value, decltype(RC)::value>),
-                         dim3(batch_grid(nblk, B.ns)), dim3(256), 0, s, L, B);
+    constexpr int PP = decltype(P)::value, RR = decltype(RC)::value;
+    const dim3 grid(batch_grid(nblk, B.ns));
+    if (v == 1) {
+      if (nt) hipLaunchKernelGGL((k_ptc_gather_mix_lds_b<PP, RR, CMX_NT>), grid, dim3(128), 0, s, L, B);
+      else hipLaunchKernelGGL((k_ptc_gather_mix_lds_b<PP, RR, 0>), grid, dim3(128), 0, s, L, B);
+    } else {
+      if (nt) hipLaunchKernelGGL((k_ptc_gather_mix_b<PP, RR, CMX_NT>), grid, dim3(256), 0, s, L, B);
+      else hipLaunchKernelGGL((k_ptc_gather_mix_b<PP, RR, 0>), grid, dim3(256), 0, s, L, B);
+    }
   });
 }
 
-void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, hipStream_t s) {
+void launch_ptc_scatter_batch(const PtcLaunch& L, const PtcBatch& B, bool nt, hipStream_t s) {
   const int nblk = (L.g.Vh + 255) / 256;
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
-    hipLaunchKernelGGL((k_ptc_scatter_b<decltype(P)::value, decltype(RC)::value>), dim3(batch_grid(nblk, B.ns)),
-                       dim3(256), 0, s, L, B);
+    constexpr int PP = decltype(P)::value, RR = decltype(RC)::value;
+    const dim3 grid(batch_grid(nblk, B.ns));
+    if (nt) hipLaunchKernelGGL((k_ptc_scatter_b<PP, RR, CMX_NT>), grid, dim3(256), 0, s, L, B);
+    else hipLaunchKernelGGL((k_ptc_scatter_b<PP, RR, 0>), grid, dim3(256), 0, s, L, B);
   });
 }
 
@@ -641,13 +652,16 @@ void launch_ptc_mix(const PtcLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(k_ptc_mix, dim3((waves + 3) / 4), dim3(256), 0, s, L);
 }
 
-void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s) {
+void launch_ptc_scatter(const PtcLaunch& L, bool adj, hipStream_t s, bool nt) {
   const dim3 grid((L.g.Vh + 255) / 256), block(256);
   dispatch_pr(L.prec, L.recon, [&](auto P, auto RC) {
+    constexpr int PP = decltype(P)::value, RR = decltype(RC)::value;
     if (adj)
-      hipLaunchKernelGGL((k_ptc_scatter<decltype(P)::value, decltype(RC)::value, true>), grid, block, 0, s, L);
+      hipLaunchKernelGGL((k_ptc_scatter<PP, RR, true, 0>), grid, block, 0, s, L);
+    else if (nt)
+      hipLaunchKernelGGL((k_ptc_scatter<PP, RR, false, CMX_NT>), grid, block, 0, s, L);
     else
-      hipLaunchKernelGGL((k_ptc_scatter<decltype(P)::value, decltype(RC)::value, false>), grid, block, 0, s, L);
+      hipLaunchKernelGGL((k_ptc_scatter<PP, RR, false, 0>), grid, block, 0, s, L);
   });
 }
 

@@ -288,9 +288,11 @@ int cmx_net_set_block(cmx_net net, int blk, const double* w1, const double* w2, 
 int cmx_net_get_weights(cmx_net net, double* w1, double* w2, double* w0);
 int cmx_net_set_weights(cmx_net net, const double* w1, const double* w2, const double* w0);
 int cmx_op_set_net(cmx_op op, cmx_net net);
-/* kernel path of the network's first layer at precision prec: 0 fused (MFMA operands in registers),
- * 1 fused (transports staged in LDS), 2 separate gather and mix kernels, -1 time all three at the next
- * application and keep the fastest (the default) */
+/* kernel path of the network at precision prec: 0 fused gather and mix (MFMA operands in registers),
+ * 1 fused (transports staged in LDS), 2 separate gather and mix kernels, 3 and 4 as 0 and 1 with the links
+ * of the gather and the scatter streamed (non-temporal loads); -1 time all five at the next application
+ * and keep the fastest (the default).  The paths agree to the network's BF16 precision but not bit for
+ * bit (the compiler contracts their FP32 arithmetic differently); each is deterministic. */
 int cmx_op_set_net_variant(cmx_op op, int prec, int variant);
 /* out = P(in) on the odd checkerboard at storage precision prec (host arrays, Vh*24 doubles) */
 int cmx_net_apply(cmx_op op, int prec, double* out, const double* in);

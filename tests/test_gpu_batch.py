@@ -113,6 +113,7 @@ def test_batch_sources_on_one_operator():
 
 FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0, 0), (cmx.PRECOND_LEARNED, cmx.HALF, 0, 1),
                 (cmx.PRECOND_LEARNED, cmx.SINGLE, 4, 1), (cmx.PRECOND_LEARNED, cmx.SINGLE, 0, 2),
+                (cmx.PRECOND_LEARNED, cmx.HALF, 0, 3), (cmx.PRECOND_LEARNED, cmx.HALF, 0, 4),
                 (cmx.PRECOND_MR, cmx.SINGLE, 4, -1), (cmx.PRECOND_NONE, cmx.DOUBLE, 4, -1),
                 (cmx.PRECOND_MG, cmx.SINGLE, 0, -1)]
 
@@ -121,7 +122,8 @@ FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0, 0), (cmx.PRECOND_LEARNED, cmx
 def test_batch_fgmres_sources_match_single(precond, sloppy, ndefl, variant):
     """The 12 point sources on one operator through the batched FGMRES-DR (a repeated operator's solves
     share its network, applied to all sources in one launch per kernel with the inference path `variant`:
-    0 operands in registers, 1 transports in LDS, 2 unfused, one source at a time; or its multigrid; the
+    0 operands in registers, 1 transports in LDS, 2 unfused, one source at a time, 3 and 4 as 0 and 1 with
+    streamed links; or its multigrid; the
     operator applied to all sources in one batched grid): each solution and
     iteration count equals the single solve's bit for bit, and reaches the FP64 true residual 1e-10."""
     L = (8, 8, 8, 16)

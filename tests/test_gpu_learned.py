@@ -55,7 +55,7 @@ def test_net_forward(L, prec):
     cmx.net_attach(op, net)
     v = random_fermion(Vh, 72)
     want = learned.ptc_apply(L, U, w1, w2, w0, v)
-    for variant in (0, 1, 2):  # fused (registers), fused (LDS), separate kernels
+    for variant in (0, 1, 2, 3, 4):  # fused (registers), fused (LDS), separate kernels; 3, 4 streamed links
         cmx.net_set_variant(op, prec, variant)
         got = cmx.net_apply(op, v, prec)
         say("  variant", variant, "rel", rel(got, want))

@@ -30,7 +30,8 @@ _lp = ctypes.POINTER(ctypes.c_long)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # CMX_ORACLE_LIB: another build of the same sources (the sanitizer build, oracle/_san)
+        path = os.environ.get("CMX_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = ctypes.CDLL(path)

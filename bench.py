@@ -559,6 +559,8 @@ def hop_kernels(cmx, op, Vh, steps, multi=True):
         ("eo_clover_op_fp32", 1, cmx.SINGLE, eo_clover_bytes(1, 12) * Vh),
         ("eo_clover_op_fp64", 1, cmx.DOUBLE, eo_clover_bytes(0, r64) * Vh),
         ("stream_copy_fp32", 3, cmx.SINGLE, 2 * 96 * Vh),
+        # five read streams and one write stream of 16 B elements carrying the FP16 hop's bytes
+        ("stream_mix_hop_fp16", 5, cmx.HALF, (hop_bytes(cmx.HALF, 12) * Vh + 95) // 96 * 96),
     ]
     if multi:
         # 12 sources: each spinor read and written once per source, each link once
@@ -769,7 +771,11 @@ def main():
                    "bytes_per_site": hop_bytes(cmx.HALF, 12), "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": hop_bytes(cmx.HALF, 12) * Vh, "traffic_source": traffic_src},
+                     "algorithmic_bytes": hop_bytes(cmx.HALF, 12) * Vh, "traffic_source": traffic_src,
+                     # the same bytes with the same read/write mix, perfectly streamed (no gathers)
+                     "streamed_mix_GB/s": extra.get("stream_mix_hop_fp16", {}).get("GB/s"),
+                     "frac_of_streamed_mix": (round(gbs / extra["stream_mix_hop_fp16"]["GB/s"], 4)
+                                              if "stream_mix_hop_fp16" in extra else None)},
         "cpu_baseline": cpu,
         "solve_time_plain_s": (learned or {}).get("plain_mixed_cg_fp16", {}).get("seconds"),
         "solve_time_learned_s": (learned or {}).get("seconds"),

@@ -538,7 +538,7 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
     require_init();
     cmx::Op& op = O(h);
     CMX_REQUIRE(ms_per_call && n > 0 && prec >= 0 && prec <= 2, CMX_ERR_ARG, "op_time: bad argument");
-    CMX_REQUIRE(what >= 0 && what <= 4, CMX_ERR_ARG, "op_time: bad selector");
+    CMX_REQUIRE(what >= 0 && what <= 5, CMX_ERR_ARG, "op_time: bad selector");
     const int Vh = (int)op.Vh;
     const int ns = op.subset_sites();
     cmx::Vec a(prec, ns), b(prec, ns);
@@ -551,6 +551,19 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
         mb.emplace_back(new cmx::Vec(prec, Vh));
         va.push_back(ma.back()->view());
         vb.push_back(mb.back()->view());
+      }
+    }
+    // what = 5: six streams of 16-byte elements, together the hopping term's algorithmic bytes at this
+    // precision with 12-real links (FP16 296, FP32 576, FP64 1152 B per site)
+    std::vector<cmx::DevBuf> mix;
+    long n_mix = 0;
+    if (what == 5) {
+      const long site_bytes = prec == cmx::P16 ? 296 : (prec == cmx::P32 ? 576 : 1152);
+      n_mix = (site_bytes * (long)Vh + 95) / 96;
+      for (int k = 0; k < 6; ++k) {
+        mix.emplace_back();
+        mix.back().alloc((size_t)n_mix * 16);
+        CMX_HIP(hipMemsetAsync(mix.back().p, 0x11 * (k + 1), (size_t)n_mix * 16, op.stream));
       }
     }
     // deterministic non-zero input
@@ -582,6 +595,11 @@ int cmx_op_time(cmx_op h, int what, int prec, int n, double* ms_per_call) {
           break;
         }
         case 3: cmx::stream_copy((long)cmx::spinor_chunks(prec) * Vh, a.buf.p, b.buf.p, op.stream); break;
+        case 5: {
+          const void* in[5] = {mix[0].p, mix[1].p, mix[2].p, mix[3].p, mix[4].p};
+          cmx::stream_mix(n_mix, in, mix[5].p, op.stream);
+          break;
+        }
         case 4:
           if (!op.dslash_multi(prec, vb.data(), va.data(), 12, +1, 1))
             for (int k = 0; k < 12; ++k) op.dslash(prec, vb[k], va[k], +1, 1);

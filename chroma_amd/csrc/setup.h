@@ -20,6 +20,9 @@ void spinor_pack(int prec, int n, const double* host_dev, void* v, float* nrm, i
 void spinor_noise(int prec, int n, unsigned seed, void* v, float* nrm, int stride, hipStream_t s);
 void spinor_unpack(int prec, int n, const void* v, const float* nrm, int stride, double* host_dev, hipStream_t s);
 void stream_copy(long n_float4, const void* a, void* b, hipStream_t s);
+// streaming reference with the hopping term's read/write mix: out[i] = a[0][i] ^ ... ^ a[4][i] over
+// n 16-byte elements (five read streams, one write stream, 5/6 of the bytes read)
+void stream_mix(long n_uint4, const void* const a[5], void* out, hipStream_t s);
 int sum_partials(int n, const double* a, double* partials, hipStream_t s);
 
 }  // namespace cmx

@@ -380,6 +380,18 @@ __global__ void __launch_bounds__(256) k_stream_copy(long n, const float4* __res
   if (i < n) b[i] = a[i];
 }
 
+// five read streams, one write stream (the read/write mix of the hopping term), 16 B per lane each
+__global__ void __launch_bounds__(256) k_stream_mix(long n, const uint4* __restrict__ a0, const uint4* __restrict__ a1,
+                                                    const uint4* __restrict__ a2, const uint4* __restrict__ a3,
+                                                    const uint4* __restrict__ a4, uint4* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint4 v0 = a0[i], v1 = a1[i], v2 = a2[i], v3 = a3[i], v4 = a4[i];
+    out[i] = make_uint4(v0.x ^ v1.x ^ v2.x ^ v3.x ^ v4.x, v0.y ^ v1.y ^ v2.y ^ v3.y ^ v4.y,
+                        v0.z ^ v1.z ^ v2.z ^ v3.z ^ v4.z, v0.w ^ v1.w ^ v2.w ^ v3.w ^ v4.w);
+  }
+}
+
 // per-site double array -> per-block partial sums
 __global__ void __launch_bounds__(256) k_sum_partials(int n, const double* a, double* partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,6 +475,11 @@ void spinor_unpack(int prec, int n, const void* v, const float* nrm, int stride,
 
 void stream_copy(long n_float4, const void* a, void* b, hipStream_t s) {
   hipLaunchKernelGGL(k_stream_copy, grid_for(n_float4), dim3(256), 0, s, n_float4, (const float4*)a, (float4*)b);
+}
+
+void stream_mix(long n_uint4, const void* const a[5], void* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_mix, grid_for(n_uint4), dim3(256), 0, s, n_uint4, (const uint4*)a[0], (const uint4*)a[1],
+                     (const uint4*)a[2], (const uint4*)a[3], (const uint4*)a[4], (uint4*)out);
 }
 
 int sum_partials(int n, const double* a, double* partials, hipStream_t s) {

@@ -386,7 +386,9 @@ int cmx_op_set_multi_group(cmx_op op, int group);
  * Average device milliseconds per call over n back-to-back launches on the operator's stream,
  * bracketed by HIP events.  what: 0 hopping term (cb 1), 1 EO operator kernel pair (PLUS),
  * 2 fused EO kernel 2 only, 3 streaming copy of one odd fermion (bandwidth reference), 4 the hopping
- * term on 12 sources (cmx_dslash_multi; one call covers all 12). */
+ * term on 12 sources (cmx_dslash_multi; one call covers all 12), 5 a streaming kernel with the hopping
+ * term's bytes and read/write mix (five read streams, one write stream of 16-byte elements, together
+ * FP16 296, FP32 576, FP64 1152 B per site: the rate the hop could reach without its gathers). */
 int cmx_op_time(cmx_op op, int what, int prec, int n, double* ms_per_call);
 
 /* ---- layout ------------------------------------------------------------------------------------

@@ -107,6 +107,7 @@ enum BlasOp {
   B_SCALE,          // v0 = S[ia] v1 (real)
   B_AXPY_CNORM,     // v0 += S[ia] v1 (complex) ; red0 = |v0|^2
   B_AXPBY,          // v0 = S[ia] v1 + S[ib] v0 (real)
+  B_MR_UPDATE,      // v0 += S[ia] v1 ; v1 += S[ib] v2 (complex) ; red0 = |v1|^2  (psi += a r ; r -= a M r)
   B_NOPS
 };
 

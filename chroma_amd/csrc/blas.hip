@@ -222,6 +222,21 @@ CMX_F(FAxpyCNorm, 1, 3, 1) {
   red[0] += nrm2(v0);
 }
 };
+// the two updates of an MR step in one pass: the same arithmetic as FCaxpy then FAxpyCNorm (v0 is
+// updated from v1 before v1 changes)
+CMX_F(FMrUpdate, 1, 7, 3) {
+  const R ar = (R)S[ia], ai = (R)S[ia + 1], br = (R)S[ib], bi = (R)S[ib + 1];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const R xr = v1[2 * k], xi = v1[2 * k + 1], yr = v2[2 * k], yi = v2[2 * k + 1];
+    v0[2 * k] += ar * xr - ai * xi;
+    v0[2 * k + 1] += ar * xi + ai * xr;
+    v1[2 * k] += br * yr - bi * yi;
+    v1[2 * k + 1] += br * yi + bi * yr;
+  }
+  red[0] += nrm2(v1);
+}
+};
 #undef CMX_F
 
 // work-group blk of the launch b
@@ -312,8 +327,20 @@ __device__ __forceinline__ void finalize_body(const FinLaunch& L) {
   int* F = L.F;
   double red[3] = {0, 0, 0};
   for (int r = 0; r < nred; ++r) {
+    // each thread sums partials threadIdx.x, + blockDim.x, ... in that order; the loads of eight terms
+    // are issued before their additions, so the sum waits for one load latency per eight terms (the
+    // partials of a 48^3x96 reduction are 20736 per entry) and the order, hence the result, is unchanged
     double acc = 0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) acc += partials[(size_t)k * nred + r];
+    const int step = blockDim.x;
+    int k = threadIdx.x;
+    for (; k + 7 * step < nb; k += 8 * step) {
+      double t[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t[j] = partials[(size_t)(k + j * step) * nred + r];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc += t[j];
+    }
+    for (; k < nb; k += step) acc += partials[(size_t)k * nred + r];
     red[r] = acc;
   }
   block_sum<3>(red);
@@ -629,7 +656,7 @@ __global__ void __launch_bounds__(256) k_finalize_b(const FinLaunch* __restrict_
 int blas_nred(int op) {
   switch (op) {
     case B_NORM2: case B_XMY_NORM: case B_XMY_NORM_COPY: case B_AXPY_NORM: case B_AXPY2_NORM: case B_BICG_UPDATE:
-    case B_AXPY_CNORM:
+    case B_AXPY_CNORM: case B_MR_UPDATE:
       return 1;
     case B_CDOT:
       return 2;
@@ -666,6 +693,7 @@ void launch_blas(int op, const BlasLaunch& b, const int prec[4], hipStream_t s, 
     case B_RBICG_UPDATE: dispatch<FRbicgUpdate>(b, prec, s, bg); break;
     case B_SCALE: dispatch<FScale>(b, prec, s, bg); break;
     case B_AXPY_CNORM: dispatch<FAxpyCNorm>(b, prec, s, bg); break;
+    case B_MR_UPDATE: dispatch<FMrUpdate>(b, prec, s, bg); break;
     default: throw Error(CMX_ERR_ARG, "blas: unknown op");
   }
 }

@@ -843,6 +843,14 @@ int cmx_mg_setup(cmx_op h, const cmx_mg_params* p, cmx_mg_result* res) {
   });
 }
 
+int cmx_mg_update(cmx_op h, const cmx_mg_params* p) {
+  return guarded([&] {
+    require_init();
+    CMX_REQUIRE(p, CMX_ERR_ARG, "mg_update: null argument");
+    cmx::mg_update(O(h), *p);
+  });
+}
+
 int cmx_mg_get_nullvecs(cmx_op h, double* out) {
   return guarded([&] {
     require_init();

@@ -33,8 +33,7 @@ struct MRPrecond {
       op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
       op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
       op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
-      op.blas(B_CAXPY, {&z, &r}, S_MR + 3, 0, F_MRDONE);              // psi += a r
-      op.blas(B_AXPY_CNORM, {&r, &mr}, S_MR + 5, 0, F_MRDONE);        // r -= a M r
+      op.blas(B_MR_UPDATE, {&z, &r, &mr}, S_MR + 3, S_MR + 5, F_MRDONE);  // psi += a r ; r -= a M r
       op.finalize(nb, 1, PG_MR_R);
     }
   }

@@ -23,6 +23,7 @@ struct MGState {
   int period[4] = {1, 1, 1, 1};
   int ncol = 1;
   std::vector<std::unique_ptr<Vec>> P;  // null vectors, full lattice FP32
+  std::vector<std::unique_ptr<Vec>> P16;  // their FP16 copies (restriction and prolongation at FP16)
   MGVecs pv{};
   DevBuf Ac, nbr, colour;
   // coarse GCR work space
@@ -279,6 +280,15 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
   op.sync();
   const auto t2 = std::chrono::steady_clock::now();
 
+  // FP16 copies of the null vectors: an FP16 V-cycle restricts and prolongs with them (half the bytes of
+  // the FP32 vectors; the coarse operator stays the Galerkin product of the FP32 ones)
+  for (int k = 0; k < m.nvec; ++k) {
+    st->P16.emplace_back(new Vec(P16, 2 * Vh));
+    op.blas(B_COPY, {st->P16.back().get(), st->P[k].get()});
+    st->pv.H[k] = st->P16.back()->buf.p;
+    st->pv.Hn[k] = st->P16.back()->nrm();
+  }
+
   // coarse GCR work space
   const size_t cv = sizeof(float2) * (size_t)m.Nb * m.D;
   st->xc.alloc(cv);
@@ -325,6 +335,31 @@ void mg_precondition(Op& op, int prec, const Vec& v, Vec& z) {
   if (st.par.nu_post > 0) MRPrecond(op, prec, n, st.par.nu_post, 0.0, st.par.omega, r, mr).apply(z, v);
 }
 
+void mg_update(Op& op, const cmx_mg_params& p) {
+  MGState& st = state(op);
+  for (int mu = 0; mu < 4; ++mu)
+    CMX_REQUIRE(p.blocking[mu] == st.par.blocking[mu], CMX_ERR_ARG,
+                "multigrid update: Blocking differs from the set-up (a new null space needs cmx_mg_setup)");
+  CMX_REQUIRE(p.nvec == st.par.nvec, CMX_ERR_ARG,
+              "multigrid update: NullVectors differs from the set-up (a new null space needs cmx_mg_setup)");
+  CMX_REQUIRE(p.nu_pre >= 0 && p.nu_post >= 0 && p.coarse_maxit >= 0, CMX_ERR_ARG,
+              "multigrid: negative iteration count");
+  CMX_REQUIRE(p.coarse_nkrylov >= 1 && p.coarse_nkrylov <= MG_MAXKRY, CMX_ERR_ARG,
+              "multigrid: PrecondGCRNKrylov must be 1..32");
+  op.sync();  // a V-cycle in flight may still read the GCR directions
+  const size_t cv = sizeof(float2) * (size_t)st.m.Nb * st.m.D;
+  for (int j = (int)st.pk.size(); j < p.coarse_nkrylov; ++j) {
+    st.pk.emplace_back(cv);
+    st.qk.emplace_back(cv);
+  }
+  // the set-up fields (setup_maxit, seed) keep the values the null space was built with
+  const int setup_maxit = st.par.setup_maxit;
+  const unsigned long long seed = st.par.seed;
+  st.par = p;
+  st.par.setup_maxit = setup_maxit;
+  st.par.seed = seed;
+}
+
 void mg_info(Op& op, int* nb, int* d) {
   MGState& st = state(op);
   *nb = st.m.Nb;
@@ -342,7 +377,15 @@ void mg_get_coarse_op(Op& op, double* out) {
   const size_t nac = (size_t)st.m.Nb * MG_NST * st.m.D * st.m.D;
   std::vector<float2> h(nac);
   CMX_HIP(hipMemcpy(h.data(), st.Ac.p, sizeof(float2) * nac, hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < nac; ++i) out[2 * i] = h[i].x, out[2 * i + 1] = h[i].y;
+  // device entries are column-major; the host copy is [block][entry][row][column]
+  const size_t D = (size_t)st.m.D;
+  for (size_t e = 0; e < (size_t)st.m.Nb * MG_NST; ++e)
+    for (size_t r = 0; r < D; ++r)
+      for (size_t c = 0; c < D; ++c) {
+        const float2 v = h[(e * D + c) * D + r];
+        const size_t o = (e * D + r) * D + c;
+        out[2 * o] = v.x, out[2 * o + 1] = v.y;
+      }
 }
 
 void mg_restrict_host(Op& op, const double* full_in, double* coarse_out) {

@@ -26,6 +26,10 @@ struct MGGeom {
 
 struct MGVecs {
   const float4* P[MG_MAXVEC];  // null vectors: full lattice (even then odd), FP32, 6 chunks of 2 Vh
+  // FP16 copies in the library's FP16 fermion format (3 chunks of 2 Vh, one norm per site), read by the
+  // restriction and prolongation of an FP16 V-cycle (half the bytes); null when absent
+  const void* H[MG_MAXVEC];
+  const float* Hn[MG_MAXVEC];
 };
 
 // block orthonormalisation of the null vectors per (block, chirality), FP64 sums
@@ -45,7 +49,10 @@ void launch_mg_noise(VRef out, int n, unsigned long long seed, hipStream_t s);
 // each block's entries whose neighbour has colour col, from the block or its two faces in that direction
 void launch_mg_face_restrict(const MGGeom& m, const MGVecs& P, const int* colour, const int* nbr, int col, int dof,
                              VRef v, float2* Ac, hipStream_t s);
-// Ac[b][d][r][dof] = y[b][r] for the first stencil entry d whose block has colour col
+// The coarse operator Ac holds per block b and stencil entry d a D x D matrix, column-major
+// (element (r, c) at ((b * 9 + d) * D + c) * D + r), so the D lanes of a block read a column together.
+// mg_scatter: element (r, dof) of entry d of block b = y[b][r], for the first stencil entry d whose
+// block has colour col
 void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int col, int dof, const float2* y,
                        float2* Ac, hipStream_t s);
 // y = A_c x; skip (nullable): a device flag, the launch does nothing while it is set
@@ -80,6 +87,9 @@ void launch_mg_update(const MGUpdate& u, hipStream_t s);
 struct MGState;
 // Builds the null space, the block-orthonormal prolongator and the coarse operator on op
 void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res);
+// the V-cycle's parameters (smoothing steps, omega, coarse tolerance, iterations and directions) changed
+// on a set-up multigrid; blocking and nvec must equal the set-up's
+void mg_update(Op& op, const cmx_mg_params& p);
 // z = K v: one V-cycle on the odd-checkerboard system at precision prec (v, z in that precision)
 void mg_precondition(Op& op, int prec, const Vec& v, Vec& z);
 // test and inspection access: null vectors (host order, FP64 copies), the coarse operator, and the

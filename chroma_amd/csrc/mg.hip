@@ -190,10 +190,31 @@ struct RestrictArgs {
   float2* out;
 };
 
-template <int PR, bool FULL> __global__ void __launch_bounds__(256) k_mg_restrict(RestrictArgs A) {
+// the 24 reals of null vector k at full-lattice site hs: the FP32 vectors (PP = P32) or their FP16
+// copies (PP = P16); reals 0-11 are chirality +, 12-23 chirality -
+template <int PP>
+__device__ __forceinline__ void load_pvec(const MGVecs& P, int k, int stride, int hs, float (&pk)[24]) {
+  if constexpr (PP == P16) {
+    SpinorIO<P16>::load(P.H[k], P.Hn[k], stride, hs, pk);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      float h[12];
+      load_half(P.P[k], stride, hs, c, h);
+#pragma unroll
+      for (int q = 0; q < 12; ++q) pk[12 * c + q] = h[q];
+    }
+  }
+}
+
+template <int PR, int PP, bool FULL> __global__ void __launch_bounds__(256) k_mg_restrict(RestrictArgs A) {
   __shared__ float2 sh[MG_MAXD * 4], tot[MG_MAXD];
   const MGGeom& m = A.m;
-  const int b = blockIdx.x;
+  // work-group g runs on XCD g mod 8: each XCD takes a contiguous range of blocks, so neighbouring
+  // blocks along x, whose sites share the cache lines of the fine fields, follow each other on one L2
+  const int per = (m.Nb + 7) / 8;
+  const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (b >= m.Nb) return;  // the whole work-group: the padding of the last XCD's range
   const int stride = 2 * m.g.Vh;
   const int nsite = FULL ? m.bvol : m.bvol / 2;
   float2 acc[MG_MAXD];
@@ -216,14 +237,15 @@ template <int PR, bool FULL> __global__ void __launch_bounds__(256) k_mg_restric
 #pragma unroll
     for (int k = 0; k < MG_MAXVEC; ++k) {
       if (k < m.nvec) {
+        float pk[24];
+        load_pvec<PP>(A.P, k, stride, hs, pk);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          float pk[12];
-          load_half(A.P.P[k], stride, hs, c, pk);
           float re = 0.f, im = 0.f;
 #pragma unroll
           for (int q = 0; q < 6; ++q) {
-            const float ar = pk[2 * q], ai = pk[2 * q + 1], br = v[12 * c + 2 * q], bi = v[12 * c + 2 * q + 1];
+            const float ar = pk[12 * c + 2 * q], ai = pk[12 * c + 2 * q + 1], br = v[12 * c + 2 * q],
+                        bi = v[12 * c + 2 * q + 1];
             re += ar * br + ai * bi;
             im += ar * bi - ai * br;
           }
@@ -249,7 +271,7 @@ struct ProlongArgs {
   int add;
 };
 
-template <int PR, bool FULL> __global__ void __launch_bounds__(256) k_mg_prolong(ProlongArgs A) {
+template <int PR, int PP, bool FULL> __global__ void __launch_bounds__(256) k_mg_prolong(ProlongArgs A) {
   const MGGeom& m = A.m;
   const int n = FULL ? 2 * m.g.Vh : m.g.Vh;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,15 +286,15 @@ template <int PR, bool FULL> __global__ void __launch_bounds__(256) k_mg_prolong
 #pragma unroll
   for (int k = 0; k < MG_MAXVEC; ++k) {
     if (k < m.nvec) {
+      float pk[24];
+      load_pvec<PP>(A.P, k, stride, hs, pk);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const float2 w = e[c * m.nvec + k];
-        float pk[12];
-        load_half(A.P.P[k], stride, hs, c, pk);
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-          v[12 * c + 2 * q] += pk[2 * q] * w.x - pk[2 * q + 1] * w.y;
-          v[12 * c + 2 * q + 1] += pk[2 * q] * w.y + pk[2 * q + 1] * w.x;
+          v[12 * c + 2 * q] += pk[12 * c + 2 * q] * w.x - pk[12 * c + 2 * q + 1] * w.y;
+          v[12 * c + 2 * q + 1] += pk[12 * c + 2 * q] * w.y + pk[12 * c + 2 * q + 1] * w.x;
         }
       }
     }
@@ -312,7 +334,7 @@ __global__ void __launch_bounds__(256) k_mg_noise(VRef out, int n, unsigned long
 // the blocks of colour cl reaches a block b of another colour only through one direction mu, from
 // b + mu across b's +mu face and from b - mu across its -mu face: restricting M probe over the whole
 // block (colour cl: the block's own entry) or over each of the two faces separates the stencil entries.
-// One work-group per block; the entries go straight into Ac[b][d][.][dof].
+// One work-group per block; the entries go straight into column dof of Ac[b][d] (column-major).
 struct FaceArgs {
   MGGeom m;
   MGVecs P;
@@ -394,7 +416,7 @@ __global__ void __launch_bounds__(256) k_mg_face_restrict(FaceArgs A) {
     const int d = mu < 0 ? 0 : 1 + 2 * mu + reg;
     if ((int)threadIdx.x < m.D) {
       const int c = threadIdx.x / m.nvec, k = threadIdx.x % m.nvec;
-      A.Ac[(((size_t)b * MG_NST + d) * m.D + threadIdx.x) * m.D + A.dof] = tot[c * MG_MAXVEC + k];
+      A.Ac[(((size_t)b * MG_NST + d) * m.D + A.dof) * m.D + threadIdx.x] = tot[c * MG_MAXVEC + k];
     }
     __syncthreads();
   }
@@ -432,7 +454,7 @@ __global__ void __launch_bounds__(256) k_mg_scatter(MGGeom m, const int* nbr, co
   const int b = t / m.D, r = t % m.D;
   for (int d = 0; d < MG_NST; ++d) {
     if (colour[nbr[b * MG_NST + d]] == col) {
-      Ac[(((size_t)b * MG_NST + d) * m.D + r) * m.D + dof] = y[t];
+      Ac[(((size_t)b * MG_NST + d) * m.D + dof) * m.D + r] = y[t];
       return;
     }
   }
@@ -446,10 +468,11 @@ __global__ void __launch_bounds__(256) k_mg_coarse_apply(MGGeom m, const int* nb
   const int b = t / m.D, r = t % m.D;
   float re = 0.f, im = 0.f;
   for (int d = 0; d < MG_NST; ++d) {
-    const float2* a = Ac + (((size_t)b * MG_NST + d) * m.D + r) * m.D;
+    // column-major entry: the D lanes of a block read one contiguous column per step
+    const float2* a = Ac + ((size_t)b * MG_NST + d) * m.D * m.D + r;
     const float2* xv = x + (size_t)nbr[b * MG_NST + d] * m.D;
     for (int c = 0; c < m.D; ++c) {
-      const float2 av = a[c], xx = xv[c];
+      const float2 av = a[(size_t)c * m.D], xx = xv[c];
       re += av.x * xx.x - av.y * xx.y;
       im += av.x * xx.y + av.y * xx.x;
     }
@@ -572,15 +595,21 @@ void launch_mg_orth(const MGGeom& m, float4* const* P, hipStream_t s) {
 
 void launch_mg_restrict(const MGGeom& m, const MGVecs& P, int prec, VRef v, bool full, float2* out, hipStream_t s) {
   RestrictArgs A{m, P, v, out};
-  auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(m.Nb), dim3(256), 0, s, A); };
+  const int per = (m.Nb + 7) / 8;  // blocks per XCD (k_mg_restrict's work-group order)
+  // one thread per site of the block where that fits a work-group of 64-256 threads (a 4^4 block's odd
+  // sites: 128 threads, two waves, instead of four waves half idle)
+  const int nsite = full ? m.bvol : m.bvol / 2;
+  const int threads = std::min(256, std::max(64, (nsite + 63) / 64 * 64));
+  auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(8 * per), dim3(threads), 0, s, A); };
   if (full) {
-    if (prec == P64) go(k_mg_restrict<P64, true>);
-    else if (prec == P32) go(k_mg_restrict<P32, true>);
-    else go(k_mg_restrict<P16, true>);
+    if (prec == P64) go(k_mg_restrict<P64, P32, true>);
+    else if (prec == P32) go(k_mg_restrict<P32, P32, true>);
+    else go(k_mg_restrict<P16, P32, true>);
   } else {
-    if (prec == P64) go(k_mg_restrict<P64, false>);
-    else if (prec == P32) go(k_mg_restrict<P32, false>);
-    else go(k_mg_restrict<P16, false>);
+    if (prec == P64) go(k_mg_restrict<P64, P32, false>);
+    else if (prec == P32) go(k_mg_restrict<P32, P32, false>);
+    else if (P.H[0]) go(k_mg_restrict<P16, P16, false>);  // FP16 V-cycle: the FP16 copies
+    else go(k_mg_restrict<P16, P32, false>);
   }
 }
 
@@ -590,13 +619,14 @@ void launch_mg_prolong(const MGGeom& m, const MGVecs& P, int prec, const float2*
   const int g = grid_of(full ? 2L * m.g.Vh : (long)m.g.Vh);
   auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, A); };
   if (full) {
-    if (prec == P64) go(k_mg_prolong<P64, true>);
-    else if (prec == P32) go(k_mg_prolong<P32, true>);
-    else go(k_mg_prolong<P16, true>);
+    if (prec == P64) go(k_mg_prolong<P64, P32, true>);
+    else if (prec == P32) go(k_mg_prolong<P32, P32, true>);
+    else go(k_mg_prolong<P16, P32, true>);
   } else {
-    if (prec == P64) go(k_mg_prolong<P64, false>);
-    else if (prec == P32) go(k_mg_prolong<P32, false>);
-    else go(k_mg_prolong<P16, false>);
+    if (prec == P64) go(k_mg_prolong<P64, P32, false>);
+    else if (prec == P32) go(k_mg_prolong<P32, P32, false>);
+    else if (P.H[0]) go(k_mg_prolong<P16, P16, false>);  // FP16 V-cycle: the FP16 copies
+    else go(k_mg_prolong<P16, P32, false>);
   }
 }
 

@@ -146,6 +146,7 @@ def lib():
         "cmx_net_train": (ctypes.c_int, [_vp, ctypes.POINTER(TrainParams), ctypes.POINTER(TrainResult)]),
         "cmx_mg_params_default": (ctypes.c_int, [ctypes.POINTER(MGParams)]),
         "cmx_mg_setup": (ctypes.c_int, [_vp, ctypes.POINTER(MGParams), ctypes.POINTER(MGResult)]),
+        "cmx_mg_update": (ctypes.c_int, [_vp, ctypes.POINTER(MGParams)]),
         "cmx_mg_get_nullvecs": (ctypes.c_int, [_vp, _dp]),
         "cmx_mg_get_coarse_op": (ctypes.c_int, [_vp, _dp]),
         "cmx_mg_restrict": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -429,7 +430,21 @@ class Operator:
         r = MGResult()
         check(lib().cmx_mg_setup(self.h, ctypes.byref(p), ctypes.byref(r)))
         self._mg = (r.n_blocks, r.coarse_dof, p.nvec)
+        self._mg_par = MGParams.from_buffer_copy(p)
         return {f: getattr(r, f) for f, _ in MGResult._fields_}
+
+    def mg_update(self, params=None, **kw):
+        """Changes the V-cycle of the set-up multigrid (cmx_mg_update): nu_pre, nu_post, omega, coarse_tol,
+        coarse_maxit, coarse_nkrylov; params: MGParams (default: those of the last mg_setup or mg_update),
+        keyword overrides on top."""
+        base = params if params is not None else getattr(self, "_mg_par", None)
+        if base is None:
+            raise CmxError(CMX_ERR_STATE, "mg_update: mg_setup has not been called on this operator")
+        p = MGParams.from_buffer_copy(base)
+        for k, v in kw.items():
+            _mg_set(p, k, v)
+        check(lib().cmx_mg_update(self.h, ctypes.byref(p)))
+        self._mg_par = p
 
     def mg_nullvecs(self):
         nb, d, nvec = self._mg

@@ -115,7 +115,7 @@ FGMRES_CASES = [(cmx.PRECOND_LEARNED, cmx.HALF, 0, 0), (cmx.PRECOND_LEARNED, cmx
                 (cmx.PRECOND_LEARNED, cmx.SINGLE, 4, 1), (cmx.PRECOND_LEARNED, cmx.SINGLE, 0, 2),
                 (cmx.PRECOND_LEARNED, cmx.HALF, 0, 3), (cmx.PRECOND_LEARNED, cmx.HALF, 0, 4),
                 (cmx.PRECOND_MR, cmx.SINGLE, 4, -1), (cmx.PRECOND_NONE, cmx.DOUBLE, 4, -1),
-                (cmx.PRECOND_MG, cmx.SINGLE, 0, -1)]
+                (cmx.PRECOND_MG, cmx.SINGLE, 0, -1), (cmx.PRECOND_MG, cmx.HALF, 0, -1)]
 
 
 @pytest.mark.parametrize("precond,sloppy,ndefl,variant", FGMRES_CASES)

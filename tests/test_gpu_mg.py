@@ -7,7 +7,9 @@
   R, P from numpy -- an independent composition that a wrong colouring, stencil entry or scatter fails.
   Covered: coarse extents 2 (the +mu and -mu entries on one block) and 4 (face-restricted probes, 2^4
   colours), and odd extents 3 (whole-block probes with 3^3 x 4 colours).
-* FGMRES-DR with the V-cycle reaches the FP64 true residual 1e-10 in fewer iterations than without it.
+* FGMRES-DR with the V-cycle reaches the FP64 true residual 1e-10 in fewer iterations than without it;
+  cmx_mg_update changes the cycle without a new set-up, and restoring the set-up's cycle restores the
+  solution bit for bit; the FP16 V-cycle (FP16 copies of the null vectors) does the same.
 """
 import sys
 
@@ -136,3 +138,21 @@ def test_fgmres_mg_preconditioner():
     assert r_mg["rel_resid"] <= 1e-10
     assert rel(psi, psi0) < 1e-8
     assert r_mg["n_count"] < r_none["n_count"] / 2
+    # cmx_mg_update: a lighter V-cycle on the same null space, then back to the set-up's cycle (same
+    # solution bit for bit: the V-cycle is deterministic)
+    op.mg_update(nu_post=2, coarse_maxit=6, coarse_nkrylov=20)
+    psi2, r2 = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
+    say("  updated cycle: iterations", r2["n_count"], "resid", r2["rel_resid"])
+    assert r2["rel_resid"] <= 1e-10 and rel(psi2, psi0) < 1e-8
+    op.mg_update(nu_post=4, coarse_maxit=12, coarse_nkrylov=12)
+    psi3, r3 = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
+    assert r3["n_count"] == r_mg["n_count"] and np.array_equal(psi3, psi)
+    with pytest.raises(cmx.CmxError):
+        op.mg_update(nvec=4)
+    # the FP16 V-cycle (restriction and prolongation with the FP16 copies of the null vectors)
+    kw16 = dict(kw, sloppy_prec=cmx.HALF)
+    _, r16n = op.solve(chi, precond=cmx.PRECOND_NONE, **kw16)
+    psi16, r16 = op.solve(chi, precond=cmx.PRECOND_MG, **kw16)
+    say("  FP16: iterations none", r16n["n_count"], "mg", r16["n_count"], "resid", r16["rel_resid"])
+    assert r16["rel_resid"] <= 1e-10 and rel(psi16, psi0) < 1e-8
+    assert r16["n_count"] < r16n["n_count"] / 2

@@ -832,6 +832,7 @@ int cmx_mg_params_default(cmx_mg_params* p) {
     p->coarse_nkrylov = 12;
     p->setup_maxit = 500;
     p->seed = 1;
+    p->prec = CMX_SINGLE;
   });
 }
 

@@ -164,6 +164,8 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
               "multigrid: negative iteration count");
   CMX_REQUIRE(p.coarse_nkrylov >= 1 && p.coarse_nkrylov <= MG_MAXKRY, CMX_ERR_ARG,
               "multigrid: PrecondGCRNKrylov must be 1..32");
+  CMX_REQUIRE(p.prec == P64 || p.prec == P32 || p.prec == P16, CMX_ERR_ARG,
+              "multigrid: Precision must be DOUBLE, SINGLE or HALF");
   std::unique_ptr<MGState> st(new MGState());
   st->par = p;
   MGGeom& m = st->m;
@@ -184,19 +186,25 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
   const int Vh = (int)op.Vh;
   std::memset(&res, 0, sizeof(res));
   op.ensure_prec(P32);
+  const int rp = p.prec;  // storage precision of the relaxation
+  op.ensure_prec(rp);
 
   // null space: relaxed random vectors on the odd checkerboard, extended to the full lattice
-  Vec x(P32, Vh), zero(P32, Vh), r(P32, Vh), mr(P32, Vh), F(P64, 2 * Vh);
+  Vec x(rp, Vh), zero(rp, Vh), r(rp, Vh), mr(rp, Vh), F(P64, 2 * Vh);
+  Vec x32(P32, rp == P32 ? 0 : Vh);
   op.blas(B_ZERO, {&zero});
   for (int k = 0; k < m.nvec; ++k) {
-    launch_mg_noise(x.view(), Vh, p.seed * 0x100000001b3ull + 7919ull * (unsigned long long)(k + 1), op.stream);
-    MRPrecond relax(op, P32, Vh, p.setup_maxit, 0.0, 1.0, r, mr);
+    // the noise start in FP32, stored at the relaxation's precision
+    Vec& start = rp == P32 ? x : x32;
+    launch_mg_noise(start.view(), Vh, p.seed * 0x100000001b3ull + 7919ull * (unsigned long long)(k + 1), op.stream);
+    if (rp != P32) convert(op, Vh, x32.view(), P32, x.view(), rp);
+    MRPrecond relax(op, rp, Vh, p.setup_maxit, 0.0, 1.0, r, mr);
     relax.apply(x, zero);
-    op.apply(P32, mr.view(), x.view(), +1, nullptr);
+    op.apply(rp, mr.view(), x.view(), +1, nullptr);
     const double ax = norm2_host(op, mr), xx = norm2_host(op, x);
     CMX_REQUIRE(xx > 0, CMX_ERR_BREAKDOWN, "multigrid: a null vector relaxed to zero");
     res.nullvec_rel_residual_max = std::max(res.nullvec_rel_residual_max, std::sqrt(ax / xx));
-    convert(op, Vh, x.view(), P32, F.view(Vh), P64);
+    convert(op, Vh, x.view(), rp, F.view(Vh), P64);
     reconstruct_even64(op, F);
     st->P.emplace_back(new Vec(P32, 2 * Vh));
     convert(op, 2 * Vh, F.view(0), P64, st->P.back()->view(0), P32);

@@ -47,7 +47,7 @@ class MGParams(ctypes.Structure):
     _fields_ = [("blocking", ctypes.c_int * 4), ("nvec", ctypes.c_int), ("nu_pre", ctypes.c_int),
                 ("nu_post", ctypes.c_int), ("omega", ctypes.c_double), ("coarse_tol", ctypes.c_double),
                 ("coarse_maxit", ctypes.c_int), ("coarse_nkrylov", ctypes.c_int), ("setup_maxit", ctypes.c_int),
-                ("seed", ctypes.c_ulonglong), ("verbose", ctypes.c_int)]
+                ("seed", ctypes.c_ulonglong), ("verbose", ctypes.c_int), ("prec", ctypes.c_int)]
 
 
 class MGResult(ctypes.Structure):
@@ -689,8 +689,9 @@ def mg_params_from_xml(xml):
     """MULTIGRIDSolverParams read from a <MULTIGRIDParams> element, with the element names and defaults of
     quda_multigrid_params.cc:44-235, restricted to two levels (one Blocking entry; per-level arrays give
     their first-level value, or their coarse-level value for the coarse solver's CoarseResidual and
-    MaxCoarseIterations; a single value is broadcast as the reader does).  Elements without a counterpart
-    here (Precision, Reconstruct, SchwarzType, SetupOnGPU, ...) are accepted and ignored."""
+    MaxCoarseIterations; a single value is broadcast as the reader does).  Precision (HALF, SINGLE,
+    DOUBLE; the reader's DEFAULT keeps the default) sets the null-space relaxation's precision.  Elements
+    without a counterpart here (Reconstruct, SchwarzType, SetupOnGPU, ...) are accepted and ignored."""
     import xml.etree.ElementTree as ET
     root = ET.fromstring(xml) if isinstance(xml, str) else xml
     if root.tag != "MULTIGRIDParams":
@@ -743,6 +744,14 @@ def mg_params_from_xml(xml):
     e = root.find("Verbosity")
     if e is not None:
         p.verbose = 1 if e.text.strip().lower() == "true" else 0
+    e = root.find("Precision")
+    if e is not None and e.text is not None:
+        names = {"HALF": HALF, "SINGLE": SINGLE, "DOUBLE": DOUBLE}
+        t = e.text.strip().upper()
+        if t in names:
+            p.prec = names[t]
+        elif t != "DEFAULT":
+            raise ValueError(f"mg_params_from_xml: Precision {e.text.strip()}")
     return p
 
 

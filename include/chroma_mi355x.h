@@ -326,7 +326,7 @@ int cmx_net_train(cmx_op op, const cmx_train_params* p, cmx_train_result* res);
  * quda_multigrid_params.cc:44-235), behind the same role as QUDA_MULTIGRID_CLOVER_INVERTER's solver: the
  * right preconditioner of CMX_SOLVER_FGMRESDR (the outer flexible solver, OuterGCRNKrylov ~ NKrylov).
  * Null vectors: nvec random starts relaxed by setup_maxit MR steps on A x = 0 (A the even-odd operator,
- * FP32), extended to the full lattice by the even-odd reconstruction, block-orthonormalised per block and
+ * at precision prec), extended to the full lattice by the even-odd reconstruction, block-orthonormalised per block and
  * chirality.  Coarse operator: the Galerkin product P^dag M P of the unpreconditioned operator M (FP64
  * probes, FP32 storage), a 9-point block stencil, 2 nvec degrees of freedom per block.  V-cycle on the
  * odd checkerboard, A^-1 = (M^-1)_oo: nu_pre MR steps, restriction of the odd residual, GCR(coarse_nkrylov)
@@ -343,6 +343,8 @@ typedef struct {
   int setup_maxit;       /* MaxIterSubspaceCreate: MR relaxation steps per null vector (500)           */
   unsigned long long seed;
   int verbose;           /* Verbosity                                                                    */
+  int prec;              /* Precision: storage precision of the null-space relaxation, CMX_SINGLE        */
+                         /* (default) or CMX_HALF (half the bytes per MR step) or CMX_DOUBLE              */
 } cmx_mg_params;
 typedef struct {
   int n_blocks, coarse_dof;        /* coarse sites, degrees of freedom per coarse site (2 nvec)       */

@@ -17,8 +17,17 @@ def test_mg_params_from_reference_xml():
     assert list(p.blocking) == [4, 4, 4, 2]
     assert (p.nvec, p.nu_pre, p.nu_post, p.coarse_nkrylov, p.verbose) == (24, 0, 6, 16, 1)
     assert p.omega == pytest.approx(0.85) and p.coarse_tol == pytest.approx(0.25) and p.coarse_maxit == 8
+    assert p.prec == cmx.SINGLE
     d = cmx.mg_params()
     assert list(d.blocking) == [4, 4, 4, 4] and d.nvec == 16 and d.coarse_maxit == 12 and d.setup_maxit == 500
+    assert d.prec == cmx.SINGLE
+    for name, want in (("HALF", cmx.HALF), ("DOUBLE", cmx.DOUBLE), ("DEFAULT", cmx.SINGLE)):
+        q = cmx.mg_params_from_xml(f"<MULTIGRIDParams><Precision>{name}</Precision>"
+                                   "<Blocking><elem>4 4 4 4</elem></Blocking></MULTIGRIDParams>")
+        assert q.prec == want
+    with pytest.raises(ValueError):
+        cmx.mg_params_from_xml("<MULTIGRIDParams><Precision>QUARTER</Precision>"
+                               "<Blocking><elem>4 4 4 4</elem></Blocking></MULTIGRIDParams>")
     with pytest.raises(ValueError):
         cmx.mg_params_from_xml("<MULTIGRIDParams><Blocking><elem>4 4 4 4</elem><elem>2 2 2 2</elem>"
                                "</Blocking></MULTIGRIDParams>")

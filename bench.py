@@ -241,7 +241,7 @@ def train_net(cmx, mass, csw, a, depth=1, pairs_per_source=4, lr=1e-3):
 FGMRES_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, basis_prec=2, n_krylov=16, cycle_reduction=3e-3)
 # two-level multigrid: 4^4 blocks, 8 null vectors (16 coarse degrees of freedom per block), MR post-smoothing
 # at FP16 storage, coarse GCR to 0.1 or 8 steps; the outer FGMRES-DR restarts every 8 steps
-MG_KW = dict(nvec=8, nu_pre=0, nu_post=6, setup_maxit=300, coarse_tol=0.1, coarse_maxit=8, seed=3)
+MG_KW = dict(nvec=8, nu_pre=0, nu_post=6, setup_maxit=100, coarse_tol=0.1, coarse_maxit=8, seed=3)
 FGMRES_MG_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, n_krylov=8)
 
 
@@ -354,8 +354,9 @@ def near_critical_solve(cmx, gauge, L, kappa, net):
 def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, variants=False):
     """The 12 spin-colour point sources of a propagator on the clover operator at kappa (Haar + 6 stout
     steps, seed 1), all to the FP64 true residual 1e-10: plain mixed-precision CG one after another and
-    batched, against the learned FGMRES-DR one after another (with `variants`: also FGMRES-DR without a
-    preconditioner and with the network's untrained Neumann start)."""
+    batched, against the learned FGMRES-DR one after another and batched (with `variants`: also FGMRES-DR
+    without a preconditioner and with the network's untrained Neumann start, and FGMRES-DR(8) with the
+    two-level multigrid, one after another and batched)."""
     import numpy as np
     mass = cmx.kappa_to_mass(kappa)
     t0 = time.time()
@@ -412,6 +413,27 @@ def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, varian
                                      "worst_true_rel_resid": max(r["rel_resid"] for r in rbl),
                                      "arnoldi_ticks": summ["n_iter"]}
     del psb
+    if variants:
+        # the two-level multigrid on the same sources (one set-up for all twelve), sequential and batched
+        info = op.mg_setup(blocking=(4, 4, 4, 4), **MG_KW)
+        kwm = dict(FGMRES_MG_KW, solver=cmx.FGMRESDR)
+        op.solve(chis[0], precond=cmx.PRECOND_MG, **kwm)
+        res = [op.solve(c, precond=cmx.PRECOND_MG, **kwm)[1] for c in chis]
+        out["multigrid_fgmres_sequential"] = {"seconds": round(sum(r["solve_seconds"] for r in res), 4),
+                                              "iterations": [r["n_count"] for r in res],
+                                              "worst_true_rel_resid": max(r["rel_resid"] for r in res),
+                                              "setup_seconds": round(info["setup_seconds"], 3)}
+        psb, rbm, summ = cmx.solve_batch([op] * 12, chis, precond=cmx.PRECOND_MG, **kwm)
+        out["multigrid_fgmres_batched"] = {"seconds": round(summ["solve_seconds"], 4),
+                                           "iterations": [r["n_count"] for r in rbm],
+                                           "worst_true_rel_resid": max(r["rel_resid"] for r in rbm),
+                                           "arnoldi_ticks": summ["n_iter"]}
+        del psb
+        out["speedup_batched_multigrid_vs_plain_batched"] = round(
+            bsecs / out["multigrid_fgmres_batched"]["seconds"], 3)
+        print(f"#   multigrid: set-up {info['setup_seconds']:.2f} s, sequential "
+              f"{out['multigrid_fgmres_sequential']['seconds']:.3f} s, batched "
+              f"{out['multigrid_fgmres_batched']['seconds']:.3f} s", file=sys.stderr, flush=True)
     ls = out["learned_fgmres_sequential"]["seconds"]
     lb = out["learned_fgmres_batched"]["seconds"]
     out["network_blocks"] = net.depth()

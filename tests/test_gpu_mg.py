@@ -121,6 +121,35 @@ def test_mg_prolongator_and_coarse_operator(L, B):
     assert rel(got, want) < 1e-4
 
 
+def test_mg_half_precision_relaxation():
+    """Null vectors relaxed in FP16 storage (Precision HALF): the aggregates stay orthonormal, the coarse
+    operator is still R M P of the downloaded vectors, and the V-cycle preconditions FGMRES-DR to 1e-10."""
+    L, B = (8, 8, 8, 16), (4, 4, 4, 4)
+    U, mass, op = make(L, 3003, 0.128)
+    nvec = 4
+    info = op.mg_setup(blocking=B, nvec=nvec, setup_maxit=60, seed=5, prec=cmx.HALF, nu_pre=0, nu_post=4,
+                       coarse_tol=0.1, coarse_maxit=8)
+    say("  half-precision setup", info)
+    blk, nb = host_blocks(L, B)
+    P = op.mg_nullvecs()
+    for c in range(2):
+        sl = slice(2 * c, 2 * c + 2)
+        for b in range(nb):
+            M = P[:, blk == b][:, :, sl, :].reshape(nvec, -1)
+            assert np.max(np.abs(np.conj(M) @ M.T - np.eye(nvec))) < 1e-5
+    rng = np.random.default_rng(11)
+    e = rng.standard_normal((nb, 2 * nvec)) + 1j * rng.standard_normal((nb, 2 * nvec))
+    ref = oracle.Op(oracle.EO_CLOVER, L, U, mass, 1.0, 1.0)
+    want = numpy_restrict(P, blk, nb, ref.piece("unprec", numpy_prolong(P, blk, e)))
+    assert rel(op.mg_coarse_apply(e), want) < 1e-4
+    Vh = int(np.prod(L)) // 2
+    chi = cmx.z2_source((Vh, 4, 3), 4)
+    kw = dict(solver=cmx.FGMRESDR, rsd=1e-10, maxit=2000, sloppy_prec=cmx.HALF, n_krylov=8)
+    _, r = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
+    say("  iterations", r["n_count"], "resid", r["rel_resid"])
+    assert r["rel_resid"] <= 1e-10
+
+
 def test_fgmres_mg_preconditioner():
     L = (16, 16, 16, 32)
     say("fgmres + mg", L)

@@ -351,12 +351,12 @@ def near_critical_solve(cmx, gauge, L, kappa, net):
     return out
 
 
-def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, variants=False):
+def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, variants=False, multigrid=False):
     """The 12 spin-colour point sources of a propagator on the clover operator at kappa (Haar + 6 stout
     steps, seed 1), all to the FP64 true residual 1e-10: plain mixed-precision CG one after another and
     batched, against the learned FGMRES-DR one after another and batched (with `variants`: also FGMRES-DR
-    without a preconditioner and with the network's untrained Neumann start, and FGMRES-DR(8) with the
-    two-level multigrid, one after another and batched)."""
+    without a preconditioner and with the network's untrained Neumann start; with `multigrid`: FGMRES-DR(8)
+    with the two-level multigrid, one after another and batched)."""
     import numpy as np
     mass = cmx.kappa_to_mass(kappa)
     t0 = time.time()
@@ -413,7 +413,7 @@ def learned_vs_plain_sources(cmx, L, kappa, csw, a, net=None, gauge=None, varian
                                      "worst_true_rel_resid": max(r["rel_resid"] for r in rbl),
                                      "arnoldi_ticks": summ["n_iter"]}
     del psb
-    if variants:
+    if multigrid:
         # the two-level multigrid on the same sources (one set-up for all twelve), sequential and batched
         info = op.mg_setup(blocking=(4, 4, 4, 4), **MG_KW)
         kwm = dict(FGMRES_MG_KW, solver=cmx.FGMRESDR)
@@ -754,7 +754,8 @@ def main():
                 "headline_lattice": near_critical_solve(cmx, gauge, L, a.near_kappa, net_nc)}
         NSL = tuple(int(v) for v in a.near_sources_lattice.split(",")) if a.near_sources_lattice else LS
         sources_nc = learned_vs_plain_sources(cmx, NSL, a.near_kappa, csw, a, net_nc,
-                                              gauge=gauge_s if NSL == LS else None, variants=NSL == LS)
+                                              gauge=gauge_s if NSL == LS else None, variants=NSL == LS,
+                                              multigrid=True)
     del gauge
 
     batched = None

@@ -26,6 +26,7 @@ struct MGState {
   std::vector<std::unique_ptr<Vec>> P16;  // their FP16 copies (restriction and prolongation at FP16)
   MGVecs pv{};
   DevBuf Ac, nbr, colour;
+  DevBuf Ah;  // FP16 copy of Ac (the coarse solve of an FP16 V-cycle)
   // coarse GCR work space
   DevBuf xc, rc, bc, part, S, F;
   std::vector<DevBuf> pk, qk;
@@ -107,7 +108,7 @@ double norm2_host(Op& op, const Vec& v) {
 
 // x = A_c^-1 b by GCR(nk) with restarts, to |r| <= tol |b| or maxit steps (x, b: coarse vectors).  Every
 // scalar stays on the device: the kernels after convergence return at once, and the host never waits.
-void coarse_gcr(MGState& st, Op& op, const float2* b, float2* x) {
+void coarse_gcr(MGState& st, Op& op, const float2* b, float2* x, bool half) {
   const MGGeom& m = st.m;
   const int n = m.Nb * m.D;
   const int nk = st.par.coarse_nkrylov;
@@ -128,7 +129,10 @@ void coarse_gcr(MGState& st, Op& op, const float2* b, float2* x) {
     float2* pj = st.pk[j].as<float2>();
     float2* qj = st.qk[j].as<float2>();
     CMX_HIP(hipMemcpyAsync(pj, r, sizeof(float2) * n, hipMemcpyDeviceToDevice, op.stream));
-    launch_mg_coarse_apply(m, st.nbr.as<int>(), st.Ac.as<float2>(), pj, qj, skip, op.stream);
+    if (half)
+      launch_mg_coarse_apply_h(m, st.nbr.as<int>(), st.Ah.p, pj, qj, skip, op.stream);
+    else
+      launch_mg_coarse_apply(m, st.nbr.as<int>(), st.Ac.as<float2>(), pj, qj, skip, op.stream);
     // <q_i, q_j> (i < j), <q_j, q_j>, conj(<q_j, r>) in one pass
     const float2* vs[MG_MAXKRY + 2];
     for (int i = 0; i <= j; ++i) vs[i] = st.qk[i].as<float2>();
@@ -297,6 +301,12 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
     st->pv.Hn[k] = st->P16.back()->nrm();
   }
 
+  // the coarse operator's FP16 copy: an FP16 V-cycle's coarse solve reads half the bytes (the solve
+  // stops at a relative residual of coarse_tol, far above FP16 rounding)
+  st->Ah.alloc(sizeof(float2) / 2 * nac);
+  launch_mg_to_half((long)nac, st->Ac.as<float2>(), st->Ah.p, op.stream);
+  CMX_HIP(hipGetLastError());
+
   // coarse GCR work space
   const size_t cv = sizeof(float2) * (size_t)m.Nb * m.D;
   st->xc.alloc(cv);
@@ -337,7 +347,7 @@ void mg_precondition(Op& op, int prec, const Vec& v, Vec& z) {
     res = &t;
   }
   launch_mg_restrict(m, st.pv, prec, res->view(), false, st.bc.as<float2>(), op.stream);
-  coarse_gcr(st, op, st.bc.as<float2>(), st.xc.as<float2>());
+  coarse_gcr(st, op, st.bc.as<float2>(), st.xc.as<float2>(), prec == P16);
   launch_mg_prolong(m, st.pv, prec, st.xc.as<float2>(), z.view(), false, true, op.stream);
   CMX_HIP(hipGetLastError());
   if (st.par.nu_post > 0) MRPrecond(op, prec, n, st.par.nu_post, 0.0, st.par.omega, r, mr).apply(z, v);

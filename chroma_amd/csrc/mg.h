@@ -58,6 +58,11 @@ void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int c
 // y = A_c x; skip (nullable): a device flag, the launch does nothing while it is set
 void launch_mg_coarse_apply(const MGGeom& m, const int* nbr, const float2* Ac, const float2* x, float2* y,
                             const int* skip, hipStream_t s);
+// the same with the entries' FP16 copy Ah (complex half pairs, same layout), for an FP16 V-cycle
+void launch_mg_coarse_apply_h(const MGGeom& m, const int* nbr, const void* Ah, const float2* x, float2* y,
+                              const int* skip, hipStream_t s);
+// h[i] = a[i] rounded to FP16 (n complex entries)
+void launch_mg_to_half(long n, const float2* a, void* h, hipStream_t s);
 // coarse vectors (n complex): partials[j][block] = sum over the block's elements of conj(v_j) w, j < m
 int mg_cdot_blocks(int n);
 void launch_mg_cdot(const float2* const* v, int m, const float2* w, int n, double2* partials, const int* skip,

@@ -460,7 +460,12 @@ __global__ void __launch_bounds__(256) k_mg_scatter(MGGeom m, const int* nbr, co
   }
 }
 
-__global__ void __launch_bounds__(256) k_mg_coarse_apply(MGGeom m, const int* nbr, const float2* Ac, const float2* x,
+__device__ __forceinline__ float2 ac_load(const float2* a, size_t i) { return a[i]; }
+__device__ __forceinline__ float2 ac_load(const __half2* a, size_t i) { return __half22float2(a[i]); }
+
+// AT: float2 (the set-up's FP32 entries) or __half2 (their FP16 copy, read by an FP16 V-cycle)
+template <typename AT>
+__global__ void __launch_bounds__(256) k_mg_coarse_apply(MGGeom m, const int* nbr, const AT* Ac, const float2* x,
                                                          float2* y, const int* skip) {
   if (skip && *skip) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,15 +474,20 @@ __global__ void __launch_bounds__(256) k_mg_coarse_apply(MGGeom m, const int* nb
   float re = 0.f, im = 0.f;
   for (int d = 0; d < MG_NST; ++d) {
     // column-major entry: the D lanes of a block read one contiguous column per step
-    const float2* a = Ac + ((size_t)b * MG_NST + d) * m.D * m.D + r;
+    const AT* a = Ac + ((size_t)b * MG_NST + d) * m.D * m.D + r;
     const float2* xv = x + (size_t)nbr[b * MG_NST + d] * m.D;
     for (int c = 0; c < m.D; ++c) {
-      const float2 av = a[(size_t)c * m.D], xx = xv[c];
+      const float2 av = ac_load(a, (size_t)c * m.D), xx = xv[c];
       re += av.x * xx.x - av.y * xx.y;
       im += av.x * xx.y + av.y * xx.x;
     }
   }
   y[t] = make_float2(re, im);
+}
+
+__global__ void __launch_bounds__(256) k_mg_to_half(long n, const float2* a, __half2* h) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) h[i] = __float22half2_rn(a[i]);
 }
 
 // ---- coarse BLAS ------------------------------------------------------------------------------------
@@ -653,7 +663,18 @@ void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int c
 
 void launch_mg_coarse_apply(const MGGeom& m, const int* nbr, const float2* Ac, const float2* x, float2* y,
                             const int* skip, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_coarse_apply, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr, Ac, x, y, skip);
+  hipLaunchKernelGGL(k_mg_coarse_apply<float2>, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr, Ac, x, y,
+                     skip);
+}
+
+void launch_mg_coarse_apply_h(const MGGeom& m, const int* nbr, const void* Ah, const float2* x, float2* y,
+                              const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_coarse_apply<__half2>, dim3(grid_of((long)m.Nb * m.D)), dim3(256), 0, s, m, nbr,
+                     static_cast<const __half2*>(Ah), x, y, skip);
+}
+
+void launch_mg_to_half(long n, const float2* a, void* h, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_to_half, dim3(grid_of(n)), dim3(256), 0, s, n, a, static_cast<__half2*>(h));
 }
 
 int mg_cdot_blocks(int n) { return std::max(1, std::min(256, grid_of(n))); }

@@ -32,7 +32,6 @@
 namespace cmx {
 
 int kry_blocks(int n);
-void learned_precond_apply(Op& op, int prec, const Vec& in, Vec& out);
 
 namespace {
 

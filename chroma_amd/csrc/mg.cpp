@@ -170,6 +170,7 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res) {
               "multigrid: PrecondGCRNKrylov must be 1..32");
   CMX_REQUIRE(p.prec == P64 || p.prec == P32 || p.prec == P16, CMX_ERR_ARG,
               "multigrid: Precision must be DOUBLE, SINGLE or HALF");
+  CMX_REQUIRE(p.smoother == 0 || p.smoother == 1, CMX_ERR_ARG, "multigrid: smoother must be 0 (MR) or 1 (network)");
   std::unique_ptr<MGState> st(new MGState());
   st->par = p;
   MGGeom& m = st->m;
@@ -350,6 +351,12 @@ void mg_precondition(Op& op, int prec, const Vec& v, Vec& z) {
   coarse_gcr(st, op, st.bc.as<float2>(), st.xc.as<float2>(), prec == P16);
   launch_mg_prolong(m, st.pv, prec, st.xc.as<float2>(), z.view(), false, true, op.stream);
   CMX_HIP(hipGetLastError());
+  if (st.par.smoother == 1) {  // the learned network on the residual of the coarse correction
+    op.apply(prec, mr.view(), z.view(), +1, nullptr);
+    op.blas(B_XMY_NORM, {&t, &v, &mr});  // t = v - A z
+    learned_precond_apply(op, prec, t, r);
+    op.blas(B_ADD, {&z, &r});            // z += N t
+  }
   if (st.par.nu_post > 0) MRPrecond(op, prec, n, st.par.nu_post, 0.0, st.par.omega, r, mr).apply(z, v);
 }
 
@@ -364,6 +371,7 @@ void mg_update(Op& op, const cmx_mg_params& p) {
               "multigrid: negative iteration count");
   CMX_REQUIRE(p.coarse_nkrylov >= 1 && p.coarse_nkrylov <= MG_MAXKRY, CMX_ERR_ARG,
               "multigrid: PrecondGCRNKrylov must be 1..32");
+  CMX_REQUIRE(p.smoother == 0 || p.smoother == 1, CMX_ERR_ARG, "multigrid: smoother must be 0 (MR) or 1 (network)");
   op.sync();  // a V-cycle in flight may still read the GCR directions
   const size_t cv = sizeof(float2) * (size_t)st.m.Nb * st.m.D;
   for (int j = (int)st.pk.size(); j < p.coarse_nkrylov; ++j) {

@@ -171,6 +171,8 @@ void net_apply_host(Op& op, int prec, double* out, const double* in);
 void net_set_variant(Op& op, int prec, int variant);
 void net_grad_host(Op& op, int prec, const double* v, const double* e, double* loss, double* gwc, double* gw0);
 void net_train(Op& op, const cmx_train_params& tp, cmx_train_result& res);
+// out = P(in): the attached network applied to one vector at precision prec (the learned preconditioner)
+void learned_precond_apply(Op& op, int prec, const Vec& in, Vec& out);
 // P(in[i]) -> out[i] for several sources: one launch per kernel and block for all of them, each source's
 // result equal to learned_precond_apply's bit for bit (fgmres uses it for the batched solves)
 void learned_precond_apply_batch(Op& op, int prec, const std::vector<const Vec*>& in, const std::vector<Vec*>& out);

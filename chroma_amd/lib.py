@@ -47,7 +47,8 @@ class MGParams(ctypes.Structure):
     _fields_ = [("blocking", ctypes.c_int * 4), ("nvec", ctypes.c_int), ("nu_pre", ctypes.c_int),
                 ("nu_post", ctypes.c_int), ("omega", ctypes.c_double), ("coarse_tol", ctypes.c_double),
                 ("coarse_maxit", ctypes.c_int), ("coarse_nkrylov", ctypes.c_int), ("setup_maxit", ctypes.c_int),
-                ("seed", ctypes.c_ulonglong), ("verbose", ctypes.c_int), ("prec", ctypes.c_int)]
+                ("seed", ctypes.c_ulonglong), ("verbose", ctypes.c_int), ("prec", ctypes.c_int),
+                ("smoother", ctypes.c_int)]
 
 
 class MGResult(ctypes.Structure):

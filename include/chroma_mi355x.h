@@ -345,6 +345,9 @@ typedef struct {
   int verbose;           /* Verbosity                                                                    */
   int prec;              /* Precision: storage precision of the null-space relaxation, CMX_SINGLE        */
                          /* (default) or CMX_HALF (half the bytes per MR step) or CMX_DOUBLE              */
+  int smoother;          /* post-smoother: 0 nu_post MR steps (default); 1 the operator's learned network */
+                         /* (cmx_net_attach) applied once to the residual of the coarse correction, then */
+                         /* nu_post MR steps.  No counterpart in the reference.                           */
 } cmx_mg_params;
 typedef struct {
   int n_blocks, coarse_dof;        /* coarse sites, degrees of freedom per coarse site (2 nvec)       */

@@ -123,7 +123,8 @@ def test_mg_prolongator_and_coarse_operator(L, B):
 
 def test_mg_half_precision_relaxation():
     """Null vectors relaxed in FP16 storage (Precision HALF): the aggregates stay orthonormal, the coarse
-    operator is still R M P of the downloaded vectors, and the V-cycle preconditions FGMRES-DR to 1e-10."""
+    operator is still R M P of the downloaded vectors, and the V-cycle preconditions FGMRES-DR to 1e-10,
+    with the MR smoother and with the learned network as the post-smoother."""
     L, B = (8, 8, 8, 16), (4, 4, 4, 4)
     U, mass, op = make(L, 3003, 0.128)
     nvec = 4
@@ -148,6 +149,18 @@ def test_mg_half_precision_relaxation():
     _, r = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
     say("  iterations", r["n_count"], "resid", r["rel_resid"])
     assert r["rel_resid"] <= 1e-10
+    # the learned network as the post-smoother (its Neumann start), alone and followed by one MR step
+    net = cmx.Net()
+    net.set_depth(2)
+    net.init_neumann(4.0 + mass)
+    cmx.net_attach(op, net)
+    for nu in (0, 1):
+        op.mg_update(smoother=1, nu_post=nu)
+        _, rs = op.solve(chi, precond=cmx.PRECOND_MG, **kw)
+        say("  network smoother, nu_post", nu, "iterations", rs["n_count"], "resid", rs["rel_resid"])
+        assert rs["rel_resid"] <= 1e-10
+    with pytest.raises(cmx.CmxError):
+        op.mg_update(smoother=2)
 
 
 def test_fgmres_mg_preconditioner():

@@ -373,6 +373,7 @@ struct HopLaunch {
   int x_stride;
   double a, b;
   double* partials;  // |out|^2 per work-group (nullable)
+  int ndot;          // with partials and an xpay epilogue: |out|^2, Re <out, x>, Im <out, x> per work-group
   const int* skip;   // device flag: return immediately when *skip != 0 (nullable)
   const int* run;    // device flag: run only when *run != 0 (nullable)
   // launch shape (chosen by the operator's tuner, see hop_config)

@@ -30,9 +30,14 @@ struct MRPrecond {
     op.blas(B_XMY_NORM, {&r, &v, &mr});      // r = chi - M psi
     op.finalize(nb, 1, PG_MR_INIT);
     for (int k = 0; k < maxit; ++k) {
-      op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
-      op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
-      op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
+      if (op.eo()) {  // M r with |Mr|^2 and <Mr, r> from the operator's second kernel
+        const int nh = op.apply(ps, mr.view(), r.view(), +1, op.partials.as<double>(), F + F_MRDONE, nullptr, true);
+        op.finalize(nh, 3, PG_MR_A);                                   // a = omega c / d
+      } else {
+        op.apply(ps, mr.view(), r.view(), +1, nullptr, F + F_MRDONE);  // M r
+        op.blas(B_TNORM_TDOT, {&mr, &r}, 0, 0, F_MRDONE);               // |Mr|^2, <Mr, r>
+        op.finalize(nb, 3, PG_MR_A);                                     // a = omega c / d
+      }
       op.blas(B_MR_UPDATE, {&z, &r, &mr}, S_MR + 3, S_MR + 5, F_MRDONE);  // psi += a r ; r -= a M r
       op.finalize(nb, 1, PG_MR_R);
     }

@@ -488,7 +488,7 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
   }
   const int plane = t * Lz + z;
   const int r = bx * blockDim.x + threadIdx.x;
-  double nsq = 0.0;
+  double nsq = 0.0, dre = 0.0, dim = 0.0;
   if (r < Pl) {
     int y = (int)((float)r * h.inv_lxh);
     int xh = r - y * Lxh;
@@ -567,12 +567,16 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
       const R a = (R)h.a;
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = fmar(a, xs[k], b * acc[k]);
+      if constexpr (NORM)
+        if (h.ndot) out_dot_x(out, xs, dre, dim);
     } else {
       R xs[24];
       SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
       clover_apply_site<P, LCP>(h.clov, Vh, i, xs, out);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = fmar(b, acc[k], out[k]);
+      if constexpr (NORM)
+        if (h.ndot) out_dot_x(out, xs, dre, dim);
     }
     SpinorIO<P>::store(h.out, h.out_nrm, h.out_stride, i, out);
     if constexpr (NORM) {
@@ -580,11 +584,7 @@ __device__ __forceinline__ void hop_fast_body(const HopLaunch& h, int blk) {
       for (int k = 0; k < 24; ++k) nsq += (double)out[k] * (double)out[k];
     }
   }
-  if constexpr (NORM) {
-    double v[1] = {nsq};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) h.partials[blk] = v[0];
-  }
+  if constexpr (NORM) write_norm_partials(h, blk, nsq, dre, dim);
 }
 
 template <int P, int ISIGN, int EPI, bool NORM, int LCP>

@@ -423,13 +423,41 @@ __device__ __forceinline__ int hop_site_index(const HopLaunch& h, int blk, int n
 // work-group blk of nblk of the launch h
 // LCP: cache policy of the loads of data read once per launch (links, clover, xpay input); the
 // double-buffered (PF) path loads its links with the default policy
+// <out, x> of one site (conj(out) x, FP64 accumulation) into (dre, dim)
+template <typename R>
+__device__ __forceinline__ void out_dot_x(const R (&out)[24], const R (&xs)[24], double& dre, double& dim) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const double orr = out[2 * k], oi = out[2 * k + 1], xr = xs[2 * k], xi = xs[2 * k + 1];
+    dre += orr * xr + oi * xi;
+    dim += orr * xi - oi * xr;
+  }
+}
+// the work-group's reductions: |out|^2 (h.partials[blk]), or with h.ndot |out|^2, Re <out, x>, Im <out, x>
+// (h.partials[3 blk + 0..2], the layout the finalize kernel reads for three reductions)
+__device__ __forceinline__ void write_norm_partials(const HopLaunch& h, int blk, double nsq, double dre, double dim) {
+  if (h.ndot) {
+    double v[3] = {nsq, dre, dim};
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+      h.partials[3 * blk] = v[0];
+      h.partials[3 * blk + 1] = v[1];
+      h.partials[3 * blk + 2] = v[2];
+    }
+  } else {
+    double v[1] = {nsq};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) h.partials[blk] = v[0];
+  }
+}
+
 template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>
 __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) {
   using R = typename SpinorIO<P>::R;
   if (h.skip && *h.skip) return;
   if (h.run && !*h.run) return;
   const int i = hop_site_index(h, blk, nblk);
-  double nsq = 0.0;
+  double nsq = 0.0, dre = 0.0, dim = 0.0;
   if (i < h.g.Vh) {
     int x[4];
     {
@@ -482,12 +510,16 @@ __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) 
       const R a = R(h.a);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] = a * xs[k] + b * acc[k];
+      if constexpr (NORM)
+        if (h.ndot) out_dot_x(out, xs, dre, dim);
     } else {
       R xs[24];
       SpinorIO<P>::template load<LCP>(h.x, h.x_nrm, h.x_stride, i, xs);
       clover_apply_site<P, LCP>(h.clov, h.g.Vh, i, xs, out);
 #pragma unroll
       for (int k = 0; k < 24; ++k) out[k] += b * acc[k];
+      if constexpr (NORM)
+        if (h.ndot) out_dot_x(out, xs, dre, dim);
     }
     SpinorIO<P>::store(h.out, h.out_nrm, h.out_stride, i, out);
     if constexpr (NORM) {
@@ -496,11 +528,7 @@ __device__ __forceinline__ void hop_body(const HopLaunch& h, int blk, int nblk) 
       for (int k = 0; k < 24; ++k) nsq += (double)out[k] * (double)out[k];
     }
   }
-  if constexpr (NORM) {
-    double v[1] = {nsq};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) h.partials[blk] = v[0];
-  }
+  if constexpr (NORM) write_norm_partials(h, blk, nsq, dre, dim);
 }
 
 template <int P, int RECON, int ISIGN, int EPI, bool NORM, bool PF, int LCP>

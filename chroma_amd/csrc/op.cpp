@@ -325,7 +325,8 @@ static void set_x(HopLaunch& h, VRef r) {
   h.x = r.v, h.x_nrm = r.nrm, h.x_stride = r.stride;
 }
 
-int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* skip, const int* run) {
+int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* skip, const int* run, bool ndot) {
+  CMX_REQUIRE(!ndot || (parts && eo()), CMX_ERR_ARG, "apply: <chi, psi> partials need an even-odd operator");
   ensure_prec(prec);
   HopLaunch h = hop_base(prec);
   const int nb = hop_num_blocks(h);
@@ -354,6 +355,7 @@ int Op::apply(int prec, VRef out, VRef in, int isign, double* parts, const int* 
     h.a = fact;
     h.b = clover() ? -0.25 : -0.25 / fact;
     h.partials = parts;
+    h.ndot = ndot ? 1 : 0;  // x = psi_o: <chi_o, psi_o> from the values the epilogue holds
     launch_hop(h, stream);
     return parts ? nb : 0;
   }

@@ -130,10 +130,11 @@ struct Op {
   Vec& tmp(int prec, int k);
   HopLaunch hop_base(int prec, int which = 0) const;
 
-  // chi = M(isign) psi on the subset; partials (optional) receives |chi|^2 per work-group;
-  // returns the number of partials written.
+  // chi = M(isign) psi on the subset; partials (optional) receives |chi|^2 per work-group, or with ndot
+  // (even-odd operators) |chi|^2, Re <chi, psi>, Im <chi, psi> per work-group (three partials each);
+  // returns the number of work-groups whose partials were written.
   int apply(int prec, VRef out, VRef in, int isign, double* partials, const int* skip = nullptr,
-            const int* run = nullptr);
+            const int* run = nullptr, bool ndot = false);
   // raw hopping term on output checkerboard cb (views of one checkerboard each), scaled by b
   void dslash(int prec, VRef out, VRef in, int isign, int cb, double b = 1.0);
   // the same on n sources at once (FP16 storage: links loaded once per group of sources); false when

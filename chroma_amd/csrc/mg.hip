@@ -367,7 +367,10 @@ __device__ __forceinline__ int face_site(const MGGeom& m, int b, int mu, int sid
 __global__ void __launch_bounds__(256) k_mg_face_restrict(FaceArgs A) {
   __shared__ float2 sh[MG_MAXD * 4], tot[MG_MAXD];
   const MGGeom& m = A.m;
-  const int b = blockIdx.x;
+  // each XCD takes a contiguous range of blocks (the order of k_mg_restrict)
+  const int per = (m.Nb + 7) / 8;
+  const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (b >= m.Nb) return;  // the whole work-group: the padding of the last XCD's range
   const int stride = 2 * m.g.Vh;
   int mu = -1;
   if (A.colour[b] != A.col) {
@@ -653,7 +656,8 @@ void launch_mg_noise(VRef out, int n, unsigned long long seed, hipStream_t s) {
 void launch_mg_face_restrict(const MGGeom& m, const MGVecs& P, const int* colour, const int* nbr, int col, int dof,
                              VRef v, float2* Ac, hipStream_t s) {
   FaceArgs A{m, P, colour, nbr, col, dof, v, Ac};
-  hipLaunchKernelGGL(k_mg_face_restrict, dim3(m.Nb), dim3(256), 0, s, A);
+  const int per = (m.Nb + 7) / 8;  // blocks per XCD (k_mg_face_restrict's work-group order)
+  hipLaunchKernelGGL(k_mg_face_restrict, dim3(8 * per), dim3(256), 0, s, A);
 }
 
 void launch_mg_scatter(const MGGeom& m, const int* nbr, const int* colour, int col, int dof, const float2* y,

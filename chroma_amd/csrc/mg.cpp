@@ -378,12 +378,13 @@ void mg_update(Op& op, const cmx_mg_params& p) {
     st.pk.emplace_back(cv);
     st.qk.emplace_back(cv);
   }
-  // the set-up fields (setup_maxit, seed) keep the values the null space was built with
-  const int setup_maxit = st.par.setup_maxit;
+  // the set-up fields (setup_maxit, seed, prec) keep the values the null space was built with
+  const int setup_maxit = st.par.setup_maxit, prec = st.par.prec;
   const unsigned long long seed = st.par.seed;
   st.par = p;
   st.par.setup_maxit = setup_maxit;
   st.par.seed = seed;
+  st.par.prec = prec;
 }
 
 void mg_info(Op& op, int* nb, int* d) {

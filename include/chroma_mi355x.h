@@ -358,9 +358,9 @@ typedef struct {
 int cmx_mg_params_default(cmx_mg_params* p);
 int cmx_mg_setup(cmx_op op, const cmx_mg_params* p, cmx_mg_result* res);
 /* Changes the V-cycle of a set-up multigrid without rebuilding it: nu_pre, nu_post, omega, coarse_tol,
- * coarse_maxit, coarse_nkrylov, verbose (the role of updateMultigridQuda, which Chroma's QUDA multigrid
+ * coarse_maxit, coarse_nkrylov, smoother, verbose (the role of updateMultigridQuda, which Chroma's QUDA multigrid
  * solver calls when it recovers a saved subspace, syssolver_linop_clover_quda_multigrid_w.h:500-508).  blocking and
- * nvec must equal the set-up's (CMX_ERR_ARG otherwise); setup_maxit and seed are ignored. */
+ * nvec must equal the set-up's (CMX_ERR_ARG otherwise); setup_maxit, seed and prec are ignored. */
 int cmx_mg_update(cmx_op op, const cmx_mg_params* p);
 /* Inspection and test access: the null vectors after block orthonormalisation (nvec * V * 24 doubles,
  * host order), the coarse operator (n_blocks * 9 * D * D complex: block, stencil entry -- the block,

@@ -384,6 +384,13 @@ struct FgmresDR::Impl {
     return true;
   }
 
+  bool mg_step_io(const Vec*& v, Vec*& z) const {
+    if (p.precond != CMX_PRECOND_MG || pb != ps) return false;
+    v = V[j];
+    z = Z[j];
+    return true;
+  }
+
   bool apply_step_io(const Vec*& z, Vec*& w) const {
     if (pb != ps) return false;
     z = Z[j];
@@ -503,6 +510,7 @@ void FgmresDR::issue_step(bool preconditioned, bool applied) { m_->issue_step(pr
 bool FgmresDR::apply_step_io(const Vec*& z, Vec*& w) const { return m_->apply_step_io(z, w); }
 void FgmresDR::issue_precondition() { m_->precondition(*m_->Z[m_->j], *m_->V[m_->j]); }
 bool FgmresDR::learned_step_io(const Vec*& v, Vec*& z) const { return m_->learned_step_io(v, z); }
+bool FgmresDR::mg_step_io(const Vec*& v, Vec*& z) const { return m_->mg_step_io(v, z); }
 bool FgmresDR::finish_step() { return m_->finish_step(); }
 void FgmresDR::end_cycle() { m_->end_cycle(); }
 void FgmresDR::result(cmx_solver_result& res) { m_->result(res); }

@@ -60,6 +60,8 @@ class FgmresDR {
   // to basis vectors (precond LEARNED, basis precision = operator precision): a batched caller applies
   // it for several solves at once and then calls issue_step(true)
   bool learned_step_io(const Vec*& v, Vec*& z) const;
+  // the same for the multigrid preconditioner (precond MG, basis precision = operator precision)
+  bool mg_step_io(const Vec*& v, Vec*& z) const;
   // z_j = M v_j of the next step alone (then issue_step(true, ...))
   void issue_precondition();
   // the next step's operator input z_j and output w when the operator acts on basis vectors directly

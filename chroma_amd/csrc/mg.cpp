@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "batch.h"
 #include "fgmres.h"
 #include "mg.h"
 #include "setup.h"
@@ -385,6 +386,41 @@ void mg_update(Op& op, const cmx_mg_params& p) {
   st.par.setup_maxit = setup_maxit;
   st.par.seed = seed;
   st.par.prec = prec;
+}
+
+void mg_precondition_batch(const std::vector<Op*>& ops, int prec, const std::vector<const Vec*>& v,
+                           const std::vector<Vec*>& z, BatchPlan& plan) {
+  MGState& st = state(*ops[0]);
+  const MGGeom& m = st.m;
+  const int n = (int)ops[0]->Vh;
+  if (ops.size() < 2 || st.par.nu_pre > 0 || st.par.smoother != 0) {
+    for (size_t i = 0; i < ops.size(); ++i) mg_precondition(*ops[i], prec, *v[i], *z[i]);
+    return;
+  }
+  // the coarse correction, one source after another on the shared stream (the coarse work space is shared)
+  for (size_t i = 0; i < ops.size(); ++i) {
+    Op& op = *ops[i];
+    CMX_REQUIRE(op.mg.get() == ops[0]->mg.get() && op.stream == ops[0]->stream, CMX_ERR_ARG,
+                "multigrid batch: contexts must share the multigrid and the stream");
+    CMX_REQUIRE(v[i]->prec == prec && z[i]->prec == prec && v[i]->n == n && z[i]->n == n, CMX_ERR_ARG,
+                "multigrid: field mismatch");
+    op.blas(B_ZERO, {z[i]});
+    launch_mg_restrict(m, st.pv, prec, v[i]->view(), false, st.bc.as<float2>(), op.stream);
+    coarse_gcr(st, op, st.bc.as<float2>(), st.xc.as<float2>(), prec == P16);
+    launch_mg_prolong(m, st.pv, prec, st.xc.as<float2>(), z[i]->view(), false, true, op.stream);
+    CMX_HIP(hipGetLastError());
+  }
+  if (st.par.nu_post <= 0) return;
+  // the MR smoothing of every source, recorded per source and run as one batched grid per kernel
+  std::vector<Recorder> recs(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) {
+    Op& op = *ops[i];
+    Vec &r = op.work(310, prec, n), &mr = op.work(311, prec, n);
+    RecordScope rs(recs[i]);
+    MRPrecond(op, prec, n, st.par.nu_post, 0.0, st.par.omega, r, mr).apply(*z[i], *v[i]);
+  }
+  plan.refresh(recs, ops[0]->stream);
+  plan.run(ops[0]->stream);
 }
 
 void mg_info(Op& op, int* nb, int* d) {

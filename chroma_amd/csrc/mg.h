@@ -97,6 +97,12 @@ void mg_setup(Op& op, const cmx_mg_params& p, cmx_mg_result& res);
 void mg_update(Op& op, const cmx_mg_params& p);
 // z = K v: one V-cycle on the odd-checkerboard system at precision prec (v, z in that precision)
 void mg_precondition(Op& op, int prec, const Vec& v, Vec& z);
+class BatchPlan;
+// z[i] = K v[i] for solver contexts ops[i] that share one multigrid, stream and launch shape: the
+// restriction, coarse solve and prolongation one source after another, the MR smoothing steps of all
+// sources in one batched grid per kernel (plan); each z[i] equals mg_precondition's bit for bit
+void mg_precondition_batch(const std::vector<Op*>& ops, int prec, const std::vector<const Vec*>& v,
+                           const std::vector<Vec*>& z, BatchPlan& plan);
 // test and inspection access: null vectors (host order, FP64 copies), the coarse operator, and the
 // restriction / prolongation / coarse operator on host arrays
 void mg_get_nullvecs(Op& op, double* out);

@@ -22,6 +22,7 @@
 
 #include "batch.h"
 #include "fgmres.h"
+#include "mg.h"
 #include "op.h"
 
 namespace cmx {
@@ -695,7 +696,7 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
     return true;
   };
 
-  BatchPlan apply_plan;
+  BatchPlan apply_plan, mg_plan;
   resume();
   long ticks = 0;
   while (!pending.empty() || !active.empty()) {
@@ -741,6 +742,32 @@ void solve_batch_fgmres(const std::vector<Op*>& ops, const cmx_solver_params& p,
       }
       if (who.size() >= 2) {
         learned_precond_apply_batch(*ctx[who[0]], p.sloppy_prec, vin, zout);
+        for (int i : who) pre[i] = 1;
+      }
+    }
+    // the multigrid V-cycles of the solves that share one multigrid: the smoothing of all of them batched
+    {
+      std::vector<const Vec*> vin;
+      std::vector<Vec*> zout;
+      std::vector<Op*> ops;
+      std::vector<int> who;
+      for (int i : active) {
+        const Vec* v = nullptr;
+        Vec* z = nullptr;
+        if (pre[i] || !run[i]->mg_step_io(v, z)) continue;
+        if (!who.empty()) {
+          const Op &a = *ctx[who[0]], &b = *ctx[i];
+          if (a.mg.get() != b.mg.get() || a.gd != b.gd || a.stream != b.stream ||
+              std::memcmp(&a.hcfg[p.sloppy_prec][0], &b.hcfg[p.sloppy_prec][0], sizeof(HopConfig)) != 0)
+            continue;
+        }
+        vin.push_back(v);
+        zout.push_back(z);
+        ops.push_back(ctx[i]);
+        who.push_back(i);
+      }
+      if (who.size() >= 2) {
+        mg_precondition_batch(ops, p.sloppy_prec, vin, zout, mg_plan);
         for (int i : who) pre[i] = 1;
       }
     }

@@ -239,9 +239,9 @@ def train_net(cmx, mass, csw, a, depth=1, pairs_per_source=4, lr=1e-3):
 
 
 FGMRES_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, basis_prec=2, n_krylov=16, cycle_reduction=3e-3)
-# two-level multigrid: 4^4 blocks, 8 null vectors (16 coarse degrees of freedom per block), 7 MR post-smoothing steps
+# two-level multigrid: 4^4 blocks, 6 null vectors (12 coarse degrees of freedom per block), 8 MR post-smoothing steps
 # at FP16 storage, coarse GCR to 0.1 or 8 steps; the outer FGMRES-DR restarts every 8 steps
-MG_KW = dict(nvec=8, nu_pre=0, nu_post=7, setup_maxit=100, coarse_tol=0.1, coarse_maxit=8, seed=3)
+MG_KW = dict(nvec=6, nu_pre=0, nu_post=8, setup_maxit=100, coarse_tol=0.1, coarse_maxit=8, seed=3)
 FGMRES_MG_KW = dict(rsd=1e-10, maxit=20000, sloppy_prec=2, n_krylov=8)
 
 
